@@ -1,0 +1,151 @@
+/* sfgs.h -- C-ABI of the MI355X (gfx950) contact-solve library libsfgs.so.
+ *
+ * The library replaces, for rigid bodies and Signorini-Coulomb contacts, the
+ * Solfec-1.0 hot path between DOM_Update_Begin and DOM_Update_End
+ * (sol.c:764-773): LOCDYN assembly, the block Gauss-Seidel solve with its
+ * local 3x3 solvers, the merit function, and the body velocity update by the
+ * reactions.  All signatures use plain pointers and sizes; host arrays are in
+ * natural contact / body order; 3x3 matrices are column-major, contact frames
+ * have columns [t1 t2 n] as in the reference (alg.h:881-953, dom.c:133-151).
+ *
+ * Every call is synchronous with respect to the host arrays it reads or
+ * writes.  A context owns its device memory and one HIP stream of default
+ * priority; calls on one context must not overlap.  Functions return 0 on
+ * success and a negative SFGS_E* code on failure (SFGS_Error_String). */
+#ifndef SFGS_H
+#define SFGS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sfgs_ctx sfgs_ctx;
+
+/* error codes */
+#define SFGS_OK            0
+#define SFGS_E_HIP        -1   /* a HIP runtime call failed */
+#define SFGS_E_ARG        -2   /* invalid argument or size */
+#define SFGS_E_STATE      -3   /* call out of order (e.g. solve before assembly) */
+#define SFGS_E_COLOURS    -4   /* contact graph needs more colours than supported */
+#define SFGS_E_SINGULAR   -5   /* a diagonal block W_ii is singular */
+
+/* local solvers: numbering of DIAS, dbs.h:27-33 */
+#define SFGS_DS_PROJECTED_GRADIENT 0
+#define SFGS_DS_DE_SAXCE_FENG      1
+#define SFGS_DS_SEMISMOOTH_NEWTON  2
+#define SFGS_DS_PROJECTED_NEWTON   3
+
+/* failure actions: GSFAIL, bgs.h:39-44 */
+#define SFGS_FAILURE_CONTINUE 0
+#define SFGS_FAILURE_EXIT     1
+#define SFGS_FAILURE_CALLBACK 2
+
+/* solver status: GSERROR, bgs.h:31-37 */
+#define SFGS_GS_OK                0
+#define SFGS_GS_DIVERGED          1
+#define SFGS_GS_DIAGONAL_DIVERGED 2
+#define SFGS_GS_DIAGONAL_FAILED   3
+
+/* sweep precision */
+#define SFGS_PREC_FP64  0   /* FP64 storage and arithmetic (the reference) */
+#define SFGS_PREC_MIXED 1   /* FP32 W/B/R sweeps, FP64 merit, FP64 refinement sweeps */
+
+/* Gauss-Seidel parameters: the fields of struct gs, bgs.h:64-111, with the
+ * defaults of GAUSS_SEIDEL_SOLVER, lng.c:4243-4349 */
+typedef struct {
+  double epsilon;      /* relative error bound */
+  int maxiter;         /* iteration bound */
+  double meritval;     /* merit bound; >= 1 switches the merit off (bgs.c:927) */
+  int failure;         /* SFGS_FAILURE_* */
+  double diagepsilon;  /* local solver accuracy */
+  int diagmaxiter;     /* local solver iteration bound */
+  int diagsolver;      /* SFGS_DS_* */
+  int reverse;         /* 1: alternate forward / backward sweeps (bgs.c:936-950) */
+  int precision;       /* SFGS_PREC_* */
+  int refine;          /* FP64 refinement sweeps after a mixed solve (default 2) */
+} SFGS_PARAM;
+
+typedef struct {
+  int iters;           /* sweeps performed (gs->iters) */
+  int error;           /* SFGS_GS_* (gs->error) */
+  double rel_error;    /* last relative error */
+  double merit;        /* last merit value (dom->merit) */
+  int ncolours;        /* colours of the contact graph */
+  int fail_rows;       /* local-solver failures (callback invocations) */
+  int aborted;         /* 1 when FAILURE_EXIT stopped the solve */
+  double solve_ms;     /* device time of the solve (HIP events) */
+  double sweep_ms;     /* device time spent in sweep kernels (HIP events) */
+  long long row_updates; /* rows visited = sweeps x rows */
+  long long nnz;       /* off-diagonal blocks */
+} SFGS_REPORT;
+
+/* context: LOCDYN_Create, ldy.h:93 (one device-resident W/B/R store) */
+sfgs_ctx* SFGS_Create(int device);
+void SFGS_Destroy(sfgs_ctx *ctx);
+const char* SFGS_Error_String(int code);
+const char* SFGS_Last_Error(sfgs_ctx *ctx);
+
+/* Rigid bodies, natural order (bod.h BODY of kind RIG/OBS): kind 0 = RIG,
+ * 1 = OBS; rot 9, x0 3 (referential centre), jinv 9 (inverse referential
+ * inertia), minv 1, velo 6 = [W v] after BODY_Dynamic_Step_Begin (bod.c:1313-1373),
+ * velo0 6 = [W v] at the start of the step. */
+int SFGS_Bodies_Upload(sfgs_ctx *ctx, int nb, const int *kind, const double *rot, const double *x0,
+                       const double *jinv, const double *minv, const double *velo, const double *velo0);
+
+/* Contacts, natural order (CON, dom.h:77-137): master/slave body indices,
+ * referential points mpnt/spnt 3 each, base 9, gap, friction, restitution,
+ * cohesion (times area), and the carried-over reactions R and relative
+ * velocities U, 3 each (zero for new contacts, mem.c:77,86). */
+int SFGS_Contacts_Upload(sfgs_ctx *ctx, int nc, const int *master, const int *slave,
+                         const double *mpnt, const double *spnt, const double *base,
+                         const double *gap, const double *friction, const double *restitution,
+                         const double *cohesion, const double *R, const double *U);
+
+/* LOCDYN_Update_Begin (ldy.h:103, ldy.c:626-876): contact adjacency through
+ * shared non-obstacle bodies (LOCDYN_Insert, ldy.c:492-568), contact-graph
+ * colouring, colour-major block-sparse W, W_ii, A = inv(W_ii), rho, B, V and
+ * the free energy, all on the device.  dynamic = 1 for DYNAMIC analysis. */
+int SFGS_Locdyn_Update_Begin(sfgs_ctx *ctx, double step, int dynamic);
+
+/* Alternative to the two uploads + assembly: install an already assembled
+ * system (natural order, off-diagonal blocks in CSR rowptr/col, 9 per block)
+ * together with R and U.  Used to solve identical W, B, R0 on CPU and GPU. */
+int SFGS_System_Upload(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, const double *W,
+                       const double *Wd, const double *A, const double *B, const double *V,
+                       const double *rho, const double *friction, const double *restitution,
+                       const double *cohesion, const double *gap, double free_energy,
+                       double step, int dynamic, const double *R, const double *U);
+
+/* GAUSS_SEIDEL_Solve (bgs.h:119, serial bgs.c:913-1069) on the current
+ * system.  rerhist / merhist (may be NULL) receive par->maxiter doubles. */
+int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *merhist, SFGS_REPORT *rep);
+
+/* MERIT_Function (mrf.h:31, mrf.c:35-184) for the current R; update_U as in
+ * the reference (U recomputed from all R); conmerit (may be NULL) receives the
+ * per-contact merit (con->merit). */
+int SFGS_Merit(sfgs_ctx *ctx, int update_U, double *merit, double *conmerit);
+
+/* Download the current R, U (natural order, 3 per contact); NULL skips. */
+int SFGS_Get_RU(sfgs_ctx *ctx, double *R, double *U);
+
+/* Download the assembled LOCDYN in natural order for inspection: rowptr
+ * (n+1), col/via (nnz), W (9 nnz), Wd, A (9 n), B, V (3 n), rho (n), the free
+ * energy and the colour of every contact; NULL skips an array. */
+int SFGS_Get_Locdyn(sfgs_ctx *ctx, int *rowptr, int *col, int *via, double *W, double *Wd,
+                    double *A, double *B, double *V, double *rho, double *free_energy, int *colour);
+int SFGS_Get_Sizes(sfgs_ctx *ctx, int *n, long long *nnz, int *ncolours);
+
+/* Colour an arbitrary graph given in CSR (no self loops) with the device
+ * colouring kernel; returns the number of colours or a negative code. */
+int SFGS_Colour(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, int *colour);
+
+/* Body velocity update of BODY_Dynamic_Step_End (bod.c:1446-1447 with
+ * rig_constraints_force, bod.c:418-435): velo += h iM r, r = sum(+-H' R) over
+ * the contacts of every rigid body, using the current R; velo_out receives 6
+ * per body, force_out (may be NULL) the generalised reaction r, 6 per body. */
+int SFGS_Body_Update(sfgs_ctx *ctx, double step, double *velo_out, double *force_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,109 @@
+/* solfec.h -- host C layer (libsolfec.so) of the MI355X contact-solve build.
+ *
+ * It keeps the reference's solver object and time-stepping contract:
+ *   GAUSS_SEIDEL_Create / Destroy / Solve and the string queries replace
+ *   bgs.h:114-140 (bgs.c:443-472, 913-1069, 1072-1155);
+ *   SOLFEC_Run replaces sol.c:665-825 for a GAUSS_SEIDEL_SOLVER, one time step
+ *   being DOM_Update_Begin, LOCDYN_Update_Begin, GAUSS_SEIDEL_Solve,
+ *   LOCDYN_Update_End, DOM_Update_End (sol.c:746-806).
+ * The scene it steps is a minimal rigid-body domain: spheres (RIG) and fixed
+ * half-space planes (OBS), Signorini-Coulomb contacts, gravity.  The contact
+ * solve itself (assembly, Gauss-Seidel, body velocity update) runs on the GPU
+ * through sfgs.h; this layer fails loudly when the GPU library cannot run.
+ *
+ * Errors: functions returning int give 0 on success; SOLFEC_Run returns the
+ * SOL_ERR_* code that the reference would THROW (err.h), and RUN in the
+ * Python layer turns it into RuntimeError(SOLFEC_Error_String(code)). */
+#ifndef SOLFEC_H
+#define SOLFEC_H
+
+#include "sfgs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct solfec SOLFEC;
+typedef struct gauss_seidel GAUSS_SEIDEL;
+typedef int (*GAUSS_SEIDEL_Callback)(void *data);  /* returns 0 to stop the program (lng.c:4213-4225) */
+
+/* error codes carried out of SOLFEC_Run (err.h names) */
+#define SOL_OK                              0
+#define SOL_ERR_GAUSS_SEIDEL_DIAGONAL_DIVERGED 1
+#define SOL_ERR_GAUSS_SEIDEL_DIVERGED          2
+#define SOL_ERR_DOM_DEPTH                      3
+#define SOL_ERR_GPU                            4
+#define SOL_ERR_CALLBACK_EXIT                  5
+#define SOL_ERR_ARG                            6
+
+/* body kinds */
+#define SOL_RIG 0
+#define SOL_OBS 1
+
+/* ---- GAUSS_SEIDEL object (bgs.h) ---- */
+GAUSS_SEIDEL* GAUSS_SEIDEL_Create(double epsilon, int maxiter, double meritval, int failure,
+                                  double diagepsilon, int diagmaxiter, int diagsolver,
+                                  void *data, GAUSS_SEIDEL_Callback callback);
+void GAUSS_SEIDEL_Destroy(GAUSS_SEIDEL *gs);
+/* member access (struct gs, bgs.h:64-111) */
+double GAUSS_SEIDEL_Get(GAUSS_SEIDEL *gs, const char *member);
+int GAUSS_SEIDEL_Set(GAUSS_SEIDEL *gs, const char *member, double value);
+const char* GAUSS_SEIDEL_Failure(GAUSS_SEIDEL *gs);
+const char* GAUSS_SEIDEL_Diagsolver(GAUSS_SEIDEL *gs);
+const char* GAUSS_SEIDEL_Error(GAUSS_SEIDEL *gs);
+const char* GAUSS_SEIDEL_Reverse(GAUSS_SEIDEL *gs);
+const char* GAUSS_SEIDEL_Variant(GAUSS_SEIDEL *gs);
+/* histories of the last solve: rerhist/merhist have 'iters' entries */
+int GAUSS_SEIDEL_History(GAUSS_SEIDEL *gs, double *rerhist, double *merhist);
+/* iteration-count history across solves (itershist, enabled by setting "itershist" = 1) */
+int GAUSS_SEIDEL_Itershist(GAUSS_SEIDEL *gs, int *out, int cap);
+/* report of the last device solve */
+int GAUSS_SEIDEL_Report(GAUSS_SEIDEL *gs, SFGS_REPORT *rep);
+
+/* ---- domain ---- */
+SOLFEC* SOLFEC_Create(int dynamic, double step, const char *outpath);
+void SOLFEC_Destroy(SOLFEC *sol);
+const char* SOLFEC_Error_String(int code);
+const char* SOLFEC_Last_Error(SOLFEC *sol);
+void SOLFEC_Gravity(SOLFEC *sol, double gx, double gy, double gz);
+/* surface pair material (SURFACE_MATERIAL, SIGNORINI_COULOMB); s1 = s2 = -1 sets the default */
+int SOLFEC_Surface_Material(SOLFEC *sol, int s1, int s2, double friction, double restitution, double cohesion);
+/* rigid sphere of given density; returns the body index */
+int SOLFEC_Sphere(SOLFEC *sol, const double *centre, double radius, double density, int surface);
+/* fixed half-space obstacle {x : (x - point).normal >= 0}; returns the body index */
+int SOLFEC_Plane(SOLFEC *sol, const double *point, const double *normal, int surface);
+/* set the velocity [W v] of a body (W referential angular, v spatial linear) */
+int SOLFEC_Set_Velocity(SOLFEC *sol, int body, const double *velo6);
+/* jittered cubic lattice of n spheres, radii U[rmin,rmax], in the box [0,L]^3
+ * bounded by 6 planes of surface 0; seeded (splitmix64). Returns 0. */
+int SOLFEC_Lattice_Scene(SOLFEC *sol, int n, double L, double rmin, double rmax, double jitter,
+                         double density, unsigned long long seed);
+
+/* SOLFEC_Run (sol.c:665-825): step until time >= t0 + duration */
+int SOLFEC_Run(SOLFEC *sol, GAUSS_SEIDEL *gs, double duration);
+/* single step phases (sol.c:761-773) for drivers that interpose work */
+int SOLFEC_Step_Begin(SOLFEC *sol);                    /* DOM_Update_Begin */
+int SOLFEC_Step_Solve(SOLFEC *sol, GAUSS_SEIDEL *gs);  /* LOCDYN_Update_Begin + solve (+ Update_End) */
+int SOLFEC_Step_End(SOLFEC *sol);                      /* DOM_Update_End */
+
+double SOLFEC_Time(SOLFEC *sol);
+int SOLFEC_Nbod(SOLFEC *sol);
+int SOLFEC_Ncon(SOLFEC *sol);
+/* wall-clock seconds accumulated per phase: "TIMINT", "CONUPD", "CONDET", "LOCDYN", "CONSOL" */
+double SOLFEC_Timing(SOLFEC *sol, const char *label);
+/* device context used by the domain (created on first use) */
+sfgs_ctx* SOLFEC_Gpu(SOLFEC *sol);
+
+/* state export (natural order); NULL skips an array */
+int SOLFEC_Get_Bodies(SOLFEC *sol, int *kind, double *rot, double *x, double *x0, double *jinv,
+                      double *minv, double *velo, double *velo0, double *radius);
+int SOLFEC_Get_Contacts(SOLFEC *sol, int *master, int *slave, double *mpnt, double *spnt, double *base,
+                        double *gap, double *friction, double *restitution, double *cohesion,
+                        double *R, double *U);
+/* replace the carried reactions / velocities (warm start) of the current contacts */
+int SOLFEC_Set_RU(SOLFEC *sol, const double *R, const double *U);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

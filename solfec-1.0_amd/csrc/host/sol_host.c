@@ -1,0 +1,759 @@
+/* sol_host.c -- minimal rigid-body domain and the time-stepping loop.
+ *
+ * Time step (sol.c:746-806):
+ *   DOM_Update_Begin   -> SOLFEC_Step_Begin: rigid half step and free velocity
+ *                         (BODY_Dynamic_Step_Begin, scheme RIG_NEG, bod.c:1313-1373),
+ *                         update of existing contacts (update_contact, dom.c:430-483),
+ *                         detection of new ones (overlap_create, dom.c:348-416;
+ *                         sphere pairs goc.c:421-455, plane-sphere goc.c:308-348),
+ *                         new contacts start with R = U = 0 (mem.c:77,86);
+ *   LOCDYN_Update_Begin + GAUSS_SEIDEL_Solve -> SOLFEC_Step_Solve (GPU, sfgs.h);
+ *   DOM_Update_End     -> SOLFEC_Step_End: body velocity update by the reactions
+ *                         (GPU) and the rigid end-of-step (BODY_Dynamic_Step_End,
+ *                         bod.c:1425-1492).
+ * Contacts are kept in an array in insertion order; persisting ones keep their
+ * slot, removed ones are compacted away, new ones are appended. */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include "solfec.h"
+#include "sol_internal.h"
+
+#define GEOMETRIC_EPSILON 1e-6   /* alg.c:24 */
+
+static double now_s(void)
+{
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return (double)t.tv_sec + 1e-9*(double)t.tv_nsec;
+}
+
+static void* xrealloc(void *p, size_t bytes)
+{
+  void *q = realloc(p, bytes ? bytes : 1);
+  if (!q) { fprintf(stderr, "solfec: out of memory (%zu bytes)\n", bytes); abort(); }
+  return q;
+}
+
+/* ---------------- small algebra ---------------- */
+
+static void mm(const double *A, const double *B, double *C)  /* C = A B, column-major */
+{
+  double T[9];
+  for (int c = 0; c < 3; c ++)
+    for (int r = 0; r < 3; r ++)
+      T[r + 3*c] = A[r]*B[3*c] + A[r + 3]*B[3*c + 1] + A[r + 6]*B[3*c + 2];
+  memcpy(C, T, sizeof(T));
+}
+static void mv(const double *A, const double *x, double *y)
+{
+  double t0 = A[0]*x[0] + A[3]*x[1] + A[6]*x[2];
+  double t1 = A[1]*x[0] + A[4]*x[1] + A[7]*x[2];
+  double t2 = A[2]*x[0] + A[5]*x[1] + A[8]*x[2];
+  y[0] = t0; y[1] = t1; y[2] = t2;
+}
+static void mtv(const double *A, const double *x, double *y)
+{
+  double t0 = A[0]*x[0] + A[1]*x[1] + A[2]*x[2];
+  double t1 = A[3]*x[0] + A[4]*x[1] + A[5]*x[2];
+  double t2 = A[6]*x[0] + A[7]*x[1] + A[8]*x[2];
+  y[0] = t0; y[1] = t1; y[2] = t2;
+}
+
+/* rotation exp([O]x) by the Rodrigues formula; series below 1e-4 rad^2 */
+static void expmap(const double *O, double *R)
+{
+  double t2 = O[0]*O[0] + O[1]*O[1] + O[2]*O[2], a, b;
+  if (t2 < 1e-4)
+  {
+    a = 1.0 - t2/6.0*(1.0 - t2/20.0*(1.0 - t2/42.0));        /* sin t / t */
+    b = 0.5*(1.0 - t2/12.0*(1.0 - t2/30.0*(1.0 - t2/56.0)));  /* (1 - cos t) / t^2 */
+  }
+  else
+  {
+    double t = sqrt(t2);
+    a = sin(t)/t;
+    b = (1.0 - cos(t))/t2;
+  }
+  double x = O[0], y = O[1], z = O[2];
+  R[0] = 1.0 - b*(y*y + z*z); R[3] = -a*z + b*x*y;          R[6] = a*y + b*x*z;
+  R[1] = a*z + b*x*y;         R[4] = 1.0 - b*(x*x + z*z);   R[7] = -a*x + b*y*z;
+  R[2] = -a*y + b*x*z;        R[5] = a*x + b*y*z;           R[8] = 1.0 - b*(x*x + y*y);
+}
+
+static void cross(const double *a, const double *b, double *c)
+{
+  double c0 = a[1]*b[2] - a[2]*b[1], c1 = a[2]*b[0] - a[0]*b[2], c2 = a[0]*b[1] - a[1]*b[0];
+  c[0] = c0; c[1] = c1; c[2] = c2;
+}
+
+/* local frame with n as the last column (localbase, dom.c:133-151) */
+static void localbase(const double *n, double *loc)
+{
+  const double e0[3] = {1.0, 0.0, 0.0}, e1[3] = {0.0, 1.0, 0.0};
+  loc[6] = n[0]; loc[7] = n[1]; loc[8] = n[2];
+  cross(e0, n, loc);
+  double len = sqrt(loc[0]*loc[0] + loc[1]*loc[1] + loc[2]*loc[2]);
+  if (len < GEOMETRIC_EPSILON)
+  {
+    cross(e1, n, loc);
+    len = sqrt(loc[0]*loc[0] + loc[1]*loc[1] + loc[2]*loc[2]);
+  }
+  loc[0] /= len; loc[1] /= len; loc[2] /= len;
+  cross(loc, n, loc + 3);
+}
+
+/* lexicographic comparison with the geometric tolerance (alg.h:35-52) */
+static int pntcmp(const double *a, const double *b)
+{
+  for (int k = 0; k < 3; k ++)
+  {
+    if (a[k] + GEOMETRIC_EPSILON < b[k] - GEOMETRIC_EPSILON) return -1;
+    if (a[k] - GEOMETRIC_EPSILON > b[k] + GEOMETRIC_EPSILON) return 1;
+  }
+  return 0;
+}
+
+/* ---------------- domain construction ---------------- */
+
+SOLFEC* SOLFEC_Create(int dynamic, double step, const char *outpath)
+{
+  if (!(step > 0.0)) return NULL;
+  SOLFEC *sol = calloc(1, sizeof(SOLFEC));
+  if (!sol) return NULL;
+  sol->dynamic = dynamic ? 1 : 0;
+  sol->step = step;
+  sol->time = 0.0;
+  sol->defmat.friction = 0.0;
+  sol->defmat.restitution = 0.0;
+  sol->defmat.cohesion = 0.0;
+  sol->device = 0;
+  snprintf(sol->outpath, sizeof(sol->outpath), "%s", outpath ? outpath : "");
+  return sol;
+}
+
+void SOLFEC_Destroy(SOLFEC *sol)
+{
+  if (!sol) return;
+  if (sol->gpu) SFGS_Destroy(sol->gpu);
+  free(sol->kind); free(sol->shape); free(sol->rad); free(sol->surf); free(sol->plane);
+  free(sol->rot); free(sol->x); free(sol->x0); free(sol->J); free(sol->jinv); free(sol->minv);
+  free(sol->velo); free(sol->velo0); free(sol->aux); free(sol->force); free(sol->mass);
+  free(sol->mat);
+  free(sol->c_master); free(sol->c_slave); free(sol->c_mpnt); free(sol->c_spnt); free(sol->c_base);
+  free(sol->c_gap); free(sol->c_fri); free(sol->c_rest); free(sol->c_coh); free(sol->c_R); free(sol->c_U);
+  free(sol->c_point);
+  free(sol->grid_head); free(sol->grid_next); free(sol->pair_keys); free(sol->pair_vals);
+  free(sol->tmp_velo);
+  free(sol);
+}
+
+const char* SOLFEC_Error_String(int code)
+{
+  switch (code)
+  {
+  case SOL_OK: return "OK";
+  case SOL_ERR_GAUSS_SEIDEL_DIAGONAL_DIVERGED: return "GAUSS_SEIDEL_SOLVER failed with error code DIAGONAL_DIVERGED";
+  case SOL_ERR_GAUSS_SEIDEL_DIVERGED: return "GAUSS_SEIDEL_SOLVER failed with error code DIVERGED";
+  case SOL_ERR_DOM_DEPTH: return "Unphysical interpenetration has occurred";
+  case SOL_ERR_GPU: return "GPU contact solver failed";
+  case SOL_ERR_CALLBACK_EXIT: return "GAUSS_SEIDEL_SOLVER failure callback requested exit";
+  case SOL_ERR_ARG: return "invalid argument";
+  }
+  return "unknown error";
+}
+
+const char* SOLFEC_Last_Error(SOLFEC *sol) { return sol ? sol->err : "null"; }
+
+void SOLFEC_Gravity(SOLFEC *sol, double gx, double gy, double gz)
+{
+  sol->gravity[0] = gx; sol->gravity[1] = gy; sol->gravity[2] = gz;
+}
+
+int SOLFEC_Surface_Material(SOLFEC *sol, int s1, int s2, double friction, double restitution, double cohesion)
+{
+  if (!sol || friction < 0.0 || restitution < 0.0 || cohesion < 0.0) return SOL_ERR_ARG;
+  if (s1 < 0 && s2 < 0)
+  {
+    sol->defmat.friction = friction;
+    sol->defmat.restitution = restitution;
+    sol->defmat.cohesion = cohesion;
+    return 0;
+  }
+  sol->mat = xrealloc(sol->mat, sizeof(MATPAIR)*(size_t)(sol->nmat + 1));
+  MATPAIR *m = &sol->mat[sol->nmat ++];
+  m->s1 = s1 < s2 ? s1 : s2;
+  m->s2 = s1 < s2 ? s2 : s1;
+  m->friction = friction; m->restitution = restitution; m->cohesion = cohesion;
+  return 0;
+}
+
+static const MATPAIR* find_material(SOLFEC *sol, int s1, int s2)
+{
+  int a = s1 < s2 ? s1 : s2, b = s1 < s2 ? s2 : s1;
+  for (int k = 0; k < sol->nmat; k ++) if (sol->mat[k].s1 == a && sol->mat[k].s2 == b) return &sol->mat[k];
+  return &sol->defmat;
+}
+
+static int add_body(SOLFEC *sol)
+{
+  if (sol->nb == sol->bcap)
+  {
+    int cap = sol->bcap ? 2*sol->bcap : 64;
+    sol->kind = xrealloc(sol->kind, sizeof(int)*(size_t)cap);
+    sol->shape = xrealloc(sol->shape, sizeof(int)*(size_t)cap);
+    sol->surf = xrealloc(sol->surf, sizeof(int)*(size_t)cap);
+    sol->rad = xrealloc(sol->rad, sizeof(double)*(size_t)cap);
+    sol->plane = xrealloc(sol->plane, sizeof(double)*6*(size_t)cap);
+    sol->rot = xrealloc(sol->rot, sizeof(double)*9*(size_t)cap);
+    sol->x = xrealloc(sol->x, sizeof(double)*3*(size_t)cap);
+    sol->x0 = xrealloc(sol->x0, sizeof(double)*3*(size_t)cap);
+    sol->J = xrealloc(sol->J, sizeof(double)*9*(size_t)cap);
+    sol->jinv = xrealloc(sol->jinv, sizeof(double)*9*(size_t)cap);
+    sol->minv = xrealloc(sol->minv, sizeof(double)*(size_t)cap);
+    sol->mass = xrealloc(sol->mass, sizeof(double)*(size_t)cap);
+    sol->velo = xrealloc(sol->velo, sizeof(double)*6*(size_t)cap);
+    sol->velo0 = xrealloc(sol->velo0, sizeof(double)*6*(size_t)cap);
+    sol->aux = xrealloc(sol->aux, sizeof(double)*3*(size_t)cap);
+    sol->force = xrealloc(sol->force, sizeof(double)*6*(size_t)cap);
+    sol->tmp_velo = xrealloc(sol->tmp_velo, sizeof(double)*6*(size_t)cap);
+    sol->bcap = cap;
+  }
+  int b = sol->nb ++;
+  static const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  memcpy(sol->rot + 9*b, I, sizeof(I));
+  memset(sol->x + 3*b, 0, 3*sizeof(double));
+  memset(sol->x0 + 3*b, 0, 3*sizeof(double));
+  memset(sol->J + 9*b, 0, 9*sizeof(double));
+  memset(sol->jinv + 9*b, 0, 9*sizeof(double));
+  memset(sol->velo + 6*b, 0, 6*sizeof(double));
+  memset(sol->velo0 + 6*b, 0, 6*sizeof(double));
+  memset(sol->aux + 3*b, 0, 3*sizeof(double));
+  memset(sol->force + 6*b, 0, 6*sizeof(double));
+  memset(sol->plane + 6*b, 0, 6*sizeof(double));
+  sol->minv[b] = 0.0;
+  sol->mass[b] = 0.0;
+  sol->rad[b] = 0.0;
+  return b;
+}
+
+int SOLFEC_Sphere(SOLFEC *sol, const double *centre, double radius, double density, int surface)
+{
+  if (!sol || !(radius > 0.0) || !(density > 0.0)) return -1;
+  int b = add_body(sol);
+  /* sphere mass characteristics (sph.c:167-238): m = rho 4/3 pi r^3, J = 2/5 m r^2 I */
+  double m = density * (4.0/3.0) * M_PI * radius*radius*radius, j = 0.4*m*radius*radius;
+  sol->kind[b] = SOL_RIG;
+  sol->shape[b] = 0;
+  sol->surf[b] = surface;
+  sol->rad[b] = radius;
+  memcpy(sol->x + 3*b, centre, 3*sizeof(double));
+  memcpy(sol->x0 + 3*b, centre, 3*sizeof(double));
+  sol->J[9*b] = sol->J[9*b + 4] = sol->J[9*b + 8] = j;
+  sol->jinv[9*b] = sol->jinv[9*b + 4] = sol->jinv[9*b + 8] = 1.0/j;
+  sol->mass[b] = m;
+  sol->minv[b] = 1.0/m;
+  return b;
+}
+
+int SOLFEC_Plane(SOLFEC *sol, const double *point, const double *normal, int surface)
+{
+  double len = sqrt(normal[0]*normal[0] + normal[1]*normal[1] + normal[2]*normal[2]);
+  if (!sol || !(len > 0.0)) return -1;
+  int b = add_body(sol);
+  sol->kind[b] = SOL_OBS;
+  sol->shape[b] = 1;
+  sol->surf[b] = surface;
+  memcpy(sol->x + 3*b, point, 3*sizeof(double));
+  memcpy(sol->x0 + 3*b, point, 3*sizeof(double));
+  memcpy(sol->plane + 6*b, point, 3*sizeof(double));
+  for (int k = 0; k < 3; k ++) sol->plane[6*b + 3 + k] = normal[k]/len;
+  return b;
+}
+
+int SOLFEC_Set_Velocity(SOLFEC *sol, int body, const double *velo6)
+{
+  if (!sol || body < 0 || body >= sol->nb) return SOL_ERR_ARG;
+  memcpy(sol->velo + 6*body, velo6, 6*sizeof(double));
+  return 0;
+}
+
+/* splitmix64 (S. Vigna) -> uniform double in [0,1) */
+static unsigned long long splitmix64(unsigned long long *s)
+{
+  unsigned long long z = (*s += 0x9e3779b97f4a7c15ULL);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+static double urand(unsigned long long *s) { return (double)(splitmix64(s) >> 11) * (1.0/9007199254740992.0); }
+
+int SOLFEC_Lattice_Scene(SOLFEC *sol, int n, double L, double rmin, double rmax, double jitter,
+                         double density, unsigned long long seed)
+{
+  if (!sol || n <= 0 || !(L > 0.0) || !(rmin > 0.0) || rmax < rmin) return SOL_ERR_ARG;
+  int side = (int)ceil(cbrt((double)n) - 1e-9);
+  double a = L / side;
+  unsigned long long s = seed;
+  for (int i = 0; i < n; i ++)
+  {
+    int ix = i % side, iy = (i / side) % side, iz = i / (side*side);
+    double c[3] = {(ix + 0.5)*a, (iy + 0.5)*a, (iz + 0.5)*a};
+    for (int k = 0; k < 3; k ++) c[k] += (2.0*urand(&s) - 1.0)*jitter;
+    double r = rmin + (rmax - rmin)*urand(&s);
+    SOLFEC_Sphere(sol, c, r, density, 1);
+  }
+  const double P[6][6] = {
+    {0, 0, 0, 1, 0, 0}, {L, 0, 0, -1, 0, 0},
+    {0, 0, 0, 0, 1, 0}, {0, L, 0, 0, -1, 0},
+    {0, 0, 0, 0, 0, 1}, {0, 0, L, 0, 0, -1}};
+  for (int k = 0; k < 6; k ++) SOLFEC_Plane(sol, P[k], P[k] + 3, 0);
+  return 0;
+}
+
+/* ---------------- rigid body integration (scheme RIG_NEG) ---------------- */
+
+/* bod.c:1313-1373 for RIG, no point forces: gravity only */
+static void body_step_begin(SOLFEC *sol, int b)
+{
+  double h = sol->step, half = 0.5*h;
+  double *W = sol->velo + 6*b, *v = W + 3, *R = sol->rot + 9*b, *x = sol->x + 3*b;
+  double *J = sol->J + 9*b, *I = sol->jinv + 9*b, *A = sol->aux + 3*b;
+  memcpy(sol->velo0 + 6*b, W, 6*sizeof(double));
+  if (sol->kind[b] == SOL_OBS) return;   /* fixed obstacles */
+  double O[3] = {half*W[0], half*W[1], half*W[2]}, DR[9], JW[3], DJW[3], W05[3], T[3] = {0, 0, 0}, f[3];
+  expmap(O, DR);
+  mm(R, DR, R);                                   /* R(t+h/2) = R(t) exp[(h/2) W(t)] */
+  for (int k = 0; k < 3; k ++) x[k] += half*v[k]; /* x(t+h/2) = x(t) + (h/2) v(t) */
+  for (int k = 0; k < 3; k ++) f[k] = sol->mass[b]*sol->gravity[k];   /* no torque loads: T = 0 */
+  mv(J, W, JW);
+  mtv(DR, JW, DJW);                               /* exp[-(h/2)W] J W */
+  for (int k = 0; k < 3; k ++) A[k] = DJW[k] + h*T[k];
+  for (int k = 0; k < 3; k ++) O[k] = half*T[k] + DJW[k];
+  mv(I, O, W05);                                  /* W05 = I [exp[-(h/2)W] J W + (h/2) T] */
+  double JW05[3], c[3], dW[3];
+  mv(J, W05, JW05);
+  cross(W05, JW05, c);
+  for (int k = 0; k < 3; k ++) T[k] -= c[k];      /* T - W05 x J W05 */
+  mv(I, T, dW);
+  for (int k = 0; k < 3; k ++) { W[k] += h*dW[k]; v[k] += h*sol->minv[b]*f[k]; }   /* u += h iM f */
+}
+
+/* bod.c:1425-1492 for RIG: velocity already updated by the reactions (GPU) */
+static void body_step_end(SOLFEC *sol, int b)
+{
+  if (sol->kind[b] == SOL_OBS) return;
+  double h = sol->step, half = 0.5*h;
+  double *W = sol->velo + 6*b, *v = W + 3, *R = sol->rot + 9*b, *x = sol->x + 3*b;
+  double *I = sol->jinv + 9*b, *A = sol->aux + 3*b, *r = sol->force + 6*b;
+  for (int k = 0; k < 3; k ++) x[k] += half*v[k];
+  double O[3] = {half*W[0], half*W[1], half*W[2]}, DR[9], t[3];
+  expmap(O, DR);
+  mm(R, DR, R);
+  for (int k = 0; k < 3; k ++) A[k] += h*r[k];
+  mtv(DR, A, t);
+  mv(I, t, W);   /* W(t+h) = I exp[-(h/2)W1][exp[-(h/2)W(t)] J W(t) + h T] */
+}
+
+/* ---------------- contacts ---------------- */
+
+static void ensure_contacts(SOLFEC *sol, int n)
+{
+  if (n <= sol->ccap) return;
+  int cap = sol->ccap ? sol->ccap : 1024;
+  while (cap < n) cap *= 2;
+  sol->c_master = xrealloc(sol->c_master, sizeof(int)*(size_t)cap);
+  sol->c_slave = xrealloc(sol->c_slave, sizeof(int)*(size_t)cap);
+  sol->c_mpnt = xrealloc(sol->c_mpnt, sizeof(double)*3*(size_t)cap);
+  sol->c_spnt = xrealloc(sol->c_spnt, sizeof(double)*3*(size_t)cap);
+  sol->c_point = xrealloc(sol->c_point, sizeof(double)*3*(size_t)cap);
+  sol->c_base = xrealloc(sol->c_base, sizeof(double)*9*(size_t)cap);
+  sol->c_gap = xrealloc(sol->c_gap, sizeof(double)*(size_t)cap);
+  sol->c_fri = xrealloc(sol->c_fri, sizeof(double)*(size_t)cap);
+  sol->c_rest = xrealloc(sol->c_rest, sizeof(double)*(size_t)cap);
+  sol->c_coh = xrealloc(sol->c_coh, sizeof(double)*(size_t)cap);
+  sol->c_R = xrealloc(sol->c_R, sizeof(double)*3*(size_t)cap);
+  sol->c_U = xrealloc(sol->c_U, sizeof(double)*3*(size_t)cap);
+  sol->ccap = cap;
+}
+
+/* referential image of a spatial point (BODY_Ref_Point): X = X0 + R'(x - xc) */
+static void ref_point(SOLFEC *sol, int b, const double *p, double *X)
+{
+  double d[3] = {p[0] - sol->x[3*b], p[1] - sol->x[3*b + 1], p[2] - sol->x[3*b + 2]}, t[3];
+  mtv(sol->rot + 9*b, d, t);
+  for (int k = 0; k < 3; k ++) X[k] = sol->x0[3*b + k] + t[k];
+}
+
+/* geometry of a sphere pair given as (slave a, master b): returns 1 when in
+ * contact (gap < GEOMETRIC_EPSILON, goc.c:658-677 / gjk.c:560-584) */
+static int sphere_pair(SOLFEC *sol, int a, int b, double *spnt, double *mpnt, double *normal, double *gap)
+{
+  const double *ca = sol->x + 3*a, *cb = sol->x + 3*b;
+  double ra = sol->rad[a], rb = sol->rad[b];
+  double d[3] = {ca[0] - cb[0], ca[1] - cb[1], ca[2] - cb[2]};
+  double dl = sqrt(d[0]*d[0] + d[1]*d[1] + d[2]*d[2]), g;
+  if (dl == 0.0) return 0;
+  for (int k = 0; k < 3; k ++) { spnt[k] = ca[k] - ra/dl*d[k]; mpnt[k] = cb[k] + rb/dl*d[k]; }
+  if (dl > ra + rb) g = dl - (ra + rb);
+  else
+  {
+    for (int k = 0; k < 3; k ++) { spnt[k] = 0.5*(spnt[k] + mpnt[k]); mpnt[k] = spnt[k]; }
+    g = 0.0;
+  }
+  if (g >= GEOMETRIC_EPSILON) return 0;
+  double nl = 0.0;
+  for (int k = 0; k < 3; k ++) { normal[k] = spnt[k] - ca[k]; nl += normal[k]*normal[k]; }
+  nl = sqrt(nl);
+  for (int k = 0; k < 3; k ++) normal[k] /= nl;
+  double dn = (cb[0] - ca[0])*normal[0] + (cb[1] - ca[1])*normal[1] + (cb[2] - ca[2])*normal[2];
+  double e = ra + rb;
+  *gap = e > dn ? dn - e : 0.0;   /* sphere_sphere_gap, goc.c:50-59 */
+  return 1;
+}
+
+/* plane (slave) and sphere (master): goc.c:308-348 with the plane as the convex */
+static int plane_sphere(SOLFEC *sol, int pl, int s, double *spnt, double *mpnt, double *normal, double *gap)
+{
+  const double *P = sol->plane + 6*pl, *n = P + 3, *c = sol->x + 3*s;
+  double r = sol->rad[s];
+  double dist = (c[0] - P[0])*n[0] + (c[1] - P[1])*n[1] + (c[2] - P[2])*n[2];
+  if (dist >= r + GEOMETRIC_EPSILON) return 0;
+  for (int k = 0; k < 3; k ++)
+  {
+    normal[k] = n[k];
+    spnt[k] = c[k] - dist*n[k];
+    mpnt[k] = c[k] - r*n[k];
+  }
+  *gap = dist - r < 0.0 ? dist - r : 0.0;
+  return 1;
+}
+
+/* open-addressing map from body pair to contact slot */
+static unsigned long long pair_key(int a, int b)
+{
+  unsigned lo = (unsigned)(a < b ? a : b), hi = (unsigned)(a < b ? b : a);
+  return ((unsigned long long)lo << 32) | hi;
+}
+static unsigned long long mix64(unsigned long long z)
+{
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+static void map_build(SOLFEC *sol)
+{
+  size_t cap = 16;
+  while (cap < 2*(size_t)sol->nc + 16) cap *= 2;
+  if (cap > sol->pair_cap)
+  {
+    sol->pair_keys = xrealloc(sol->pair_keys, sizeof(unsigned long long)*cap);
+    sol->pair_vals = xrealloc(sol->pair_vals, sizeof(int)*cap);
+    sol->pair_cap = cap;
+  }
+  sol->pair_mask = cap - 1;
+  for (size_t k = 0; k <= sol->pair_mask; k ++) sol->pair_keys[k] = ~0ULL;
+  for (int i = 0; i < sol->nc; i ++)
+  {
+    unsigned long long key = pair_key(sol->c_master[i], sol->c_slave[i]);
+    size_t h = mix64(key) & sol->pair_mask;
+    while (sol->pair_keys[h] != ~0ULL) h = (h + 1) & sol->pair_mask;
+    sol->pair_keys[h] = key;
+    sol->pair_vals[h] = i;
+  }
+}
+static int map_find(SOLFEC *sol, int a, int b)
+{
+  unsigned long long key = pair_key(a, b);
+  size_t h = mix64(key) & sol->pair_mask;
+  while (sol->pair_keys[h] != ~0ULL)
+  {
+    if (sol->pair_keys[h] == key) return sol->pair_vals[h];
+    h = (h + 1) & sol->pair_mask;
+  }
+  return -1;
+}
+
+static void set_contact(SOLFEC *sol, int i, int master, int slave, const double *mp, const double *sp,
+                        const double *normal, double gap)
+{
+  sol->c_master[i] = master;
+  sol->c_slave[i] = slave;
+  memcpy(sol->c_point + 3*i, mp, 3*sizeof(double));
+  ref_point(sol, master, mp, sol->c_mpnt + 3*i);
+  ref_point(sol, slave, sp, sol->c_spnt + 3*i);
+  localbase(normal, sol->c_base + 9*i);
+  sol->c_gap[i] = gap;
+  const MATPAIR *m = find_material(sol, sol->surf[master], sol->surf[slave]);
+  sol->c_fri[i] = m->friction;
+  sol->c_rest[i] = m->restitution;
+  sol->c_coh[i] = m->cohesion;   /* area = 1 for sphere contacts (goc.c:450) */
+}
+
+/* update existing contacts (keep slot, R, U; remove separated), then detect
+ * new ones with a uniform grid over sphere centres */
+static int update_and_detect(SOLFEC *sol)
+{
+  double t0 = now_s();
+  int nc = sol->nc, kept = 0;
+  double sp[3], mp[3], nrm[3], g;
+  for (int i = 0; i < nc; i ++)
+  {
+    int m = sol->c_master[i], s = sol->c_slave[i], ok;
+    if (sol->shape[s] == 1) ok = plane_sphere(sol, s, m, sp, mp, nrm, &g);
+    else ok = sphere_pair(sol, s, m, sp, mp, nrm, &g);
+    if (!ok) continue;
+    if (kept != i)
+    {
+      memcpy(sol->c_R + 3*kept, sol->c_R + 3*i, 3*sizeof(double));
+      memcpy(sol->c_U + 3*kept, sol->c_U + 3*i, 3*sizeof(double));
+    }
+    set_contact(sol, kept, m, s, mp, sp, nrm, g);
+    kept ++;
+  }
+  sol->nc = kept;
+  double t1 = now_s();
+  sol->t_conupd += t1 - t0;
+
+  /* broad phase: spheres hashed into cells of size 2 rmax + eps */
+  int nb = sol->nb;
+  double rmax = 0.0, lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int b = 0; b < nb; b ++)
+    if (sol->shape[b] == 0)
+    {
+      if (sol->rad[b] > rmax) rmax = sol->rad[b];
+      for (int k = 0; k < 3; k ++)
+      {
+        if (sol->x[3*b + k] < lo[k]) lo[k] = sol->x[3*b + k];
+        if (sol->x[3*b + k] > hi[k]) hi[k] = sol->x[3*b + k];
+      }
+    }
+  map_build(sol);
+  if (rmax > 0.0)
+  {
+    double cell = 2.0*rmax + 2.0*GEOMETRIC_EPSILON;
+    int dim[3];
+    long long ncell = 1;
+    for (int k = 0; k < 3; k ++)
+    {
+      dim[k] = (int)floor((hi[k] - lo[k])/cell) + 1;
+      if (dim[k] < 1) dim[k] = 1;
+      ncell *= dim[k];
+    }
+    while (ncell > 4LL*nb + 1024)   /* very sparse scenes: coarser grid */
+    {
+      cell *= 2.0;
+      ncell = 1;
+      for (int k = 0; k < 3; k ++) { dim[k] = (int)floor((hi[k] - lo[k])/cell) + 1; ncell *= dim[k]; }
+    }
+    if ((size_t)ncell > sol->grid_cap)
+    {
+      sol->grid_head = xrealloc(sol->grid_head, sizeof(int)*(size_t)ncell);
+      sol->grid_cap = (size_t)ncell;
+    }
+    if ((size_t)nb > sol->gnext_cap)
+    {
+      sol->grid_next = xrealloc(sol->grid_next, sizeof(int)*(size_t)nb);
+      sol->gnext_cap = (size_t)nb;
+    }
+    for (long long c = 0; c < ncell; c ++) sol->grid_head[c] = -1;
+    /* insert in descending body order so that cell lists run in ascending order */
+    for (int b = nb - 1; b >= 0; b --)
+    {
+      if (sol->shape[b] != 0) continue;
+      int ci[3];
+      for (int k = 0; k < 3; k ++) ci[k] = (int)floor((sol->x[3*b + k] - lo[k])/cell);
+      long long c = ((long long)ci[2]*dim[1] + ci[1])*dim[0] + ci[0];
+      sol->grid_next[b] = sol->grid_head[c];
+      sol->grid_head[c] = b;
+    }
+    for (int a = 0; a < nb; a ++)
+    {
+      if (sol->shape[a] != 0) continue;
+      int ci[3];
+      for (int k = 0; k < 3; k ++) ci[k] = (int)floor((sol->x[3*a + k] - lo[k])/cell);
+      for (int dz = -1; dz <= 1; dz ++)
+        for (int dy = -1; dy <= 1; dy ++)
+          for (int dx = -1; dx <= 1; dx ++)
+          {
+            int cx = ci[0] + dx, cy = ci[1] + dy, cz = ci[2] + dz;
+            if (cx < 0 || cy < 0 || cz < 0 || cx >= dim[0] || cy >= dim[1] || cz >= dim[2]) continue;
+            for (int b = sol->grid_head[((long long)cz*dim[1] + cy)*dim[0] + cx]; b >= 0; b = sol->grid_next[b])
+            {
+              if (b <= a) continue;
+              if (map_find(sol, a, b) >= 0) continue;
+              /* orientation rule of detect_sphere_sphere (goc.c:421-455): the body with the
+               * lexicographically larger centre is the master */
+              int slave = a, master = b;
+              if (pntcmp(sol->x + 3*a, sol->x + 3*b) > 0) { slave = b; master = a; }
+              if (!sphere_pair(sol, slave, master, sp, mp, nrm, &g)) continue;
+              ensure_contacts(sol, sol->nc + 1);
+              int i = sol->nc ++;
+              set_contact(sol, i, master, slave, mp, sp, nrm, g);
+              memset(sol->c_R + 3*i, 0, 3*sizeof(double));
+              memset(sol->c_U + 3*i, 0, 3*sizeof(double));
+            }
+          }
+    }
+  }
+  /* planes against all spheres */
+  for (int p = 0; p < nb; p ++)
+  {
+    if (sol->shape[p] != 1) continue;
+    for (int s = 0; s < nb; s ++)
+    {
+      if (sol->shape[s] != 0) continue;
+      const double *P = sol->plane + 6*p, *c = sol->x + 3*s;
+      double dist = (c[0] - P[0])*P[3] + (c[1] - P[1])*P[4] + (c[2] - P[2])*P[5];
+      if (dist >= sol->rad[s] + GEOMETRIC_EPSILON) continue;
+      if (map_find(sol, p, s) >= 0) continue;
+      if (!plane_sphere(sol, p, s, sp, mp, nrm, &g)) continue;
+      ensure_contacts(sol, sol->nc + 1);
+      int i = sol->nc ++;
+      set_contact(sol, i, s, p, mp, sp, nrm, g);
+      memset(sol->c_R + 3*i, 0, 3*sizeof(double));
+      memset(sol->c_U + 3*i, 0, 3*sizeof(double));
+    }
+  }
+  sol->t_condet += now_s() - t1;
+  return 0;
+}
+
+/* ---------------- GPU context ---------------- */
+
+sfgs_ctx* SOLFEC_Gpu(SOLFEC *sol)
+{
+  if (!sol->gpu)
+  {
+    sol->gpu = SFGS_Create(sol->device);
+    if (!sol->gpu) snprintf(sol->err, sizeof(sol->err), "no HIP device available for the contact solver");
+  }
+  return sol->gpu;
+}
+
+/* ---------------- time stepping ---------------- */
+
+int SOLFEC_Step_Begin(SOLFEC *sol)
+{
+  double t0 = now_s();
+  for (int b = 0; b < sol->nb; b ++) body_step_begin(sol, b);
+  sol->t_timint += now_s() - t0;
+  ensure_contacts(sol, 1);
+  return update_and_detect(sol);
+}
+
+int SOLFEC_Step_End(SOLFEC *sol)
+{
+  double t0 = now_s();
+  for (int b = 0; b < sol->nb; b ++) body_step_end(sol, b);
+  sol->time += sol->step;
+  sol->t_timint += now_s() - t0;
+  return 0;
+}
+
+double SOLFEC_Time(SOLFEC *sol) { return sol->time; }
+int SOLFEC_Nbod(SOLFEC *sol) { return sol->nb; }
+int SOLFEC_Ncon(SOLFEC *sol) { return sol->nc; }
+
+double SOLFEC_Timing(SOLFEC *sol, const char *label)
+{
+  if (!strcmp(label, "TIMINT")) return sol->t_timint;
+  if (!strcmp(label, "CONUPD")) return sol->t_conupd;
+  if (!strcmp(label, "CONDET")) return sol->t_condet;
+  if (!strcmp(label, "LOCDYN")) return sol->t_locdyn;
+  if (!strcmp(label, "CONSOL")) return sol->t_consol;
+  return 0.0;
+}
+
+int SOLFEC_Get_Bodies(SOLFEC *sol, int *kind, double *rot, double *x, double *x0, double *jinv,
+                      double *minv, double *velo, double *velo0, double *radius)
+{
+  int nb = sol->nb;
+  if (kind) memcpy(kind, sol->kind, sizeof(int)*(size_t)nb);
+  if (rot) memcpy(rot, sol->rot, sizeof(double)*9*(size_t)nb);
+  if (x) memcpy(x, sol->x, sizeof(double)*3*(size_t)nb);
+  if (x0) memcpy(x0, sol->x0, sizeof(double)*3*(size_t)nb);
+  if (jinv) memcpy(jinv, sol->jinv, sizeof(double)*9*(size_t)nb);
+  if (minv) memcpy(minv, sol->minv, sizeof(double)*(size_t)nb);
+  if (velo) memcpy(velo, sol->velo, sizeof(double)*6*(size_t)nb);
+  if (velo0) memcpy(velo0, sol->velo0, sizeof(double)*6*(size_t)nb);
+  if (radius) memcpy(radius, sol->rad, sizeof(double)*(size_t)nb);
+  return 0;
+}
+
+int SOLFEC_Get_Contacts(SOLFEC *sol, int *master, int *slave, double *mpnt, double *spnt, double *base,
+                        double *gap, double *friction, double *restitution, double *cohesion,
+                        double *R, double *U)
+{
+  size_t n = (size_t)sol->nc;
+  if (master) memcpy(master, sol->c_master, sizeof(int)*n);
+  if (slave) memcpy(slave, sol->c_slave, sizeof(int)*n);
+  if (mpnt) memcpy(mpnt, sol->c_mpnt, sizeof(double)*3*n);
+  if (spnt) memcpy(spnt, sol->c_spnt, sizeof(double)*3*n);
+  if (base) memcpy(base, sol->c_base, sizeof(double)*9*n);
+  if (gap) memcpy(gap, sol->c_gap, sizeof(double)*n);
+  if (friction) memcpy(friction, sol->c_fri, sizeof(double)*n);
+  if (restitution) memcpy(restitution, sol->c_rest, sizeof(double)*n);
+  if (cohesion) memcpy(cohesion, sol->c_coh, sizeof(double)*n);
+  if (R) memcpy(R, sol->c_R, sizeof(double)*3*n);
+  if (U) memcpy(U, sol->c_U, sizeof(double)*3*n);
+  return 0;
+}
+
+int SOLFEC_Set_RU(SOLFEC *sol, const double *R, const double *U)
+{
+  size_t n = (size_t)sol->nc;
+  if (R) memcpy(sol->c_R, R, sizeof(double)*3*n);
+  if (U) memcpy(sol->c_U, U, sizeof(double)*3*n);
+  return 0;
+}
+
+/* upload, assemble, solve and update body velocities on the GPU */
+int SOLFEC_Step_Solve(SOLFEC *sol, GAUSS_SEIDEL *gs)
+{
+  sfgs_ctx *g = SOLFEC_Gpu(sol);
+  if (!g) return SOL_ERR_GPU;
+  double t0 = now_s();
+  int rc = SFGS_Bodies_Upload(g, sol->nb, sol->kind, sol->rot, sol->x0, sol->jinv, sol->minv, sol->velo, sol->velo0);
+  if (!rc) rc = SFGS_Contacts_Upload(g, sol->nc, sol->c_master, sol->c_slave, sol->c_mpnt, sol->c_spnt, sol->c_base,
+                                     sol->c_gap, sol->c_fri, sol->c_rest, sol->c_coh, sol->c_R, sol->c_U);
+  if (!rc) rc = SFGS_Locdyn_Update_Begin(g, sol->step, sol->dynamic);
+  double t1 = now_s();
+  sol->t_locdyn += t1 - t0;
+  if (rc)
+  {
+    snprintf(sol->err, sizeof(sol->err), "LOCDYN on GPU failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));
+    return SOL_ERR_GPU;
+  }
+  int err = GAUSS_SEIDEL_Solve_Device(gs, g, sol->err, sizeof(sol->err));
+  if (err == SOL_ERR_GPU) return err;
+  rc = SFGS_Get_RU(g, sol->c_R, sol->c_U);
+  if (!rc) rc = SFGS_Body_Update(g, sol->step, sol->tmp_velo, sol->force);
+  if (rc)
+  {
+    snprintf(sol->err, sizeof(sol->err), "GPU body update failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));
+    return SOL_ERR_GPU;
+  }
+  memcpy(sol->velo, sol->tmp_velo, sizeof(double)*6*(size_t)sol->nb);
+  sol->t_consol += now_s() - t1;
+  return err;
+}
+
+int SOLFEC_Run(SOLFEC *sol, GAUSS_SEIDEL *gs, double duration)
+{
+  if (!sol || !gs || !(duration > 0.0)) return SOL_ERR_ARG;
+  double t0 = sol->time;
+  while (sol->time < t0 + duration - 1e-12*sol->step)
+  {
+    int rc = SOLFEC_Step_Begin(sol);
+    if (rc) return rc;
+    rc = SOLFEC_Step_Solve(sol, gs);
+    if (rc) return rc;
+    rc = SOLFEC_Step_End(sol);
+    if (rc) return rc;
+  }
+  return SOL_OK;
+}

@@ -1,0 +1,66 @@
+/* sol_internal.h -- private state of the host domain and the solver object. */
+#ifndef SOL_INTERNAL_H
+#define SOL_INTERNAL_H
+
+#include <stddef.h>
+#include "solfec.h"
+
+typedef struct {
+  int s1, s2;
+  double friction, restitution, cohesion;
+} MATPAIR;
+
+struct solfec {
+  int dynamic;
+  double step, time;
+  double gravity[3];
+  char outpath[512];
+  char err[512];
+  int device;
+  sfgs_ctx *gpu;
+
+  /* bodies */
+  int nb, bcap;
+  int *kind, *shape, *surf;        /* shape: 0 sphere, 1 plane */
+  double *rad, *plane;             /* plane: point 3, unit normal 3 */
+  double *rot, *x, *x0, *J, *jinv, *minv, *mass;
+  double *velo, *velo0, *aux, *force, *tmp_velo;
+
+  /* materials */
+  MATPAIR defmat;
+  MATPAIR *mat;
+  int nmat;
+
+  /* contacts */
+  int nc, ccap;
+  int *c_master, *c_slave;
+  double *c_mpnt, *c_spnt, *c_point, *c_base, *c_gap, *c_fri, *c_rest, *c_coh, *c_R, *c_U;
+
+  /* detection scratch */
+  int *grid_head, *grid_next;
+  size_t grid_cap, gnext_cap;
+  unsigned long long *pair_keys;
+  int *pair_vals;
+  size_t pair_cap, pair_mask;
+
+  /* phase timers (sol.c:392-439 labels), seconds */
+  double t_timint, t_conupd, t_condet, t_locdyn, t_consol;
+};
+
+struct gauss_seidel {
+  SFGS_PARAM par;
+  int variant, innerloops;
+  void *data;
+  GAUSS_SEIDEL_Callback callback;
+  int iters, error;
+  double *rerhist, *merhist;
+  int hist_cap;
+  int *itershist;
+  int itershistcount, itershistsize;
+  SFGS_REPORT rep;
+};
+
+/* run the device solve for the current system of g; maps failures to SOL_ERR_* */
+int GAUSS_SEIDEL_Solve_Device(GAUSS_SEIDEL *gs, sfgs_ctx *g, char *err, size_t errlen);
+
+#endif

@@ -1,0 +1,88 @@
+// sfgs_ctx.hpp -- device-resident LOCDYN store (the W/B/R store) of one context.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include <cstdint>
+#include "../../include/sfgs.h"
+
+namespace sfgs {
+
+// grow-only device buffer
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t cap = 0;
+  size_t n = 0;
+};
+
+// device control block of one solve
+struct Ctrl {
+  int iters;        // completed sweeps
+  int done;         // terminated (converged, maxiter or abort)
+  int aborted;      // FAILURE_EXIT raised in a sweep
+  int refine;       // refinement sweeps still to run (mixed precision)
+  double error;     // last relative error
+  double merit;     // last merit
+  double eps, meritval;
+  int maxiter, nomerit;
+  double energy;    // free energy (merit denominator source)
+};
+
+struct RowData;  // forward
+
+} // namespace sfgs
+
+struct sfgs_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // ---- bodies, natural order
+  int nb = 0;
+  sfgs::DBuf<int> kind;
+  sfgs::DBuf<double> rot, x0, jinv, minv, velo, velo0;
+
+  // ---- contacts, natural order
+  int nc = 0;
+  sfgs::DBuf<int> master, slave;
+  sfgs::DBuf<double> mpnt, spnt, base, gap, fri, rest, coh, Rn, Un;
+
+  // ---- body -> contact incidence
+  sfgs::DBuf<int> bptr, blist, bkeys, bkeys2, bvals, bvals2;
+
+  // ---- natural adjacency CSR
+  long long nnz = 0;
+  sfgs::DBuf<int> rowptr, rowlen_n, col, via;
+
+  // ---- colouring and permutation
+  int ncolours = 0;
+  sfgs::DBuf<int> colour, colour2, perm, iperm, counter;
+  sfgs::DBuf<unsigned long long> skeys, skeys2;
+  sfgs::DBuf<int> svals, svals2;
+  std::vector<int> h_colptr, h_cslice;   // rows / slices per colour (host)
+  sfgs::DBuf<unsigned char> scratch;     // hipcub temporary storage
+
+  // ---- slices (SELL-64, colour-major)
+  int nslices = 0;
+  sfgs::DBuf<int> slice_row0, slice_nrows, slice_len;
+  sfgs::DBuf<long long> slice_off;       // block offset of slice (in blocks, multiple of 64)
+  long long sell_blocks = 0;             // total block slots incl. padding
+
+  // ---- permuted row data (colour-major order)
+  sfgs::DBuf<int> rowlen;                // per permuted row
+  sfgs::DBuf<double> W, Wd, A, B, V, rho, pfri, pcoh, shift, R, U, Rsave, Usave;
+  sfgs::DBuf<unsigned char> sep;
+  sfgs::DBuf<int> wcol;
+  sfgs::DBuf<float> W32, Wd32, B32, R32, U32, rho32, pfri32, pcoh32, shift32;
+  double energy = 0.0, step = 0.0;
+  int dynamic = 1;
+  int have_bodies = 0, have_system = 0, assembled = 0, from_bodies = 0;
+
+  // ---- reductions / control
+  sfgs::DBuf<double> partials, mpartials, hist;
+  sfgs::DBuf<unsigned long long> failkey;
+  sfgs::DBuf<int> failcount;
+  sfgs::DBuf<sfgs::Ctrl> ctrl;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+};

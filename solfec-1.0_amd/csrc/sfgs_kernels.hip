@@ -1,0 +1,1491 @@
+// sfgs_kernels.hip -- LOCDYN assembly, contact-graph colouring, colour-major
+// block Gauss-Seidel sweeps, merit and body update on MI355X (gfx950).
+//
+// Data layout (all device resident, rebuilt every step):
+//  * natural order (contact index i): the uploaded contact records and R, U;
+//  * colour-major order (row p = iperm[i]): rows sorted by (colour, -row length,
+//    i); per-row scalars in SoA planes (Wd[e*n + p], B[k*n + p], U[k*n + p]),
+//    reactions R as 4 doubles per row (32-byte aligned gathers);
+//  * off-diagonal blocks in SELL-64: a slice is 64 consecutive rows of one
+//    colour (one wavefront, one row per lane); slot k of lane l stores element e
+//    of its 3x3 block at W[9*off + (9k + e)*64 + l] and the neighbour's row at
+//    wcol[off + 64k + l], so each load instruction of a wave reads 512
+//    contiguous bytes.
+// One kernel launch sweeps one colour: rows of a colour are not coupled, so the
+// launch order of colours reproduces a serial Gauss-Seidel sweep over the
+// colour-major row order exactly.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
+#include <cfloat>
+#include <climits>
+#include <algorithm>
+#include "sfgs_ctx.hpp"
+#include "sfgs_math.hpp"
+#include "sfgs_prims.hpp"
+
+using namespace sfgs;
+
+#define HIPCHK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) { \
+    ctx->err = std::string(#call) + ": " + hipGetErrorString(e_); return SFGS_E_HIP; } } while (0)
+#define CHK(call) do { int r_ = (call); if (r_) return r_; } while (0)
+
+static const int TPB = 256;          // threads per block (4 wavefronts)
+static const int SLICE = 64;         // rows per slice (one wavefront)
+
+template <typename T>
+static int ensure(sfgs_ctx *ctx, DBuf<T> &b, size_t n)
+{
+  if (n == 0) n = 1;
+  if (b.cap < n)
+  {
+    if (b.p) { (void)hipStreamSynchronize(ctx->stream); (void)hipFree(b.p); }
+    b.p = nullptr; b.cap = 0;
+    size_t cap = n + n/4 + 16;
+    hipError_t e = hipMalloc(&b.p, cap * sizeof(T));
+    if (e != hipSuccess)
+    {
+      b.p = nullptr;
+      ctx->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+      return SFGS_E_HIP;
+    }
+    b.cap = cap;
+  }
+  b.n = n;
+  return 0;
+}
+
+template <typename T>
+static void release(DBuf<T> &b) { if (b.p) (void)hipFree(b.p); b.p = nullptr; b.cap = b.n = 0; }
+
+template <typename T>
+static int upload(sfgs_ctx *ctx, DBuf<T> &b, const T *h, size_t n)
+{
+  CHK(ensure(ctx, b, n));
+  if (n && h) HIPCHK(hipMemcpyAsync(b.p, h, n*sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return 0;
+}
+
+static inline int nblk(long long n, int t = TPB) { return (int)((n + t - 1) / t); }
+
+// =====================================================================
+// device helpers
+// =====================================================================
+
+__device__ __forceinline__ unsigned hash32(unsigned x)
+{
+  // 32-bit integer mixer "lowbias32" (C. Wellons, hash-prospector)
+  x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+  return x;
+}
+
+__device__ __forceinline__ bool couples(const int *kind, int b) { return b >= 0 && kind[b] == 0; }
+
+// H operator of a rigid body at referential point X (bod.c:216-228):
+// H = [ -base' R skew(X - X0) | base' ], returned as Hr (3x3) and the frame.
+__device__ __forceinline__ void operator_Hr(const double *R, const double *X0, const double *X,
+                                            const double *base, double *Hr)
+{
+  double a0 = X[0] - X0[0], a1 = X[1] - X0[1], a2 = X[2] - X0[2];
+  double Q[9];   // Q = base' R
+#pragma unroll
+  for (int c = 0; c < 3; c ++)
+#pragma unroll
+    for (int r = 0; r < 3; r ++)
+      Q[r + 3*c] = base[3*r]*R[3*c] + base[3*r + 1]*R[3*c + 1] + base[3*r + 2]*R[3*c + 2];
+  // Hr = -Q skew(a): columns  -(a2 Q1 - a1 Q2), -(a0 Q2 - a2 Q0), -(a1 Q0 - a0 Q1)
+#pragma unroll
+  for (int r = 0; r < 3; r ++)
+  {
+    double q0 = Q[r], q1 = Q[r + 3], q2 = Q[r + 6];
+    Hr[r]     = a1*q2 - a2*q1;
+    Hr[r + 3] = a2*q0 - a0*q2;
+    Hr[r + 6] = a0*q1 - a1*q0;
+  }
+}
+
+// P = H1 iM H2' = Hr1 J Hr2' + minv base1' base2  (3x3, column-major)
+__device__ __forceinline__ void h_im_ht(const double *Hr1, const double *b1, const double *Hr2, const double *b2,
+                                        const double *J, double minv, double *P)
+{
+  double G[9];   // G = J Hr2'
+#pragma unroll
+  for (int c = 0; c < 3; c ++)
+#pragma unroll
+    for (int r = 0; r < 3; r ++)
+      G[r + 3*c] = J[r]*Hr2[c] + J[r + 3]*Hr2[c + 3] + J[r + 6]*Hr2[c + 6];
+#pragma unroll
+  for (int c = 0; c < 3; c ++)
+#pragma unroll
+    for (int r = 0; r < 3; r ++)
+      P[r + 3*c] = (Hr1[r]*G[3*c] + Hr1[r + 3]*G[3*c + 1] + Hr1[r + 6]*G[3*c + 2])
+                 + minv*(b1[3*r]*b2[3*c] + b1[3*r + 1]*b2[3*c + 1] + b1[3*r + 2]*b2[3*c + 2]);
+}
+
+// y = H u, u = [W v]
+__device__ __forceinline__ void h_u(const double *Hr, const double *base, const double *u, double *y)
+{
+#pragma unroll
+  for (int r = 0; r < 3; r ++)
+    y[r] = Hr[r]*u[0] + Hr[r + 3]*u[1] + Hr[r + 6]*u[2]
+         + base[3*r]*u[3] + base[3*r + 1]*u[4] + base[3*r + 2]*u[5];
+}
+
+// cyclic Jacobi eigenvalues of a symmetric 3x3 (upper triangle), maximum returned
+__device__ double eig3_max(const double *W)
+{
+  double a00 = W[0], a01 = W[3], a02 = W[6], a11 = W[4], a12 = W[7], a22 = W[8];
+  double a[3][3] = {{a00, a01, a02}, {a01, a11, a12}, {a02, a12, a22}};
+  for (int sweep = 0; sweep < 64; sweep ++)
+  {
+    double off = a[0][1]*a[0][1] + a[0][2]*a[0][2] + a[1][2]*a[1][2];
+    double dia = a[0][0]*a[0][0] + a[1][1]*a[1][1] + a[2][2]*a[2][2];
+    if (off <= 1e-34*dia || off == 0.0) break;
+    for (int p = 0; p < 2; p ++)
+      for (int q = p + 1; q < 3; q ++)
+      {
+        if (a[p][q] == 0.0) continue;
+        double theta = (a[q][q] - a[p][p]) / (2.0*a[p][q]);
+        double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta*theta + 1.0));
+        double c = 1.0/sqrt(t*t + 1.0), s = t*c;
+        for (int k = 0; k < 3; k ++)
+        {
+          double akp = a[k][p], akq = a[k][q];
+          a[k][p] = c*akp - s*akq;
+          a[k][q] = s*akp + c*akq;
+        }
+        for (int k = 0; k < 3; k ++)
+        {
+          double apk = a[p][k], aqk = a[q][k];
+          a[p][k] = c*apk - s*aqk;
+          a[q][k] = s*apk + c*aqk;
+        }
+      }
+  }
+  return fmax(a[0][0], fmax(a[1][1], a[2][2]));
+}
+
+// 3x3 inverse by LU with partial pivoting; 0 on success
+__device__ int inverse3(const double *W, double *A)
+{
+#pragma unroll
+  for (int c = 0; c < 3; c ++)
+  {
+    double M[9], x[3] = {c == 0 ? 1.0 : 0.0, c == 1 ? 1.0 : 0.0, c == 2 ? 1.0 : 0.0};
+#pragma unroll
+    for (int k = 0; k < 9; k ++) M[k] = W[k];
+    if (solve3(M, x)) return -1;
+    A[3*c] = x[0]; A[3*c + 1] = x[1]; A[3*c + 2] = x[2];
+  }
+  return 0;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// deterministic block sum of two values; thread 0 returns them
+__device__ __forceinline__ void block_sum2(double &a, double &b)
+{
+  __shared__ double sa[TPB/64], sb[TPB/64];
+  a = wave_sum(a); b = wave_sum(b);
+  int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sa[w] = a; sb[w] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    a = sa[0]; b = sb[0];
+    for (int k = 1; k < TPB/64; k ++) { a += sa[k]; b += sb[k]; }
+  }
+}
+
+// =====================================================================
+// incidence and adjacency (LOCDYN_Insert, ldy.c:492-568)
+// =====================================================================
+
+__global__ void k_inc_keys(int nc, const int *master, const int *slave, const int *kind, int *keys, int *vals)
+{
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  int m = master[i], s = slave[i];
+  keys[2*i] = couples(kind, m) ? m : INT_MAX;
+  keys[2*i + 1] = (couples(kind, s) && s != m) ? s : INT_MAX;
+  vals[2*i] = i;
+  vals[2*i + 1] = i;
+}
+
+__global__ void k_inc_count(int n2, const int *keys, int *count)
+{
+  int q = blockIdx.x*blockDim.x + threadIdx.x;
+  if (q >= n2) return;
+  int b = keys[q];
+  if (b != INT_MAX) atomicAdd(&count[b], 1);
+}
+
+__global__ void k_rowlen(int nc, const int *master, const int *slave, const int *kind, const int *bptr, int *len)
+{
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  int m = master[i], s = slave[i], k = 0;
+  if (couples(kind, m)) k += bptr[m + 1] - bptr[m] - 1;
+  if (couples(kind, s) && s != m) k += bptr[s + 1] - bptr[s] - 1;
+  len[i] = k;
+}
+
+__global__ void k_adj_fill(int nc, const int *master, const int *slave, const int *kind, const int *bptr,
+                           const int *blist, const int *rowptr, int *col, int *via)
+{
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  int p = rowptr[i], side[2] = {master[i], slave[i]};
+  for (int t = 0; t < 2; t ++)
+  {
+    int b = side[t];
+    if (!couples(kind, b) || (t == 1 && b == side[0])) continue;
+    for (int q = bptr[b]; q < bptr[b + 1]; q ++)
+    {
+      int j = blist[q];
+      if (j != i) { col[p] = j; via[p] = b; p ++; }
+    }
+  }
+}
+
+// =====================================================================
+// Jones-Plassmann colouring: priority (row length << 32 | hash(v)), ties by
+// larger index; a vertex is coloured when no uncoloured neighbour outranks it
+// and takes the smallest colour absent from its coloured neighbours.  Reads
+// cin only (the previous round), writes cout: the result equals greedy
+// first-fit in descending priority and does not depend on scheduling.
+// =====================================================================
+
+#define SFGS_MAX_COLOURS 256
+
+__global__ void k_jp_round(int n, const int *rowptr, const int *col, const int *cin, int *cout,
+                           int *remaining, int *maxcol)
+{
+  int v = blockIdx.x*blockDim.x + threadIdx.x;
+  if (v >= n || cin[v] >= 0) return;
+  unsigned long long pv = ((unsigned long long)(unsigned)(rowptr[v + 1] - rowptr[v]) << 32) | hash32((unsigned)v);
+  unsigned long long mask[SFGS_MAX_COLOURS/64] = {0ull, 0ull, 0ull, 0ull};
+  for (int p = rowptr[v]; p < rowptr[v + 1]; p ++)
+  {
+    int u = col[p], cu = cin[u];
+    if (cu < 0)
+    {
+      unsigned long long pu = ((unsigned long long)(unsigned)(rowptr[u + 1] - rowptr[u]) << 32) | hash32((unsigned)u);
+      if (pu > pv || (pu == pv && u > v)) { atomicAdd(remaining, 1); return; }
+    }
+    else if (cu < SFGS_MAX_COLOURS) mask[cu >> 6] |= 1ull << (cu & 63);
+  }
+  int c = SFGS_MAX_COLOURS;
+  for (int w = 0; w < SFGS_MAX_COLOURS/64; w ++)
+    if (~mask[w]) { c = 64*w + __ffsll((long long)~mask[w]) - 1; break; }
+  cout[v] = c;
+  atomicMax(maxcol, c);
+}
+
+__global__ void k_copy_int(int n, const int *a, int *b)
+{
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+__global__ void k_fill_int(int n, int v, int *b)
+{
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i < n) b[i] = v;
+}
+
+__global__ void k_sort_keys(int n, const int *colour, const int *rowptr, unsigned long long *keys, int *vals)
+{
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int len = rowptr[i + 1] - rowptr[i];
+  keys[i] = ((unsigned long long)(unsigned)colour[i] << 32) | (unsigned)(0x7fffffff - len);
+  vals[i] = i;
+}
+
+__global__ void k_iperm(int n, const int *perm, int *iperm)
+{
+  int p = blockIdx.x*blockDim.x + threadIdx.x;
+  if (p < n) iperm[perm[p]] = p;
+}
+
+__global__ void k_colour_count(int n, const int *colour, int *count)
+{
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&count[colour[i]], 1);
+}
+
+__global__ void k_slice_len(int ns, const int *row0, const int *perm, const int *rowptr, int *slen, long long *slen_ll)
+{
+  int s = blockIdx.x*blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  int i = perm[row0[s]];
+  int len = rowptr[i + 1] - rowptr[i];
+  slen[s] = len;
+  slen_ll[s] = (long long)len * SLICE;
+}
+
+// =====================================================================
+// LOCDYN assembly (ldy.c:364-397, 626-844)
+// =====================================================================
+
+struct BodyView {
+  const int *kind;
+  const double *rot, *x0, *jinv, *minv, *velo, *velo0;
+};
+
+struct ConView {
+  const int *master, *slave;
+  const double *mpnt, *spnt, *base, *gap, *fri, *rest, *coh, *Rn, *Un;
+};
+
+struct RowsOut {
+  int n;
+  int *rowlen;
+  double *Wd, *A, *B, *V, *rho, *pfri, *pcoh, *shift, *R, *U;
+  unsigned char *sep;
+  double *energy_part;
+  int *singular;
+};
+
+// per permuted row: V, B (ldy.c:364-397), W_ii, rho, A, free energy (ldy.c:654-736)
+__global__ __launch_bounds__(TPB) void k_rows(BodyView bv, ConView cv, RowsOut o, const int *perm,
+                                               const int *rowptr, double step, int dynamic)
+{
+  int p = blockIdx.x*blockDim.x + threadIdx.x;
+  double e_part = 0.0, dummy = 0.0;
+  if (p < o.n)
+  {
+    int i = perm[p], n = o.n;
+    int m = cv.master[i], s = cv.slave[i];
+    const double *base = cv.base + 9*(size_t)i;
+    double bs[9];
+#pragma unroll
+    for (int k = 0; k < 9; k ++) bs[k] = base[k];
+    double Hr[9], Wi[9], P[9], y[3], Vv[3], Bv[3];
+
+    operator_Hr(bv.rot + 9*(size_t)m, bv.x0 + 3*(size_t)m, cv.mpnt + 3*(size_t)i, bs, Hr);
+    h_u(Hr, bs, bv.velo0 + 6*(size_t)m, Vv);
+    h_u(Hr, bs, bv.velo + 6*(size_t)m, Bv);
+#pragma unroll
+    for (int k = 0; k < 9; k ++) Wi[k] = 0.0;
+    if (couples(bv.kind, m))
+    {
+      h_im_ht(Hr, bs, Hr, bs, bv.jinv + 9*(size_t)m, bv.minv[m], P);
+#pragma unroll
+      for (int k = 0; k < 9; k ++) Wi[k] += P[k];
+    }
+    if (s >= 0)
+    {
+      operator_Hr(bv.rot + 9*(size_t)s, bv.x0 + 3*(size_t)s, cv.spnt + 3*(size_t)i, bs, Hr);
+      h_u(Hr, bs, bv.velo0 + 6*(size_t)s, y);
+      Vv[0] -= y[0]; Vv[1] -= y[1]; Vv[2] -= y[2];
+      h_u(Hr, bs, bv.velo + 6*(size_t)s, y);
+      Bv[0] -= y[0]; Bv[1] -= y[1]; Bv[2] -= y[2];
+      if (couples(bv.kind, s))
+      {
+        h_im_ht(Hr, bs, Hr, bs, bv.jinv + 9*(size_t)s, bv.minv[s], P);
+#pragma unroll
+        for (int k = 0; k < 9; k ++) Wi[k] += P[k];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k ++) Wi[k] *= step;
+
+    double lmax = eig3_max(Wi), Ai[9];
+    if (inverse3(Wi, Ai)) atomicAdd(o.singular, 1);
+#pragma unroll
+    for (int k = 0; k < 9; k ++) { o.Wd[(size_t)k*n + p] = Wi[k]; o.A[(size_t)k*n + p] = Ai[k]; }
+#pragma unroll
+    for (int k = 0; k < 3; k ++) { o.B[(size_t)k*n + p] = Bv[k]; o.V[(size_t)k*n + p] = Vv[k]; }
+    o.rho[p] = 1.0 / lmax;
+    double g = cv.gap[i], e = cv.rest[i];
+    o.pfri[p] = cv.fri[i];
+    o.pcoh[p] = cv.coh[i];
+    o.shift[p] = dynamic ? e * fmin(Vv[2], 0.0) : fmax(g, 0.0) / step;
+    o.sep[p] = (dynamic && g > 0.0) ? 1 : 0;
+    o.rowlen[p] = rowptr[i + 1] - rowptr[i];
+#pragma unroll
+    for (int k = 0; k < 3; k ++)
+    {
+      o.R[4*(size_t)p + k] = cv.Rn[3*(size_t)i + k];
+      o.U[(size_t)k*n + p] = cv.Un[3*(size_t)i + k];
+    }
+    o.R[4*(size_t)p + 3] = 0.0;
+    double x0 = Ai[0]*Bv[0] + Ai[3]*Bv[1] + Ai[6]*Bv[2];
+    double x1 = Ai[1]*Bv[0] + Ai[4]*Bv[1] + Ai[7]*Bv[2];
+    double x2 = Ai[2]*Bv[0] + Ai[5]*Bv[1] + Ai[8]*Bv[2];
+    e_part = x0*Bv[0] + x1*Bv[1] + x2*Bv[2];
+  }
+  block_sum2(e_part, dummy);
+  if (threadIdx.x == 0) o.energy_part[blockIdx.x] = e_part;
+}
+
+struct SellOut {
+  const int *row0, *nrows;
+  const long long *off;
+  double *W;
+  int *wcol;
+};
+
+// off-diagonal blocks W_ij = h s_i s_j H_i^b iM_b H_j^b' (ldy.c:738-844), one
+// lane per row of a slice; the block for the pair is formed from the lower
+// index as "upper" and transposed for the other, as the reference's symmetry copy
+__global__ __launch_bounds__(TPB) void k_offdiag(BodyView bv, ConView cv, SellOut so, const int *perm,
+                                                  const int *iperm, const int *rowptr, const int *col,
+                                                  const int *via, int nslices, double step)
+{
+  int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (s >= nslices || lane >= so.nrows[s]) return;
+  int p = so.row0[s] + lane, i = perm[p];
+  long long off = so.off[s];
+  double *Wb = so.W + 9*off + lane;
+  int *cb = so.wcol + off + lane;
+  for (int q = rowptr[i], k = 0; q < rowptr[i + 1]; q ++, k ++)
+  {
+    int j = col[q], b = via[q];
+    int lo = i < j ? i : j, hi = i < j ? j : i;
+    double sl = cv.master[lo] == b ? 1.0 : -1.0, sh = cv.master[hi] == b ? 1.0 : -1.0;
+    const double *xl = sl > 0.0 ? cv.mpnt + 3*(size_t)lo : cv.spnt + 3*(size_t)lo;
+    const double *xh = sh > 0.0 ? cv.mpnt + 3*(size_t)hi : cv.spnt + 3*(size_t)hi;
+    double bl[9], bh[9], Hl[9], Hh[9], P[9];
+#pragma unroll
+    for (int e = 0; e < 9; e ++) { bl[e] = cv.base[9*(size_t)lo + e]; bh[e] = cv.base[9*(size_t)hi + e]; }
+    operator_Hr(bv.rot + 9*(size_t)b, bv.x0 + 3*(size_t)b, xl, bl, Hl);
+    operator_Hr(bv.rot + 9*(size_t)b, bv.x0 + 3*(size_t)b, xh, bh, Hh);
+    h_im_ht(Hl, bl, Hh, bh, bv.jinv + 9*(size_t)b, bv.minv[b], P);
+    double f = step * sl * sh;
+    if (i == lo)
+    {
+#pragma unroll
+      for (int e = 0; e < 9; e ++) Wb[(9*(size_t)k + e)*SLICE] = f * P[e];
+    }
+    else
+    {
+#pragma unroll
+      for (int r = 0; r < 3; r ++)
+#pragma unroll
+        for (int c = 0; c < 3; c ++)
+          Wb[(9*(size_t)k + r + 3*c)*SLICE] = f * P[c + 3*r];
+    }
+    cb[(size_t)k*SLICE] = iperm[j];
+  }
+}
+
+// ---- installation of a pre-assembled system (natural CSR) ----
+
+struct SysIn {
+  const int *rowptr, *col;
+  const double *W, *Wd, *A, *B, *V, *rho, *fri, *rest, *coh, *gap, *Rn, *Un;
+};
+
+__global__ __launch_bounds__(TPB) void k_rows_sys(SysIn si, RowsOut o, const int *perm, double step, int dynamic)
+{
+  int p = blockIdx.x*blockDim.x + threadIdx.x;
+  if (p >= o.n) return;
+  int i = perm[p], n = o.n;
+#pragma unroll
+  for (int k = 0; k < 9; k ++)
+  {
+    o.Wd[(size_t)k*n + p] = si.Wd[9*(size_t)i + k];
+    o.A[(size_t)k*n + p] = si.A[9*(size_t)i + k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k ++)
+  {
+    o.B[(size_t)k*n + p] = si.B[3*(size_t)i + k];
+    o.V[(size_t)k*n + p] = si.V[3*(size_t)i + k];
+    o.R[4*(size_t)p + k] = si.Rn[3*(size_t)i + k];
+    o.U[(size_t)k*n + p] = si.Un[3*(size_t)i + k];
+  }
+  o.R[4*(size_t)p + 3] = 0.0;
+  double g = si.gap[i], e = si.rest[i], v2 = si.V[3*(size_t)i + 2];
+  o.rho[p] = si.rho[i];
+  o.pfri[p] = si.fri[i];
+  o.pcoh[p] = si.coh[i];
+  o.shift[p] = dynamic ? e * fmin(v2, 0.0) : fmax(g, 0.0) / step;
+  o.sep[p] = (dynamic && g > 0.0) ? 1 : 0;
+  o.rowlen[p] = si.rowptr[i + 1] - si.rowptr[i];
+}
+
+__global__ __launch_bounds__(TPB) void k_offdiag_sys(SysIn si, SellOut so, const int *perm, const int *iperm, int nslices)
+{
+  int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (s >= nslices || lane >= so.nrows[s]) return;
+  int p = so.row0[s] + lane, i = perm[p];
+  long long off = so.off[s];
+  double *Wb = so.W + 9*off + lane;
+  int *cb = so.wcol + off + lane;
+  for (int q = si.rowptr[i], k = 0; q < si.rowptr[i + 1]; q ++, k ++)
+  {
+#pragma unroll
+    for (int e = 0; e < 9; e ++) Wb[(9*(size_t)k + e)*SLICE] = si.W[9*(size_t)q + e];
+    cb[(size_t)k*SLICE] = iperm[si.col[q]];
+  }
+}
+
+// =====================================================================
+// Gauss-Seidel sweep of one colour (bgs.c:950-1019), FP64
+// =====================================================================
+
+struct SweepArgs {
+  const int *row0, *nrows, *rowlen;
+  const long long *off;
+  const double *W;
+  const int *wcol;
+  const double *Wd, *B, *rho, *fri, *coh, *shift;
+  const unsigned char *sep;
+  double *R, *U;
+  int n;
+  double diageps;
+  int diagmaxiter, diagsolver, failure;
+  double *partials;
+  unsigned long long *failkey;
+  int *failcount;
+  Ctrl *ctrl;
+};
+
+__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs a, int s0, int s1, int pbase, int backward)
+{
+  int s = s0 + blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  double eu = 0.0, el = 0.0;
+  const bool live = !a.ctrl->done;
+  if (live && s < s1 && lane < a.nrows[s])
+  {
+    const int n = a.n, p = a.row0[s] + lane, len = a.rowlen[p];
+    const long long off = a.off[s];
+    const double *Wb = a.W + 9*off + lane;
+    const int *cb = a.wcol + off + lane;
+
+    // B' = B_i + sum_j W_ij R_j  (bgs.c:957-963)
+    double Bp[3] = {a.B[p], a.B[(size_t)n + p], a.B[2*(size_t)n + p]};
+    for (int k = 0; k < len; k ++)
+    {
+      const int j = cb[(size_t)k*SLICE];
+      const double *w = Wb + 9*(size_t)k*SLICE;
+      const double r0 = a.R[4*(size_t)j], r1 = a.R[4*(size_t)j + 1], r2 = a.R[4*(size_t)j + 2];
+      Bp[0] = Bp[0] + (w[0]*r0 + w[3*SLICE]*r1 + w[6*SLICE]*r2);
+      Bp[1] = Bp[1] + (w[1*SLICE]*r0 + w[4*SLICE]*r1 + w[7*SLICE]*r2);
+      Bp[2] = Bp[2] + (w[2*SLICE]*r0 + w[5*SLICE]*r1 + w[8*SLICE]*r2);
+    }
+
+    double Wi[9], R[3], U[3], R0[3];
+#pragma unroll
+    for (int e = 0; e < 9; e ++) Wi[e] = a.Wd[(size_t)e*n + p];
+#pragma unroll
+    for (int k = 0; k < 3; k ++) { R[k] = a.R[4*(size_t)p + k]; U[k] = a.U[(size_t)k*n + p]; R0[k] = R[k]; }
+    RowLaw<double> L{a.fri[p], a.coh[p], a.shift[p], a.rho[p], a.sep[p] != 0};
+
+    int d = local_solve(a.diagsolver, L, a.diageps, a.diagmaxiter, Wi, Bp, U, R);
+    if (d >= a.diagmaxiter || d < 0)
+    {
+      unsigned code = d < 0 ? SFGS_GS_DIAGONAL_FAILED : SFGS_GS_DIAGONAL_DIVERGED;
+      unsigned long long pos = backward ? (unsigned long long)(n - 1 - p) : (unsigned long long)p;
+      atomicMax(a.failkey, ((unsigned long long)(a.ctrl->iters + 1) << 40) | (pos << 2) | code);
+      atomicAdd(a.failcount, 1);
+      if (a.failure == SFGS_FAILURE_CONTINUE)
+      {
+        const int chain[4] = {SFGS_DS_SEMISMOOTH_NEWTON, SFGS_DS_PROJECTED_GRADIENT,
+                              SFGS_DS_DE_SAXCE_FENG, SFGS_DS_PROJECTED_NEWTON};
+        for (int c = 0; c < 4; c ++)
+        {
+          if (chain[c] == a.diagsolver) continue;
+          R[0] = R0[0]; R[1] = R0[1]; R[2] = R0[2];
+          d = local_solve(chain[c], L, a.diageps, a.diagmaxiter, Wi, Bp, U, R);
+          if (d < a.diagmaxiter && d >= 0) break;
+        }
+        if (d >= a.diagmaxiter || d < 0) { R[0] = R0[0]; R[1] = R0[1]; R[2] = R0[2]; }
+      }
+      else if (a.failure == SFGS_FAILURE_EXIT)
+      {
+        atomicExch(&a.ctrl->aborted, 1);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k ++) { a.R[4*(size_t)p + k] = R[k]; a.U[(size_t)k*n + p] = U[k]; }
+    const double d0 = R[0] - R0[0], d1 = R[1] - R0[1], d2 = R[2] - R0[2];
+    eu = d0*d0 + d1*d1 + d2*d2;
+    el = R[0]*R[0] + R[1]*R[1] + R[2]*R[2];
+  }
+  if (!live) return;   // uniform over the grid
+  block_sum2(eu, el);
+  if (threadIdx.x == 0)
+  {
+    a.partials[2*(size_t)(pbase + blockIdx.x)] = eu;
+    a.partials[2*(size_t)(pbase + blockIdx.x) + 1] = el;
+  }
+}
+
+// =====================================================================
+// merit (mrf.c:35-184): U = B + W_ii R + sum W_ij R_j, up = C'AC
+// =====================================================================
+
+struct MeritArgs {
+  const int *row0, *nrows, *rowlen;
+  const long long *off;
+  const double *W;
+  const int *wcol;
+  const double *Wd, *A, *B, *fri, *coh, *shift;
+  const double *R;
+  double *U;
+  int n, update_U;
+  double *mpartials;
+  double *conmerit;   // may be null: per-row up (colour-major)
+  Ctrl *ctrl;
+};
+
+__global__ __launch_bounds__(TPB) void k_merit(MeritArgs a, int nslices, int check_done)
+{
+  int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  double up = 0.0, dummy = 0.0;
+  const bool live = !(check_done && a.ctrl->done);
+  if (live && s < nslices && lane < a.nrows[s])
+  {
+    const int n = a.n, p = a.row0[s] + lane;
+    double R[3] = {a.R[4*(size_t)p], a.R[4*(size_t)p + 1], a.R[4*(size_t)p + 2]}, U[3];
+    if (a.update_U)
+    {
+      const int len = a.rowlen[p];
+      const long long off = a.off[s];
+      const double *Wb = a.W + 9*off + lane;
+      const int *cb = a.wcol + off + lane;
+      double Wi[9];
+#pragma unroll
+      for (int e = 0; e < 9; e ++) Wi[e] = a.Wd[(size_t)e*n + p];
+      double Bv[3] = {a.B[p], a.B[(size_t)n + p], a.B[2*(size_t)n + p]};
+      mv_add(Bv, Wi, R, U);
+      for (int k = 0; k < len; k ++)
+      {
+        const int j = cb[(size_t)k*SLICE];
+        const double *w = Wb + 9*(size_t)k*SLICE;
+        const double r0 = a.R[4*(size_t)j], r1 = a.R[4*(size_t)j + 1], r2 = a.R[4*(size_t)j + 2];
+        U[0] = U[0] + (w[0]*r0 + w[3*SLICE]*r1 + w[6*SLICE]*r2);
+        U[1] = U[1] + (w[1*SLICE]*r0 + w[4*SLICE]*r1 + w[7*SLICE]*r2);
+        U[2] = U[2] + (w[2*SLICE]*r0 + w[5*SLICE]*r1 + w[8*SLICE]*r2);
+      }
+#pragma unroll
+      for (int k = 0; k < 3; k ++) a.U[(size_t)k*n + p] = U[k];
+    }
+    else
+    {
+#pragma unroll
+      for (int k = 0; k < 3; k ++) U[k] = a.U[(size_t)k*n + p];
+    }
+    double C[3], Ai[9];
+    scf_residual<double>(a.fri[p], a.coh[p], a.shift[p], U, R, C, nullptr, nullptr);
+#pragma unroll
+    for (int e = 0; e < 9; e ++) Ai[e] = a.A[(size_t)e*n + p];
+    const double q0 = Ai[0]*C[0] + Ai[3]*C[1] + Ai[6]*C[2];
+    const double q1 = Ai[1]*C[0] + Ai[4]*C[1] + Ai[7]*C[2];
+    const double q2 = Ai[2]*C[0] + Ai[5]*C[1] + Ai[8]*C[2];
+    up = q0*C[0] + q1*C[1] + q2*C[2];
+    if (a.conmerit) a.conmerit[p] = up;
+  }
+  if (!live) return;
+  block_sum2(up, dummy);
+  if (threadIdx.x == 0) a.mpartials[blockIdx.x] = up;
+}
+
+// =====================================================================
+// end of one iteration: errors, merit, histories, termination (bgs.c:1021-1036)
+// =====================================================================
+
+__global__ __launch_bounds__(TPB) void k_finish(Ctrl *ctrl, const double *partials, int np,
+                                                 const double *mpartials, int nm, double *hist)
+{
+  if (ctrl->done) return;
+  double eu = 0.0, el = 0.0, up = 0.0, dummy = 0.0;
+  for (int k = threadIdx.x; k < np; k += TPB) { eu += partials[2*(size_t)k]; el += partials[2*(size_t)k + 1]; }
+  block_sum2(eu, el);
+  __syncthreads();
+  if (!ctrl->nomerit) for (int k = threadIdx.x; k < nm; k += TPB) up += mpartials[k];
+  block_sum2(up, dummy);
+  if (threadIdx.x != 0) return;
+  int it = ctrl->iters;
+  double error = sqrt(eu) / sqrt(el == 0.0 ? 1.0 : el);
+  double merit = 0.0;
+  if (!ctrl->nomerit)
+  {
+    double lo = ctrl->energy < DBL_EPSILON ? 1.0 : ctrl->energy;
+    merit = (0.5*up) / lo;
+  }
+  hist[it] = error;
+  hist[ctrl->maxiter + it] = merit;
+  ctrl->error = error;
+  ctrl->merit = merit;
+  ctrl->iters = it + 1;
+  if (ctrl->aborted) ctrl->done = 1;
+  else if (!(it + 1 < ctrl->maxiter && (error > ctrl->eps || merit > ctrl->meritval))) ctrl->done = 1;
+}
+
+// =====================================================================
+// results back to natural order, body update (bod.c:418-435, 1446-1447)
+// =====================================================================
+
+__global__ void k_unperm(int n, const int *perm, const double *R, const double *U, double *Rn, double *Un)
+{
+  int p = blockIdx.x*blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int i = perm[p];
+#pragma unroll
+  for (int k = 0; k < 3; k ++)
+  {
+    Rn[3*(size_t)i + k] = R[4*(size_t)p + k];
+    Un[3*(size_t)i + k] = U[(size_t)k*n + p];
+  }
+}
+
+__global__ void k_body_update(BodyView bv, ConView cv, int nb, const int *bptr, const int *blist,
+                              const int *iperm, const double *R, double step, double *velo_out, double *force_out)
+{
+  int b = blockIdx.x*blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double u[6];
+#pragma unroll
+  for (int k = 0; k < 6; k ++) u[k] = bv.velo[6*(size_t)b + k];
+  if (bv.kind[b] == 0)
+  {
+    double f[6] = {0, 0, 0, 0, 0, 0};
+    const double *Rot = bv.rot + 9*(size_t)b, *X0 = bv.x0 + 3*(size_t)b;
+    for (int q = bptr[b]; q < bptr[b + 1]; q ++)
+    {
+      int i = blist[q];
+      double sg = cv.master[i] == b ? 1.0 : -1.0, bs[9], Hr[9];
+      const double *X = sg > 0.0 ? cv.mpnt + 3*(size_t)i : cv.spnt + 3*(size_t)i;
+#pragma unroll
+      for (int e = 0; e < 9; e ++) bs[e] = cv.base[9*(size_t)i + e];
+      operator_Hr(Rot, X0, X, bs, Hr);
+      int p = iperm[i];
+      double r0 = R[4*(size_t)p], r1 = R[4*(size_t)p + 1], r2 = R[4*(size_t)p + 2];
+#pragma unroll
+      for (int c = 0; c < 3; c ++)
+      {
+        f[c] += sg * (Hr[3*c]*r0 + Hr[3*c + 1]*r1 + Hr[3*c + 2]*r2);
+        f[3 + c] += sg * (bs[c]*r0 + bs[c + 3]*r1 + bs[c + 6]*r2);
+      }
+    }
+    const double *J = bv.jinv + 9*(size_t)b;
+    double minv = bv.minv[b];
+#pragma unroll
+    for (int r = 0; r < 3; r ++)
+    {
+      u[r] += step * (J[r]*f[0] + J[r + 3]*f[1] + J[r + 6]*f[2]);
+      u[3 + r] += step * minv * f[3 + r];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k ++) force_out[6*(size_t)b + k] = f[k];
+  }
+  else
+  {
+#pragma unroll
+    for (int k = 0; k < 6; k ++) force_out[6*(size_t)b + k] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k ++) velo_out[6*(size_t)b + k] = u[k];
+}
+
+// =====================================================================
+// host side
+// =====================================================================
+
+static int scan_int(sfgs_ctx *ctx, const int *in, int *out, int n)
+{
+  size_t bytes = 0;
+  HIPCHK(sfgs::scan_int(nullptr, bytes, in, out, n, ctx->stream));
+  CHK(ensure(ctx, ctx->scratch, bytes));
+  HIPCHK(sfgs::scan_int(ctx->scratch.p, bytes, in, out, n, ctx->stream));
+  return 0;
+}
+
+static int scan_ll(sfgs_ctx *ctx, const long long *in, long long *out, int n)
+{
+  size_t bytes = 0;
+  HIPCHK(sfgs::scan_ll(nullptr, bytes, in, out, n, ctx->stream));
+  CHK(ensure(ctx, ctx->scratch, bytes));
+  HIPCHK(sfgs::scan_ll(ctx->scratch.p, bytes, in, out, n, ctx->stream));
+  return 0;
+}
+
+// colour the natural CSR (ctx->rowptr/col, n rows) into ctx->colour; sets ncolours
+static int colour_graph(sfgs_ctx *ctx, int n)
+{
+  ctx->ncolours = 0;
+  if (n == 0) return 0;
+  CHK(ensure(ctx, ctx->colour, n));
+  CHK(ensure(ctx, ctx->colour2, n));
+  CHK(ensure(ctx, ctx->counter, 4));
+  hipLaunchKernelGGL(k_fill_int, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, -1, ctx->colour.p);
+  hipLaunchKernelGGL(k_fill_int, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, -1, ctx->colour2.p);
+  HIPCHK(hipMemsetAsync(ctx->counter.p, 0, 4*sizeof(int), ctx->stream));
+  int h[2] = {0, 0};
+  for (int round = 0; ; round ++)
+  {
+    if (round > 100000) { ctx->err = "colouring did not finish"; return SFGS_E_STATE; }
+    HIPCHK(hipMemsetAsync(ctx->counter.p, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_jp_round, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->rowptr.p, ctx->col.p,
+                       ctx->colour.p, ctx->colour2.p, ctx->counter.p, ctx->counter.p + 1);
+    hipLaunchKernelGGL(k_copy_int, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->colour2.p, ctx->colour.p);
+    HIPCHK(hipMemcpyAsync(h, ctx->counter.p, 2*sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (h[0] == 0) break;
+  }
+  if (h[1] >= SFGS_MAX_COLOURS) { ctx->err = "contact graph needs more than 256 colours"; return SFGS_E_COLOURS; }
+  ctx->ncolours = h[1] + 1;
+  return 0;
+}
+
+// permutation into colour-major order and the SELL-64 slice table
+static int build_order(sfgs_ctx *ctx, int n)
+{
+  CHK(ensure(ctx, ctx->skeys, n));
+  CHK(ensure(ctx, ctx->skeys2, n));
+  CHK(ensure(ctx, ctx->svals, n));
+  CHK(ensure(ctx, ctx->perm, n));
+  CHK(ensure(ctx, ctx->iperm, n));
+  if (n > 0)
+  {
+    hipLaunchKernelGGL(k_sort_keys, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->colour.p, ctx->rowptr.p,
+                       ctx->skeys.p, ctx->svals.p);
+    size_t bytes = 0;
+    HIPCHK(sfgs::sort_pairs_u64(nullptr, bytes, ctx->skeys.p, ctx->skeys2.p, ctx->svals.p, ctx->perm.p, n, 41, ctx->stream));
+    CHK(ensure(ctx, ctx->scratch, bytes));
+    HIPCHK(sfgs::sort_pairs_u64(ctx->scratch.p, bytes, ctx->skeys.p, ctx->skeys2.p, ctx->svals.p, ctx->perm.p, n, 41, ctx->stream));
+    hipLaunchKernelGGL(k_iperm, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->perm.p, ctx->iperm.p);
+  }
+  // rows per colour
+  int nc = ctx->ncolours;
+  CHK(ensure(ctx, ctx->counter, (size_t)std::max(nc, 4)));
+  HIPCHK(hipMemsetAsync(ctx->counter.p, 0, sizeof(int)*(size_t)std::max(nc, 4), ctx->stream));
+  if (n > 0) hipLaunchKernelGGL(k_colour_count, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->colour.p, ctx->counter.p);
+  std::vector<int> cnt((size_t)std::max(nc, 1), 0);
+  if (nc > 0) HIPCHK(hipMemcpyAsync(cnt.data(), ctx->counter.p, sizeof(int)*(size_t)nc, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->h_colptr.assign((size_t)nc + 1, 0);
+  ctx->h_cslice.assign((size_t)nc + 1, 0);
+  std::vector<int> row0, nrows;
+  for (int c = 0; c < nc; c ++)
+  {
+    ctx->h_colptr[c + 1] = ctx->h_colptr[c] + cnt[c];
+    for (int r = ctx->h_colptr[c]; r < ctx->h_colptr[c + 1]; r += SLICE)
+    {
+      row0.push_back(r);
+      nrows.push_back(std::min(SLICE, ctx->h_colptr[c + 1] - r));
+    }
+    ctx->h_cslice[c + 1] = (int)row0.size();
+  }
+  int ns = (int)row0.size();
+  ctx->nslices = ns;
+  CHK(upload(ctx, ctx->slice_row0, row0.data(), (size_t)ns));
+  CHK(upload(ctx, ctx->slice_nrows, nrows.data(), (size_t)ns));
+  CHK(ensure(ctx, ctx->slice_len, (size_t)ns + 1));
+  DBuf<long long> &tmp = ctx->slice_off;  // reuse: lengths first, then offsets
+  CHK(ensure(ctx, tmp, (size_t)ns + 1));
+  DBuf<long long> lens;
+  CHK(ensure(ctx, lens, (size_t)ns + 1));
+  HIPCHK(hipMemsetAsync(lens.p, 0, sizeof(long long)*((size_t)ns + 1), ctx->stream));
+  if (ns > 0)
+    hipLaunchKernelGGL(k_slice_len, dim3(nblk(ns)), dim3(TPB), 0, ctx->stream, ns, ctx->slice_row0.p, ctx->perm.p,
+                       ctx->rowptr.p, ctx->slice_len.p, lens.p);
+  CHK(scan_ll(ctx, lens.p, ctx->slice_off.p, ns + 1));
+  long long total = 0;
+  HIPCHK(hipMemcpyAsync(&total, ctx->slice_off.p + ns, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  release(lens);
+  ctx->sell_blocks = total;
+  CHK(ensure(ctx, ctx->W, (size_t)(9*total)));
+  CHK(ensure(ctx, ctx->wcol, (size_t)total));
+  if (total > 0)
+  {
+    HIPCHK(hipMemsetAsync(ctx->W.p, 0, sizeof(double)*(size_t)(9*total), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->wcol.p, 0, sizeof(int)*(size_t)total, ctx->stream));
+  }
+  return 0;
+}
+
+static int ensure_rows(sfgs_ctx *ctx, int n)
+{
+  CHK(ensure(ctx, ctx->rowlen, n));
+  CHK(ensure(ctx, ctx->Wd, 9*(size_t)n));
+  CHK(ensure(ctx, ctx->A, 9*(size_t)n));
+  CHK(ensure(ctx, ctx->B, 3*(size_t)n));
+  CHK(ensure(ctx, ctx->V, 3*(size_t)n));
+  CHK(ensure(ctx, ctx->rho, n));
+  CHK(ensure(ctx, ctx->pfri, n));
+  CHK(ensure(ctx, ctx->pcoh, n));
+  CHK(ensure(ctx, ctx->shift, n));
+  CHK(ensure(ctx, ctx->sep, n));
+  CHK(ensure(ctx, ctx->R, 4*(size_t)n));
+  CHK(ensure(ctx, ctx->U, 3*(size_t)n));
+  return 0;
+}
+
+static RowsOut rows_out(sfgs_ctx *ctx, int n, double *epart, int *singular)
+{
+  RowsOut o;
+  o.n = n; o.rowlen = ctx->rowlen.p; o.Wd = ctx->Wd.p; o.A = ctx->A.p; o.B = ctx->B.p; o.V = ctx->V.p;
+  o.rho = ctx->rho.p; o.pfri = ctx->pfri.p; o.pcoh = ctx->pcoh.p; o.shift = ctx->shift.p; o.R = ctx->R.p;
+  o.U = ctx->U.p; o.sep = ctx->sep.p; o.energy_part = epart; o.singular = singular;
+  return o;
+}
+
+static SellOut sell_out(sfgs_ctx *ctx)
+{
+  SellOut so;
+  so.row0 = ctx->slice_row0.p; so.nrows = ctx->slice_nrows.p; so.off = ctx->slice_off.p;
+  so.W = ctx->W.p; so.wcol = ctx->wcol.p;
+  return so;
+}
+
+extern "C" {
+
+const char* SFGS_Error_String(int code)
+{
+  switch (code)
+  {
+  case SFGS_OK: return "ok";
+  case SFGS_E_HIP: return "HIP runtime error";
+  case SFGS_E_ARG: return "invalid argument";
+  case SFGS_E_STATE: return "call out of order";
+  case SFGS_E_COLOURS: return "too many colours";
+  case SFGS_E_SINGULAR: return "singular diagonal block";
+  }
+  return "unknown error";
+}
+
+sfgs_ctx* SFGS_Create(int device)
+{
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0 || device < 0 || device >= count) return nullptr;
+  if (hipSetDevice(device) != hipSuccess) return nullptr;
+  sfgs_ctx *ctx = new sfgs_ctx();
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return nullptr; }
+  (void)hipEventCreate(&ctx->ev0);
+  (void)hipEventCreate(&ctx->ev1);
+  (void)hipEventCreate(&ctx->ev2);
+  (void)hipEventCreate(&ctx->ev3);
+  return ctx;
+}
+
+void SFGS_Destroy(sfgs_ctx *ctx)
+{
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  release(ctx->kind); release(ctx->rot); release(ctx->x0); release(ctx->jinv); release(ctx->minv);
+  release(ctx->velo); release(ctx->velo0);
+  release(ctx->master); release(ctx->slave); release(ctx->mpnt); release(ctx->spnt); release(ctx->base);
+  release(ctx->gap); release(ctx->fri); release(ctx->rest); release(ctx->coh); release(ctx->Rn); release(ctx->Un);
+  release(ctx->bptr); release(ctx->blist); release(ctx->bkeys); release(ctx->bkeys2); release(ctx->bvals); release(ctx->bvals2);
+  release(ctx->rowptr); release(ctx->rowlen_n); release(ctx->col); release(ctx->via);
+  release(ctx->colour); release(ctx->colour2); release(ctx->perm); release(ctx->iperm); release(ctx->counter);
+  release(ctx->skeys); release(ctx->skeys2); release(ctx->svals); release(ctx->svals2); release(ctx->scratch);
+  release(ctx->slice_row0); release(ctx->slice_nrows); release(ctx->slice_len); release(ctx->slice_off);
+  release(ctx->rowlen); release(ctx->W); release(ctx->Wd); release(ctx->A); release(ctx->B); release(ctx->V);
+  release(ctx->rho); release(ctx->pfri); release(ctx->pcoh); release(ctx->shift); release(ctx->R); release(ctx->U);
+  release(ctx->Rsave); release(ctx->Usave); release(ctx->sep); release(ctx->wcol);
+  release(ctx->W32); release(ctx->Wd32); release(ctx->B32); release(ctx->R32); release(ctx->U32);
+  release(ctx->rho32); release(ctx->pfri32); release(ctx->pcoh32); release(ctx->shift32);
+  release(ctx->partials); release(ctx->mpartials); release(ctx->hist); release(ctx->failkey);
+  release(ctx->failcount); release(ctx->ctrl);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
+  if (ctx->ev3) (void)hipEventDestroy(ctx->ev3);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* SFGS_Last_Error(sfgs_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int SFGS_Bodies_Upload(sfgs_ctx *ctx, int nb, const int *kind, const double *rot, const double *x0,
+                       const double *jinv, const double *minv, const double *velo, const double *velo0)
+{
+  if (!ctx || nb < 0 || (nb > 0 && (!kind || !rot || !x0 || !jinv || !minv || !velo || !velo0))) return SFGS_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  ctx->nb = nb;
+  CHK(upload(ctx, ctx->kind, kind, (size_t)nb));
+  CHK(upload(ctx, ctx->rot, rot, 9*(size_t)nb));
+  CHK(upload(ctx, ctx->x0, x0, 3*(size_t)nb));
+  CHK(upload(ctx, ctx->jinv, jinv, 9*(size_t)nb));
+  CHK(upload(ctx, ctx->minv, minv, (size_t)nb));
+  CHK(upload(ctx, ctx->velo, velo, 6*(size_t)nb));
+  CHK(upload(ctx, ctx->velo0, velo0, 6*(size_t)nb));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->have_bodies = 1;
+  ctx->assembled = 0;
+  return 0;
+}
+
+int SFGS_Contacts_Upload(sfgs_ctx *ctx, int nc, const int *master, const int *slave,
+                         const double *mpnt, const double *spnt, const double *base,
+                         const double *gap, const double *friction, const double *restitution,
+                         const double *cohesion, const double *R, const double *U)
+{
+  if (!ctx || nc < 0) return SFGS_E_ARG;
+  if (nc > 0 && (!master || !slave || !mpnt || !spnt || !base || !gap || !friction || !restitution || !cohesion || !R || !U))
+    return SFGS_E_ARG;
+  if (!ctx->have_bodies) { ctx->err = "bodies must be uploaded before contacts"; return SFGS_E_STATE; }
+  for (int i = 0; i < nc; i ++)
+  {
+    if (master[i] < 0 || master[i] >= ctx->nb || slave[i] < -1 || slave[i] >= ctx->nb || slave[i] == master[i])
+    {
+      ctx->err = "contact " + std::to_string(i) + " has an invalid body pair";
+      return SFGS_E_ARG;
+    }
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  ctx->nc = nc;
+  CHK(upload(ctx, ctx->master, master, (size_t)nc));
+  CHK(upload(ctx, ctx->slave, slave, (size_t)nc));
+  CHK(upload(ctx, ctx->mpnt, mpnt, 3*(size_t)nc));
+  CHK(upload(ctx, ctx->spnt, spnt, 3*(size_t)nc));
+  CHK(upload(ctx, ctx->base, base, 9*(size_t)nc));
+  CHK(upload(ctx, ctx->gap, gap, (size_t)nc));
+  CHK(upload(ctx, ctx->fri, friction, (size_t)nc));
+  CHK(upload(ctx, ctx->rest, restitution, (size_t)nc));
+  CHK(upload(ctx, ctx->coh, cohesion, (size_t)nc));
+  CHK(upload(ctx, ctx->Rn, R, 3*(size_t)nc));
+  CHK(upload(ctx, ctx->Un, U, 3*(size_t)nc));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->assembled = 0;
+  return 0;
+}
+
+int SFGS_Locdyn_Update_Begin(sfgs_ctx *ctx, double step, int dynamic)
+{
+  if (!ctx || !(step > 0.0)) return SFGS_E_ARG;
+  if (!ctx->have_bodies) { ctx->err = "no bodies"; return SFGS_E_STATE; }
+  HIPCHK(hipSetDevice(ctx->device));
+  const int nc = ctx->nc, nb = ctx->nb;
+  hipStream_t st = ctx->stream;
+
+  // body -> contact incidence, ascending contact index per body (stable sort)
+  CHK(ensure(ctx, ctx->bkeys, 2*(size_t)nc));
+  CHK(ensure(ctx, ctx->bkeys2, 2*(size_t)nc));
+  CHK(ensure(ctx, ctx->bvals, 2*(size_t)nc));
+  CHK(ensure(ctx, ctx->blist, 2*(size_t)nc));
+  CHK(ensure(ctx, ctx->bptr, (size_t)nb + 1));
+  CHK(ensure(ctx, ctx->counter, (size_t)nb + 1));
+  if (nc > 0)
+  {
+    hipLaunchKernelGGL(k_inc_keys, dim3(nblk(nc)), dim3(TPB), 0, st, nc, ctx->master.p, ctx->slave.p, ctx->kind.p,
+                       ctx->bkeys.p, ctx->bvals.p);
+    size_t bytes = 0;
+    HIPCHK(sfgs::sort_pairs_i32(nullptr, bytes, ctx->bkeys.p, ctx->bkeys2.p, ctx->bvals.p, ctx->blist.p, 2*nc, 32, st));
+    CHK(ensure(ctx, ctx->scratch, bytes));
+    HIPCHK(sfgs::sort_pairs_i32(ctx->scratch.p, bytes, ctx->bkeys.p, ctx->bkeys2.p, ctx->bvals.p, ctx->blist.p, 2*nc, 32, st));
+  }
+  HIPCHK(hipMemsetAsync(ctx->counter.p, 0, sizeof(int)*((size_t)nb + 1), st));
+  if (nc > 0) hipLaunchKernelGGL(k_inc_count, dim3(nblk(2*nc)), dim3(TPB), 0, st, 2*nc, ctx->bkeys2.p, ctx->counter.p);
+  CHK(scan_int(ctx, ctx->counter.p, ctx->bptr.p, nb + 1));
+
+  // contact adjacency CSR (natural order)
+  CHK(ensure(ctx, ctx->rowlen_n, (size_t)nc + 1));
+  CHK(ensure(ctx, ctx->rowptr, (size_t)nc + 1));
+  HIPCHK(hipMemsetAsync(ctx->rowlen_n.p, 0, sizeof(int)*((size_t)nc + 1), st));
+  if (nc > 0) hipLaunchKernelGGL(k_rowlen, dim3(nblk(nc)), dim3(TPB), 0, st, nc, ctx->master.p, ctx->slave.p, ctx->kind.p,
+                                 ctx->bptr.p, ctx->rowlen_n.p);
+  CHK(scan_int(ctx, ctx->rowlen_n.p, ctx->rowptr.p, nc + 1));
+  int nnz = 0;
+  HIPCHK(hipMemcpyAsync(&nnz, ctx->rowptr.p + nc, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  ctx->nnz = nnz;
+  CHK(ensure(ctx, ctx->col, (size_t)nnz));
+  CHK(ensure(ctx, ctx->via, (size_t)nnz));
+  if (nc > 0) hipLaunchKernelGGL(k_adj_fill, dim3(nblk(nc)), dim3(TPB), 0, st, nc, ctx->master.p, ctx->slave.p, ctx->kind.p,
+                                 ctx->bptr.p, ctx->blist.p, ctx->rowptr.p, ctx->col.p, ctx->via.p);
+
+  // colouring and colour-major layout
+  CHK(colour_graph(ctx, nc));
+  CHK(build_order(ctx, nc));
+  CHK(ensure_rows(ctx, nc));
+
+  // rows and blocks
+  int nbk = nblk(nc);
+  CHK(ensure(ctx, ctx->partials, 2*(size_t)std::max(nbk, 1)));
+  CHK(ensure(ctx, ctx->failcount, 4));
+  HIPCHK(hipMemsetAsync(ctx->failcount.p, 0, sizeof(int)*4, st));
+  BodyView bv{ctx->kind.p, ctx->rot.p, ctx->x0.p, ctx->jinv.p, ctx->minv.p, ctx->velo.p, ctx->velo0.p};
+  ConView cv{ctx->master.p, ctx->slave.p, ctx->mpnt.p, ctx->spnt.p, ctx->base.p, ctx->gap.p, ctx->fri.p,
+             ctx->rest.p, ctx->coh.p, ctx->Rn.p, ctx->Un.p};
+  if (nc > 0)
+  {
+    hipLaunchKernelGGL(k_rows, dim3(nbk), dim3(TPB), 0, st, bv, cv, rows_out(ctx, nc, ctx->partials.p, ctx->failcount.p),
+                       ctx->perm.p, ctx->rowptr.p, step, dynamic);
+    if (ctx->nslices > 0)
+      hipLaunchKernelGGL(k_offdiag, dim3(nblk((long long)ctx->nslices*SLICE)), dim3(TPB), 0, st, bv, cv, sell_out(ctx),
+                         ctx->perm.p, ctx->iperm.p, ctx->rowptr.p, ctx->col.p, ctx->via.p, ctx->nslices, step);
+  }
+  std::vector<double> ep((size_t)std::max(nbk, 1), 0.0);
+  int singular = 0;
+  if (nc > 0) HIPCHK(hipMemcpyAsync(ep.data(), ctx->partials.p, sizeof(double)*(size_t)nbk, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&singular, ctx->failcount.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  double e = 0.0;
+  for (int k = 0; k < nbk; k ++) e += ep[k];
+  ctx->energy = 0.5 * e;
+  ctx->step = step;
+  ctx->dynamic = dynamic;
+  ctx->assembled = 1;
+  ctx->from_bodies = 1;
+  if (singular) { ctx->err = std::to_string(singular) + " singular diagonal blocks"; return SFGS_E_SINGULAR; }
+  return 0;
+}
+
+int SFGS_System_Upload(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, const double *W,
+                       const double *Wd, const double *A, const double *B, const double *V,
+                       const double *rho, const double *friction, const double *restitution,
+                       const double *cohesion, const double *gap, double free_energy,
+                       double step, int dynamic, const double *R, const double *U)
+{
+  if (!ctx || n < 0 || !(step > 0.0) || !rowptr) return SFGS_E_ARG;
+  if (n > 0 && (!Wd || !A || !B || !V || !rho || !friction || !restitution || !cohesion || !gap || !R || !U)) return SFGS_E_ARG;
+  long long nnz = rowptr[n];
+  if (rowptr[0] != 0 || nnz < 0 || (nnz > 0 && (!col || !W))) return SFGS_E_ARG;
+  for (int i = 0; i < n; i ++) if (rowptr[i + 1] < rowptr[i]) return SFGS_E_ARG;
+  for (long long q = 0; q < nnz; q ++) if (col[q] < 0 || col[q] >= n || col[q] == -1) return SFGS_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  ctx->nc = n;
+  ctx->nnz = nnz;
+  CHK(upload(ctx, ctx->rowptr, rowptr, (size_t)n + 1));
+  CHK(upload(ctx, ctx->col, col, (size_t)nnz));
+  DBuf<double> dW, dWd, dA, dB, dV, drho, dfri, drest, dcoh, dgap;
+  CHK(upload(ctx, dW, W, 9*(size_t)nnz));
+  CHK(upload(ctx, dWd, Wd, 9*(size_t)n));
+  CHK(upload(ctx, dA, A, 9*(size_t)n));
+  CHK(upload(ctx, dB, B, 3*(size_t)n));
+  CHK(upload(ctx, dV, V, 3*(size_t)n));
+  CHK(upload(ctx, drho, rho, (size_t)n));
+  CHK(upload(ctx, dfri, friction, (size_t)n));
+  CHK(upload(ctx, drest, restitution, (size_t)n));
+  CHK(upload(ctx, dcoh, cohesion, (size_t)n));
+  CHK(upload(ctx, dgap, gap, (size_t)n));
+  CHK(upload(ctx, ctx->Rn, R, 3*(size_t)n));
+  CHK(upload(ctx, ctx->Un, U, 3*(size_t)n));
+  int rc = colour_graph(ctx, n);
+  if (!rc) rc = build_order(ctx, n);
+  if (!rc) rc = ensure_rows(ctx, n);
+  if (!rc && n > 0)
+  {
+    SysIn si{ctx->rowptr.p, ctx->col.p, dW.p, dWd.p, dA.p, dB.p, dV.p, drho.p, dfri.p, drest.p, dcoh.p, dgap.p,
+             ctx->Rn.p, ctx->Un.p};
+    hipLaunchKernelGGL(k_rows_sys, dim3(nblk(n)), dim3(TPB), 0, st, si, rows_out(ctx, n, nullptr, nullptr),
+                       ctx->perm.p, step, dynamic);
+    if (ctx->nslices > 0)
+      hipLaunchKernelGGL(k_offdiag_sys, dim3(nblk((long long)ctx->nslices*SLICE)), dim3(TPB), 0, st, si, sell_out(ctx),
+                         ctx->perm.p, ctx->iperm.p, ctx->nslices);
+  }
+  hipError_t e = hipStreamSynchronize(st);
+  release(dW); release(dWd); release(dA); release(dB); release(dV); release(drho); release(dfri);
+  release(drest); release(dcoh); release(dgap);
+  if (rc) return rc;
+  if (e != hipSuccess) { ctx->err = hipGetErrorString(e); return SFGS_E_HIP; }
+  ctx->energy = free_energy;
+  ctx->step = step;
+  ctx->dynamic = dynamic;
+  ctx->assembled = 1;
+  ctx->from_bodies = 0;
+  return 0;
+}
+
+static SweepArgs sweep_args(sfgs_ctx *ctx, const SFGS_PARAM *par)
+{
+  SweepArgs a;
+  a.row0 = ctx->slice_row0.p; a.nrows = ctx->slice_nrows.p; a.rowlen = ctx->rowlen.p; a.off = ctx->slice_off.p;
+  a.W = ctx->W.p; a.wcol = ctx->wcol.p; a.Wd = ctx->Wd.p; a.B = ctx->B.p; a.rho = ctx->rho.p; a.fri = ctx->pfri.p;
+  a.coh = ctx->pcoh.p; a.shift = ctx->shift.p; a.sep = ctx->sep.p; a.R = ctx->R.p; a.U = ctx->U.p; a.n = ctx->nc;
+  a.diageps = par->diagepsilon; a.diagmaxiter = par->diagmaxiter; a.diagsolver = par->diagsolver;
+  a.failure = par->failure; a.partials = ctx->partials.p; a.failkey = ctx->failkey.p; a.failcount = ctx->failcount.p;
+  a.ctrl = ctx->ctrl.p;
+  return a;
+}
+
+static MeritArgs merit_args(sfgs_ctx *ctx, int update_U, double *conmerit)
+{
+  MeritArgs m;
+  m.row0 = ctx->slice_row0.p; m.nrows = ctx->slice_nrows.p; m.rowlen = ctx->rowlen.p; m.off = ctx->slice_off.p;
+  m.W = ctx->W.p; m.wcol = ctx->wcol.p; m.Wd = ctx->Wd.p; m.A = ctx->A.p; m.B = ctx->B.p; m.fri = ctx->pfri.p;
+  m.coh = ctx->pcoh.p; m.shift = ctx->shift.p; m.R = ctx->R.p; m.U = ctx->U.p; m.n = ctx->nc; m.update_U = update_U;
+  m.mpartials = ctx->mpartials.p; m.conmerit = conmerit; m.ctrl = ctx->ctrl.p;
+  return m;
+}
+
+int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *merhist, SFGS_REPORT *rep)
+{
+  if (!ctx || !par || par->maxiter <= 0 || !(par->epsilon > 0.0) || par->diagmaxiter <= 0) return SFGS_E_ARG;
+  if (par->diagsolver < 0 || par->diagsolver > 3 || par->failure < 0 || par->failure > 2) return SFGS_E_ARG;
+  if (!ctx->assembled) { ctx->err = "solve before LOCDYN assembly"; return SFGS_E_STATE; }
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int n = ctx->nc, ncol = ctx->ncolours, maxiter = par->maxiter;
+  const int nomerit = par->meritval >= 1.0 ? 1 : 0;
+
+  // partial-sum slots: per colour, one per block
+  std::vector<int> pbase((size_t)ncol + 1, 0), grid((size_t)ncol, 0);
+  for (int c = 0; c < ncol; c ++)
+  {
+    int ns = ctx->h_cslice[c + 1] - ctx->h_cslice[c];
+    grid[c] = (ns + TPB/SLICE - 1) / (TPB/SLICE);
+    pbase[c + 1] = pbase[c] + grid[c];
+  }
+  const int np = pbase[ncol];
+  const int mgrid = (ctx->nslices + TPB/SLICE - 1) / (TPB/SLICE);
+  CHK(ensure(ctx, ctx->partials, 2*(size_t)std::max(np, 1)));
+  CHK(ensure(ctx, ctx->mpartials, (size_t)std::max(mgrid, 1)));
+  CHK(ensure(ctx, ctx->hist, 2*(size_t)maxiter));
+  CHK(ensure(ctx, ctx->failkey, 1));
+  CHK(ensure(ctx, ctx->failcount, 4));
+  CHK(ensure(ctx, ctx->ctrl, 1));
+  HIPCHK(hipMemsetAsync(ctx->partials.p, 0, sizeof(double)*2*(size_t)std::max(np, 1), st));
+  HIPCHK(hipMemsetAsync(ctx->mpartials.p, 0, sizeof(double)*(size_t)std::max(mgrid, 1), st));
+  HIPCHK(hipMemsetAsync(ctx->failkey.p, 0, sizeof(unsigned long long), st));
+  HIPCHK(hipMemsetAsync(ctx->failcount.p, 0, sizeof(int)*4, st));
+  Ctrl hc;
+  std::memset(&hc, 0, sizeof(hc));
+  hc.eps = par->epsilon; hc.meritval = par->meritval; hc.maxiter = maxiter; hc.nomerit = nomerit;
+  hc.energy = ctx->energy;
+  HIPCHK(hipMemcpyAsync(ctx->ctrl.p, &hc, sizeof(Ctrl), hipMemcpyHostToDevice, st));
+
+  SweepArgs sa = sweep_args(ctx, par);
+  MeritArgs ma = merit_args(ctx, 1, nullptr);
+  HIPCHK(hipEventRecord(ctx->ev0, st));
+  int batch = 2, it = 0;
+  while (it < maxiter)
+  {
+    int nbt = std::min(batch, maxiter - it);
+    for (int j = 0; j < nbt; j ++)
+    {
+      int backward = par->reverse && ((it + j) % 2);
+      for (int ci = 0; ci < ncol; ci ++)
+      {
+        int c = backward ? ncol - 1 - ci : ci;
+        if (grid[c] == 0) continue;
+        hipLaunchKernelGGL(k_sweep, dim3(grid[c]), dim3(TPB), 0, st, sa, ctx->h_cslice[c], ctx->h_cslice[c + 1],
+                           pbase[c], backward);
+      }
+      if (!nomerit && mgrid > 0) hipLaunchKernelGGL(k_merit, dim3(mgrid), dim3(TPB), 0, st, ma, ctx->nslices, 1);
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(TPB), 0, st, ctx->ctrl.p, ctx->partials.p, np, ctx->mpartials.p,
+                         mgrid, ctx->hist.p);
+    }
+    it += nbt;
+    HIPCHK(hipMemcpyAsync(&hc, ctx->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipGetLastError());
+    if (hc.done) break;
+    batch = std::min(batch*2, 32);
+  }
+  HIPCHK(hipEventRecord(ctx->ev1, st));
+  unsigned long long fk = 0;
+  int fc = 0;
+  std::vector<double> hist(2*(size_t)maxiter, 0.0);
+  HIPCHK(hipMemcpyAsync(&fk, ctx->failkey.p, sizeof(fk), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&fc, ctx->failcount.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(hist.data(), ctx->hist.p, sizeof(double)*2*(size_t)maxiter, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (rerhist) std::memcpy(rerhist, hist.data(), sizeof(double)*(size_t)maxiter);
+  if (merhist) std::memcpy(merhist, hist.data() + maxiter, sizeof(double)*(size_t)maxiter);
+  if (rep)
+  {
+    std::memset(rep, 0, sizeof(*rep));
+    rep->iters = hc.iters;
+    rep->rel_error = hc.error;
+    rep->merit = hc.merit;
+    rep->ncolours = ncol;
+    rep->fail_rows = fc;
+    rep->aborted = hc.aborted;
+    rep->solve_ms = ms;
+    rep->row_updates = (long long)hc.iters * n;
+    rep->nnz = ctx->nnz;
+    rep->error = fk ? (int)(fk & 3ull) : SFGS_GS_OK;
+    if (!hc.aborted && hc.iters >= maxiter) rep->error = SFGS_GS_DIVERGED;
+  }
+  return 0;
+}
+
+int SFGS_Merit(sfgs_ctx *ctx, int update_U, double *merit, double *conmerit)
+{
+  if (!ctx) return SFGS_E_ARG;
+  if (!ctx->assembled) { ctx->err = "merit before LOCDYN assembly"; return SFGS_E_STATE; }
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int n = ctx->nc, mgrid = (ctx->nslices + TPB/SLICE - 1) / (TPB/SLICE);
+  CHK(ensure(ctx, ctx->mpartials, (size_t)std::max(mgrid, 1)));
+  CHK(ensure(ctx, ctx->ctrl, 1));
+  DBuf<double> cm;
+  if (conmerit) { CHK(ensure(ctx, cm, (size_t)n)); }
+  MeritArgs ma = merit_args(ctx, update_U, conmerit ? cm.p : nullptr);
+  if (mgrid > 0) hipLaunchKernelGGL(k_merit, dim3(mgrid), dim3(TPB), 0, st, ma, ctx->nslices, 0);
+  std::vector<double> mp((size_t)std::max(mgrid, 1), 0.0);
+  std::vector<double> hcm(conmerit ? (size_t)n : 0), hnat(conmerit ? (size_t)n : 0);
+  std::vector<int> hperm(conmerit ? (size_t)n : 0);
+  if (mgrid > 0) HIPCHK(hipMemcpyAsync(mp.data(), ctx->mpartials.p, sizeof(double)*(size_t)mgrid, hipMemcpyDeviceToHost, st));
+  if (conmerit && n > 0)
+  {
+    HIPCHK(hipMemcpyAsync(hcm.data(), cm.p, sizeof(double)*(size_t)n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hperm.data(), ctx->perm.p, sizeof(int)*(size_t)n, hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  release(cm);
+  double up = 0.0;
+  for (int k = 0; k < mgrid; k ++) up += mp[k];
+  double lo = ctx->energy < DBL_EPSILON ? 1.0 : ctx->energy;
+  if (merit) *merit = 0.5*up / lo;
+  if (conmerit) for (int p = 0; p < n; p ++) conmerit[hperm[p]] = hcm[p] / lo;
+  return 0;
+}
+
+int SFGS_Get_RU(sfgs_ctx *ctx, double *R, double *U)
+{
+  if (!ctx) return SFGS_E_ARG;
+  if (!ctx->assembled) { ctx->err = "no system"; return SFGS_E_STATE; }
+  HIPCHK(hipSetDevice(ctx->device));
+  const int n = ctx->nc;
+  if (n == 0) return 0;
+  CHK(ensure(ctx, ctx->Rn, 3*(size_t)n));
+  CHK(ensure(ctx, ctx->Un, 3*(size_t)n));
+  hipLaunchKernelGGL(k_unperm, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->perm.p, ctx->R.p, ctx->U.p, ctx->Rn.p, ctx->Un.p);
+  if (R) HIPCHK(hipMemcpyAsync(R, ctx->Rn.p, sizeof(double)*3*(size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+  if (U) HIPCHK(hipMemcpyAsync(U, ctx->Un.p, sizeof(double)*3*(size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int SFGS_Get_Sizes(sfgs_ctx *ctx, int *n, long long *nnz, int *ncolours)
+{
+  if (!ctx) return SFGS_E_ARG;
+  if (n) *n = ctx->nc;
+  if (nnz) *nnz = ctx->nnz;
+  if (ncolours) *ncolours = ctx->ncolours;
+  return 0;
+}
+
+int SFGS_Get_Locdyn(sfgs_ctx *ctx, int *rowptr, int *col, int *via, double *W, double *Wd,
+                    double *A, double *B, double *V, double *rho, double *free_energy, int *colour)
+{
+  if (!ctx) return SFGS_E_ARG;
+  if (!ctx->assembled) { ctx->err = "no system"; return SFGS_E_STATE; }
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int n = ctx->nc;
+  const long long nnz = ctx->nnz;
+  std::vector<int> hperm((size_t)n), hrowptr((size_t)n + 1), hrow0((size_t)ctx->nslices), hnr((size_t)ctx->nslices);
+  std::vector<long long> hoff((size_t)ctx->nslices + 1);
+  std::vector<double> hWd(9*(size_t)n), hA(9*(size_t)n), hB(3*(size_t)n), hV(3*(size_t)n), hrho((size_t)n);
+  std::vector<double> hW(9*(size_t)ctx->sell_blocks);
+  std::vector<int> hcol((size_t)nnz), hvia((size_t)nnz), hwcol((size_t)ctx->sell_blocks);
+  if (n > 0)
+  {
+    HIPCHK(hipMemcpyAsync(hperm.data(), ctx->perm.p, sizeof(int)*(size_t)n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hrowptr.data(), ctx->rowptr.p, sizeof(int)*((size_t)n + 1), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hWd.data(), ctx->Wd.p, sizeof(double)*9*(size_t)n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hA.data(), ctx->A.p, sizeof(double)*9*(size_t)n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hB.data(), ctx->B.p, sizeof(double)*3*(size_t)n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hV.data(), ctx->V.p, sizeof(double)*3*(size_t)n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hrho.data(), ctx->rho.p, sizeof(double)*(size_t)n, hipMemcpyDeviceToHost, st));
+  }
+  if (nnz > 0)
+  {
+    HIPCHK(hipMemcpyAsync(hcol.data(), ctx->col.p, sizeof(int)*(size_t)nnz, hipMemcpyDeviceToHost, st));
+    if (ctx->from_bodies) HIPCHK(hipMemcpyAsync(hvia.data(), ctx->via.p, sizeof(int)*(size_t)nnz, hipMemcpyDeviceToHost, st));
+  }
+  if (ctx->nslices > 0)
+  {
+    HIPCHK(hipMemcpyAsync(hrow0.data(), ctx->slice_row0.p, sizeof(int)*(size_t)ctx->nslices, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hnr.data(), ctx->slice_nrows.p, sizeof(int)*(size_t)ctx->nslices, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hoff.data(), ctx->slice_off.p, sizeof(long long)*(size_t)ctx->nslices, hipMemcpyDeviceToHost, st));
+  }
+  if (ctx->sell_blocks > 0)
+  {
+    HIPCHK(hipMemcpyAsync(hW.data(), ctx->W.p, sizeof(double)*9*(size_t)ctx->sell_blocks, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hwcol.data(), ctx->wcol.p, sizeof(int)*(size_t)ctx->sell_blocks, hipMemcpyDeviceToHost, st));
+  }
+  std::vector<int> hcolour((size_t)n);
+  if (n > 0) HIPCHK(hipMemcpyAsync(hcolour.data(), ctx->colour.p, sizeof(int)*(size_t)n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (rowptr) std::memcpy(rowptr, hrowptr.data(), sizeof(int)*((size_t)n + 1));
+  if (col) std::memcpy(col, hcol.data(), sizeof(int)*(size_t)nnz);
+  if (via) std::memcpy(via, hvia.data(), sizeof(int)*(size_t)nnz);
+  if (colour) std::memcpy(colour, hcolour.data(), sizeof(int)*(size_t)n);
+  if (free_energy) *free_energy = ctx->energy;
+  for (int p = 0; p < n; p ++)
+  {
+    int i = hperm[p];
+    for (int e = 0; e < 9; e ++)
+    {
+      if (Wd) Wd[9*(size_t)i + e] = hWd[(size_t)e*n + p];
+      if (A) A[9*(size_t)i + e] = hA[(size_t)e*n + p];
+    }
+    for (int k = 0; k < 3; k ++)
+    {
+      if (B) B[3*(size_t)i + k] = hB[(size_t)k*n + p];
+      if (V) V[3*(size_t)i + k] = hV[(size_t)k*n + p];
+    }
+    if (rho) rho[i] = hrho[p];
+  }
+  if (W)
+  {
+    // SELL slot k of row p holds the k-th block of the natural row perm[p]
+    for (int s = 0; s < ctx->nslices; s ++)
+      for (int l = 0; l < hnr[s]; l ++)
+      {
+        int p = hrow0[s] + l, i = hperm[p];
+        for (int q = hrowptr[i], k = 0; q < hrowptr[i + 1]; q ++, k ++)
+          for (int e = 0; e < 9; e ++)
+            W[9*(size_t)q + e] = hW[9*(size_t)hoff[s] + (9*(size_t)k + e)*SLICE + l];
+      }
+  }
+  return 0;
+}
+
+int SFGS_Colour(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, int *colour)
+{
+  if (!ctx || n < 0 || !rowptr || (n > 0 && !colour)) return SFGS_E_ARG;
+  long long nnz = rowptr[n];
+  if (rowptr[0] != 0 || nnz < 0 || (nnz > 0 && !col)) return SFGS_E_ARG;
+  for (long long q = 0; q < nnz; q ++) if (col[q] < 0 || col[q] >= n) return SFGS_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  CHK(upload(ctx, ctx->rowptr, rowptr, (size_t)n + 1));
+  CHK(upload(ctx, ctx->col, col, (size_t)nnz));
+  ctx->assembled = 0;
+  CHK(colour_graph(ctx, n));
+  if (n > 0) HIPCHK(hipMemcpyAsync(colour, ctx->colour.p, sizeof(int)*(size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return ctx->ncolours;
+}
+
+int SFGS_Body_Update(sfgs_ctx *ctx, double step, double *velo_out, double *force_out)
+{
+  if (!ctx || !velo_out) return SFGS_E_ARG;
+  if (!ctx->assembled || !ctx->from_bodies) { ctx->err = "body update needs a system assembled from bodies"; return SFGS_E_STATE; }
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int nb = ctx->nb;
+  if (nb == 0) return 0;
+  DBuf<double> out;
+  CHK(ensure(ctx, out, 12*(size_t)nb));
+  BodyView bv{ctx->kind.p, ctx->rot.p, ctx->x0.p, ctx->jinv.p, ctx->minv.p, ctx->velo.p, ctx->velo0.p};
+  ConView cv{ctx->master.p, ctx->slave.p, ctx->mpnt.p, ctx->spnt.p, ctx->base.p, ctx->gap.p, ctx->fri.p,
+             ctx->rest.p, ctx->coh.p, ctx->Rn.p, ctx->Un.p};
+  hipLaunchKernelGGL(k_body_update, dim3(nblk(nb)), dim3(TPB), 0, st, bv, cv, nb, ctx->bptr.p, ctx->blist.p,
+                     ctx->iperm.p, ctx->R.p, step, out.p, out.p + 6*(size_t)nb);
+  HIPCHK(hipMemcpyAsync(velo_out, out.p, sizeof(double)*6*(size_t)nb, hipMemcpyDeviceToHost, st));
+  if (force_out) HIPCHK(hipMemcpyAsync(force_out, out.p + 6*(size_t)nb, sizeof(double)*6*(size_t)nb, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  release(out);
+  return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,151 @@
+"""Thin Python handle over one libsfgs.so context (one GPU W/B/R store).
+
+Every method calls straight into the HIP library; there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import SFGS_PARAM, SFGS_REPORT, f64, i32, ptr, sfgs
+
+FP64, MIXED = 0, 1
+DIAGSOLVERS = {"PROJECTED_GRADIENT": 0, "DE_SAXCE_FENG": 1, "SEMISMOOTH_NEWTON": 2, "PROJECTED_NEWTON": 3}
+FAILURES = {"CONTINUE": 0, "EXIT": 1, "CALLBACK": 2}
+GS_ERRORS = {0: "OK", 1: "DIVERGED", 2: "DIAGONAL_DIVERGED", 3: "DIAGONAL_FAILED"}
+
+
+class SfgsError(RuntimeError):
+    pass
+
+
+def default_diag(epsilon, maxiter, meritval=1.0):
+    """diagepsilon / diagmaxiter defaults of GAUSS_SEIDEL_SOLVER (lng.c:4336-4342)."""
+    de = 0.01 * min(epsilon, min(meritval, 1e-4))
+    de = max(de, np.finfo(np.float64).eps)
+    return de, max(100, maxiter // 100)
+
+
+def make_param(epsilon, maxiter, meritval=1.0, failure=0, diagepsilon=None, diagmaxiter=None,
+               diagsolver=2, reverse=0, precision=FP64, refine=2):
+    de, dm = default_diag(epsilon, maxiter, meritval)
+    p = SFGS_PARAM()
+    p.epsilon = epsilon
+    p.maxiter = maxiter
+    p.meritval = meritval
+    p.failure = failure
+    p.diagepsilon = de if diagepsilon is None else diagepsilon
+    p.diagmaxiter = dm if diagmaxiter is None else diagmaxiter
+    p.diagsolver = diagsolver
+    p.reverse = reverse
+    p.precision = precision
+    p.refine = refine
+    return p
+
+
+class Device:
+    """One device context (``SFGS_Create``)."""
+
+    def __init__(self, device=0):
+        self.lib = sfgs()
+        self.ctx = self.lib.SFGS_Create(device)
+        if not self.ctx:
+            raise SfgsError("SFGS_Create failed: no HIP device")
+
+    def close(self):
+        if self.ctx:
+            self.lib.SFGS_Destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc < 0:
+            msg = self.lib.SFGS_Last_Error(self.ctx).decode()
+            raise SfgsError(f"{what}: {self.lib.SFGS_Error_String(rc).decode()} ({msg})")
+        return rc
+
+    # ---- inputs -------------------------------------------------------
+    def bodies(self, kind, rot, x0, jinv, minv, velo, velo0):
+        nb = len(kind)
+        a = [i32(kind, nb), f64(rot, 9 * nb), f64(x0, 3 * nb), f64(jinv, 9 * nb), f64(minv, nb),
+             f64(velo, 6 * nb), f64(velo0, 6 * nb)]
+        self._check(self.lib.SFGS_Bodies_Upload(self.ctx, nb, *[ptr(x) for x in a]), "SFGS_Bodies_Upload")
+
+    def contacts(self, master, slave, mpnt, spnt, base, gap, friction, restitution, cohesion, R, U):
+        n = len(master)
+        a = [i32(master, n), i32(slave, n), f64(mpnt, 3 * n), f64(spnt, 3 * n), f64(base, 9 * n), f64(gap, n),
+             f64(friction, n), f64(restitution, n), f64(cohesion, n), f64(R, 3 * n), f64(U, 3 * n)]
+        self._check(self.lib.SFGS_Contacts_Upload(self.ctx, n, *[ptr(x) for x in a]), "SFGS_Contacts_Upload")
+
+    def assemble(self, step, dynamic=1):
+        self._check(self.lib.SFGS_Locdyn_Update_Begin(self.ctx, step, dynamic), "SFGS_Locdyn_Update_Begin")
+
+    def system(self, rowptr, col, W, Wd, A, B, V, rho, friction, restitution, cohesion, gap,
+               free_energy, step, R, U, dynamic=1):
+        n = len(rowptr) - 1
+        rowptr = i32(rowptr, n + 1)
+        nnz = int(rowptr[-1])
+        a = [rowptr, i32(col, nnz), f64(W, 9 * nnz), f64(Wd, 9 * n), f64(A, 9 * n), f64(B, 3 * n),
+             f64(V, 3 * n), f64(rho, n), f64(friction, n), f64(restitution, n), f64(cohesion, n), f64(gap, n)]
+        Rr, Ur = f64(R, 3 * n), f64(U, 3 * n)
+        rc = self.lib.SFGS_System_Upload(self.ctx, n, *[ptr(x) for x in a], float(free_energy), float(step),
+                                         int(dynamic), ptr(Rr), ptr(Ur))
+        self._check(rc, "SFGS_System_Upload")
+
+    # ---- solve --------------------------------------------------------
+    def solve(self, param, history=True):
+        rep = SFGS_REPORT()
+        rer = np.zeros(param.maxiter) if history else None
+        mer = np.zeros(param.maxiter) if history else None
+        self._check(self.lib.SFGS_Solve(self.ctx, C.byref(param), ptr(rer), ptr(mer), C.byref(rep)), "SFGS_Solve")
+        if history:
+            rer, mer = rer[:rep.iters].copy(), mer[:rep.iters].copy()
+        return rep, rer, mer
+
+    def merit(self, update_U=1, conmerit=False):
+        m = C.c_double(0.0)
+        cm = np.zeros(self.sizes()[0]) if conmerit else None
+        self._check(self.lib.SFGS_Merit(self.ctx, update_U, C.byref(m), ptr(cm)), "SFGS_Merit")
+        return (m.value, cm) if conmerit else m.value
+
+    # ---- outputs ------------------------------------------------------
+    def sizes(self):
+        n, nnz, nc = C.c_int(0), C.c_longlong(0), C.c_int(0)
+        self._check(self.lib.SFGS_Get_Sizes(self.ctx, C.byref(n), C.byref(nnz), C.byref(nc)), "SFGS_Get_Sizes")
+        return n.value, nnz.value, nc.value
+
+    def get_RU(self):
+        n = self.sizes()[0]
+        R, U = np.zeros(3 * n), np.zeros(3 * n)
+        self._check(self.lib.SFGS_Get_RU(self.ctx, ptr(R), ptr(U)), "SFGS_Get_RU")
+        return R.reshape(n, 3), U.reshape(n, 3)
+
+    def locdyn(self):
+        n, nnz, _ = self.sizes()
+        out = dict(rowptr=np.zeros(n + 1, np.int32), col=np.zeros(nnz, np.int32), via=np.zeros(nnz, np.int32),
+                   W=np.zeros(9 * nnz), Wd=np.zeros(9 * n), A=np.zeros(9 * n), B=np.zeros(3 * n),
+                   V=np.zeros(3 * n), rho=np.zeros(n), colour=np.zeros(n, np.int32))
+        e = C.c_double(0.0)
+        rc = self.lib.SFGS_Get_Locdyn(self.ctx, ptr(out["rowptr"]), ptr(out["col"]), ptr(out["via"]),
+                                      ptr(out["W"]), ptr(out["Wd"]), ptr(out["A"]), ptr(out["B"]),
+                                      ptr(out["V"]), ptr(out["rho"]), C.byref(e), ptr(out["colour"]))
+        self._check(rc, "SFGS_Get_Locdyn")
+        out["free_energy"] = e.value
+        return out
+
+    def colour(self, rowptr, col):
+        n = len(rowptr) - 1
+        rowptr = i32(rowptr, n + 1)
+        col = i32(col, int(rowptr[-1]))
+        out = np.zeros(max(n, 1), np.int32)
+        nc = self._check(self.lib.SFGS_Colour(self.ctx, n, ptr(rowptr), ptr(col), ptr(out)), "SFGS_Colour")
+        return nc, out[:n]
+
+    def body_update(self, step, nb):
+        velo, force = np.zeros(6 * nb), np.zeros(6 * nb)
+        self._check(self.lib.SFGS_Body_Update(self.ctx, step, ptr(velo), ptr(force)), "SFGS_Body_Update")
+        return velo.reshape(nb, 6), force.reshape(nb, 6)

@@ -1,0 +1,193 @@
+"""Benchmark of the contact-solve hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 200,000 rigid
+spheres, radii U[0.045, 0.055] m, density 2000 kg/m^3, on a seeded jittered
+cubic lattice (+-0.005 m) in a 6x6x6 m box of 6 fixed planes, gravity 9.81,
+Coulomb mu = 0.5, restitution 0, h = 1e-3 s, settled for 500 steps through the
+product RUN loop (GAUSS_SEIDEL_SOLVER(1e-4, 1000) while settling).  The
+settled state of the next step (bodies, detected contacts, previous-step R and
+U as warm start) is then uploaded once and stays resident in HBM.
+
+One timed step = LOCDYN assembly on the device + GAUSS_SEIDEL_SOLVER with
+epsilon = 1e-6, meritval = 1e-6, maxiter = 5000 + body velocity update, all
+on the GPU; each step starts from the same resident state.  Every library
+call returns after synchronising its stream, so the host clock brackets
+completed device work.
+
+Printed line: value = contact-block updates/s (row updates of the GS sweeps per
+second of whole-step time), ms_per_step, the roofline of the sweep kernel
+(algorithmic bytes per row s(9k+26)+4k+4, k = off-diagonal blocks, over the
+HIP-event duration of every sweep launch) and the CPU oracle timed on a
+bounded sample of the same system.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "solfec-1.0_amd"), ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "ms/step to merit<=1e-6 & contact-block updates/s: learned+mixed vs plain GPU GS"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def settle(n, L, steps, seed, friction):
+    import solfec
+    s = solfec.SOLFEC("DYNAMIC", 1e-3, "/tmp/solfec_bench")
+    solfec.SURFACE_MATERIAL(s, friction=friction, restitution=0.0)
+    solfec.GRAVITY(s, (0.0, 0.0, -9.81))
+    solfec.LATTICE(s, n, L, 0.045, 0.055, 0.005, 2000.0, seed)
+    gs = solfec.GAUSS_SEIDEL_SOLVER(1e-4, 1000)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        rc = s.step_begin() or s.step_solve(gs) or s.step_end()
+        if rc:
+            raise RuntimeError(f"settling step {k} failed: {s.lib.SOLFEC_Last_Error(s.sol).decode()}")
+        if (k + 1) % 25 == 0 or k == 0:
+            log(f"[settle] step {k + 1}/{steps} contacts={s.ncon} gs_iters={gs.iters} "
+                f"elapsed={time.perf_counter() - t0:.1f}s")
+    s.step_begin()   # contacts of the next step; R, U carried from the last solve
+    return s
+
+
+def cpu_baseline(b, c, colours, n, seconds):
+    """The oracle (C restatement, 1 core) on the same system: GS iterations of
+    sweep + merit, as the GPU solve does, for about `seconds` of CPU time."""
+    import oracle
+    O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), 1e-3)
+    sysobj = oracle.System(O)
+    order, _ = oracle.colour_order(colours, int(colours.max()) + 1)
+    R0, U0 = c["R"].copy(), c["U"].copy()
+    p = oracle.gs_param(1e-6, 1, meritval=1e-6)
+    t0 = time.perf_counter()
+    oracle.gs_solve(sysobj, p, order, R0, U0, history=False)
+    t1 = time.perf_counter() - t0
+    k = max(1, int(seconds / max(t1, 1e-6)))
+    p = oracle.gs_param(1e-300, k, meritval=1e-300)
+    t0 = time.perf_counter()
+    rc, rep, _, _, _, _ = oracle.gs_solve(sysobj, p, order, R0, U0, history=False)
+    dt = time.perf_counter() - t0
+    return dict(value=rep.iters * n / dt, unit="contact-block updates/s", cores=1, kind="port",
+                sample=f"{rep.iters} GS iterations (colour-major sweep + merit) of the same {n}-contact "
+                       f"system, oracle/liboracle.so on 1 host core, {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spheres", type=int, default=200000)
+    ap.add_argument("--box", type=float, default=6.0)
+    ap.add_argument("--settle", type=int, default=500)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--friction", type=float, default=0.5)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lanes", type=int, default=1, help="lanes of a wavefront per contact row (1, 2, 4, 8)")
+    ap.add_argument("--save-snapshot", default=None, help="write the settled state to this .npz")
+    ap.add_argument("--snapshot", default=None, help="load a settled state instead of settling")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs the contact solve on a single MI355X "
+              f"(multi-GPU sharding is a design only); got --gpus {args.gpus}", file=sys.stderr)
+        sys.exit(2)
+
+    import solfec
+    h = 1e-3
+    if args.snapshot:
+        z = np.load(args.snapshot)
+        b = {k[2:]: z[k] for k in z.files if k.startswith("b_")}
+        c = {k[2:]: z[k] for k in z.files if k.startswith("c_")}
+    else:
+        s = settle(args.spheres, args.box, args.settle, args.seed, args.friction)
+        b, c = s.bodies(), s.contacts()
+        if args.save_snapshot:
+            np.savez(args.save_snapshot, **{"b_" + k: v for k, v in b.items()}, **{"c_" + k: v for k, v in c.items()})
+    nb = len(b["kind"])
+    dev = solfec.Device(0)
+    dev.set_layout(args.lanes)
+    dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
+    dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
+                 c["restitution"], c["cohesion"], c["R"], c["U"])
+    par = solfec.make_param(1e-6, 5000, meritval=1e-6)
+
+    def step():
+        dev.assemble(h)
+        rep, _, _ = dev.solve(par, history=False)
+        dev.body_update(h, nb, download=False)
+        return rep
+
+    for _ in range(args.warmup):
+        rep = step()
+    log(f"[bench] warmup done: iters={rep.iters} merit={rep.merit:.3e} error={rep.rel_error:.3e}")
+    reps = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        reps.append(step())
+    dt = time.perf_counter() - t0
+    n, nnz, ncol = dev.sizes()
+    iters = [r.iters for r in reps]
+    updates = sum(iters) * n
+    ms_per_step = 1e3 * dt / args.steps
+    value = updates / dt
+    log(f"[bench] timed {args.steps} steps: {ms_per_step:.2f} ms/step, iters={iters}")
+
+    # roofline of the sweep kernel: one profiled solve, HIP events around every launch
+    dev.assemble(h)
+    prof = solfec.make_param(1e-6, 5000, meritval=1e-6, profile=1)
+    prep, _, _ = dev.solve(prof, history=False)
+    bytes_per_sweep = 8 * (9 * nnz + 26 * n) + 4 * nnz + 4 * n
+    sweeps_run = prep.sweep_launches / max(ncol, 1)
+    achieved = bytes_per_sweep * sweeps_run / (prep.sweep_ms * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel="k_sweep", avg_launch_us=round(1e3 * prep.sweep_ms / max(prep.sweep_launches, 1), 3),
+                launches=int(prep.sweep_launches), bytes_per_sweep=int(bytes_per_sweep))
+    log(f"[bench] sweep kernel: {prep.sweep_launches} launches, {prep.sweep_ms:.2f} ms, {achieved:.1f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        colours = dev.locdyn()["colour"]
+        cpu = cpu_baseline(b, c, colours, n, args.cpu_seconds)
+        log(f"[bench] cpu baseline: {cpu['value']:.3e} updates/s")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "contact-block updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.spheres}-sphere packing settled {args.settle} steps, plain FP64 GPU GS",
+            "spheres": args.spheres, "contacts": n, "offdiag_blocks": nnz, "colours": ncol,
+            "epsilon": 1e-6, "meritval": 1e-6, "maxiter": 5000, "friction": args.friction,
+            "iters_per_step": iters, "merit": reps[-1].merit, "rel_error": reps[-1].rel_error,
+            "precision": "fp64", "warm_start": "previous-step R", "lanes_per_row": args.lanes,
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    dev.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -63,6 +63,7 @@ typedef struct {
   int reverse;         /* 1: alternate forward / backward sweeps (bgs.c:936-950) */
   int precision;       /* SFGS_PREC_* */
   int refine;          /* FP64 refinement sweeps after a mixed solve (default 2) */
+  int profile;         /* 1: time every sweep-kernel launch with HIP events (slower) */
 } SFGS_PARAM;
 
 typedef struct {
@@ -74,7 +75,8 @@ typedef struct {
   int fail_rows;       /* local-solver failures (callback invocations) */
   int aborted;         /* 1 when FAILURE_EXIT stopped the solve */
   double solve_ms;     /* device time of the solve (HIP events) */
-  double sweep_ms;     /* device time spent in sweep kernels (HIP events) */
+  double sweep_ms;     /* device time of the sweep-kernel launches (profile mode) */
+  long long sweep_launches; /* sweep-kernel launches timed in profile mode */
   long long row_updates; /* rows visited = sweeps x rows */
   long long nnz;       /* off-diagonal blocks */
 } SFGS_REPORT;
@@ -135,6 +137,11 @@ int SFGS_Get_Locdyn(sfgs_ctx *ctx, int *rowptr, int *col, int *via, double *W, d
                     double *A, double *B, double *V, double *rho, double *free_energy, int *colour);
 int SFGS_Get_Sizes(sfgs_ctx *ctx, int *n, long long *nnz, int *ncolours);
 
+/* Lanes of a wavefront that serve one contact row in the sweep layout (1, 2,
+ * 4 or 8; default 1).  Takes effect at the next LOCDYN assembly or system
+ * upload.  No counterpart in the reference (a tuning knob of the GPU layout). */
+int SFGS_Set_Layout(sfgs_ctx *ctx, int lanes_per_row);
+
 /* Colour an arbitrary graph given in CSR (no self loops) with the device
  * colouring kernel; returns the number of colours or a negative code. */
 int SFGS_Colour(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, int *colour);
@@ -142,7 +149,8 @@ int SFGS_Colour(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, int *co
 /* Body velocity update of BODY_Dynamic_Step_End (bod.c:1446-1447 with
  * rig_constraints_force, bod.c:418-435): velo += h iM r, r = sum(+-H' R) over
  * the contacts of every rigid body, using the current R; velo_out receives 6
- * per body, force_out (may be NULL) the generalised reaction r, 6 per body. */
+ * per body, force_out the generalised reaction r, 6 per body; either may be
+ * NULL, in which case that result stays on the device. */
 int SFGS_Body_Update(sfgs_ctx *ctx, double step, double *velo_out, double *force_out);
 
 #ifdef __cplusplus

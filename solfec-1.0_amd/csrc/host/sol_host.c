@@ -22,6 +22,15 @@
 #include "solfec.h"
 #include "sol_internal.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+static int omp_get_num_threads_(void) { return omp_get_num_threads(); }
+static int omp_get_thread_num_(void) { return omp_get_thread_num(); }
+#else
+static int omp_get_num_threads_(void) { return 1; }
+static int omp_get_thread_num_(void) { return 0; }
+#endif
+
 #define GEOMETRIC_EPSILON 1e-6   /* alg.c:24 */
 
 static double now_s(void)
@@ -148,6 +157,9 @@ void SOLFEC_Destroy(SOLFEC *sol)
   free(sol->c_point);
   free(sol->grid_head); free(sol->grid_next); free(sol->pair_keys); free(sol->pair_vals);
   free(sol->tmp_velo);
+  free(sol->upd_geo); free(sol->upd_ok);
+  for (int t = 0; t < sol->cand_nthr; t ++) free(sol->cand[t]);
+  free(sol->cand); free(sol->cand_n); free(sol->cand_cap);
   free(sol);
 }
 
@@ -500,19 +512,40 @@ static int update_and_detect(SOLFEC *sol)
   double t0 = now_s();
   int nc = sol->nc, kept = 0;
   double sp[3], mp[3], nrm[3], g;
+
+  /* geometry of every existing contact (independent per contact) */
+  if ((size_t)nc > sol->upd_cap)
+  {
+    sol->upd_geo = xrealloc(sol->upd_geo, sizeof(double)*10*(size_t)nc);
+    sol->upd_ok = xrealloc(sol->upd_ok, sizeof(int)*(size_t)nc);
+    sol->upd_cap = (size_t)nc;
+  }
+  #pragma omp parallel for schedule(static)
   for (int i = 0; i < nc; i ++)
   {
-    int m = sol->c_master[i], s = sol->c_slave[i], ok;
-    if (sol->shape[s] == 1) ok = plane_sphere(sol, s, m, sp, mp, nrm, &g);
-    else ok = sphere_pair(sol, s, m, sp, mp, nrm, &g);
-    if (!ok) continue;
+    int m = sol->c_master[i], s = sol->c_slave[i];
+    double *G = sol->upd_geo + 10*(size_t)i;
+    if (sol->shape[s] == 1) sol->upd_ok[i] = plane_sphere(sol, s, m, G, G + 3, G + 6, G + 9);
+    else sol->upd_ok[i] = sphere_pair(sol, s, m, G, G + 3, G + 6, G + 9);
+  }
+  /* compaction keeps the insertion order; upd_ok receives the old index of each kept slot */
+  for (int i = 0; i < nc; i ++)
+  {
+    if (!sol->upd_ok[i]) continue;
     if (kept != i)
     {
+      sol->c_master[kept] = sol->c_master[i];
+      sol->c_slave[kept] = sol->c_slave[i];
       memcpy(sol->c_R + 3*kept, sol->c_R + 3*i, 3*sizeof(double));
       memcpy(sol->c_U + 3*kept, sol->c_U + 3*i, 3*sizeof(double));
     }
-    set_contact(sol, kept, m, s, mp, sp, nrm, g);
-    kept ++;
+    sol->upd_ok[kept ++] = i;
+  }
+  #pragma omp parallel for schedule(static)
+  for (int k = 0; k < kept; k ++)
+  {
+    const double *G = sol->upd_geo + 10*(size_t)sol->upd_ok[k];
+    set_contact(sol, k, sol->c_master[k], sol->c_slave[k], G + 3, G, G + 6, G[9]);
   }
   sol->nc = kept;
   double t1 = now_s();
@@ -570,34 +603,70 @@ static int update_and_detect(SOLFEC *sol)
       sol->grid_next[b] = sol->grid_head[c];
       sol->grid_head[c] = b;
     }
-    for (int a = 0; a < nb; a ++)
+    /* candidate new pairs per thread, merged in ascending order of a */
+    int nthr = 1;
+    #pragma omp parallel
     {
-      if (sol->shape[a] != 0) continue;
-      int ci[3];
-      for (int k = 0; k < 3; k ++) ci[k] = (int)floor((sol->x[3*a + k] - lo[k])/cell);
-      for (int dz = -1; dz <= 1; dz ++)
-        for (int dy = -1; dy <= 1; dy ++)
-          for (int dx = -1; dx <= 1; dx ++)
-          {
-            int cx = ci[0] + dx, cy = ci[1] + dy, cz = ci[2] + dz;
-            if (cx < 0 || cy < 0 || cz < 0 || cx >= dim[0] || cy >= dim[1] || cz >= dim[2]) continue;
-            for (int b = sol->grid_head[((long long)cz*dim[1] + cy)*dim[0] + cx]; b >= 0; b = sol->grid_next[b])
-            {
-              if (b <= a) continue;
-              if (map_find(sol, a, b) >= 0) continue;
-              /* orientation rule of detect_sphere_sphere (goc.c:421-455): the body with the
-               * lexicographically larger centre is the master */
-              int slave = a, master = b;
-              if (pntcmp(sol->x + 3*a, sol->x + 3*b) > 0) { slave = b; master = a; }
-              if (!sphere_pair(sol, slave, master, sp, mp, nrm, &g)) continue;
-              ensure_contacts(sol, sol->nc + 1);
-              int i = sol->nc ++;
-              set_contact(sol, i, master, slave, mp, sp, nrm, g);
-              memset(sol->c_R + 3*i, 0, 3*sizeof(double));
-              memset(sol->c_U + 3*i, 0, 3*sizeof(double));
-            }
-          }
+      #pragma omp single
+      nthr = omp_get_num_threads_();
     }
+    if (sol->cand_nthr < nthr)
+    {
+      sol->cand = xrealloc(sol->cand, sizeof(int*)*(size_t)nthr);
+      sol->cand_n = xrealloc(sol->cand_n, sizeof(int)*(size_t)nthr);
+      sol->cand_cap = xrealloc(sol->cand_cap, sizeof(int)*(size_t)nthr);
+      for (int t = sol->cand_nthr; t < nthr; t ++) { sol->cand[t] = NULL; sol->cand_cap[t] = 0; }
+      sol->cand_nthr = nthr;
+    }
+    #pragma omp parallel
+    {
+      int t = omp_get_thread_num_();
+      int chunk = (nb + nthr - 1) / nthr, a0 = t*chunk, a1 = a0 + chunk < nb ? a0 + chunk : nb;
+      sol->cand_n[t] = 0;
+      for (int a = a0; a < a1; a ++)
+      {
+        if (sol->shape[a] != 0) continue;
+        int ci[3];
+        for (int k = 0; k < 3; k ++) ci[k] = (int)floor((sol->x[3*a + k] - lo[k])/cell);
+        for (int dz = -1; dz <= 1; dz ++)
+          for (int dy = -1; dy <= 1; dy ++)
+            for (int dx = -1; dx <= 1; dx ++)
+            {
+              int cx = ci[0] + dx, cy = ci[1] + dy, cz = ci[2] + dz;
+              if (cx < 0 || cy < 0 || cz < 0 || cx >= dim[0] || cy >= dim[1] || cz >= dim[2]) continue;
+              for (int b = sol->grid_head[((long long)cz*dim[1] + cy)*dim[0] + cx]; b >= 0; b = sol->grid_next[b])
+              {
+                if (b <= a) continue;
+                double dx_ = sol->x[3*b] - sol->x[3*a], dy_ = sol->x[3*b + 1] - sol->x[3*a + 1],
+                       dz_ = sol->x[3*b + 2] - sol->x[3*a + 2], e_ = sol->rad[a] + sol->rad[b] + GEOMETRIC_EPSILON;
+                if (dx_*dx_ + dy_*dy_ + dz_*dz_ >= e_*e_) continue;
+                if (map_find(sol, a, b) >= 0) continue;
+                if (sol->cand_n[t] + 2 > sol->cand_cap[t])
+                {
+                  sol->cand_cap[t] = sol->cand_cap[t] ? 2*sol->cand_cap[t] : 1024;
+                  sol->cand[t] = xrealloc(sol->cand[t], sizeof(int)*(size_t)sol->cand_cap[t]);
+                }
+                sol->cand[t][sol->cand_n[t] ++] = a;
+                sol->cand[t][sol->cand_n[t] ++] = b;
+              }
+            }
+      }
+    }
+    for (int t = 0; t < nthr; t ++)
+      for (int q = 0; q < sol->cand_n[t]; q += 2)
+      {
+        int a = sol->cand[t][q], b = sol->cand[t][q + 1];
+        /* orientation rule of detect_sphere_sphere (goc.c:421-455): the body with the
+         * lexicographically larger centre is the master */
+        int slave = a, master = b;
+        if (pntcmp(sol->x + 3*a, sol->x + 3*b) > 0) { slave = b; master = a; }
+        if (!sphere_pair(sol, slave, master, sp, mp, nrm, &g)) continue;
+        ensure_contacts(sol, sol->nc + 1);
+        int i = sol->nc ++;
+        set_contact(sol, i, master, slave, mp, sp, nrm, g);
+        memset(sol->c_R + 3*i, 0, 3*sizeof(double));
+        memset(sol->c_U + 3*i, 0, 3*sizeof(double));
+      }
   }
   /* planes against all spheres */
   for (int p = 0; p < nb; p ++)
@@ -639,6 +708,7 @@ sfgs_ctx* SOLFEC_Gpu(SOLFEC *sol)
 int SOLFEC_Step_Begin(SOLFEC *sol)
 {
   double t0 = now_s();
+  #pragma omp parallel for schedule(static)
   for (int b = 0; b < sol->nb; b ++) body_step_begin(sol, b);
   sol->t_timint += now_s() - t0;
   ensure_contacts(sol, 1);
@@ -648,6 +718,7 @@ int SOLFEC_Step_Begin(SOLFEC *sol)
 int SOLFEC_Step_End(SOLFEC *sol)
 {
   double t0 = now_s();
+  #pragma omp parallel for schedule(static)
   for (int b = 0; b < sol->nb; b ++) body_step_end(sol, b);
   sol->time += sol->step;
   sol->t_timint += now_s() - t0;

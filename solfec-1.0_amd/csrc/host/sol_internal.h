@@ -42,6 +42,10 @@ struct solfec {
   unsigned long long *pair_keys;
   int *pair_vals;
   size_t pair_cap, pair_mask;
+  double *upd_geo;
+  int *upd_ok;
+  size_t upd_cap;
+  int **cand, *cand_n, *cand_cap, cand_nthr;
 
   /* phase timers (sol.c:392-439 labels), seconds */
   double t_timint, t_conupd, t_condet, t_locdyn, t_consol;

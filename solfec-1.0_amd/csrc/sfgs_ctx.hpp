@@ -18,14 +18,17 @@ struct DBuf {
 
 // device control block of one solve
 struct Ctrl {
-  int iters;        // completed sweeps
+  int iters;        // completed Gauss-Seidel iterations (gs->iters)
+  int sweeps;       // sweep kernels run so far
   int done;         // terminated (converged, maxiter or abort)
   int aborted;      // FAILURE_EXIT raised in a sweep
-  int refine;       // refinement sweeps still to run (mixed precision)
+  int final_buf;    // R buffer (0 / 1) holding the final reactions
+  int final_U;      // 0: U, 1: Um holds the final relative velocities
+  int maxiter, nomerit;
   double error;     // last relative error
   double merit;     // last merit
+  double prev_error;
   double eps, meritval;
-  int maxiter, nomerit;
   double energy;    // free energy (merit denominator source)
 };
 
@@ -71,7 +74,9 @@ struct sfgs_ctx {
 
   // ---- permuted row data (colour-major order)
   sfgs::DBuf<int> rowlen;                // per permuted row
-  sfgs::DBuf<double> W, Wd, A, B, V, rho, pfri, pcoh, shift, R, U, Rsave, Usave;
+  sfgs::DBuf<double> W, Wd, A, B, V, rho, pfri, pcoh, shift, R, U, Rb, Um;
+  int G = 1;                             // lanes per row of the sweep layout (1, 2, 4, 8)
+  int G_built = 1;                       // layout of the current store
   sfgs::DBuf<unsigned char> sep;
   sfgs::DBuf<int> wcol;
   sfgs::DBuf<float> W32, Wd32, B32, R32, U32, rho32, pfri32, pcoh32, shift32;
@@ -85,4 +90,6 @@ struct sfgs_ctx {
   sfgs::DBuf<int> failcount;
   sfgs::DBuf<sfgs::Ctrl> ctrl;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  std::vector<hipEvent_t> prof_ev;        // per-launch sweep timing (profile mode)
+  sfgs::DBuf<double> body_out;            // body update results (velo, force)
 };

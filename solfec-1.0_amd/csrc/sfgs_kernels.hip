@@ -321,14 +321,15 @@ __global__ void k_colour_count(int n, const int *colour, int *count)
   if (i < n) atomicAdd(&count[colour[i]], 1);
 }
 
-__global__ void k_slice_len(int ns, const int *row0, const int *perm, const int *rowptr, int *slen, long long *slen_ll)
+__global__ void k_slice_len(int ns, const int *row0, const int *perm, const int *rowptr, int G, int *slen, long long *slen_ll)
 {
   int s = blockIdx.x*blockDim.x + threadIdx.x;
   if (s >= ns) return;
   int i = perm[row0[s]];
   int len = rowptr[i + 1] - rowptr[i];
-  slen[s] = len;
-  slen_ll[s] = (long long)len * SLICE;
+  int nslot = (len + G - 1) / G;   // slots of one block per lane
+  slen[s] = nslot;
+  slen_ll[s] = (long long)nslot * SLICE;
 }
 
 // =====================================================================
@@ -434,22 +435,25 @@ struct SellOut {
   int *wcol;
 };
 
-// off-diagonal blocks W_ij = h s_i s_j H_i^b iM_b H_j^b' (ldy.c:738-844), one
-// lane per row of a slice; the block for the pair is formed from the lower
-// index as "upper" and transposed for the other, as the reference's symmetry copy
+// off-diagonal blocks W_ij = h s_i s_j H_i^b iM_b H_j^b' (ldy.c:738-844).  G
+// lanes serve one row: lane g of row r (lane l = rG + g) holds blocks k = mG + g
+// in slot m, element e at W[9 off + (9m + e) 64 + l].  The block of a pair is
+// formed with the lower contact index as the "upper" side and transposed for
+// the other, as the reference's symmetry copy (ldy.c:830-844).
+template <int G>
 __global__ __launch_bounds__(TPB) void k_offdiag(BodyView bv, ConView cv, SellOut so, const int *perm,
                                                   const int *iperm, const int *rowptr, const int *col,
                                                   const int *via, int nslices, double step)
 {
-  int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (s >= nslices || lane >= so.nrows[s]) return;
-  int p = so.row0[s] + lane, i = perm[p];
+  int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6), l = threadIdx.x & 63, r = l / G, g = l % G;
+  if (s >= nslices || r >= so.nrows[s]) return;
+  int p = so.row0[s] + r, i = perm[p];
   long long off = so.off[s];
-  double *Wb = so.W + 9*off + lane;
-  int *cb = so.wcol + off + lane;
-  for (int q = rowptr[i], k = 0; q < rowptr[i + 1]; q ++, k ++)
+  double *Wb = so.W + 9*off + l;
+  int *cb = so.wcol + off + l;
+  for (int k = g, m = 0; k < rowptr[i + 1] - rowptr[i]; k += G, m ++)
   {
-    int j = col[q], b = via[q];
+    int q = rowptr[i] + k, j = col[q], b = via[q];
     int lo = i < j ? i : j, hi = i < j ? j : i;
     double sl = cv.master[lo] == b ? 1.0 : -1.0, sh = cv.master[hi] == b ? 1.0 : -1.0;
     const double *xl = sl > 0.0 ? cv.mpnt + 3*(size_t)lo : cv.spnt + 3*(size_t)lo;
@@ -464,17 +468,17 @@ __global__ __launch_bounds__(TPB) void k_offdiag(BodyView bv, ConView cv, SellOu
     if (i == lo)
     {
 #pragma unroll
-      for (int e = 0; e < 9; e ++) Wb[(9*(size_t)k + e)*SLICE] = f * P[e];
+      for (int e = 0; e < 9; e ++) Wb[(9*(size_t)m + e)*SLICE] = f * P[e];
     }
     else
     {
 #pragma unroll
-      for (int r = 0; r < 3; r ++)
+      for (int rr = 0; rr < 3; rr ++)
 #pragma unroll
         for (int c = 0; c < 3; c ++)
-          Wb[(9*(size_t)k + r + 3*c)*SLICE] = f * P[c + 3*r];
+          Wb[(9*(size_t)m + rr + 3*c)*SLICE] = f * P[c + 3*rr];
     }
-    cb[(size_t)k*SLICE] = iperm[j];
+    cb[(size_t)m*SLICE] = iperm[j];
   }
 }
 
@@ -514,24 +518,40 @@ __global__ __launch_bounds__(TPB) void k_rows_sys(SysIn si, RowsOut o, const int
   o.rowlen[p] = si.rowptr[i + 1] - si.rowptr[i];
 }
 
+template <int G>
 __global__ __launch_bounds__(TPB) void k_offdiag_sys(SysIn si, SellOut so, const int *perm, const int *iperm, int nslices)
 {
-  int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (s >= nslices || lane >= so.nrows[s]) return;
-  int p = so.row0[s] + lane, i = perm[p];
+  int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6), l = threadIdx.x & 63, r = l / G, g = l % G;
+  if (s >= nslices || r >= so.nrows[s]) return;
+  int p = so.row0[s] + r, i = perm[p];
   long long off = so.off[s];
-  double *Wb = so.W + 9*off + lane;
-  int *cb = so.wcol + off + lane;
-  for (int q = si.rowptr[i], k = 0; q < si.rowptr[i + 1]; q ++, k ++)
+  double *Wb = so.W + 9*off + l;
+  int *cb = so.wcol + off + l;
+  for (int k = g, m = 0; k < si.rowptr[i + 1] - si.rowptr[i]; k += G, m ++)
   {
+    int q = si.rowptr[i] + k;
 #pragma unroll
-    for (int e = 0; e < 9; e ++) Wb[(9*(size_t)k + e)*SLICE] = si.W[9*(size_t)q + e];
-    cb[(size_t)k*SLICE] = iperm[si.col[q]];
+    for (int e = 0; e < 9; e ++) Wb[(9*(size_t)m + e)*SLICE] = si.W[9*(size_t)q + e];
+    cb[(size_t)m*SLICE] = iperm[si.col[q]];
   }
 }
 
 // =====================================================================
-// Gauss-Seidel sweep of one colour (bgs.c:950-1019), FP64
+// Gauss-Seidel sweep of one colour (bgs.c:950-1019) with the merit of the
+// previous iteration fused in (mrf.c:35-184)
+//
+// R lives in two buffers.  Sweep k reads P (R at the end of sweep k-1, all
+// rows) and writes Q.  Row i gathers Q_j for neighbours already updated in
+// this sweep (permuted index in [upd_lo, upd_hi)) and P_j otherwise, which is
+// the serial colour-major Gauss-Seidel order.  When the merit is on, sweep
+// k >= 1 also forms U_i = B_i + W_ii P_i + sum_j W_ij P_j, i.e. the merit
+// update of U after iteration k-1 (mrf.c:61-76), accumulates C'AC for it, keeps
+// that U in Um and starts the local solve from it, exactly as the reference's
+// next sweep does after MERIT_Function.  The termination test of iteration k-1
+// therefore completes after sweep k; when it stops, P and Um are the state.
+// G lanes cooperate on one row (blocks k = g, g+G, ...), reduce B' across the
+// group with cross-lane adds, and run the local solve redundantly; lane g = 0
+// writes.
 // =====================================================================
 
 struct SweepArgs {
@@ -539,56 +559,151 @@ struct SweepArgs {
   const long long *off;
   const double *W;
   const int *wcol;
-  const double *Wd, *B, *rho, *fri, *coh, *shift;
+  const double *Wd, *A, *B, *rho, *fri, *coh, *shift;
   const unsigned char *sep;
-  double *R, *U;
+  const double *P;
+  double *Q, *U, *Um;
   int n;
   double diageps;
   int diagmaxiter, diagsolver, failure;
-  double *partials;
+  double *partials;     // 3 per block: errup, errlo, merit numerator
   unsigned long long *failkey;
   int *failcount;
   Ctrl *ctrl;
 };
 
-__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs a, int s0, int s1, int pbase, int backward)
+template <int G>
+__device__ __forceinline__ double gsum(double v)
 {
-  int s = s0 + blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  double eu = 0.0, el = 0.0;
-  const bool live = !a.ctrl->done;
-  if (live && s < s1 && lane < a.nrows[s])
+  if (G >= 2) v += __shfl_xor(v, 1, 64);
+  if (G >= 4) v += __shfl_xor(v, 2, 64);
+  if (G >= 8) v += __shfl_xor(v, 4, 64);
+  if (G >= 16) v += __shfl_xor(v, 8, 64);
+  return v;
+}
+
+// deterministic block sum of three values; thread 0 returns them
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c)
+{
+  __shared__ double sa[TPB/64], sb[TPB/64], sc[TPB/64];
+  a = wave_sum(a); b = wave_sum(b); c = wave_sum(c);
+  int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sa[w] = a; sb[w] = b; sc[w] = c; }
+  __syncthreads();
+  if (threadIdx.x == 0)
   {
-    const int n = a.n, p = a.row0[s] + lane, len = a.rowlen[p];
-    const long long off = a.off[s];
-    const double *Wb = a.W + 9*off + lane;
-    const int *cb = a.wcol + off + lane;
+    a = sa[0]; b = sb[0]; c = sc[0];
+    for (int k = 1; k < TPB/64; k ++) { a += sa[k]; b += sb[k]; c += sc[k]; }
+  }
+}
 
-    // B' = B_i + sum_j W_ij R_j  (bgs.c:957-963)
-    double Bp[3] = {a.B[p], a.B[(size_t)n + p], a.B[2*(size_t)n + p]};
-    for (int k = 0; k < len; k ++)
-    {
-      const int j = cb[(size_t)k*SLICE];
-      const double *w = Wb + 9*(size_t)k*SLICE;
-      const double r0 = a.R[4*(size_t)j], r1 = a.R[4*(size_t)j + 1], r2 = a.R[4*(size_t)j + 2];
-      Bp[0] = Bp[0] + (w[0]*r0 + w[3*SLICE]*r1 + w[6*SLICE]*r2);
-      Bp[1] = Bp[1] + (w[1*SLICE]*r0 + w[4*SLICE]*r1 + w[7*SLICE]*r2);
-      Bp[2] = Bp[2] + (w[2*SLICE]*r0 + w[5*SLICE]*r1 + w[8*SLICE]*r2);
-    }
-
-    double Wi[9], R[3], U[3], R0[3];
+template <int G, bool MERIT>
+__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs a, int s0, int s1, int pbase, int upd_lo, int upd_hi,
+                                                int backward)
+{
+  if (a.ctrl->done) return;   // uniform: set only by k_finish between launches
+  const int l = threadIdx.x & 63, r = l / G, g = l % G;
+  const int s = s0 + blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6);
+  const int n = a.n;
+  bool act = false;
+  int p = 0, len = 0;
+  long long off = 0;
+  if (s < s1)
+  {
+    act = r < a.nrows[s];
+    off = a.off[s];
+    if (act) { p = a.row0[s] + r; len = a.rowlen[p]; }
+  }
+  // the row's own data first, so that its loads overlap the gather below
+  double Bi[3] = {0.0, 0.0, 0.0}, Wi[9], R[3] = {0.0, 0.0, 0.0}, U[3] = {0.0, 0.0, 0.0}, Ai[9];
+  RowLaw<double> L{0.0, 0.0, 0.0, 0.0, false};
+  if (act)
+  {
+    Bi[0] = a.B[p]; Bi[1] = a.B[(size_t)n + p]; Bi[2] = a.B[2*(size_t)n + p];
 #pragma unroll
     for (int e = 0; e < 9; e ++) Wi[e] = a.Wd[(size_t)e*n + p];
 #pragma unroll
-    for (int k = 0; k < 3; k ++) { R[k] = a.R[4*(size_t)p + k]; U[k] = a.U[(size_t)k*n + p]; R0[k] = R[k]; }
-    RowLaw<double> L{a.fri[p], a.coh[p], a.shift[p], a.rho[p], a.sep[p] != 0};
+    for (int k = 0; k < 3; k ++) R[k] = a.P[4*(size_t)p + k];
+    if (MERIT)
+    {
+#pragma unroll
+      for (int e = 0; e < 9; e ++) Ai[e] = a.A[(size_t)e*n + p];
+    }
+    else
+    {
+#pragma unroll
+      for (int k = 0; k < 3; k ++) U[k] = a.U[(size_t)k*n + p];
+    }
+    L = RowLaw<double>{a.fri[p], a.coh[p], a.shift[p], a.rho[p], a.sep[p] != 0};
+  }
+  double b0 = 0.0, b1 = 0.0, b2 = 0.0, u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  if (act)
+  {
+    const double *Wb = a.W + 9*off + l;
+    const int *cb = a.wcol + off + l;
+#pragma unroll 4
+    for (int k = g, m = 0; k < len; k += G, m ++)
+    {
+      const int j = cb[(size_t)m*SLICE];
+      const double *w = Wb + 9*(size_t)m*SLICE;
+      const double w0 = w[0], w1 = w[SLICE], w2 = w[2*SLICE], w3 = w[3*SLICE], w4 = w[4*SLICE],
+                   w5 = w[5*SLICE], w6 = w[6*SLICE], w7 = w[7*SLICE], w8 = w[8*SLICE];
+      const bool upd = j >= upd_lo && j < upd_hi;
+      const double *xj = (upd ? a.Q : a.P) + 4*(size_t)j;
+      const double r0 = xj[0], r1 = xj[1], r2 = xj[2];
+      b0 += w0*r0 + w3*r1 + w6*r2;
+      b1 += w1*r0 + w4*r1 + w7*r2;
+      b2 += w2*r0 + w5*r1 + w8*r2;
+      if (MERIT)
+      {
+        double p0 = r0, p1 = r1, p2 = r2;
+        if (upd)
+        {
+          const double *pj = a.P + 4*(size_t)j;
+          p0 = pj[0]; p1 = pj[1]; p2 = pj[2];
+        }
+        u0 += w0*p0 + w3*p1 + w6*p2;
+        u1 += w1*p0 + w4*p1 + w7*p2;
+        u2 += w2*p0 + w5*p1 + w8*p2;
+      }
+    }
+  }
+  // sum over the G lanes of each row; every lane of the wave takes part
+  b0 = gsum<G>(b0); b1 = gsum<G>(b1); b2 = gsum<G>(b2);
+  if (MERIT) { u0 = gsum<G>(u0); u1 = gsum<G>(u1); u2 = gsum<G>(u2); }
+
+  double eu = 0.0, el = 0.0, up = 0.0;
+  if (act)
+  {
+    double Bp[3] = {Bi[0] + b0, Bi[1] + b1, Bi[2] + b2};
+    const double R0[3] = {R[0], R[1], R[2]};
+    if (MERIT)
+    {
+      // merit of the previous iteration at this row (mrf.c:63-84)
+      mv_add(Bi, Wi, R, U);
+      U[0] += u0; U[1] += u1; U[2] += u2;
+      double C[3];
+      scf_residual<double>(L.fri, L.coh, L.shift, U, R, C, nullptr, nullptr);
+      const double q0 = Ai[0]*C[0] + Ai[3]*C[1] + Ai[6]*C[2];
+      const double q1 = Ai[1]*C[0] + Ai[4]*C[1] + Ai[7]*C[2];
+      const double q2 = Ai[2]*C[0] + Ai[5]*C[1] + Ai[8]*C[2];
+      up = q0*C[0] + q1*C[1] + q2*C[2];
+      if (g == 0)
+#pragma unroll
+        for (int k = 0; k < 3; k ++) a.Um[(size_t)k*n + p] = U[k];
+    }
 
     int d = local_solve(a.diagsolver, L, a.diageps, a.diagmaxiter, Wi, Bp, U, R);
     if (d >= a.diagmaxiter || d < 0)
     {
-      unsigned code = d < 0 ? SFGS_GS_DIAGONAL_FAILED : SFGS_GS_DIAGONAL_DIVERGED;
-      unsigned long long pos = backward ? (unsigned long long)(n - 1 - p) : (unsigned long long)p;
-      atomicMax(a.failkey, ((unsigned long long)(a.ctrl->iters + 1) << 40) | (pos << 2) | code);
-      atomicAdd(a.failcount, 1);
+      if (g == 0)
+      {
+        unsigned code = d < 0 ? SFGS_GS_DIAGONAL_FAILED : SFGS_GS_DIAGONAL_DIVERGED;
+        unsigned long long pos = backward ? (unsigned long long)(n - 1 - p) : (unsigned long long)p;
+        atomicMax(a.failkey, ((unsigned long long)(a.ctrl->sweeps + 1) << 40) | (pos << 2) | code);
+        atomicAdd(a.failcount, 1);
+        if (a.failure == SFGS_FAILURE_EXIT) atomicExch(&a.ctrl->aborted, 1);
+      }
       if (a.failure == SFGS_FAILURE_CONTINUE)
       {
         const int chain[4] = {SFGS_DS_SEMISMOOTH_NEWTON, SFGS_DS_PROJECTED_GRADIENT,
@@ -602,30 +717,28 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs a, int s0, int s1, int 
         }
         if (d >= a.diagmaxiter || d < 0) { R[0] = R0[0]; R[1] = R0[1]; R[2] = R0[2]; }
       }
-      else if (a.failure == SFGS_FAILURE_EXIT)
-      {
-        atomicExch(&a.ctrl->aborted, 1);
-      }
     }
+    if (g == 0)
+    {
 #pragma unroll
-    for (int k = 0; k < 3; k ++) { a.R[4*(size_t)p + k] = R[k]; a.U[(size_t)k*n + p] = U[k]; }
-    const double d0 = R[0] - R0[0], d1 = R[1] - R0[1], d2 = R[2] - R0[2];
-    eu = d0*d0 + d1*d1 + d2*d2;
-    el = R[0]*R[0] + R[1]*R[1] + R[2]*R[2];
+      for (int k = 0; k < 3; k ++) { a.Q[4*(size_t)p + k] = R[k]; a.U[(size_t)k*n + p] = U[k]; }
+      a.Q[4*(size_t)p + 3] = 0.0;
+      const double d0 = R[0] - R0[0], d1 = R[1] - R0[1], d2 = R[2] - R0[2];
+      eu = d0*d0 + d1*d1 + d2*d2;
+      el = R[0]*R[0] + R[1]*R[1] + R[2]*R[2];
+    }
+    else up = 0.0;
   }
-  if (!live) return;   // uniform over the grid
-  block_sum2(eu, el);
+  block_sum3(eu, el, up);
   if (threadIdx.x == 0)
   {
-    a.partials[2*(size_t)(pbase + blockIdx.x)] = eu;
-    a.partials[2*(size_t)(pbase + blockIdx.x) + 1] = el;
+    double *pt = a.partials + 3*(size_t)(pbase + blockIdx.x);
+    pt[0] = eu; pt[1] = el; pt[2] = up;
   }
 }
 
-// =====================================================================
-// merit (mrf.c:35-184): U = B + W_ii R + sum W_ij R_j, up = C'AC
-// =====================================================================
-
+// standalone merit (MERIT_Function with update_U, mrf.c:35-184) on the
+// canonical R buffer; one launch over all slices
 struct MeritArgs {
   const int *row0, *nrows, *rowlen;
   const long long *off;
@@ -636,41 +749,53 @@ struct MeritArgs {
   double *U;
   int n, update_U;
   double *mpartials;
-  double *conmerit;   // may be null: per-row up (colour-major)
-  Ctrl *ctrl;
+  double *conmerit;   // may be null: per-row C'AC (colour-major)
 };
 
-__global__ __launch_bounds__(TPB) void k_merit(MeritArgs a, int nslices, int check_done)
+template <int G>
+__global__ __launch_bounds__(TPB) void k_merit(MeritArgs a, int nslices)
 {
-  int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  double up = 0.0, dummy = 0.0;
-  const bool live = !(check_done && a.ctrl->done);
-  if (live && s < nslices && lane < a.nrows[s])
+  const int l = threadIdx.x & 63, r = l / G, g = l % G;
+  const int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6), n = a.n;
+  bool act = false;
+  int p = 0, len = 0;
+  long long off = 0;
+  if (s < nslices)
   {
-    const int n = a.n, p = a.row0[s] + lane;
+    act = r < a.nrows[s];
+    off = a.off[s];
+    if (act) { p = a.row0[s] + r; len = a.update_U ? a.rowlen[p] : 0; }
+  }
+  double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  if (act)
+  {
+    const double *Wb = a.W + 9*off + l;
+    const int *cb = a.wcol + off + l;
+    for (int k = g, m = 0; k < len; k += G, m ++)
+    {
+      const int j = cb[(size_t)m*SLICE];
+      const double *w = Wb + 9*(size_t)m*SLICE;
+      const double r0 = a.R[4*(size_t)j], r1 = a.R[4*(size_t)j + 1], r2 = a.R[4*(size_t)j + 2];
+      u0 += w[0]*r0 + w[3*SLICE]*r1 + w[6*SLICE]*r2;
+      u1 += w[SLICE]*r0 + w[4*SLICE]*r1 + w[7*SLICE]*r2;
+      u2 += w[2*SLICE]*r0 + w[5*SLICE]*r1 + w[8*SLICE]*r2;
+    }
+  }
+  u0 = gsum<G>(u0); u1 = gsum<G>(u1); u2 = gsum<G>(u2);
+  double up = 0.0, z1 = 0.0, z2 = 0.0;
+  if (act)
+  {
     double R[3] = {a.R[4*(size_t)p], a.R[4*(size_t)p + 1], a.R[4*(size_t)p + 2]}, U[3];
     if (a.update_U)
     {
-      const int len = a.rowlen[p];
-      const long long off = a.off[s];
-      const double *Wb = a.W + 9*off + lane;
-      const int *cb = a.wcol + off + lane;
-      double Wi[9];
+      double Wi[9], Bv[3] = {a.B[p], a.B[(size_t)n + p], a.B[2*(size_t)n + p]};
 #pragma unroll
       for (int e = 0; e < 9; e ++) Wi[e] = a.Wd[(size_t)e*n + p];
-      double Bv[3] = {a.B[p], a.B[(size_t)n + p], a.B[2*(size_t)n + p]};
       mv_add(Bv, Wi, R, U);
-      for (int k = 0; k < len; k ++)
-      {
-        const int j = cb[(size_t)k*SLICE];
-        const double *w = Wb + 9*(size_t)k*SLICE;
-        const double r0 = a.R[4*(size_t)j], r1 = a.R[4*(size_t)j + 1], r2 = a.R[4*(size_t)j + 2];
-        U[0] = U[0] + (w[0]*r0 + w[3*SLICE]*r1 + w[6*SLICE]*r2);
-        U[1] = U[1] + (w[1*SLICE]*r0 + w[4*SLICE]*r1 + w[7*SLICE]*r2);
-        U[2] = U[2] + (w[2*SLICE]*r0 + w[5*SLICE]*r1 + w[8*SLICE]*r2);
-      }
+      U[0] += u0; U[1] += u1; U[2] += u2;
+      if (g == 0)
 #pragma unroll
-      for (int k = 0; k < 3; k ++) a.U[(size_t)k*n + p] = U[k];
+        for (int k = 0; k < 3; k ++) a.U[(size_t)k*n + p] = U[k];
     }
     else
     {
@@ -684,44 +809,68 @@ __global__ __launch_bounds__(TPB) void k_merit(MeritArgs a, int nslices, int che
     const double q0 = Ai[0]*C[0] + Ai[3]*C[1] + Ai[6]*C[2];
     const double q1 = Ai[1]*C[0] + Ai[4]*C[1] + Ai[7]*C[2];
     const double q2 = Ai[2]*C[0] + Ai[5]*C[1] + Ai[8]*C[2];
-    up = q0*C[0] + q1*C[1] + q2*C[2];
-    if (a.conmerit) a.conmerit[p] = up;
+    if (g == 0)
+    {
+      up = q0*C[0] + q1*C[1] + q2*C[2];
+      if (a.conmerit) a.conmerit[p] = up;
+    }
   }
-  if (!live) return;
-  block_sum2(up, dummy);
+  block_sum3(up, z1, z2);
   if (threadIdx.x == 0) a.mpartials[blockIdx.x] = up;
 }
 
-// =====================================================================
-// end of one iteration: errors, merit, histories, termination (bgs.c:1021-1036)
-// =====================================================================
-
-__global__ __launch_bounds__(TPB) void k_finish(Ctrl *ctrl, const double *partials, int np,
-                                                 const double *mpartials, int nm, double *hist)
+// end of a sweep (bgs.c:1021-1036): errors, merit, histories, termination.
+// Without merit, sweep k closes iteration k.  With merit, sweep k >= 1 closes
+// iteration k-1 (its merit was formed during sweep k); sweep 0 only records its
+// error.  ctrl->final_buf / final_U name the buffers holding the final state.
+__global__ __launch_bounds__(TPB) void k_finish(Ctrl *ctrl, const double *partials, int np, double *hist)
 {
   if (ctrl->done) return;
-  double eu = 0.0, el = 0.0, up = 0.0, dummy = 0.0;
-  for (int k = threadIdx.x; k < np; k += TPB) { eu += partials[2*(size_t)k]; el += partials[2*(size_t)k + 1]; }
-  block_sum2(eu, el);
-  __syncthreads();
-  if (!ctrl->nomerit) for (int k = threadIdx.x; k < nm; k += TPB) up += mpartials[k];
-  block_sum2(up, dummy);
-  if (threadIdx.x != 0) return;
-  int it = ctrl->iters;
-  double error = sqrt(eu) / sqrt(el == 0.0 ? 1.0 : el);
-  double merit = 0.0;
-  if (!ctrl->nomerit)
+  double eu = 0.0, el = 0.0, up = 0.0;
+  for (int k = threadIdx.x; k < np; k += TPB)
   {
-    double lo = ctrl->energy < DBL_EPSILON ? 1.0 : ctrl->energy;
-    merit = (0.5*up) / lo;
+    eu += partials[3*(size_t)k]; el += partials[3*(size_t)k + 1]; up += partials[3*(size_t)k + 2];
   }
-  hist[it] = error;
-  hist[ctrl->maxiter + it] = merit;
+  block_sum3(eu, el, up);
+  if (threadIdx.x != 0) return;
+  const int k = ctrl->sweeps;
+  const double error_k = sqrt(eu) / sqrt(el == 0.0 ? 1.0 : el);
+  ctrl->sweeps = k + 1;
+  if (ctrl->nomerit)
+  {
+    hist[k] = error_k;
+    hist[ctrl->maxiter + k] = 0.0;
+    ctrl->iters = k + 1;
+    ctrl->error = error_k;
+    ctrl->merit = 0.0;
+    ctrl->final_buf = (k + 1) & 1;
+    ctrl->final_U = 0;
+    if (ctrl->aborted || !(k + 1 < ctrl->maxiter && error_k > ctrl->eps)) ctrl->done = 1;
+    return;
+  }
+  if (ctrl->aborted)
+  {
+    ctrl->iters = k + 1;
+    ctrl->final_buf = (k + 1) & 1;
+    ctrl->final_U = 0;
+    ctrl->done = 1;
+    return;
+  }
+  if (k == 0) { ctrl->prev_error = error_k; return; }
+  const double lo = ctrl->energy < DBL_EPSILON ? 1.0 : ctrl->energy;
+  const double merit = (0.5*up) / lo, error = ctrl->prev_error;
+  hist[k - 1] = error;
+  hist[ctrl->maxiter + k - 1] = merit;
+  ctrl->iters = k;
   ctrl->error = error;
   ctrl->merit = merit;
-  ctrl->iters = it + 1;
-  if (ctrl->aborted) ctrl->done = 1;
-  else if (!(it + 1 < ctrl->maxiter && (error > ctrl->eps || merit > ctrl->meritval))) ctrl->done = 1;
+  if (!(k < ctrl->maxiter && (error > ctrl->eps || merit > ctrl->meritval)))
+  {
+    ctrl->final_buf = k & 1;   // P of sweep k = R after sweep k-1
+    ctrl->final_U = 1;         // U after the merit of iteration k-1
+    ctrl->done = 1;
+  }
+  else ctrl->prev_error = error_k;
 }
 
 // =====================================================================
@@ -843,6 +992,8 @@ static int colour_graph(sfgs_ctx *ctx, int n)
 // permutation into colour-major order and the SELL-64 slice table
 static int build_order(sfgs_ctx *ctx, int n)
 {
+  const int G = ctx->G, RPS = SLICE / G;
+  ctx->G_built = G;
   CHK(ensure(ctx, ctx->skeys, n));
   CHK(ensure(ctx, ctx->skeys2, n));
   CHK(ensure(ctx, ctx->svals, n));
@@ -872,10 +1023,10 @@ static int build_order(sfgs_ctx *ctx, int n)
   for (int c = 0; c < nc; c ++)
   {
     ctx->h_colptr[c + 1] = ctx->h_colptr[c] + cnt[c];
-    for (int r = ctx->h_colptr[c]; r < ctx->h_colptr[c + 1]; r += SLICE)
+    for (int r = ctx->h_colptr[c]; r < ctx->h_colptr[c + 1]; r += RPS)
     {
       row0.push_back(r);
-      nrows.push_back(std::min(SLICE, ctx->h_colptr[c + 1] - r));
+      nrows.push_back(std::min(RPS, ctx->h_colptr[c + 1] - r));
     }
     ctx->h_cslice[c + 1] = (int)row0.size();
   }
@@ -891,7 +1042,7 @@ static int build_order(sfgs_ctx *ctx, int n)
   HIPCHK(hipMemsetAsync(lens.p, 0, sizeof(long long)*((size_t)ns + 1), ctx->stream));
   if (ns > 0)
     hipLaunchKernelGGL(k_slice_len, dim3(nblk(ns)), dim3(TPB), 0, ctx->stream, ns, ctx->slice_row0.p, ctx->perm.p,
-                       ctx->rowptr.p, ctx->slice_len.p, lens.p);
+                       ctx->rowptr.p, G, ctx->slice_len.p, lens.p);
   CHK(scan_ll(ctx, lens.p, ctx->slice_off.p, ns + 1));
   long long total = 0;
   HIPCHK(hipMemcpyAsync(&total, ctx->slice_off.p + ns, sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
@@ -921,9 +1072,19 @@ static int ensure_rows(sfgs_ctx *ctx, int n)
   CHK(ensure(ctx, ctx->shift, n));
   CHK(ensure(ctx, ctx->sep, n));
   CHK(ensure(ctx, ctx->R, 4*(size_t)n));
+  CHK(ensure(ctx, ctx->Rb, 4*(size_t)n));
   CHK(ensure(ctx, ctx->U, 3*(size_t)n));
+  CHK(ensure(ctx, ctx->Um, 3*(size_t)n));
   return 0;
 }
+
+#define COMMA ,
+#define SFGS_DISPATCH_G(G, CALL) \
+  switch (G) { \
+  case 1: { constexpr int GG = 1; CALL; } break; \
+  case 2: { constexpr int GG = 2; CALL; } break; \
+  case 8: { constexpr int GG = 8; CALL; } break; \
+  default: { constexpr int GG = 4; CALL; } break; }
 
 static RowsOut rows_out(sfgs_ctx *ctx, int n, double *epart, int *singular)
 {
@@ -989,7 +1150,7 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   release(ctx->slice_row0); release(ctx->slice_nrows); release(ctx->slice_len); release(ctx->slice_off);
   release(ctx->rowlen); release(ctx->W); release(ctx->Wd); release(ctx->A); release(ctx->B); release(ctx->V);
   release(ctx->rho); release(ctx->pfri); release(ctx->pcoh); release(ctx->shift); release(ctx->R); release(ctx->U);
-  release(ctx->Rsave); release(ctx->Usave); release(ctx->sep); release(ctx->wcol);
+  release(ctx->Rb); release(ctx->Um); release(ctx->sep); release(ctx->wcol);
   release(ctx->W32); release(ctx->Wd32); release(ctx->B32); release(ctx->R32); release(ctx->U32);
   release(ctx->rho32); release(ctx->pfri32); release(ctx->pcoh32); release(ctx->shift32);
   release(ctx->partials); release(ctx->mpartials); release(ctx->hist); release(ctx->failkey);
@@ -998,6 +1159,8 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
   if (ctx->ev3) (void)hipEventDestroy(ctx->ev3);
+  for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
+  release(ctx->body_out);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1120,8 +1283,9 @@ int SFGS_Locdyn_Update_Begin(sfgs_ctx *ctx, double step, int dynamic)
     hipLaunchKernelGGL(k_rows, dim3(nbk), dim3(TPB), 0, st, bv, cv, rows_out(ctx, nc, ctx->partials.p, ctx->failcount.p),
                        ctx->perm.p, ctx->rowptr.p, step, dynamic);
     if (ctx->nslices > 0)
-      hipLaunchKernelGGL(k_offdiag, dim3(nblk((long long)ctx->nslices*SLICE)), dim3(TPB), 0, st, bv, cv, sell_out(ctx),
-                         ctx->perm.p, ctx->iperm.p, ctx->rowptr.p, ctx->col.p, ctx->via.p, ctx->nslices, step);
+      SFGS_DISPATCH_G(ctx->G_built, auto kern = k_offdiag<GG>; hipLaunchKernelGGL(kern, dim3(nblk((long long)ctx->nslices*SLICE)), dim3(TPB),
+                      0, st, bv, cv, sell_out(ctx), ctx->perm.p, ctx->iperm.p, ctx->rowptr.p, ctx->col.p, ctx->via.p,
+                      ctx->nslices, step));
   }
   std::vector<double> ep((size_t)std::max(nbk, 1), 0.0);
   int singular = 0;
@@ -1181,8 +1345,8 @@ int SFGS_System_Upload(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, 
     hipLaunchKernelGGL(k_rows_sys, dim3(nblk(n)), dim3(TPB), 0, st, si, rows_out(ctx, n, nullptr, nullptr),
                        ctx->perm.p, step, dynamic);
     if (ctx->nslices > 0)
-      hipLaunchKernelGGL(k_offdiag_sys, dim3(nblk((long long)ctx->nslices*SLICE)), dim3(TPB), 0, st, si, sell_out(ctx),
-                         ctx->perm.p, ctx->iperm.p, ctx->nslices);
+      SFGS_DISPATCH_G(ctx->G_built, auto kern = k_offdiag_sys<GG>; hipLaunchKernelGGL(kern, dim3(nblk((long long)ctx->nslices*SLICE)),
+                      dim3(TPB), 0, st, si, sell_out(ctx), ctx->perm.p, ctx->iperm.p, ctx->nslices));
   }
   hipError_t e = hipStreamSynchronize(st);
   release(dW); release(dWd); release(dA); release(dB); release(dV); release(drho); release(dfri);
@@ -1201,8 +1365,9 @@ static SweepArgs sweep_args(sfgs_ctx *ctx, const SFGS_PARAM *par)
 {
   SweepArgs a;
   a.row0 = ctx->slice_row0.p; a.nrows = ctx->slice_nrows.p; a.rowlen = ctx->rowlen.p; a.off = ctx->slice_off.p;
-  a.W = ctx->W.p; a.wcol = ctx->wcol.p; a.Wd = ctx->Wd.p; a.B = ctx->B.p; a.rho = ctx->rho.p; a.fri = ctx->pfri.p;
-  a.coh = ctx->pcoh.p; a.shift = ctx->shift.p; a.sep = ctx->sep.p; a.R = ctx->R.p; a.U = ctx->U.p; a.n = ctx->nc;
+  a.W = ctx->W.p; a.wcol = ctx->wcol.p; a.Wd = ctx->Wd.p; a.A = ctx->A.p; a.B = ctx->B.p; a.rho = ctx->rho.p;
+  a.fri = ctx->pfri.p; a.coh = ctx->pcoh.p; a.shift = ctx->shift.p; a.sep = ctx->sep.p;
+  a.P = ctx->R.p; a.Q = ctx->Rb.p; a.U = ctx->U.p; a.Um = ctx->Um.p; a.n = ctx->nc;
   a.diageps = par->diagepsilon; a.diagmaxiter = par->diagmaxiter; a.diagsolver = par->diagsolver;
   a.failure = par->failure; a.partials = ctx->partials.p; a.failkey = ctx->failkey.p; a.failcount = ctx->failcount.p;
   a.ctrl = ctx->ctrl.p;
@@ -1215,8 +1380,36 @@ static MeritArgs merit_args(sfgs_ctx *ctx, int update_U, double *conmerit)
   m.row0 = ctx->slice_row0.p; m.nrows = ctx->slice_nrows.p; m.rowlen = ctx->rowlen.p; m.off = ctx->slice_off.p;
   m.W = ctx->W.p; m.wcol = ctx->wcol.p; m.Wd = ctx->Wd.p; m.A = ctx->A.p; m.B = ctx->B.p; m.fri = ctx->pfri.p;
   m.coh = ctx->pcoh.p; m.shift = ctx->shift.p; m.R = ctx->R.p; m.U = ctx->U.p; m.n = ctx->nc; m.update_U = update_U;
-  m.mpartials = ctx->mpartials.p; m.conmerit = conmerit; m.ctrl = ctx->ctrl.p;
+  m.mpartials = ctx->mpartials.p; m.conmerit = conmerit;
   return m;
+}
+
+// one sweep over all colours (forward or backward) with or without the fused merit
+static void launch_sweep(sfgs_ctx *ctx, const SweepArgs &base, int k, int backward, bool merit,
+                         const std::vector<int> &grid, const std::vector<int> &pbase, bool prof)
+{
+  const int ncol = ctx->ncolours, n = ctx->nc;
+  SweepArgs a = base;
+  if (k & 1) { a.P = ctx->Rb.p; a.Q = ctx->R.p; }
+  else { a.P = ctx->R.p; a.Q = ctx->Rb.p; }
+  for (int ci = 0; ci < ncol; ci ++)
+  {
+    int c = backward ? ncol - 1 - ci : ci;
+    if (grid[c] == 0) continue;
+    int lo = backward ? ctx->h_colptr[c + 1] : 0, hi = backward ? n : ctx->h_colptr[c];
+    if (prof) (void)hipEventRecord(ctx->prof_ev[2*ci], ctx->stream);
+    if (merit)
+    {
+      SFGS_DISPATCH_G(ctx->G_built, auto kern = k_sweep<GG COMMA true>; hipLaunchKernelGGL(kern, dim3(grid[c]), dim3(TPB), 0,
+                      ctx->stream, a, ctx->h_cslice[c], ctx->h_cslice[c + 1], pbase[c], lo, hi, backward));
+    }
+    else
+    {
+      SFGS_DISPATCH_G(ctx->G_built, auto kern = k_sweep<GG COMMA false>; hipLaunchKernelGGL(kern, dim3(grid[c]), dim3(TPB), 0,
+                      ctx->stream, a, ctx->h_cslice[c], ctx->h_cslice[c + 1], pbase[c], lo, hi, backward));
+    }
+    if (prof) (void)hipEventRecord(ctx->prof_ev[2*ci + 1], ctx->stream);
+  }
 }
 
 int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *merhist, SFGS_REPORT *rep)
@@ -1226,7 +1419,7 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   if (!ctx->assembled) { ctx->err = "solve before LOCDYN assembly"; return SFGS_E_STATE; }
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
-  const int n = ctx->nc, ncol = ctx->ncolours, maxiter = par->maxiter;
+  const int ncol = ctx->ncolours, maxiter = par->maxiter;
   const int nomerit = par->meritval >= 1.0 ? 1 : 0;
 
   // partial-sum slots: per colour, one per block
@@ -1238,15 +1431,12 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
     pbase[c + 1] = pbase[c] + grid[c];
   }
   const int np = pbase[ncol];
-  const int mgrid = (ctx->nslices + TPB/SLICE - 1) / (TPB/SLICE);
-  CHK(ensure(ctx, ctx->partials, 2*(size_t)std::max(np, 1)));
-  CHK(ensure(ctx, ctx->mpartials, (size_t)std::max(mgrid, 1)));
+  CHK(ensure(ctx, ctx->partials, 3*(size_t)std::max(np, 1)));
   CHK(ensure(ctx, ctx->hist, 2*(size_t)maxiter));
   CHK(ensure(ctx, ctx->failkey, 1));
   CHK(ensure(ctx, ctx->failcount, 4));
   CHK(ensure(ctx, ctx->ctrl, 1));
-  HIPCHK(hipMemsetAsync(ctx->partials.p, 0, sizeof(double)*2*(size_t)std::max(np, 1), st));
-  HIPCHK(hipMemsetAsync(ctx->mpartials.p, 0, sizeof(double)*(size_t)std::max(mgrid, 1), st));
+  HIPCHK(hipMemsetAsync(ctx->partials.p, 0, sizeof(double)*3*(size_t)std::max(np, 1), st));
   HIPCHK(hipMemsetAsync(ctx->failkey.p, 0, sizeof(unsigned long long), st));
   HIPCHK(hipMemsetAsync(ctx->failcount.p, 0, sizeof(int)*4, st));
   Ctrl hc;
@@ -1255,34 +1445,62 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   hc.energy = ctx->energy;
   HIPCHK(hipMemcpyAsync(ctx->ctrl.p, &hc, sizeof(Ctrl), hipMemcpyHostToDevice, st));
 
-  SweepArgs sa = sweep_args(ctx, par);
-  MeritArgs ma = merit_args(ctx, 1, nullptr);
-  HIPCHK(hipEventRecord(ctx->ev0, st));
-  int batch = 2, it = 0;
-  while (it < maxiter)
+  const int prof = par->profile != 0;
+  if (prof && (int)ctx->prof_ev.size() < 2*ncol)
   {
-    int nbt = std::min(batch, maxiter - it);
+    for (int k = (int)ctx->prof_ev.size(); k < 2*ncol; k ++)
+    {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      ctx->prof_ev.push_back(e);
+    }
+  }
+  SweepArgs sa = sweep_args(ctx, par);
+  double sweep_ms = 0.0;
+  long long launches = 0;
+  // with the merit on, sweep k closes iteration k-1: up to maxiter + 1 sweeps
+  const int maxsweeps = nomerit ? maxiter : maxiter + 1;
+  HIPCHK(hipEventRecord(ctx->ev0, st));
+  int batch = prof ? 1 : 2, k = 0;
+  while (k < maxsweeps)
+  {
+    int nbt = std::min(batch, maxsweeps - k);
     for (int j = 0; j < nbt; j ++)
     {
-      int backward = par->reverse && ((it + j) % 2);
-      for (int ci = 0; ci < ncol; ci ++)
-      {
-        int c = backward ? ncol - 1 - ci : ci;
-        if (grid[c] == 0) continue;
-        hipLaunchKernelGGL(k_sweep, dim3(grid[c]), dim3(TPB), 0, st, sa, ctx->h_cslice[c], ctx->h_cslice[c + 1],
-                           pbase[c], backward);
-      }
-      if (!nomerit && mgrid > 0) hipLaunchKernelGGL(k_merit, dim3(mgrid), dim3(TPB), 0, st, ma, ctx->nslices, 1);
-      hipLaunchKernelGGL(k_finish, dim3(1), dim3(TPB), 0, st, ctx->ctrl.p, ctx->partials.p, np, ctx->mpartials.p,
-                         mgrid, ctx->hist.p);
+      int kk = k + j;
+      int backward = par->reverse && (kk % 2);
+      launch_sweep(ctx, sa, kk, backward, !nomerit && kk > 0, grid, pbase, prof);
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(TPB), 0, st, ctx->ctrl.p, ctx->partials.p, np, ctx->hist.p);
     }
-    it += nbt;
+    k += nbt;
     HIPCHK(hipMemcpyAsync(&hc, ctx->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     HIPCHK(hipGetLastError());
+    if (prof)
+    {
+      for (int ci = 0; ci < ncol; ci ++)
+      {
+        float e = 0.0f;
+        HIPCHK(hipEventElapsedTime(&e, ctx->prof_ev[2*ci], ctx->prof_ev[2*ci + 1]));
+        sweep_ms += e;
+        launches ++;
+      }
+    }
     if (hc.done) break;
-    batch = std::min(batch*2, 32);
+    batch = prof ? 1 : std::min(batch*2, 32);
   }
+  if (!hc.done)
+  {
+    // all sweeps ran without a stop: only possible with no rows; state is the last sweep
+    hc.final_buf = maxsweeps & 1;
+    hc.final_U = 0;
+  }
+  // canonical state: R in buffer 0, U in U
+  const int n = ctx->nc;
+  if (n > 0 && hc.final_buf == 1)
+    HIPCHK(hipMemcpyAsync(ctx->R.p, ctx->Rb.p, sizeof(double)*4*(size_t)n, hipMemcpyDeviceToDevice, st));
+  if (n > 0 && hc.final_U == 1)
+    HIPCHK(hipMemcpyAsync(ctx->U.p, ctx->Um.p, sizeof(double)*3*(size_t)n, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipEventRecord(ctx->ev1, st));
   unsigned long long fk = 0;
   int fc = 0;
@@ -1305,6 +1523,8 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
     rep->fail_rows = fc;
     rep->aborted = hc.aborted;
     rep->solve_ms = ms;
+    rep->sweep_ms = sweep_ms;
+    rep->sweep_launches = launches;
     rep->row_updates = (long long)hc.iters * n;
     rep->nnz = ctx->nnz;
     rep->error = fk ? (int)(fk & 3ull) : SFGS_GS_OK;
@@ -1321,13 +1541,12 @@ int SFGS_Merit(sfgs_ctx *ctx, int update_U, double *merit, double *conmerit)
   hipStream_t st = ctx->stream;
   const int n = ctx->nc, mgrid = (ctx->nslices + TPB/SLICE - 1) / (TPB/SLICE);
   CHK(ensure(ctx, ctx->mpartials, (size_t)std::max(mgrid, 1)));
-  CHK(ensure(ctx, ctx->ctrl, 1));
   DBuf<double> cm;
   if (conmerit) { CHK(ensure(ctx, cm, (size_t)n)); }
   MeritArgs ma = merit_args(ctx, update_U, conmerit ? cm.p : nullptr);
-  if (mgrid > 0) hipLaunchKernelGGL(k_merit, dim3(mgrid), dim3(TPB), 0, st, ma, ctx->nslices, 0);
+  if (mgrid > 0) { SFGS_DISPATCH_G(ctx->G_built, auto kern = k_merit<GG>; hipLaunchKernelGGL(kern, dim3(mgrid), dim3(TPB), 0, st, ma, ctx->nslices)); }
   std::vector<double> mp((size_t)std::max(mgrid, 1), 0.0);
-  std::vector<double> hcm(conmerit ? (size_t)n : 0), hnat(conmerit ? (size_t)n : 0);
+  std::vector<double> hcm(conmerit ? (size_t)n : 0);
   std::vector<int> hperm(conmerit ? (size_t)n : 0);
   if (mgrid > 0) HIPCHK(hipMemcpyAsync(mp.data(), ctx->mpartials.p, sizeof(double)*(size_t)mgrid, hipMemcpyDeviceToHost, st));
   if (conmerit && n > 0)
@@ -1343,6 +1562,13 @@ int SFGS_Merit(sfgs_ctx *ctx, int update_U, double *merit, double *conmerit)
   double lo = ctx->energy < DBL_EPSILON ? 1.0 : ctx->energy;
   if (merit) *merit = 0.5*up / lo;
   if (conmerit) for (int p = 0; p < n; p ++) conmerit[hperm[p]] = hcm[p] / lo;
+  return 0;
+}
+
+int SFGS_Set_Layout(sfgs_ctx *ctx, int lanes_per_row)
+{
+  if (!ctx || (lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4 && lanes_per_row != 8)) return SFGS_E_ARG;
+  ctx->G = lanes_per_row;
   return 0;
 }
 
@@ -1442,8 +1668,11 @@ int SFGS_Get_Locdyn(sfgs_ctx *ctx, int *rowptr, int *col, int *via, double *W, d
       {
         int p = hrow0[s] + l, i = hperm[p];
         for (int q = hrowptr[i], k = 0; q < hrowptr[i + 1]; q ++, k ++)
+        {
+          int G = ctx->G_built, m = k / G, lane = l*G + k % G;
           for (int e = 0; e < 9; e ++)
-            W[9*(size_t)q + e] = hW[9*(size_t)hoff[s] + (9*(size_t)k + e)*SLICE + l];
+            W[9*(size_t)q + e] = hW[9*(size_t)hoff[s] + (9*(size_t)m + e)*SLICE + lane];
+        }
       }
   }
   return 0;
@@ -1467,24 +1696,23 @@ int SFGS_Colour(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, int *co
 
 int SFGS_Body_Update(sfgs_ctx *ctx, double step, double *velo_out, double *force_out)
 {
-  if (!ctx || !velo_out) return SFGS_E_ARG;
+  if (!ctx) return SFGS_E_ARG;
   if (!ctx->assembled || !ctx->from_bodies) { ctx->err = "body update needs a system assembled from bodies"; return SFGS_E_STATE; }
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const int nb = ctx->nb;
   if (nb == 0) return 0;
-  DBuf<double> out;
+  DBuf<double> &out = ctx->body_out;
   CHK(ensure(ctx, out, 12*(size_t)nb));
   BodyView bv{ctx->kind.p, ctx->rot.p, ctx->x0.p, ctx->jinv.p, ctx->minv.p, ctx->velo.p, ctx->velo0.p};
   ConView cv{ctx->master.p, ctx->slave.p, ctx->mpnt.p, ctx->spnt.p, ctx->base.p, ctx->gap.p, ctx->fri.p,
              ctx->rest.p, ctx->coh.p, ctx->Rn.p, ctx->Un.p};
   hipLaunchKernelGGL(k_body_update, dim3(nblk(nb)), dim3(TPB), 0, st, bv, cv, nb, ctx->bptr.p, ctx->blist.p,
                      ctx->iperm.p, ctx->R.p, step, out.p, out.p + 6*(size_t)nb);
-  HIPCHK(hipMemcpyAsync(velo_out, out.p, sizeof(double)*6*(size_t)nb, hipMemcpyDeviceToHost, st));
+  if (velo_out) HIPCHK(hipMemcpyAsync(velo_out, out.p, sizeof(double)*6*(size_t)nb, hipMemcpyDeviceToHost, st));
   if (force_out) HIPCHK(hipMemcpyAsync(force_out, out.p + 6*(size_t)nb, sizeof(double)*6*(size_t)nb, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());
-  release(out);
   return 0;
 }
 

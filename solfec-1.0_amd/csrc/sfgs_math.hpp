@@ -74,6 +74,7 @@ __device__ __forceinline__ int solve3(T *a, T *b)
       for (int i = k + 1; i < 3; i ++) if (i == p) b[i] = bk;
     }
     T inv = T(1) / a[k + 3*k];
+    a[k + 3*k] = inv;   // kept for the back substitution
 #pragma unroll
     for (int i = k + 1; i < 3; i ++)
     {
@@ -86,7 +87,7 @@ __device__ __forceinline__ int solve3(T *a, T *b)
 #pragma unroll
   for (int k = 2; k >= 0; k --)
   {
-    b[k] /= a[k + 3*k];
+    b[k] *= a[k + 3*k];
 #pragma unroll
     for (int i = 0; i < k; i ++) b[i] -= b[k] * a[i + 3*k];
   }
@@ -254,8 +255,10 @@ __device__ __forceinline__ int semismooth_newton(const RowLaw<T> &L, T eps, int 
         if (lim > T(0))
         {
           T len = sqrt(R[0]*R[0] + R[1]*R[1]);
+          const T inorm = T(1) / norm;
           T den = tmax(lim, len) * norm;
-          T e = lim / norm, beta;
+          const T iden = T(1) / den;
+          T e = lim * inorm, beta;
           if (len == T(0)) beta = T(1);
           else
           {
@@ -263,20 +266,21 @@ __device__ __forceinline__ int semismooth_newton(const RowLaw<T> &L, T eps, int 
             T delta = tmin(len/lim, T(1));
             beta = alfa < T(0) ? T(1)/(T(1) - alfa*delta) : T(1);
           }
-          T F0 = (R[0]*d[0])/den, F1 = (R[1]*d[0])/den, F2 = (R[0]*d[1])/den, F3 = (R[1]*d[1])/den;
+          T F0 = (R[0]*d[0])*iden, F1 = (R[1]*d[0])*iden, F2 = (R[0]*d[1])*iden, F3 = (R[1]*d[1])*iden;
           T M0 = e*(T(1) - F0), M1 = -e*F1, M2 = -e*F2, M3 = e*(T(1) - F3);
           T H0 = T(1) - beta*M0, H1 = -beta*M1, H2 = -beta*M2, H3 = T(1) - beta*M3;
+          const T t0 = d[0]*inorm, t1 = d[1]*inorm;
           a[0] = H0 + rho*(M0*W[0] + M2*W[1]);
           a[1] = H1 + rho*(M1*W[0] + M3*W[1]);
           a[2] = W[2];
           a[3] = H2 + rho*(M0*W[3] + M2*W[4]);
           a[4] = H3 + rho*(M1*W[3] + M3*W[4]);
           a[5] = W[5];
-          a[6] = rho*(M0*W[6] + M2*W[7]) - fri*(d[0]/norm);
-          a[7] = rho*(M1*W[6] + M3*W[7]) - fri*(d[1]/norm);
+          a[6] = rho*(M0*W[6] + M2*W[7]) - fri*t0;
+          a[7] = rho*(M1*W[6] + M3*W[7]) - fri*t1;
           a[8] = W[8];
-          b[0] = fri*(d[0]/norm)*(R[2] + coh) - R[0] - rho*(M0*RES[0] + M2*RES[1]);
-          b[1] = fri*(d[1]/norm)*(R[2] + coh) - R[1] - rho*(M1*RES[0] + M3*RES[1]);
+          b[0] = fri*t0*(R[2] + coh) - R[0] - rho*(M0*RES[0] + M2*RES[1]);
+          b[1] = fri*t1*(R[2] + coh) - R[1] - rho*(M1*RES[0] + M3*RES[1]);
           b[2] = -UN - RES[2];
         }
         else
@@ -293,9 +297,10 @@ __device__ __forceinline__ int semismooth_newton(const RowLaw<T> &L, T eps, int 
       {
 #pragma unroll
         for (int k = 0; k < 9; k ++) a[k] = W[k];
-        a[6] += U[0]/d[2];
-        a[7] += U[1]/d[2];
-        T s = T(1) + rho*U[2]/d[2];
+        const T id2 = T(1) / d[2];
+        a[6] += U[0]*id2;
+        a[7] += U[1]*id2;
+        T s = T(1) + rho*U[2]*id2;
         b[0] = -s*U[0] - RES[0];
         b[1] = -s*U[1] - RES[1];
         b[2] = -UN - RES[2];

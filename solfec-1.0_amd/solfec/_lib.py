@@ -23,13 +23,13 @@ _s = C.c_char_p
 class SFGS_PARAM(C.Structure):
     _fields_ = [("epsilon", _d), ("maxiter", _i), ("meritval", _d), ("failure", _i),
                 ("diagepsilon", _d), ("diagmaxiter", _i), ("diagsolver", _i), ("reverse", _i),
-                ("precision", _i), ("refine", _i)]
+                ("precision", _i), ("refine", _i), ("profile", _i)]
 
 
 class SFGS_REPORT(C.Structure):
     _fields_ = [("iters", _i), ("error", _i), ("rel_error", _d), ("merit", _d), ("ncolours", _i),
                 ("fail_rows", _i), ("aborted", _i), ("solve_ms", _d), ("sweep_ms", _d),
-                ("row_updates", _ll), ("nnz", _ll)]
+                ("sweep_launches", _ll), ("row_updates", _ll), ("nnz", _ll)]
 
 
 _SFGS_PROTOS = {
@@ -48,6 +48,7 @@ _SFGS_PROTOS = {
     "SFGS_Get_Sizes": (_i, [_p, _p, _p, _p]),
     "SFGS_Colour": (_i, [_p, _i, _p, _p, _p]),
     "SFGS_Body_Update": (_i, [_p, _d, _p, _p]),
+    "SFGS_Set_Layout": (_i, [_p, _i]),
 }
 
 _SOL_PROTOS = {

@@ -26,7 +26,7 @@ def default_diag(epsilon, maxiter, meritval=1.0):
 
 
 def make_param(epsilon, maxiter, meritval=1.0, failure=0, diagepsilon=None, diagmaxiter=None,
-               diagsolver=2, reverse=0, precision=FP64, refine=2):
+               diagsolver=2, reverse=0, precision=FP64, refine=2, profile=0):
     de, dm = default_diag(epsilon, maxiter, meritval)
     p = SFGS_PARAM()
     p.epsilon = epsilon
@@ -39,6 +39,7 @@ def make_param(epsilon, maxiter, meritval=1.0, failure=0, diagepsilon=None, diag
     p.reverse = reverse
     p.precision = precision
     p.refine = refine
+    p.profile = profile
     return p
 
 
@@ -67,6 +68,9 @@ class Device:
             msg = self.lib.SFGS_Last_Error(self.ctx).decode()
             raise SfgsError(f"{what}: {self.lib.SFGS_Error_String(rc).decode()} ({msg})")
         return rc
+
+    def set_layout(self, lanes_per_row):
+        self._check(self.lib.SFGS_Set_Layout(self.ctx, int(lanes_per_row)), "SFGS_Set_Layout")
 
     # ---- inputs -------------------------------------------------------
     def bodies(self, kind, rot, x0, jinv, minv, velo, velo0):
@@ -145,7 +149,10 @@ class Device:
         nc = self._check(self.lib.SFGS_Colour(self.ctx, n, ptr(rowptr), ptr(col), ptr(out)), "SFGS_Colour")
         return nc, out[:n]
 
-    def body_update(self, step, nb):
+    def body_update(self, step, nb, download=True):
+        if not download:
+            self._check(self.lib.SFGS_Body_Update(self.ctx, step, None, None), "SFGS_Body_Update")
+            return None, None
         velo, force = np.zeros(6 * nb), np.zeros(6 * nb)
         self._check(self.lib.SFGS_Body_Update(self.ctx, step, ptr(velo), ptr(force)), "SFGS_Body_Update")
         return velo.reshape(nb, 6), force.reshape(nb, 6)

@@ -1,0 +1,110 @@
+"""The Python surface keeps the reference's GAUSS_SEIDEL_SOLVER / RUN contract
+(lng.c:4173-4641, 8973-9025): keywords, defaults, derived diagepsilon and
+diagmaxiter, read-only and read/write members, validation errors.  The host
+domain (scene, contact detection) is checked against brute force.  No GPU."""
+import numpy as np
+import pytest
+
+import solfec
+from scenes import lattice
+
+
+def test_defaults_and_members():
+    gs = solfec.GAUSS_SEIDEL_SOLVER(1e-4, 1000)
+    assert gs.epsilon == 1e-4 and gs.maxiter == 1000 and gs.meritval == 1.0
+    assert gs.failure == "FAILURE_CONTINUE" and gs.diagsolver == "SEMISMOOTH_NEWTON"
+    # lng.c:4336-4342: 0.01 min(eps, meritval, 1e-4), max(100, maxiter/100)
+    assert gs.diagepsilon == pytest.approx(1e-6) and gs.diagmaxiter == 100
+    gs2 = solfec.GAUSS_SEIDEL_SOLVER(1e-6, 50000, meritval=1e-8)
+    assert gs2.diagepsilon == pytest.approx(1e-10) and gs2.diagmaxiter == 500
+    gs3 = solfec.GAUSS_SEIDEL_SOLVER(1e-20, 10)
+    assert gs3.diagepsilon == pytest.approx(np.finfo(float).eps)
+    assert gs.reverse == "OFF" and gs.variant == "FULL" and gs.innerloops == 1
+    assert gs.iters == 0 and gs.error == "OK" and gs.rerhist == [] and gs.itershist is None
+
+
+def test_read_write_members():
+    gs = solfec.GAUSS_SEIDEL_SOLVER(1e-4, 1000)
+    gs.epsilon = 1e-5
+    gs.maxiter = 20
+    gs.diagepsilon = 1e-9
+    gs.diagmaxiter = 7
+    gs.diagsolver = "PROJECTED_GRADIENT"
+    gs.reverse = "ON"
+    gs.variant = "MIDDLE_JACOBI"
+    gs.innerloops = 3
+    gs.precision = "MIXED"
+    assert (gs.epsilon, gs.maxiter, gs.diagepsilon, gs.diagmaxiter) == (1e-5, 20, 1e-9, 7)
+    assert gs.diagsolver == "PROJECTED_GRADIENT" and gs.reverse == "ON" and gs.variant == "MIDDLE_JACOBI"
+    assert gs.innerloops == 3 and gs.precision == "MIXED"
+    for member in ("failure", "error", "iters", "rerhist", "merhist"):
+        with pytest.raises(ValueError):
+            setattr(gs, member, 1)
+    with pytest.raises(ValueError):
+        gs.diagsolver = "NEWTON"
+    with pytest.raises(ValueError):
+        gs.reverse = "MAYBE"
+    with pytest.raises(ValueError):
+        gs.epsilon = -1.0
+
+
+def test_constructor_validation():
+    with pytest.raises(ValueError):
+        solfec.GAUSS_SEIDEL_SOLVER(-1.0, 10)
+    with pytest.raises(ValueError):
+        solfec.GAUSS_SEIDEL_SOLVER(1e-4, 0)
+    with pytest.raises(ValueError):
+        solfec.GAUSS_SEIDEL_SOLVER(1e-4, 10, failure="ABORT")
+    with pytest.raises(ValueError):
+        solfec.GAUSS_SEIDEL_SOLVER(1e-4, 10, failure="CALLBACK")
+    with pytest.raises(ValueError):
+        solfec.GAUSS_SEIDEL_SOLVER(1e-4, 10, diagsolver="X")
+    gs = solfec.GAUSS_SEIDEL_SOLVER(1e-4, 10, failure="CALLBACK", callback=lambda s: 1, data=(1, 2))
+    assert gs.failure == "FAILURE_CALLBACK"
+    with pytest.raises(TypeError):
+        solfec.RUN(object(), gs, 1.0)
+
+
+def test_scene_is_seeded_and_contacts_match_brute_force():
+    s1 = lattice(343, seed=5)
+    s2 = lattice(343, seed=5)
+    b1, c1 = s1.bodies(), s1.contacts()
+    b2, c2 = s2.bodies(), s2.contacts()
+    assert np.array_equal(b1["x"], b2["x"]) and np.array_equal(c1["master"], c2["master"])
+    x = b1["x"].reshape(-1, 3)
+    r = b1["radius"]
+    sph = np.where(b1["kind"] == 0)[0]
+    want = set()
+    for a in sph:
+        d = np.linalg.norm(x[sph] - x[a], axis=1) - (r[sph] + r[a])
+        for bb in sph[(d < 1e-6) & (sph > a)]:
+            want.add((int(a), int(bb)))
+    got = {(min(m, s), max(m, s)) for m, s in zip(c1["master"], c1["slave"]) if b1["kind"][s] == 0}
+    assert got == want
+    # normal from slave to master, unit, gap <= 0, frame orthonormal
+    n = len(c1["master"])
+    for i in range(n):
+        base = c1["base"][9 * i:9 * i + 9].reshape(3, 3).T
+        assert np.allclose(base.T @ base, np.eye(3), atol=1e-12)
+        assert c1["gap"][i] <= 0.0
+        m, s = c1["master"][i], c1["slave"][i]
+        if b1["kind"][s] == 0:
+            d = x[m] - x[s]
+            assert np.allclose(base[:, 2], d / np.linalg.norm(d), atol=1e-12)
+
+
+def test_contacts_persist_with_their_reactions():
+    s = lattice(125, seed=2)
+    c0 = s.contacts()
+    n = len(c0["master"])
+    s.set_RU(np.arange(3 * n, dtype=float), None)
+    s.step_end()
+    s.step_begin()
+    c1 = s.contacts()
+    keep = {(m, sl): k for k, (m, sl) in enumerate(zip(c0["master"], c0["slave"]))}
+    for i, (m, sl) in enumerate(zip(c1["master"], c1["slave"])):
+        if (m, sl) in keep:
+            k = keep[(m, sl)]
+            assert np.array_equal(c1["R"][3 * i:3 * i + 3], np.arange(3 * k, 3 * k + 3, dtype=float))
+        else:
+            assert np.all(c1["R"][3 * i:3 * i + 3] == 0.0)

@@ -90,7 +90,27 @@ def test_gs_converged_reactions_match_oracle(dev, n, seed, kw):
     if kw.get("meritval", 1.0) < 1.0:
         assert rep.merit <= kw["meritval"] or rep.iters == kw["maxiter"]
         assert np.allclose(mer[:k], omer[:k], rtol=1e-6, atol=1e-14)
-    assert rep.error == orep.error
+    # DIVERGED must agree; a local-solver failure recovered by the fallback chain
+    # (DIAGONAL_*) is a rounding-sensitive rare event and may show on one side only
+    assert (rep.error == 1) == (orep.error == 1)
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8])
+def test_gs_layouts_agree(dev, lanes):
+    """Every lanes-per-row layout of the sweep reproduces the oracle (merit on and off)."""
+    print(f"[layout] lanes={lanes}", flush=True)
+    s = lattice(729, seed=12)
+    b, c = s.bodies(), s.contacts()
+    O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), 1e-3)
+    dev.set_layout(lanes)
+    try:
+        for kw in (dict(epsilon=1e-5, maxiter=2000), dict(epsilon=1e-6, maxiter=2000, meritval=1e-8, reverse=1)):
+            rep, rer, mer, Rg, Ug, orep, orer, omer, Ro, Uo = _solve_both(dev, O, kw)
+            assert np.abs(Rg - Ro).max() / np.abs(Ro).max() < RTOL_R
+            assert np.abs(Ug - Uo).max() / np.abs(Uo).max() < RTOL_R
+            assert abs(rep.iters - orep.iters) <= 1, (rep.iters, orep.iters)
+    finally:
+        dev.set_layout(1)
 
 
 def test_gs_warm_start_and_U_state(dev):

@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=1, help="lanes of a wavefront per contact row (1, 2, 4, 8)")
     ap.add_argument("--save-snapshot", default=None, help="write the settled state to this .npz")
     ap.add_argument("--snapshot", default=None, help="load a settled state instead of settling")
+    ap.add_argument("--no-mixed", dest="mixed", action="store_false", help="skip the mixed-precision mode")
+    ap.add_argument("--headline", default="mixed", choices=["mixed", "fp64"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the contact solve on a single MI355X "
@@ -120,39 +122,55 @@ def main():
     dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
     dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
                  c["restitution"], c["cohesion"], c["R"], c["U"])
-    par = solfec.make_param(1e-6, 5000, meritval=1e-6)
+    n, nnz, ncol = 0, 0, 0
 
-    def step():
-        dev.assemble(h)
-        rep, _, _ = dev.solve(par, history=False)
-        dev.body_update(h, nb, download=False)
-        return rep
+    def run_mode(name, precision, steps, warmup):
+        par = solfec.make_param(1e-6, 5000, meritval=1e-6, precision=precision, refine=2)
 
-    for _ in range(args.warmup):
-        rep = step()
-    log(f"[bench] warmup done: iters={rep.iters} merit={rep.merit:.3e} error={rep.rel_error:.3e}")
-    reps = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        reps.append(step())
-    dt = time.perf_counter() - t0
+        def step():
+            dev.assemble(h)
+            rep, _, _ = dev.solve(par, history=False)
+            dev.body_update(h, nb, download=False)
+            return rep
+
+        for _ in range(warmup):
+            rep = step()
+        reps = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            reps.append(step())
+        dt = time.perf_counter() - t0
+        R, _ = dev.get_RU()
+        iters = [r.iters for r in reps]
+        res = dict(ms_per_step=1e3 * dt / steps, iters=iters, merit=reps[-1].merit, rel_error=reps[-1].rel_error,
+                   updates_per_s=sum(iters) * dev.sizes()[0] / dt)
+        log(f"[bench] {name}: {res['ms_per_step']:.2f} ms/step, iters={iters}, merit={res['merit']:.3e}")
+        return res, R
+
+    modes = {}
+    modes["fp64"], R64 = run_mode("plain FP64 GS", solfec.device.FP64, args.steps, args.warmup)
+    if args.mixed:
+        modes["mixed"], Rmx = run_mode("mixed FP32/FP64 GS", solfec.device.MIXED, args.steps, args.warmup)
+        modes["mixed"]["err_vs_fp64"] = float(np.abs(Rmx - R64).max() / max(np.abs(R64).max(), 1e-300))
+    head = "mixed" if args.mixed and args.headline == "mixed" else "fp64"
     n, nnz, ncol = dev.sizes()
-    iters = [r.iters for r in reps]
-    updates = sum(iters) * n
-    ms_per_step = 1e3 * dt / args.steps
-    value = updates / dt
-    log(f"[bench] timed {args.steps} steps: {ms_per_step:.2f} ms/step, iters={iters}")
+    value = modes[head]["updates_per_s"]
+    ms_per_step = modes[head]["ms_per_step"]
+    iters = modes[head]["iters"]
 
-    # roofline of the sweep kernel: one profiled solve, HIP events around every launch
+    # roofline of the sweep kernel of the headline mode: one profiled solve, HIP events around every launch
     dev.assemble(h)
-    prof = solfec.make_param(1e-6, 5000, meritval=1e-6, profile=1)
+    prec = solfec.device.MIXED if head == "mixed" else solfec.device.FP64
+    prof = solfec.make_param(1e-6, 5000, meritval=1e-6, profile=1, precision=prec, refine=2)
     prep, _, _ = dev.solve(prof, history=False)
-    bytes_per_sweep = 8 * (9 * nnz + 26 * n) + 4 * nnz + 4 * n
+    sz = 4 if head == "mixed" else 8
+    bytes_per_sweep = sz * (9 * nnz + 26 * n) + 4 * nnz + 4 * n
     sweeps_run = prep.sweep_launches / max(ncol, 1)
     achieved = bytes_per_sweep * sweeps_run / (prep.sweep_ms * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                kernel="k_sweep", avg_launch_us=round(1e3 * prep.sweep_ms / max(prep.sweep_launches, 1), 3),
+                kernel="k_sweep<%s>" % ("float" if head == "mixed" else "double"),
+                avg_launch_us=round(1e3 * prep.sweep_ms / max(prep.sweep_launches, 1), 3),
                 launches=int(prep.sweep_launches), bytes_per_sweep=int(bytes_per_sweep))
     log(f"[bench] sweep kernel: {prep.sweep_launches} launches, {prep.sweep_ms:.2f} ms, {achieved:.1f} GB/s")
 
@@ -173,14 +191,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.spheres}-sphere packing settled {args.settle} steps, plain FP64 GPU GS",
+            "workload": f"{args.spheres}-sphere packing settled {args.settle} steps, "
+                        + ("mixed FP32/FP64 GPU GS" if head == "mixed" else "plain FP64 GPU GS"),
             "spheres": args.spheres, "contacts": n, "offdiag_blocks": nnz, "colours": ncol,
             "epsilon": 1e-6, "meritval": 1e-6, "maxiter": 5000, "friction": args.friction,
-            "iters_per_step": iters, "merit": reps[-1].merit, "rel_error": reps[-1].rel_error,
-            "precision": "fp64", "warm_start": "previous-step R", "lanes_per_row": args.lanes,
+            "iters_per_step": iters, "merit": modes[head]["merit"], "rel_error": modes[head]["rel_error"],
+            "precision": head, "warm_start": "previous-step R", "lanes_per_row": args.lanes,
+            "modes": modes,
+            "speedup_vs_plain_fp64": modes["fp64"]["ms_per_step"] / ms_per_step,
         },
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -153,6 +153,29 @@ int SFGS_Colour(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, int *co
  * NULL, in which case that result stays on the device. */
 int SFGS_Body_Update(sfgs_ctx *ctx, double step, double *velo_out, double *force_out);
 
+/* ---- learned warm start (no counterpart in the reference) ----
+ * A 18 -> 128 -> 128 -> 128 -> 3 GELU perceptron (input padded to 32, output to
+ * 32), bf16 weights, FP32 accumulation, trained and run with the library's own
+ * bf16 matrix-core kernels.  Features of contact i: gap, B (3), W_ii (9),
+ * R_prev (3), number of off-diagonal blocks, friction.  Output: R - R_prev in
+ * units of the output scale.  Parameters are FP32, layout per layer [in][out]:
+ * W1 32x128, b1 128, W2 128x128, b2 128, W3 128x128, b3 128, W4 128x32, b4 32
+ * (35,872 values); norm = 18 feature means, 18 feature scales, 3 output scales. */
+int SFGS_Mlp_Set(sfgs_ctx *ctx, const float *params, long long nparams, const float *norm);
+int SFGS_Mlp_Get(sfgs_ctx *ctx, float *params, long long *nparams);
+/* prediction for host features (n x 18): out n x 3, units of R */
+int SFGS_Mlp_Forward(sfgs_ctx *ctx, const float *features, int n, float *out);
+/* loss (mean squared error in normalised units) and its gradient for one batch */
+int SFGS_Mlp_Grad(sfgs_ctx *ctx, const float *features, const float *targets, int n, float *grad, float *loss);
+/* Adam training on the device over 'epochs' passes of seeded random minibatches */
+int SFGS_Mlp_Train(sfgs_ctx *ctx, const float *features, const float *targets, long long n, int epochs, int batch,
+                   float lr, unsigned long long seed, float *loss_hist);
+/* features of the current system (natural order, n x 18): the snapshot writer */
+int SFGS_Features(sfgs_ctx *ctx, float *features);
+/* replace the carried R of the current system by R_prev + MLP(features) (call
+ * after SFGS_Locdyn_Update_Begin, before SFGS_Solve) */
+int SFGS_Warm_Start(sfgs_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

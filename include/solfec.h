@@ -102,6 +102,11 @@ int SOLFEC_Get_Contacts(SOLFEC *sol, int *master, int *slave, double *mpnt, doub
                         double *R, double *U);
 /* replace the carried reactions / velocities (warm start) of the current contacts */
 int SOLFEC_Set_RU(SOLFEC *sol, const double *R, const double *U);
+/* snapshot writer: after each assembly, the features of every contact (18
+ * floats, natural order, see SFGS_Features) are copied to buffer when the step
+ * has at most cap contacts; SOLFEC_Captured gives the count of the last step */
+int SOLFEC_Capture(SOLFEC *sol, float *buffer, int cap);
+int SOLFEC_Captured(SOLFEC *sol);
 
 #ifdef __cplusplus
 }

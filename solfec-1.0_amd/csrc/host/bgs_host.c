@@ -74,6 +74,7 @@ double GAUSS_SEIDEL_Get(GAUSS_SEIDEL *gs, const char *m)
   if (!strcmp(m, "error")) return gs->error;
   if (!strcmp(m, "precision")) return gs->par.precision;
   if (!strcmp(m, "refine")) return gs->par.refine;
+  if (!strcmp(m, "warmstart")) return gs->warmstart;
   return -1.0;
 }
 
@@ -91,6 +92,7 @@ int GAUSS_SEIDEL_Set(GAUSS_SEIDEL *gs, const char *m, double v)
   if (!strcmp(m, "itershist")) { gs->itershistcount = v != 0.0 ? 0 : -1; return 0; }
   if (!strcmp(m, "precision")) { if (v != SFGS_PREC_FP64 && v != SFGS_PREC_MIXED) return -1; gs->par.precision = (int)v; return 0; }
   if (!strcmp(m, "refine")) { if (v < 0) return -1; gs->par.refine = (int)v; return 0; }
+  if (!strcmp(m, "warmstart")) { if (v != 0 && v != 1) return -1; gs->warmstart = (int)v; return 0; }
   return -1;
 }
 

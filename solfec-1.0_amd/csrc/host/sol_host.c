@@ -725,6 +725,16 @@ int SOLFEC_Step_End(SOLFEC *sol)
   return 0;
 }
 
+int SOLFEC_Capture(SOLFEC *sol, float *buffer, int cap)
+{
+  sol->capture = buffer;
+  sol->capture_cap = buffer ? cap : 0;
+  sol->captured = 0;
+  return 0;
+}
+
+int SOLFEC_Captured(SOLFEC *sol) { return sol->captured; }
+
 double SOLFEC_Time(SOLFEC *sol) { return sol->time; }
 int SOLFEC_Nbod(SOLFEC *sol) { return sol->nb; }
 int SOLFEC_Ncon(SOLFEC *sol) { return sol->nc; }
@@ -794,6 +804,12 @@ int SOLFEC_Step_Solve(SOLFEC *sol, GAUSS_SEIDEL *gs)
   if (!rc) rc = SFGS_Locdyn_Update_Begin(g, sol->step, sol->dynamic);
   double t1 = now_s();
   sol->t_locdyn += t1 - t0;
+  if (!rc && sol->capture)
+  {
+    sol->captured = sol->nc <= sol->capture_cap ? sol->nc : 0;
+    if (sol->captured) rc = SFGS_Features(g, sol->capture);
+  }
+  if (!rc && gs->warmstart) rc = SFGS_Warm_Start(g);
   if (rc)
   {
     snprintf(sol->err, sizeof(sol->err), "LOCDYN on GPU failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));

@@ -47,6 +47,10 @@ struct solfec {
   size_t upd_cap;
   int **cand, *cand_n, *cand_cap, cand_nthr;
 
+  /* learned warm start: features of the last assembled step (snapshot writer) */
+  float *capture;
+  int capture_cap, captured;
+
   /* phase timers (sol.c:392-439 labels), seconds */
   double t_timint, t_conupd, t_condet, t_locdyn, t_consol;
 };
@@ -54,6 +58,7 @@ struct solfec {
 struct gauss_seidel {
   SFGS_PARAM par;
   int variant, innerloops;
+  int warmstart;          /* 0: previous-step R (the reference), 1: learned (SFGS_Warm_Start) */
   void *data;
   GAUSS_SEIDEL_Callback callback;
   int iters, error;

@@ -36,6 +36,10 @@ struct RowData;  // forward
 
 } // namespace sfgs
 
+struct sfgs_ctx;
+void sfgs_mlp_release(sfgs_ctx *ctx);
+int sfgs_mlp_apply(sfgs_ctx *ctx, const float *feat, int n, double *R);
+
 struct sfgs_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -79,7 +83,8 @@ struct sfgs_ctx {
   int G_built = 1;                       // layout of the current store
   sfgs::DBuf<unsigned char> sep;
   sfgs::DBuf<int> wcol;
-  sfgs::DBuf<float> W32, Wd32, B32, R32, U32, rho32, pfri32, pcoh32, shift32;
+  sfgs::DBuf<float> W32, Wd32, B32, R32, R32b, U32, Um32, rho32, pfri32, pcoh32, shift32;
+  int mirror_valid = 0;                  // FP32 copies match the FP64 store
   double energy = 0.0, step = 0.0;
   int dynamic = 1;
   int have_bodies = 0, have_system = 0, assembled = 0, from_bodies = 0;
@@ -92,4 +97,6 @@ struct sfgs_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   std::vector<hipEvent_t> prof_ev;        // per-launch sweep timing (profile mode)
   sfgs::DBuf<double> body_out;            // body update results (velo, force)
+  sfgs::DBuf<float> feat;                 // learned warm start: per-row features (18 per row)
+  void *mlp = nullptr;                    // learned warm start model (sfgs_mlp.hip)
 };

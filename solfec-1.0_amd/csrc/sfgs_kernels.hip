@@ -315,10 +315,16 @@ __global__ void k_iperm(int n, const int *perm, int *iperm)
   if (p < n) iperm[perm[p]] = p;
 }
 
-__global__ void k_colour_count(int n, const int *colour, int *count)
+// rows per colour: block histogram in LDS, one global add per colour and block
+__global__ void k_colour_count(int n, const int *colour, int *count, int ncolours)
 {
+  __shared__ int h[SFGS_MAX_COLOURS];
+  for (int c = threadIdx.x; c < ncolours; c += blockDim.x) h[c] = 0;
+  __syncthreads();
   int i = blockIdx.x*blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&count[colour[i]], 1);
+  if (i < n) atomicAdd(&h[colour[i]], 1);
+  __syncthreads();
+  for (int c = threadIdx.x; c < ncolours; c += blockDim.x) if (h[c]) atomicAdd(&count[c], h[c]);
 }
 
 __global__ void k_slice_len(int ns, const int *row0, const int *perm, const int *rowptr, int G, int *slen, long long *slen_ll)
@@ -554,17 +560,19 @@ __global__ __launch_bounds__(TPB) void k_offdiag_sys(SysIn si, SellOut so, const
 // writes.
 // =====================================================================
 
+template <typename T>
 struct SweepArgs {
   const int *row0, *nrows, *rowlen;
   const long long *off;
-  const double *W;
+  const T *W;
   const int *wcol;
-  const double *Wd, *A, *B, *rho, *fri, *coh, *shift;
+  const T *Wd, *B, *rho, *fri, *coh, *shift;
+  const double *A;      // merit weights stay FP64
   const unsigned char *sep;
-  const double *P;
-  double *Q, *U, *Um;
+  const T *P;
+  T *Q, *U, *Um;
   int n;
-  double diageps;
+  T diageps;
   int diagmaxiter, diagsolver, failure;
   double *partials;     // 3 per block: errup, errlo, merit numerator
   unsigned long long *failkey;
@@ -572,8 +580,8 @@ struct SweepArgs {
   Ctrl *ctrl;
 };
 
-template <int G>
-__device__ __forceinline__ double gsum(double v)
+template <int G, typename T>
+__device__ __forceinline__ T gsum(T v)
 {
   if (G >= 2) v += __shfl_xor(v, 1, 64);
   if (G >= 4) v += __shfl_xor(v, 2, 64);
@@ -597,8 +605,8 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c)
   }
 }
 
-template <int G, bool MERIT>
-__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs a, int s0, int s1, int pbase, int upd_lo, int upd_hi,
+template <typename T, int G, bool MERIT>
+__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<T> a, int s0, int s1, int pbase, int upd_lo, int upd_hi,
                                                 int backward)
 {
   if (a.ctrl->done) return;   // uniform: set only by k_finish between launches
@@ -615,8 +623,9 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs a, int s0, int s1, int 
     if (act) { p = a.row0[s] + r; len = a.rowlen[p]; }
   }
   // the row's own data first, so that its loads overlap the gather below
-  double Bi[3] = {0.0, 0.0, 0.0}, Wi[9], R[3] = {0.0, 0.0, 0.0}, U[3] = {0.0, 0.0, 0.0}, Ai[9];
-  RowLaw<double> L{0.0, 0.0, 0.0, 0.0, false};
+  T Bi[3] = {T(0), T(0), T(0)}, Wi[9], R[3] = {T(0), T(0), T(0)}, U[3] = {T(0), T(0), T(0)};
+  double Ai[9];
+  RowLaw<T> L{T(0), T(0), T(0), T(0), false};
   if (act)
   {
     Bi[0] = a.B[p]; Bi[1] = a.B[(size_t)n + p]; Bi[2] = a.B[2*(size_t)n + p];
@@ -634,32 +643,32 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs a, int s0, int s1, int 
 #pragma unroll
       for (int k = 0; k < 3; k ++) U[k] = a.U[(size_t)k*n + p];
     }
-    L = RowLaw<double>{a.fri[p], a.coh[p], a.shift[p], a.rho[p], a.sep[p] != 0};
+    L = RowLaw<T>{a.fri[p], a.coh[p], a.shift[p], a.rho[p], a.sep[p] != 0};
   }
-  double b0 = 0.0, b1 = 0.0, b2 = 0.0, u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  T b0 = T(0), b1 = T(0), b2 = T(0), u0 = T(0), u1 = T(0), u2 = T(0);
   if (act)
   {
-    const double *Wb = a.W + 9*off + l;
+    const T *Wb = a.W + 9*off + l;
     const int *cb = a.wcol + off + l;
 #pragma unroll 4
     for (int k = g, m = 0; k < len; k += G, m ++)
     {
       const int j = cb[(size_t)m*SLICE];
-      const double *w = Wb + 9*(size_t)m*SLICE;
-      const double w0 = w[0], w1 = w[SLICE], w2 = w[2*SLICE], w3 = w[3*SLICE], w4 = w[4*SLICE],
-                   w5 = w[5*SLICE], w6 = w[6*SLICE], w7 = w[7*SLICE], w8 = w[8*SLICE];
+      const T *w = Wb + 9*(size_t)m*SLICE;
+      const T w0 = w[0], w1 = w[SLICE], w2 = w[2*SLICE], w3 = w[3*SLICE], w4 = w[4*SLICE],
+              w5 = w[5*SLICE], w6 = w[6*SLICE], w7 = w[7*SLICE], w8 = w[8*SLICE];
       const bool upd = j >= upd_lo && j < upd_hi;
-      const double *xj = (upd ? a.Q : a.P) + 4*(size_t)j;
-      const double r0 = xj[0], r1 = xj[1], r2 = xj[2];
+      const T *xj = (upd ? a.Q : a.P) + 4*(size_t)j;
+      const T r0 = xj[0], r1 = xj[1], r2 = xj[2];
       b0 += w0*r0 + w3*r1 + w6*r2;
       b1 += w1*r0 + w4*r1 + w7*r2;
       b2 += w2*r0 + w5*r1 + w8*r2;
       if (MERIT)
       {
-        double p0 = r0, p1 = r1, p2 = r2;
+        T p0 = r0, p1 = r1, p2 = r2;
         if (upd)
         {
-          const double *pj = a.P + 4*(size_t)j;
+          const T *pj = a.P + 4*(size_t)j;
           p0 = pj[0]; p1 = pj[1]; p2 = pj[2];
         }
         u0 += w0*p0 + w3*p1 + w6*p2;
@@ -675,15 +684,16 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs a, int s0, int s1, int 
   double eu = 0.0, el = 0.0, up = 0.0;
   if (act)
   {
-    double Bp[3] = {Bi[0] + b0, Bi[1] + b1, Bi[2] + b2};
-    const double R0[3] = {R[0], R[1], R[2]};
+    T Bp[3] = {Bi[0] + b0, Bi[1] + b1, Bi[2] + b2};
+    const T R0[3] = {R[0], R[1], R[2]};
     if (MERIT)
     {
       // merit of the previous iteration at this row (mrf.c:63-84)
       mv_add(Bi, Wi, R, U);
       U[0] += u0; U[1] += u1; U[2] += u2;
-      double C[3];
-      scf_residual<double>(L.fri, L.coh, L.shift, U, R, C, nullptr, nullptr);
+      // the merit is evaluated in FP64 whatever the sweep type
+      double C[3], Ud[3] = {U[0], U[1], U[2]}, Rd[3] = {R[0], R[1], R[2]};
+      scf_residual<double>(L.fri, L.coh, L.shift, Ud, Rd, C, nullptr, nullptr);
       const double q0 = Ai[0]*C[0] + Ai[3]*C[1] + Ai[6]*C[2];
       const double q1 = Ai[1]*C[0] + Ai[4]*C[1] + Ai[7]*C[2];
       const double q2 = Ai[2]*C[0] + Ai[5]*C[1] + Ai[8]*C[2];
@@ -722,10 +732,10 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs a, int s0, int s1, int 
     {
 #pragma unroll
       for (int k = 0; k < 3; k ++) { a.Q[4*(size_t)p + k] = R[k]; a.U[(size_t)k*n + p] = U[k]; }
-      a.Q[4*(size_t)p + 3] = 0.0;
-      const double d0 = R[0] - R0[0], d1 = R[1] - R0[1], d2 = R[2] - R0[2];
+      a.Q[4*(size_t)p + 3] = T(0);
+      const double d0 = (double)R[0] - R0[0], d1 = (double)R[1] - R0[1], d2 = (double)R[2] - R0[2];
       eu = d0*d0 + d1*d1 + d2*d2;
-      el = R[0]*R[0] + R[1]*R[1] + R[2]*R[2];
+      el = (double)R[0]*R[0] + (double)R[1]*R[1] + (double)R[2]*R[2];
     }
     else up = 0.0;
   }
@@ -873,6 +883,35 @@ __global__ __launch_bounds__(TPB) void k_finish(Ctrl *ctrl, const double *partia
   else ctrl->prev_error = error_k;
 }
 
+// learned warm-start features of row p: gap, B (3), W_ii (9), R_prev (3), row
+// length, friction; written at row p (dst_nat = 0) or at the natural index
+__global__ void k_features(int n, const int *perm, const double *gap, const double *B, const double *Wd,
+                           const double *R, const int *rowlen, const double *fri, int natural, float *out)
+{
+  int p = blockIdx.x*blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int i = perm[p], o = natural ? i : p;
+  float *f = out + 18*(size_t)o;
+  f[0] = (float)gap[i];
+  for (int k = 0; k < 3; k ++) f[1 + k] = (float)B[(size_t)k*n + p];
+  for (int e = 0; e < 9; e ++) f[4 + e] = (float)Wd[(size_t)e*n + p];
+  for (int k = 0; k < 3; k ++) f[13 + k] = (float)R[4*(size_t)p + k];
+  f[16] = (float)rowlen[p];
+  f[17] = (float)fri[p];
+}
+
+__global__ void k_to_f32(long long n, const double *src, float *dst)
+{
+  long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
+__global__ void k_to_f64(long long n, const float *src, double *dst)
+{
+  long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (double)src[i];
+}
+
 // =====================================================================
 // results back to natural order, body update (bod.c:418-435, 1446-1447)
 // =====================================================================
@@ -1013,7 +1052,7 @@ static int build_order(sfgs_ctx *ctx, int n)
   int nc = ctx->ncolours;
   CHK(ensure(ctx, ctx->counter, (size_t)std::max(nc, 4)));
   HIPCHK(hipMemsetAsync(ctx->counter.p, 0, sizeof(int)*(size_t)std::max(nc, 4), ctx->stream));
-  if (n > 0) hipLaunchKernelGGL(k_colour_count, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->colour.p, ctx->counter.p);
+  if (n > 0) hipLaunchKernelGGL(k_colour_count, dim3(nblk(n, 1024)), dim3(1024), 0, ctx->stream, n, ctx->colour.p, ctx->counter.p, nc);
   std::vector<int> cnt((size_t)std::max(nc, 1), 0);
   if (nc > 0) HIPCHK(hipMemcpyAsync(cnt.data(), ctx->counter.p, sizeof(int)*(size_t)nc, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1151,7 +1190,7 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   release(ctx->rowlen); release(ctx->W); release(ctx->Wd); release(ctx->A); release(ctx->B); release(ctx->V);
   release(ctx->rho); release(ctx->pfri); release(ctx->pcoh); release(ctx->shift); release(ctx->R); release(ctx->U);
   release(ctx->Rb); release(ctx->Um); release(ctx->sep); release(ctx->wcol);
-  release(ctx->W32); release(ctx->Wd32); release(ctx->B32); release(ctx->R32); release(ctx->U32);
+  release(ctx->W32); release(ctx->Wd32); release(ctx->B32); release(ctx->R32); release(ctx->R32b); release(ctx->U32); release(ctx->Um32);
   release(ctx->rho32); release(ctx->pfri32); release(ctx->pcoh32); release(ctx->shift32);
   release(ctx->partials); release(ctx->mpartials); release(ctx->hist); release(ctx->failkey);
   release(ctx->failcount); release(ctx->ctrl);
@@ -1161,6 +1200,8 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   if (ctx->ev3) (void)hipEventDestroy(ctx->ev3);
   for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
   release(ctx->body_out);
+  release(ctx->feat);
+  sfgs_mlp_release(ctx);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1300,6 +1341,7 @@ int SFGS_Locdyn_Update_Begin(sfgs_ctx *ctx, double step, int dynamic)
   ctx->dynamic = dynamic;
   ctx->assembled = 1;
   ctx->from_bodies = 1;
+  ctx->mirror_valid = 0;
   if (singular) { ctx->err = std::to_string(singular) + " singular diagonal blocks"; return SFGS_E_SINGULAR; }
   return 0;
 }
@@ -1358,17 +1400,22 @@ int SFGS_System_Upload(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, 
   ctx->dynamic = dynamic;
   ctx->assembled = 1;
   ctx->from_bodies = 0;
+  ctx->mirror_valid = 0;
   return 0;
 }
 
-static SweepArgs sweep_args(sfgs_ctx *ctx, const SFGS_PARAM *par)
+} // extern "C"
+
+template <typename T>
+static SweepArgs<T> sweep_args(sfgs_ctx *ctx, const SFGS_PARAM *par, const T *W, const T *Wd, const T *B,
+                               const T *rho, const T *fri, const T *coh, const T *shift, T *U, T *Um, double diageps)
 {
-  SweepArgs a;
+  SweepArgs<T> a;
   a.row0 = ctx->slice_row0.p; a.nrows = ctx->slice_nrows.p; a.rowlen = ctx->rowlen.p; a.off = ctx->slice_off.p;
-  a.W = ctx->W.p; a.wcol = ctx->wcol.p; a.Wd = ctx->Wd.p; a.A = ctx->A.p; a.B = ctx->B.p; a.rho = ctx->rho.p;
-  a.fri = ctx->pfri.p; a.coh = ctx->pcoh.p; a.shift = ctx->shift.p; a.sep = ctx->sep.p;
-  a.P = ctx->R.p; a.Q = ctx->Rb.p; a.U = ctx->U.p; a.Um = ctx->Um.p; a.n = ctx->nc;
-  a.diageps = par->diagepsilon; a.diagmaxiter = par->diagmaxiter; a.diagsolver = par->diagsolver;
+  a.W = W; a.wcol = ctx->wcol.p; a.Wd = Wd; a.A = ctx->A.p; a.B = B; a.rho = rho;
+  a.fri = fri; a.coh = coh; a.shift = shift; a.sep = ctx->sep.p;
+  a.P = nullptr; a.Q = nullptr; a.U = U; a.Um = Um; a.n = ctx->nc;
+  a.diageps = (T)diageps; a.diagmaxiter = par->diagmaxiter; a.diagsolver = par->diagsolver;
   a.failure = par->failure; a.partials = ctx->partials.p; a.failkey = ctx->failkey.p; a.failcount = ctx->failcount.p;
   a.ctrl = ctx->ctrl.p;
   return a;
@@ -1384,14 +1431,16 @@ static MeritArgs merit_args(sfgs_ctx *ctx, int update_U, double *conmerit)
   return m;
 }
 
-// one sweep over all colours (forward or backward) with or without the fused merit
-static void launch_sweep(sfgs_ctx *ctx, const SweepArgs &base, int k, int backward, bool merit,
+// one sweep over all colours (forward or backward) with or without the fused merit;
+// sweep k reads buffer X[k & 1] and writes X[(k + 1) & 1]
+template <typename T>
+static void launch_sweep(sfgs_ctx *ctx, const SweepArgs<T> &base, T *X0, T *X1, int k, int backward, bool merit,
                          const std::vector<int> &grid, const std::vector<int> &pbase, bool prof)
 {
   const int ncol = ctx->ncolours, n = ctx->nc;
-  SweepArgs a = base;
-  if (k & 1) { a.P = ctx->Rb.p; a.Q = ctx->R.p; }
-  else { a.P = ctx->R.p; a.Q = ctx->Rb.p; }
+  SweepArgs<T> a = base;
+  if (k & 1) { a.P = X1; a.Q = X0; }
+  else { a.P = X0; a.Q = X1; }
   for (int ci = 0; ci < ncol; ci ++)
   {
     int c = backward ? ncol - 1 - ci : ci;
@@ -1400,53 +1449,139 @@ static void launch_sweep(sfgs_ctx *ctx, const SweepArgs &base, int k, int backwa
     if (prof) (void)hipEventRecord(ctx->prof_ev[2*ci], ctx->stream);
     if (merit)
     {
-      SFGS_DISPATCH_G(ctx->G_built, auto kern = k_sweep<GG COMMA true>; hipLaunchKernelGGL(kern, dim3(grid[c]), dim3(TPB), 0,
-                      ctx->stream, a, ctx->h_cslice[c], ctx->h_cslice[c + 1], pbase[c], lo, hi, backward));
+      SFGS_DISPATCH_G(ctx->G_built, auto kern = k_sweep<T COMMA GG COMMA true>; hipLaunchKernelGGL(kern, dim3(grid[c]),
+                      dim3(TPB), 0, ctx->stream, a, ctx->h_cslice[c], ctx->h_cslice[c + 1], pbase[c], lo, hi, backward));
     }
     else
     {
-      SFGS_DISPATCH_G(ctx->G_built, auto kern = k_sweep<GG COMMA false>; hipLaunchKernelGGL(kern, dim3(grid[c]), dim3(TPB), 0,
-                      ctx->stream, a, ctx->h_cslice[c], ctx->h_cslice[c + 1], pbase[c], lo, hi, backward));
+      SFGS_DISPATCH_G(ctx->G_built, auto kern = k_sweep<T COMMA GG COMMA false>; hipLaunchKernelGGL(kern, dim3(grid[c]),
+                      dim3(TPB), 0, ctx->stream, a, ctx->h_cslice[c], ctx->h_cslice[c + 1], pbase[c], lo, hi, backward));
     }
     if (prof) (void)hipEventRecord(ctx->prof_ev[2*ci + 1], ctx->stream);
   }
 }
 
+struct GsPlan {
+  std::vector<int> grid, pbase;
+  int np = 0;
+  bool prof = false;
+  double sweep_ms = 0.0;
+  long long launches = 0;
+};
+
+// Gauss-Seidel iterations until k_finish stops them; hc receives the final control block
+template <typename T>
+static int run_gs(sfgs_ctx *ctx, SweepArgs<T> sa, T *X0, T *X1, int maxiter, int nomerit, double eps,
+                  double meritval, int reverse, double *hist, GsPlan &pl, Ctrl &hc)
+{
+  hipStream_t st = ctx->stream;
+  const int ncol = ctx->ncolours;
+  std::memset(&hc, 0, sizeof(hc));
+  hc.eps = eps; hc.meritval = meritval; hc.maxiter = maxiter; hc.nomerit = nomerit; hc.energy = ctx->energy;
+  HIPCHK(hipMemcpyAsync(ctx->ctrl.p, &hc, sizeof(Ctrl), hipMemcpyHostToDevice, st));
+  const int maxsweeps = nomerit ? maxiter : maxiter + 1;   // with merit, sweep k closes iteration k-1
+  int batch = pl.prof ? 1 : 2, k = 0;
+  while (k < maxsweeps)
+  {
+    int nbt = std::min(batch, maxsweeps - k);
+    for (int j = 0; j < nbt; j ++)
+    {
+      int kk = k + j;
+      launch_sweep<T>(ctx, sa, X0, X1, kk, reverse && (kk % 2), !nomerit && kk > 0, pl.grid, pl.pbase, pl.prof);
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(TPB), 0, st, ctx->ctrl.p, ctx->partials.p, pl.np, hist);
+    }
+    k += nbt;
+    HIPCHK(hipMemcpyAsync(&hc, ctx->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipGetLastError());
+    if (pl.prof)
+    {
+      for (int ci = 0; ci < ncol; ci ++)
+      {
+        float e = 0.0f;
+        HIPCHK(hipEventElapsedTime(&e, ctx->prof_ev[2*ci], ctx->prof_ev[2*ci + 1]));
+        pl.sweep_ms += e;
+        pl.launches ++;
+      }
+    }
+    if (hc.done) break;
+    batch = pl.prof ? 1 : std::min(batch*2, 32);
+  }
+  if (!hc.done) { hc.final_buf = maxsweeps & 1; hc.final_U = 0; }
+  return 0;
+}
+
+static inline int nblk_ll(long long n) { return (int)((n + TPB - 1) / TPB); }
+
+// FP32 copies of the sweep data (built once per assembled system)
+static int build_f32_mirror(sfgs_ctx *ctx)
+{
+  if (ctx->mirror_valid) return 0;
+  const long long n = ctx->nc, nb = 9*ctx->sell_blocks;
+  hipStream_t st = ctx->stream;
+  CHK(ensure(ctx, ctx->W32, (size_t)std::max(nb, 1LL)));
+  CHK(ensure(ctx, ctx->Wd32, 9*(size_t)n));
+  CHK(ensure(ctx, ctx->B32, 3*(size_t)n));
+  CHK(ensure(ctx, ctx->rho32, (size_t)n));
+  CHK(ensure(ctx, ctx->pfri32, (size_t)n));
+  CHK(ensure(ctx, ctx->pcoh32, (size_t)n));
+  CHK(ensure(ctx, ctx->shift32, (size_t)n));
+  CHK(ensure(ctx, ctx->R32, 4*(size_t)n));
+  CHK(ensure(ctx, ctx->R32b, 4*(size_t)n));
+  CHK(ensure(ctx, ctx->U32, 3*(size_t)n));
+  CHK(ensure(ctx, ctx->Um32, 3*(size_t)n));
+  if (nb > 0) hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(nb)), dim3(TPB), 0, st, nb, ctx->W.p, ctx->W32.p);
+  if (n > 0)
+  {
+    hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(9*n)), dim3(TPB), 0, st, 9*n, ctx->Wd.p, ctx->Wd32.p);
+    hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(3*n)), dim3(TPB), 0, st, 3*n, ctx->B.p, ctx->B32.p);
+    hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(n)), dim3(TPB), 0, st, n, ctx->rho.p, ctx->rho32.p);
+    hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(n)), dim3(TPB), 0, st, n, ctx->pfri.p, ctx->pfri32.p);
+    hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(n)), dim3(TPB), 0, st, n, ctx->pcoh.p, ctx->pcoh32.p);
+    hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(n)), dim3(TPB), 0, st, n, ctx->shift.p, ctx->shift32.p);
+  }
+  ctx->mirror_valid = 1;
+  return 0;
+}
+
+extern "C" {
+
 int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *merhist, SFGS_REPORT *rep)
 {
   if (!ctx || !par || par->maxiter <= 0 || !(par->epsilon > 0.0) || par->diagmaxiter <= 0) return SFGS_E_ARG;
   if (par->diagsolver < 0 || par->diagsolver > 3 || par->failure < 0 || par->failure > 2) return SFGS_E_ARG;
+  if (par->precision != SFGS_PREC_FP64 && par->precision != SFGS_PREC_MIXED) return SFGS_E_ARG;
+  if (par->precision == SFGS_PREC_MIXED && par->refine < 0) return SFGS_E_ARG;
   if (!ctx->assembled) { ctx->err = "solve before LOCDYN assembly"; return SFGS_E_STATE; }
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
-  const int ncol = ctx->ncolours, maxiter = par->maxiter;
+  const int ncol = ctx->ncolours, maxiter = par->maxiter, n = ctx->nc;
   const int nomerit = par->meritval >= 1.0 ? 1 : 0;
+  const bool mixed = par->precision == SFGS_PREC_MIXED;
+  const int refine = mixed ? par->refine : 0;
 
-  // partial-sum slots: per colour, one per block
-  std::vector<int> pbase((size_t)ncol + 1, 0), grid((size_t)ncol, 0);
+  GsPlan pl;
+  pl.grid.assign((size_t)ncol, 0);
+  pl.pbase.assign((size_t)ncol + 1, 0);
   for (int c = 0; c < ncol; c ++)
   {
     int ns = ctx->h_cslice[c + 1] - ctx->h_cslice[c];
-    grid[c] = (ns + TPB/SLICE - 1) / (TPB/SLICE);
-    pbase[c + 1] = pbase[c] + grid[c];
+    pl.grid[c] = (ns + TPB/SLICE - 1) / (TPB/SLICE);
+    pl.pbase[c + 1] = pl.pbase[c] + pl.grid[c];
   }
-  const int np = pbase[ncol];
-  CHK(ensure(ctx, ctx->partials, 3*(size_t)std::max(np, 1)));
-  CHK(ensure(ctx, ctx->hist, 2*(size_t)maxiter));
+  pl.np = pl.pbase[ncol];
+  pl.prof = par->profile != 0;
+  const int mgrid = (ctx->nslices + TPB/SLICE - 1) / (TPB/SLICE);
+  CHK(ensure(ctx, ctx->partials, 3*(size_t)std::max(pl.np, 1)));
+  CHK(ensure(ctx, ctx->mpartials, (size_t)std::max(mgrid, 1)));
+  CHK(ensure(ctx, ctx->hist, 2*(size_t)(maxiter + refine + 1)));
   CHK(ensure(ctx, ctx->failkey, 1));
   CHK(ensure(ctx, ctx->failcount, 4));
   CHK(ensure(ctx, ctx->ctrl, 1));
-  HIPCHK(hipMemsetAsync(ctx->partials.p, 0, sizeof(double)*3*(size_t)std::max(np, 1), st));
+  HIPCHK(hipMemsetAsync(ctx->partials.p, 0, sizeof(double)*3*(size_t)std::max(pl.np, 1), st));
   HIPCHK(hipMemsetAsync(ctx->failkey.p, 0, sizeof(unsigned long long), st));
   HIPCHK(hipMemsetAsync(ctx->failcount.p, 0, sizeof(int)*4, st));
-  Ctrl hc;
-  std::memset(&hc, 0, sizeof(hc));
-  hc.eps = par->epsilon; hc.meritval = par->meritval; hc.maxiter = maxiter; hc.nomerit = nomerit;
-  hc.energy = ctx->energy;
-  HIPCHK(hipMemcpyAsync(ctx->ctrl.p, &hc, sizeof(Ctrl), hipMemcpyHostToDevice, st));
-
-  const int prof = par->profile != 0;
-  if (prof && (int)ctx->prof_ev.size() < 2*ncol)
+  if (pl.prof && (int)ctx->prof_ev.size() < 2*ncol)
   {
     for (int k = (int)ctx->prof_ev.size(); k < 2*ncol; k ++)
     {
@@ -1455,77 +1590,113 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
       ctx->prof_ev.push_back(e);
     }
   }
-  SweepArgs sa = sweep_args(ctx, par);
-  double sweep_ms = 0.0;
-  long long launches = 0;
-  // with the merit on, sweep k closes iteration k-1: up to maxiter + 1 sweeps
-  const int maxsweeps = nomerit ? maxiter : maxiter + 1;
+
+  Ctrl hc, hr;
+  std::memset(&hr, 0, sizeof(hr));
   HIPCHK(hipEventRecord(ctx->ev0, st));
-  int batch = prof ? 1 : 2, k = 0;
-  while (k < maxsweeps)
+  if (!mixed)
   {
-    int nbt = std::min(batch, maxsweeps - k);
-    for (int j = 0; j < nbt; j ++)
-    {
-      int kk = k + j;
-      int backward = par->reverse && (kk % 2);
-      launch_sweep(ctx, sa, kk, backward, !nomerit && kk > 0, grid, pbase, prof);
-      hipLaunchKernelGGL(k_finish, dim3(1), dim3(TPB), 0, st, ctx->ctrl.p, ctx->partials.p, np, ctx->hist.p);
-    }
-    k += nbt;
-    HIPCHK(hipMemcpyAsync(&hc, ctx->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    HIPCHK(hipGetLastError());
-    if (prof)
-    {
-      for (int ci = 0; ci < ncol; ci ++)
-      {
-        float e = 0.0f;
-        HIPCHK(hipEventElapsedTime(&e, ctx->prof_ev[2*ci], ctx->prof_ev[2*ci + 1]));
-        sweep_ms += e;
-        launches ++;
-      }
-    }
-    if (hc.done) break;
-    batch = prof ? 1 : std::min(batch*2, 32);
+    SweepArgs<double> sa = sweep_args<double>(ctx, par, ctx->W.p, ctx->Wd.p, ctx->B.p, ctx->rho.p, ctx->pfri.p,
+                                              ctx->pcoh.p, ctx->shift.p, ctx->U.p, ctx->Um.p, par->diagepsilon);
+    CHK(run_gs<double>(ctx, sa, ctx->R.p, ctx->Rb.p, maxiter, nomerit, par->epsilon, par->meritval, par->reverse,
+                       ctx->hist.p, pl, hc));
+    if (n > 0 && hc.final_buf == 1)
+      HIPCHK(hipMemcpyAsync(ctx->R.p, ctx->Rb.p, sizeof(double)*4*(size_t)n, hipMemcpyDeviceToDevice, st));
+    if (n > 0 && hc.final_U == 1)
+      HIPCHK(hipMemcpyAsync(ctx->U.p, ctx->Um.p, sizeof(double)*3*(size_t)n, hipMemcpyDeviceToDevice, st));
   }
-  if (!hc.done)
+  else
   {
-    // all sweeps ran without a stop: only possible with no rows; state is the last sweep
-    hc.final_buf = maxsweeps & 1;
-    hc.final_U = 0;
+    // FP32 sweeps (FP64 merit, errors and termination), then FP64 refinement sweeps
+    CHK(build_f32_mirror(ctx));
+    if (n > 0)
+    {
+      hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(4LL*n)), dim3(TPB), 0, st, 4LL*n, ctx->R.p, ctx->R32.p);
+      hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(3LL*n)), dim3(TPB), 0, st, 3LL*n, ctx->U.p, ctx->U32.p);
+    }
+    // FP32 resolves a relative change of about 1e-7: keep the local tolerance above it
+    const double de32 = std::max(par->diagepsilon, 1e-6);
+    SweepArgs<float> sa = sweep_args<float>(ctx, par, ctx->W32.p, ctx->Wd32.p, ctx->B32.p, ctx->rho32.p,
+                                            ctx->pfri32.p, ctx->pcoh32.p, ctx->shift32.p, ctx->U32.p, ctx->Um32.p, de32);
+    CHK(run_gs<float>(ctx, sa, ctx->R32.p, ctx->R32b.p, maxiter, nomerit, par->epsilon, par->meritval, par->reverse,
+                      ctx->hist.p, pl, hc));
+    if (n > 0)
+    {
+      float *Rf = hc.final_buf == 1 ? ctx->R32b.p : ctx->R32.p, *Uf = hc.final_U == 1 ? ctx->Um32.p : ctx->U32.p;
+      hipLaunchKernelGGL(k_to_f64, dim3(nblk_ll(4LL*n)), dim3(TPB), 0, st, 4LL*n, Rf, ctx->R.p);
+      hipLaunchKernelGGL(k_to_f64, dim3(nblk_ll(3LL*n)), dim3(TPB), 0, st, 3LL*n, Uf, ctx->U.p);
+    }
+    if (refine > 0 && !hc.aborted)
+    {
+      SweepArgs<double> sd = sweep_args<double>(ctx, par, ctx->W.p, ctx->Wd.p, ctx->B.p, ctx->rho.p, ctx->pfri.p,
+                                                ctx->pcoh.p, ctx->shift.p, ctx->U.p, ctx->Um.p, par->diagepsilon);
+      // exactly 'refine' FP64 sweeps (eps 0 never stops them early), histories after the FP32 ones
+      CHK(run_gs<double>(ctx, sd, ctx->R.p, ctx->Rb.p, refine, 1, 0.0, 1.0, 0, ctx->hist.p + 2*(size_t)maxiter, pl, hr));
+      if (n > 0 && hr.final_buf == 1)
+        HIPCHK(hipMemcpyAsync(ctx->R.p, ctx->Rb.p, sizeof(double)*4*(size_t)n, hipMemcpyDeviceToDevice, st));
+    }
+    // final FP64 merit with the merit's update of U (mrf.c:61-76)
+    if (!nomerit && mgrid > 0)
+    {
+      MeritArgs ma = merit_args(ctx, 1, nullptr);
+      SFGS_DISPATCH_G(ctx->G_built, auto kern = k_merit<GG>; hipLaunchKernelGGL(kern, dim3(mgrid), dim3(TPB), 0, st, ma,
+                      ctx->nslices));
+    }
   }
-  // canonical state: R in buffer 0, U in U
-  const int n = ctx->nc;
-  if (n > 0 && hc.final_buf == 1)
-    HIPCHK(hipMemcpyAsync(ctx->R.p, ctx->Rb.p, sizeof(double)*4*(size_t)n, hipMemcpyDeviceToDevice, st));
-  if (n > 0 && hc.final_U == 1)
-    HIPCHK(hipMemcpyAsync(ctx->U.p, ctx->Um.p, sizeof(double)*3*(size_t)n, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipEventRecord(ctx->ev1, st));
   unsigned long long fk = 0;
   int fc = 0;
-  std::vector<double> hist(2*(size_t)maxiter, 0.0);
+  std::vector<double> hist(2*(size_t)(maxiter + refine + 1), 0.0), mp((size_t)std::max(mgrid, 1), 0.0);
   HIPCHK(hipMemcpyAsync(&fk, ctx->failkey.p, sizeof(fk), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&fc, ctx->failcount.p, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(hist.data(), ctx->hist.p, sizeof(double)*2*(size_t)maxiter, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(hist.data(), ctx->hist.p, sizeof(double)*hist.size(), hipMemcpyDeviceToHost, st));
+  if (mixed && !nomerit && mgrid > 0)
+    HIPCHK(hipMemcpyAsync(mp.data(), ctx->mpartials.p, sizeof(double)*(size_t)mgrid, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
   float ms = 0.0f;
   HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-  if (rerhist) std::memcpy(rerhist, hist.data(), sizeof(double)*(size_t)maxiter);
-  if (merhist) std::memcpy(merhist, hist.data() + maxiter, sizeof(double)*(size_t)maxiter);
+
+  int iters = hc.iters;
+  double rel_error = hc.error, merit = hc.merit;
+  if (mixed)
+  {
+    const double *h2 = hist.data() + 2*(size_t)maxiter;   // refinement block: errors then merits (maxiter = refine)
+    int nref = hr.iters;
+    if (!nomerit)
+    {
+      double up = 0.0;
+      for (int k = 0; k < mgrid; k ++) up += mp[k];
+      double lo = ctx->energy < DBL_EPSILON ? 1.0 : ctx->energy;
+      merit = 0.5*up / lo;
+    }
+    if (nref > 0) rel_error = h2[nref - 1];
+    // append the refinement sweeps to the histories
+    for (int k = 0; k < nref && iters + k < maxiter + refine; k ++)
+    {
+      hist[(size_t)iters + k] = h2[k];
+    }
+    iters += nref;
+  }
+  if (rerhist) std::memcpy(rerhist, hist.data(), sizeof(double)*(size_t)std::min(iters, maxiter));
+  if (merhist)
+  {
+    std::memcpy(merhist, hist.data() + maxiter, sizeof(double)*(size_t)std::min(hc.iters, maxiter));
+    for (int k = hc.iters; k < std::min(iters, maxiter); k ++) merhist[k] = merit;
+  }
   if (rep)
   {
     std::memset(rep, 0, sizeof(*rep));
-    rep->iters = hc.iters;
-    rep->rel_error = hc.error;
-    rep->merit = hc.merit;
+    rep->iters = iters;
+    rep->rel_error = rel_error;
+    rep->merit = merit;
     rep->ncolours = ncol;
     rep->fail_rows = fc;
     rep->aborted = hc.aborted;
     rep->solve_ms = ms;
-    rep->sweep_ms = sweep_ms;
-    rep->sweep_launches = launches;
-    rep->row_updates = (long long)hc.iters * n;
+    rep->sweep_ms = pl.sweep_ms;
+    rep->sweep_launches = pl.launches;
+    rep->row_updates = (long long)iters * n;
     rep->nnz = ctx->nnz;
     rep->error = fk ? (int)(fk & 3ull) : SFGS_GS_OK;
     if (!hc.aborted && hc.iters >= maxiter) rep->error = SFGS_GS_DIVERGED;
@@ -1563,6 +1734,40 @@ int SFGS_Merit(sfgs_ctx *ctx, int update_U, double *merit, double *conmerit)
   if (merit) *merit = 0.5*up / lo;
   if (conmerit) for (int p = 0; p < n; p ++) conmerit[hperm[p]] = hcm[p] / lo;
   return 0;
+}
+
+static int compute_features(sfgs_ctx *ctx, int natural)
+{
+  const int n = ctx->nc;
+  CHK(ensure(ctx, ctx->feat, 18*(size_t)std::max(n, 1)));
+  if (n > 0)
+  {
+    const double *gap = ctx->gap.p;
+    hipLaunchKernelGGL(k_features, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->perm.p, gap, ctx->B.p, ctx->Wd.p,
+                       ctx->R.p, ctx->rowlen.p, ctx->pfri.p, natural, ctx->feat.p);
+  }
+  return 0;
+}
+
+int SFGS_Features(sfgs_ctx *ctx, float *features)
+{
+  if (!ctx || !features) return SFGS_E_ARG;
+  if (!ctx->assembled || !ctx->from_bodies) { ctx->err = "features need a system assembled from bodies"; return SFGS_E_STATE; }
+  HIPCHK(hipSetDevice(ctx->device));
+  CHK(compute_features(ctx, 1));
+  if (ctx->nc > 0)
+    HIPCHK(hipMemcpyAsync(features, ctx->feat.p, sizeof(float)*18*(size_t)ctx->nc, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int SFGS_Warm_Start(sfgs_ctx *ctx)
+{
+  if (!ctx) return SFGS_E_ARG;
+  if (!ctx->assembled || !ctx->from_bodies) { ctx->err = "warm start needs a system assembled from bodies"; return SFGS_E_STATE; }
+  HIPCHK(hipSetDevice(ctx->device));
+  CHK(compute_features(ctx, 0));
+  return sfgs_mlp_apply(ctx, ctx->feat.p, ctx->nc, ctx->R.p);
 }
 
 int SFGS_Set_Layout(sfgs_ctx *ctx, int lanes_per_row)

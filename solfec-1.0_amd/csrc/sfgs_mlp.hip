@@ -1,0 +1,584 @@
+// sfgs_mlp.hip -- learned warm start for the Gauss-Seidel solve.
+//
+// A multilayer perceptron 32 -> 128 -> 128 -> 128 -> 32 (18 input features and
+// 3 outputs used, the rest zero padding), GELU on the hidden layers, bf16
+// weights and activations, FP32 accumulation.  It proposes each contact's
+// reaction as R0 = R_prev + s * f(features); the plain Gauss-Seidel solve then
+// runs from R0 as usual, so the network only changes where the solve starts.
+//
+// Every matrix product (inference, the backward pass, the weight gradients)
+// goes through one tiled GEMM on the gfx950 matrix cores,
+// v_mfma_f32_16x16x32_bf16: a 256-thread workgroup computes a 64x64 tile of C
+// in four 32x32 wave tiles, staging 64x32 slices of A and B in LDS.  Lane l of
+// a wave holds, for one 16x16x32 product, A[l%16][8(l/16)..+7] and
+// B[8(l/16)..+7][l%16] and receives D[4(l/16)..+3][l%16].  Only the row and
+// column placement matters for the result: A and B give their k values to the
+// same lanes in the same order, so any assignment of the 32 k values to
+// (lane group, element) is a permutation of the contraction and sums the same.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include "sfgs_ctx.hpp"
+
+namespace sfgs_mlp {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+static const int BM = 64, BN = 64, BK = 32, LPAD = 8;
+
+enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS_F32 = 2, EPI_GELU_BWD = 3, EPI_BF16 = 4 };
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f*x*(1.0f + erff(x*0.70710678118654752f)); }
+__device__ __forceinline__ float dgelu(float x)
+{
+  return 0.5f*(1.0f + erff(x*0.70710678118654752f)) + x*0.39894228040143268f*expf(-0.5f*x*x);
+}
+
+struct GemmArgs {
+  int M, N, K;           // C is M x N, contraction over K (multiple of 32)
+  const bf16 *A; int lda; int transA;   // transA = 0: A[m*lda + k]; 1: A[k*lda + m]
+  const bf16 *B; int ldb; int transB;   // transB = 0: B[k*ldb + n]; 1: B[n*ldb + k]
+  float *C; int ldc;                    // FP32 output (EPI_F32, EPI_BIAS_F32, EPI_BIAS_GELU keeps Z here)
+  bf16 *H; int ldh;                     // bf16 output (EPI_BIAS_GELU: GELU(Z); EPI_GELU_BWD: dZ; EPI_BF16)
+  const float *bias;                    // length N (EPI_BIAS_*)
+  const float *Z; int ldz;              // pre-activations for EPI_GELU_BWD
+  int epi;
+};
+
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
+{
+  __shared__ bf16 As[BM][BK + LPAD];
+  __shared__ bf16 Bt[BN][BK + LPAD];   // B tile stored transposed: Bt[n][k]
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int m0 = blockIdx.y*BM, n0 = blockIdx.x*BN;
+  const int wm = (wave >> 1)*32, wn = (wave & 1)*32;
+  floatx4 acc[2][2];
+  for (int i = 0; i < 2; i ++) for (int j = 0; j < 2; j ++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = 0; k0 < g.K; k0 += BK)
+  {
+    // stage A (64 x 32) and B (32 x 64): 2048 elements each, 8 per thread
+    {
+      int e = tid*8;
+      int r = e / BK, c = e % BK;            // A tile row r, k c..c+7
+      int m = m0 + r;
+      for (int t = 0; t < 8; t ++)
+      {
+        int k = k0 + c + t;
+        bf16 v = (bf16)0.0f;
+        if (m < g.M && k < g.K) v = g.transA ? g.A[(size_t)k*g.lda + m] : g.A[(size_t)m*g.lda + k];
+        As[r][c + t] = v;
+      }
+      int nn = e / BK, kk = e % BK;          // B tile column nn, k kk..kk+7
+      int n = n0 + nn;
+      for (int t = 0; t < 8; t ++)
+      {
+        int k = k0 + kk + t;
+        bf16 v = (bf16)0.0f;
+        if (n < g.N && k < g.K) v = g.transB ? g.B[(size_t)n*g.ldb + k] : g.B[(size_t)k*g.ldb + n];
+        Bt[nn][kk + t] = v;
+      }
+    }
+    __syncthreads();
+    const int fr = lane & 15, fk = 8*(lane >> 4);
+    bf16x8 a[2], b[2];
+    for (int i = 0; i < 2; i ++)
+      for (int t = 0; t < 8; t ++) a[i][t] = As[wm + 16*i + fr][fk + t];
+    for (int j = 0; j < 2; j ++)
+      for (int t = 0; t < 8; t ++) b[j][t] = Bt[wn + 16*j + fr][fk + t];
+    for (int i = 0; i < 2; i ++)
+      for (int j = 0; j < 2; j ++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+
+  // epilogue: lane holds D[4(l/16)+q][l%16] of each 16x16 tile
+  for (int i = 0; i < 2; i ++)
+    for (int j = 0; j < 2; j ++)
+      for (int q = 0; q < 4; q ++)
+      {
+        int m = m0 + wm + 16*i + 4*(lane >> 4) + q, n = n0 + wn + 16*j + (lane & 15);
+        if (m >= g.M || n >= g.N) continue;
+        float v = acc[i][j][q];
+        switch (g.epi)
+        {
+        case EPI_F32: g.C[(size_t)m*g.ldc + n] = v; break;
+        case EPI_BIAS_F32: g.C[(size_t)m*g.ldc + n] = v + g.bias[n]; break;
+        case EPI_BIAS_GELU:
+        {
+          float z = v + g.bias[n];
+          g.C[(size_t)m*g.ldc + n] = z;
+          g.H[(size_t)m*g.ldh + n] = (bf16)gelu(z);
+        }
+        break;
+        case EPI_GELU_BWD: g.H[(size_t)m*g.ldh + n] = (bf16)(v * dgelu(g.Z[(size_t)m*g.ldz + n])); break;
+        case EPI_BF16: g.H[(size_t)m*g.ldh + n] = (bf16)v; break;
+        }
+      }
+}
+
+// ---- model state ----
+
+static const int NF = 18, NI = 32, NH = 128, NO = 3, NOP = 32;
+
+struct Model {
+  // FP32 master parameters, layout [in][out] per layer, then biases
+  // W1 32x128, b1 128, W2 128x128, b2 128, W3 128x128, b3 128, W4 128x32, b4 32
+  size_t off[8], total;
+  float *param = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;  // Adam moments
+  bf16 *wbf = nullptr;      // bf16 copies of W1..W4 (same offsets as the W parts of param)
+  float norm[2*NF + NO];   // feature mean, feature std, output scale
+  float *dnorm = nullptr;  // device copy of norm
+  int have = 0;
+  long long step = 0;
+  // activation workspace for a batch
+  int cap = 0;
+  bf16 *X = nullptr, *H1 = nullptr, *H2 = nullptr, *H3 = nullptr, *dO = nullptr, *dZ = nullptr, *dZ2 = nullptr;
+  float *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr, *O = nullptr, *Y = nullptr, *loss = nullptr;
+  // device-resident training set
+  float *tX = nullptr, *tY = nullptr;
+  long long tn = 0, tcap = 0;
+  int *idx = nullptr;
+};
+
+static void layout(Model &m)
+{
+  size_t o = 0;
+  m.off[0] = o; o += (size_t)NI*NH;  m.off[1] = o; o += NH;
+  m.off[2] = o; o += (size_t)NH*NH;  m.off[3] = o; o += NH;
+  m.off[4] = o; o += (size_t)NH*NH;  m.off[5] = o; o += NH;
+  m.off[6] = o; o += (size_t)NH*NOP; m.off[7] = o; o += NOP;
+  m.total = o;
+}
+
+__global__ void k_to_bf16(size_t n, const float *src, bf16 *dst)
+{
+  size_t i = (size_t)blockIdx.x*blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (bf16)src[i];
+}
+
+// MSE loss on the 3 used outputs; dO = d loss / d out (bf16, padded columns zero)
+__global__ void k_loss(int B, const float *O, const float *Y, bf16 *dO, float *loss_part)
+{
+  __shared__ float red[256];
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (i < B)
+  {
+    for (int c = 0; c < NOP; c ++)
+    {
+      float g = 0.f;
+      if (c < NO)
+      {
+        float d = O[(size_t)i*NOP + c] - Y[(size_t)i*NO + c];
+        l += d*d;
+        g = 2.f*d / (float)(B*NO);
+      }
+      dO[(size_t)i*NOP + c] = (bf16)g;
+    }
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1)
+  {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+}
+
+// bias gradient: column sums of a bf16 B x N matrix
+__global__ void k_colsum(int B, int N, const bf16 *D, float *g)
+{
+  int n = blockIdx.x*blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.f;
+  for (int i = 0; i < B; i ++) s += (float)D[(size_t)i*N + n];
+  g[n] = s;
+}
+
+__global__ void k_adam(size_t n, float *p, const float *g, float *m1, float *m2, float lr, float b1, float b2,
+                       float c1, float c2, float eps)
+{
+  size_t i = (size_t)blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gi = g[i];
+  float a = b1*m1[i] + (1.f - b1)*gi, v = b2*m2[i] + (1.f - b2)*gi*gi;
+  m1[i] = a; m2[i] = v;
+  p[i] -= lr * (a/c1) / (sqrtf(v/c2) + eps);
+}
+
+// minibatch gather from the resident training set: normalised features -> bf16 X (B x 32), targets Y
+__global__ void k_gather(int B, const int *idx, const float *tX, const float *tY, const float *norm, bf16 *X, float *Y)
+{
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= B) return;
+  int s = idx[i];
+  for (int f = 0; f < NI; f ++)
+  {
+    float v = 0.f;
+    if (f < NF) v = (tX[(size_t)s*NF + f] - norm[f]) / norm[NF + f];
+    X[(size_t)i*NI + f] = (bf16)v;
+  }
+  for (int c = 0; c < NO; c ++) Y[(size_t)i*NO + c] = tY[(size_t)s*NO + c] / norm[2*NF + c];
+}
+
+// R0 = R_prev + s * f(features), rows in colour-major order (R has 4 doubles per row)
+__global__ void k_apply(int n, const float *O, const float *norm, double *R)
+{
+  int p = blockIdx.x*blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  for (int c = 0; c < NO; c ++) R[4*(size_t)p + c] += (double)(O[(size_t)p*NOP + c] * norm[2*NF + c]);
+}
+
+__global__ void k_iota(int n, int *idx)
+{
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i < n) idx[i] = i;
+}
+
+} // namespace sfgs_mlp
+
+using namespace sfgs_mlp;
+
+#define HIPCHK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) { \
+    ctx->err = std::string(#call) + ": " + hipGetErrorString(e_); return SFGS_E_HIP; } } while (0)
+
+static Model* model_of(sfgs_ctx *ctx)
+{
+  if (!ctx->mlp)
+  {
+    Model *m = new Model();
+    layout(*m);
+    ctx->mlp = m;
+  }
+  return (Model*)ctx->mlp;
+}
+
+void sfgs_mlp_release(sfgs_ctx *ctx)
+{
+  Model *m = (Model*)ctx->mlp;
+  if (!m) return;
+  void *ptrs[] = {m->param, m->grad, m->m1, m->m2, m->wbf, m->dnorm, m->X, m->H1, m->H2, m->H3, m->dO, m->dZ, m->dZ2,
+                  m->Z1, m->Z2, m->Z3, m->O, m->Y, m->loss, m->tX, m->tY, m->idx};
+  for (void *p : ptrs) if (p) (void)hipFree(p);
+  delete m;
+  ctx->mlp = nullptr;
+}
+
+static int gemm(sfgs_ctx *ctx, const GemmArgs &g)
+{
+  dim3 grid((g.N + BN - 1)/BN, (g.M + BM - 1)/BM);
+  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, ctx->stream, g);
+  return 0;
+}
+
+static GemmArgs gargs(int M, int N, int K, const bf16 *A, int lda, int tA, const bf16 *B, int ldb, int tB, int epi)
+{
+  GemmArgs g;
+  std::memset(&g, 0, sizeof(g));
+  g.M = M; g.N = N; g.K = K; g.A = A; g.lda = lda; g.transA = tA; g.B = B; g.ldb = ldb; g.transB = tB; g.epi = epi;
+  return g;
+}
+
+static int ensure_batch(sfgs_ctx *ctx, Model *m, int B)
+{
+  if (B <= m->cap) return 0;
+  (void)hipStreamSynchronize(ctx->stream);
+  void **bufs[] = {(void**)&m->X, (void**)&m->H1, (void**)&m->H2, (void**)&m->H3, (void**)&m->dO, (void**)&m->dZ,
+                   (void**)&m->dZ2, (void**)&m->Z1, (void**)&m->Z2, (void**)&m->Z3, (void**)&m->O, (void**)&m->Y,
+                   (void**)&m->loss};
+  size_t sz[] = {sizeof(bf16)*NI, sizeof(bf16)*NH, sizeof(bf16)*NH, sizeof(bf16)*NH, sizeof(bf16)*NOP, sizeof(bf16)*NH,
+                 sizeof(bf16)*NH, sizeof(float)*NH, sizeof(float)*NH, sizeof(float)*NH, sizeof(float)*NOP,
+                 sizeof(float)*NO, sizeof(float)};
+  int cap = std::max(B, 1024);
+  for (int k = 0; k < 13; k ++)
+  {
+    if (*bufs[k]) (void)hipFree(*bufs[k]);
+    *bufs[k] = nullptr;
+    HIPCHK(hipMalloc(bufs[k], sz[k]*(size_t)cap));
+  }
+  m->cap = cap;
+  return 0;
+}
+
+// forward pass of B rows of m->X (bf16, B x 32) into m->O (FP32, B x 32)
+static int forward(sfgs_ctx *ctx, Model *m, int B)
+{
+  const bf16 *W1 = m->wbf + m->off[0], *W2 = m->wbf + m->off[2], *W3 = m->wbf + m->off[4], *W4 = m->wbf + m->off[6];
+  const float *b1 = m->param + m->off[1], *b2 = m->param + m->off[3], *b3 = m->param + m->off[5], *b4 = m->param + m->off[7];
+  GemmArgs g = gargs(B, NH, NI, m->X, NI, 0, W1, NH, 0, EPI_BIAS_GELU);
+  g.C = m->Z1; g.ldc = NH; g.H = m->H1; g.ldh = NH; g.bias = b1;
+  gemm(ctx, g);
+  g = gargs(B, NH, NH, m->H1, NH, 0, W2, NH, 0, EPI_BIAS_GELU);
+  g.C = m->Z2; g.ldc = NH; g.H = m->H2; g.ldh = NH; g.bias = b2;
+  gemm(ctx, g);
+  g = gargs(B, NH, NH, m->H2, NH, 0, W3, NH, 0, EPI_BIAS_GELU);
+  g.C = m->Z3; g.ldc = NH; g.H = m->H3; g.ldh = NH; g.bias = b3;
+  gemm(ctx, g);
+  g = gargs(B, NOP, NH, m->H3, NH, 0, W4, NOP, 0, EPI_BIAS_F32);
+  g.C = m->O; g.ldc = NOP; g.bias = b4;
+  gemm(ctx, g);
+  return 0;
+}
+
+// backward pass from m->dO (bf16) into m->grad (FP32); forward activations must be current
+static int backward(sfgs_ctx *ctx, Model *m, int B)
+{
+  const bf16 *W2 = m->wbf + m->off[2], *W3 = m->wbf + m->off[4], *W4 = m->wbf + m->off[6];
+  float *G = m->grad;
+  // layer 4
+  GemmArgs g = gargs(NH, NOP, B, m->H3, NH, 1, m->dO, NOP, 0, EPI_F32);     // dW4 = H3' dO
+  g.C = G + m->off[6]; g.ldc = NOP;
+  gemm(ctx, g);
+  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, ctx->stream, B, NOP, m->dO, G + m->off[7]);
+  g = gargs(B, NH, NOP, m->dO, NOP, 0, W4, NOP, 1, EPI_GELU_BWD);          // dZ3 = (dO W4') * gelu'(Z3)
+  g.H = m->dZ; g.ldh = NH; g.Z = m->Z3; g.ldz = NH;
+  gemm(ctx, g);
+  // layer 3
+  g = gargs(NH, NH, B, m->H2, NH, 1, m->dZ, NH, 0, EPI_F32);               // dW3 = H2' dZ3
+  g.C = G + m->off[4]; g.ldc = NH;
+  gemm(ctx, g);
+  hipLaunchKernelGGL(k_colsum, dim3(2), dim3(64), 0, ctx->stream, B, NH, m->dZ, G + m->off[5]);
+  g = gargs(B, NH, NH, m->dZ, NH, 0, W3, NH, 1, EPI_GELU_BWD);             // dZ2 = (dZ3 W3') * gelu'(Z2)
+  g.H = m->dZ2; g.ldh = NH; g.Z = m->Z2; g.ldz = NH;
+  gemm(ctx, g);
+  // layer 2
+  g = gargs(NH, NH, B, m->H1, NH, 1, m->dZ2, NH, 0, EPI_F32);              // dW2 = H1' dZ2
+  g.C = G + m->off[2]; g.ldc = NH;
+  gemm(ctx, g);
+  hipLaunchKernelGGL(k_colsum, dim3(2), dim3(64), 0, ctx->stream, B, NH, m->dZ2, G + m->off[3]);
+  g = gargs(B, NH, NH, m->dZ2, NH, 0, W2, NH, 1, EPI_GELU_BWD);            // dZ1 = (dZ2 W2') * gelu'(Z1)
+  g.H = m->dZ; g.ldh = NH; g.Z = m->Z1; g.ldz = NH;
+  gemm(ctx, g);
+  // layer 1
+  g = gargs(NI, NH, B, m->X, NI, 1, m->dZ, NH, 0, EPI_F32);                // dW1 = X' dZ1
+  g.C = G + m->off[0]; g.ldc = NH;
+  gemm(ctx, g);
+  hipLaunchKernelGGL(k_colsum, dim3(2), dim3(64), 0, ctx->stream, B, NH, m->dZ, G + m->off[1]);
+  return 0;
+}
+
+static int refresh_bf16(sfgs_ctx *ctx, Model *m)
+{
+  size_t n = m->total;
+  hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((n + 255)/256)), dim3(256), 0, ctx->stream, n, m->param, m->wbf);
+  return 0;
+}
+
+// apply the model to device features (n x 18, row order of R) and shift R by the prediction
+int sfgs_mlp_apply(sfgs_ctx *ctx, const float *feat, int n, double *R)
+{
+  Model *m = model_of(ctx);
+  if (!m->have) { ctx->err = "no MLP parameters"; return SFGS_E_STATE; }
+  if (n == 0) return 0;
+  if (ensure_batch(ctx, m, n)) return SFGS_E_HIP;
+  if (n > m->tcap)
+  {
+    if (m->idx) (void)hipFree(m->idx);
+    m->idx = nullptr;
+    if (m->tX) (void)hipFree(m->tX);
+    if (m->tY) (void)hipFree(m->tY);
+    m->tX = m->tY = nullptr;
+    HIPCHK(hipMalloc(&m->tX, sizeof(float)*NF*(size_t)n));
+    HIPCHK(hipMalloc(&m->tY, sizeof(float)*NO*(size_t)n));
+    HIPCHK(hipMalloc(&m->idx, sizeof(int)*(size_t)n));
+    m->tcap = n;
+  }
+  const float *dn = m->dnorm;
+  hipLaunchKernelGGL(k_iota, dim3((n + 255)/256), dim3(256), 0, ctx->stream, n, m->idx);
+  hipLaunchKernelGGL(k_gather, dim3((n + 255)/256), dim3(256), 0, ctx->stream, n, m->idx, feat, feat, dn, m->X, m->Y);
+  forward(ctx, m, n);
+  hipLaunchKernelGGL(k_apply, dim3((n + 255)/256), dim3(256), 0, ctx->stream, n, m->O, dn, R);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" {
+
+int SFGS_Mlp_Set(sfgs_ctx *ctx, const float *params, long long nparams, const float *norm)
+{
+  if (!ctx || !params || !norm) return SFGS_E_ARG;
+  Model *m = model_of(ctx);
+  if ((size_t)nparams != m->total) { ctx->err = "MLP parameter count must be " + std::to_string(m->total); return SFGS_E_ARG; }
+  HIPCHK(hipSetDevice(ctx->device));
+  if (!m->param)
+  {
+    HIPCHK(hipMalloc(&m->param, sizeof(float)*m->total));
+    HIPCHK(hipMalloc(&m->grad, sizeof(float)*m->total));
+    HIPCHK(hipMalloc(&m->m1, sizeof(float)*m->total));
+    HIPCHK(hipMalloc(&m->m2, sizeof(float)*m->total));
+    HIPCHK(hipMalloc(&m->wbf, sizeof(bf16)*m->total));
+    HIPCHK(hipMalloc(&m->dnorm, sizeof(m->norm)));
+  }
+  HIPCHK(hipMemcpyAsync(m->param, params, sizeof(float)*m->total, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(m->m1, 0, sizeof(float)*m->total, ctx->stream));
+  HIPCHK(hipMemsetAsync(m->m2, 0, sizeof(float)*m->total, ctx->stream));
+  std::memcpy(m->norm, norm, sizeof(m->norm));
+  for (int f = 0; f < NF; f ++) if (!(m->norm[NF + f] > 0.f)) m->norm[NF + f] = 1.f;
+  for (int c = 0; c < NO; c ++) if (!(m->norm[2*NF + c] > 0.f)) m->norm[2*NF + c] = 1.f;
+  HIPCHK(hipMemcpyAsync(m->dnorm, m->norm, sizeof(m->norm), hipMemcpyHostToDevice, ctx->stream));
+  refresh_bf16(ctx, m);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  m->have = 1;
+  m->step = 0;
+  return 0;
+}
+
+int SFGS_Mlp_Get(sfgs_ctx *ctx, float *params, long long *nparams)
+{
+  if (!ctx) return SFGS_E_ARG;
+  Model *m = model_of(ctx);
+  if (nparams) *nparams = (long long)m->total;
+  if (!params) return 0;
+  if (!m->have) { ctx->err = "no MLP parameters"; return SFGS_E_STATE; }
+  HIPCHK(hipMemcpyAsync(params, m->param, sizeof(float)*m->total, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+// forward pass on host features (n x 18, raw): out (n x 3) in target units (R - R_prev)
+int SFGS_Mlp_Forward(sfgs_ctx *ctx, const float *features, int n, float *out)
+{
+  if (!ctx || n < 0 || (n > 0 && (!features || !out))) return SFGS_E_ARG;
+  Model *m = model_of(ctx);
+  if (!m->have) { ctx->err = "no MLP parameters"; return SFGS_E_STATE; }
+  if (n == 0) return 0;
+  HIPCHK(hipSetDevice(ctx->device));
+  if (ensure_batch(ctx, m, n)) return SFGS_E_HIP;
+  float *tX = nullptr;
+  int *idx = nullptr;
+  const float *dn = m->dnorm;
+  HIPCHK(hipMalloc(&tX, sizeof(float)*NF*(size_t)n));
+  HIPCHK(hipMalloc(&idx, sizeof(int)*(size_t)n));
+  std::vector<int> hid((size_t)n);
+  for (int i = 0; i < n; i ++) hid[i] = i;
+  HIPCHK(hipMemcpyAsync(tX, features, sizeof(float)*NF*(size_t)n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(idx, hid.data(), sizeof(int)*(size_t)n, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_gather, dim3((n + 255)/256), dim3(256), 0, ctx->stream, n, idx, tX, tX, dn, m->X, m->Y);
+  forward(ctx, m, n);
+  std::vector<float> o((size_t)n*NOP);
+  HIPCHK(hipMemcpyAsync(o.data(), m->O, sizeof(float)*NOP*(size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipGetLastError());
+  (void)hipFree(tX); (void)hipFree(idx);
+  for (int i = 0; i < n; i ++)
+    for (int c = 0; c < NO; c ++) out[(size_t)i*NO + c] = o[(size_t)i*NOP + c] * m->norm[2*NF + c];
+  return 0;
+}
+
+// loss and gradient of one batch of host samples (features n x 18 raw, targets n x 3 in
+// target units): grad receives the FP32 gradient of the mean squared error in
+// normalised units with respect to every parameter (layout of SFGS_Mlp_Set)
+int SFGS_Mlp_Grad(sfgs_ctx *ctx, const float *features, const float *targets, int n, float *grad, float *loss)
+{
+  if (!ctx || n <= 0 || !features || !targets) return SFGS_E_ARG;
+  Model *m = model_of(ctx);
+  if (!m->have) { ctx->err = "no MLP parameters"; return SFGS_E_STATE; }
+  HIPCHK(hipSetDevice(ctx->device));
+  if (ensure_batch(ctx, m, n)) return SFGS_E_HIP;
+  float *tX = nullptr, *tY = nullptr;
+  int *idx = nullptr;
+  const float *dn = m->dnorm;
+  HIPCHK(hipMalloc(&tX, sizeof(float)*NF*(size_t)n));
+  HIPCHK(hipMalloc(&tY, sizeof(float)*NO*(size_t)n));
+  HIPCHK(hipMalloc(&idx, sizeof(int)*(size_t)n));
+  std::vector<int> hid((size_t)n);
+  for (int i = 0; i < n; i ++) hid[i] = i;
+  HIPCHK(hipMemcpyAsync(tX, features, sizeof(float)*NF*(size_t)n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(tY, targets, sizeof(float)*NO*(size_t)n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(idx, hid.data(), sizeof(int)*(size_t)n, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_gather, dim3((n + 255)/256), dim3(256), 0, ctx->stream, n, idx, tX, tY, dn, m->X, m->Y);
+  forward(ctx, m, n);
+  int lb = (n + 255)/256;
+  hipLaunchKernelGGL(k_loss, dim3(lb), dim3(256), 0, ctx->stream, n, m->O, m->Y, m->dO, m->loss);
+  backward(ctx, m, n);
+  std::vector<float> lp((size_t)lb);
+  HIPCHK(hipMemcpyAsync(lp.data(), m->loss, sizeof(float)*(size_t)lb, hipMemcpyDeviceToHost, ctx->stream));
+  if (grad) HIPCHK(hipMemcpyAsync(grad, m->grad, sizeof(float)*m->total, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipGetLastError());
+  (void)hipFree(tX); (void)hipFree(tY); (void)hipFree(idx);
+  if (loss)
+  {
+    double s = 0.0;
+    for (float v : lp) s += v;
+    *loss = (float)(s / ((double)n*NO));
+  }
+  return 0;
+}
+
+// Adam training on a device-resident copy of the samples: epochs over random
+// minibatches (seeded permutation on the host), loss per epoch into loss_hist
+int SFGS_Mlp_Train(sfgs_ctx *ctx, const float *features, const float *targets, long long n, int epochs, int batch,
+                   float lr, unsigned long long seed, float *loss_hist)
+{
+  if (!ctx || n <= 0 || epochs <= 0 || batch <= 0 || !features || !targets) return SFGS_E_ARG;
+  Model *m = model_of(ctx);
+  if (!m->have) { ctx->err = "no MLP parameters"; return SFGS_E_STATE; }
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (batch > n) batch = (int)n;
+  if (ensure_batch(ctx, m, batch)) return SFGS_E_HIP;
+  if (n > m->tcap)
+  {
+    if (m->tX) (void)hipFree(m->tX);
+    if (m->tY) (void)hipFree(m->tY);
+    if (m->idx) (void)hipFree(m->idx);
+    m->tX = m->tY = nullptr; m->idx = nullptr;
+    HIPCHK(hipMalloc(&m->tX, sizeof(float)*NF*(size_t)n));
+    HIPCHK(hipMalloc(&m->tY, sizeof(float)*NO*(size_t)n));
+    HIPCHK(hipMalloc(&m->idx, sizeof(int)*(size_t)n));
+    m->tcap = n;
+  }
+  const float *dn = m->dnorm;
+  HIPCHK(hipMemcpyAsync(m->tX, features, sizeof(float)*NF*(size_t)n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(m->tY, targets, sizeof(float)*NO*(size_t)n, hipMemcpyHostToDevice, st));
+  std::vector<int> perm((size_t)n);
+  unsigned long long s = seed;
+  const int lbmax = (batch + 255)/256;
+  std::vector<float> lp((size_t)lbmax);
+  const float b1 = 0.9f, b2 = 0.999f;
+  for (int e = 0; e < epochs; e ++)
+  {
+    for (long long i = 0; i < n; i ++) perm[i] = (int)i;
+    for (long long i = n - 1; i > 0; i --)   // Fisher-Yates with splitmix64
+    {
+      s += 0x9e3779b97f4a7c15ULL;
+      unsigned long long z = s;
+      z = (z ^ (z >> 30))*0xbf58476d1ce4e5b9ULL; z = (z ^ (z >> 27))*0x94d049bb133111ebULL; z ^= z >> 31;
+      long long j = (long long)(z % (unsigned long long)(i + 1));
+      std::swap(perm[i], perm[j]);
+    }
+    HIPCHK(hipMemcpyAsync(m->idx, perm.data(), sizeof(int)*(size_t)n, hipMemcpyHostToDevice, st));
+    double esum = 0.0;
+    long long cnt = 0;
+    for (long long b0 = 0; b0 + batch <= n; b0 += batch)
+    {
+      hipLaunchKernelGGL(k_gather, dim3((batch + 255)/256), dim3(256), 0, st, batch, m->idx + b0, m->tX, m->tY, dn,
+                         m->X, m->Y);
+      forward(ctx, m, batch);
+      hipLaunchKernelGGL(k_loss, dim3(lbmax), dim3(256), 0, st, batch, m->O, m->Y, m->dO, m->loss);
+      backward(ctx, m, batch);
+      m->step ++;
+      float c1 = 1.f - powf(b1, (float)m->step), c2 = 1.f - powf(b2, (float)m->step);
+      hipLaunchKernelGGL(k_adam, dim3((unsigned)((m->total + 255)/256)), dim3(256), 0, st, m->total, m->param, m->grad,
+                         m->m1, m->m2, lr, b1, b2, c1, c2, 1e-8f);
+      refresh_bf16(ctx, m);
+      HIPCHK(hipMemcpyAsync(lp.data(), m->loss, sizeof(float)*(size_t)lbmax, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      for (float v : lp) esum += v;
+      cnt += (long long)batch*NO;
+    }
+    if (loss_hist) loss_hist[e] = cnt ? (float)(esum / (double)cnt) : 0.f;
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+} // extern "C"

@@ -194,7 +194,10 @@ class GAUSS_SEIDEL_SOLVER:
     Read-only members: failure, error, iters, rerhist, merhist (and itershist).
     Read/write members: epsilon, maxiter, meritval, diagepsilon, diagmaxiter, diagsolver,
     reverse ('ON'/'OFF'), variant, innerloops, itershist (set 'ON'); and, for this
-    build, precision ('FP64'/'MIXED') and refine (FP64 refinement sweeps of MIXED).
+    build, precision ('FP64'/'MIXED'), refine (FP64 refinement sweeps of MIXED)
+    and warmstart ('PREVIOUS' = previous-step R as the reference, 'LEARNED' =
+    previous-step R corrected by the network loaded into the domain's device,
+    see solfec.learned).
     """
 
     _RO = ("failure", "error", "iters", "rerhist", "merhist")
@@ -280,6 +283,8 @@ class GAUSS_SEIDEL_SOLVER:
             return lib.GAUSS_SEIDEL_Variant(gs).decode()
         if name == "precision":
             return "MIXED" if int(self._get("precision")) == 1 else "FP64"
+        if name == "warmstart":
+            return "LEARNED" if int(self._get("warmstart")) == 1 else "PREVIOUS"
         raise AttributeError(name)
 
     def __setattr__(self, name, value):
@@ -318,6 +323,10 @@ class GAUSS_SEIDEL_SOLVER:
             if not value >= 0:
                 raise ValueError("'refine' must be >= 0")
             lib.GAUSS_SEIDEL_Set(gs, b"refine", float(value))
+        elif name == "warmstart":
+            if value not in ("PREVIOUS", "LEARNED"):
+                raise ValueError("Invalid warm start (PREVIOUS/LEARNED accepted)")
+            lib.GAUSS_SEIDEL_Set(gs, b"warmstart", 1.0 if value == "LEARNED" else 0.0)
         else:
             raise AttributeError(name)
 

@@ -49,6 +49,13 @@ _SFGS_PROTOS = {
     "SFGS_Colour": (_i, [_p, _i, _p, _p, _p]),
     "SFGS_Body_Update": (_i, [_p, _d, _p, _p]),
     "SFGS_Set_Layout": (_i, [_p, _i]),
+    "SFGS_Mlp_Set": (_i, [_p, _p, _ll, _p]),
+    "SFGS_Mlp_Get": (_i, [_p, _p, _p]),
+    "SFGS_Mlp_Forward": (_i, [_p, _p, _i, _p]),
+    "SFGS_Mlp_Grad": (_i, [_p, _p, _p, _i, _p, _p]),
+    "SFGS_Mlp_Train": (_i, [_p, _p, _p, _ll, _i, _i, C.c_float, C.c_ulonglong, _p]),
+    "SFGS_Features": (_i, [_p, _p]),
+    "SFGS_Warm_Start": (_i, [_p]),
 }
 
 _SOL_PROTOS = {
@@ -86,6 +93,8 @@ _SOL_PROTOS = {
     "SOLFEC_Get_Bodies": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "SOLFEC_Get_Contacts": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "SOLFEC_Set_RU": (_i, [_p, _p, _p]),
+    "SOLFEC_Capture": (_i, [_p, _p, _i]),
+    "SOLFEC_Captured": (_i, [_p]),
 }
 
 _cache = {}

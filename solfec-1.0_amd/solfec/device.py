@@ -149,6 +149,56 @@ class Device:
         nc = self._check(self.lib.SFGS_Colour(self.ctx, n, ptr(rowptr), ptr(col), ptr(out)), "SFGS_Colour")
         return nc, out[:n]
 
+    # ---- learned warm start --------------------------------------------
+    def features(self):
+        n = self.sizes()[0]
+        out = np.zeros(18 * max(n, 1), np.float32)
+        self._check(self.lib.SFGS_Features(self.ctx, ptr(out)), "SFGS_Features")
+        return out[:18 * n].reshape(n, 18)
+
+    def warm_start(self):
+        self._check(self.lib.SFGS_Warm_Start(self.ctx), "SFGS_Warm_Start")
+
+    def mlp_set(self, params, norm):
+        params = np.ascontiguousarray(params, np.float32)
+        norm = np.ascontiguousarray(norm, np.float32)
+        self._check(self.lib.SFGS_Mlp_Set(self.ctx, ptr(params), params.size, ptr(norm)), "SFGS_Mlp_Set")
+
+    def mlp_get(self):
+        nparam = C.c_longlong(0)
+        self._check(self.lib.SFGS_Mlp_Get(self.ctx, None, C.byref(nparam)), "SFGS_Mlp_Get")
+        out = np.zeros(nparam.value, np.float32)
+        self._check(self.lib.SFGS_Mlp_Get(self.ctx, ptr(out), None), "SFGS_Mlp_Get")
+        return out
+
+    def mlp_forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros((len(X), 3), np.float32)
+        self._check(self.lib.SFGS_Mlp_Forward(self.ctx, ptr(X), len(X), ptr(out)), "SFGS_Mlp_Forward")
+        return out
+
+    def mlp_grad(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        g = np.zeros(self.mlp_nparam(), np.float32)
+        loss = C.c_float(0.0)
+        self._check(self.lib.SFGS_Mlp_Grad(self.ctx, ptr(X), ptr(Y), len(X), ptr(g), C.byref(loss)), "SFGS_Mlp_Grad")
+        return loss.value, g
+
+    def mlp_nparam(self):
+        nparam = C.c_longlong(0)
+        self._check(self.lib.SFGS_Mlp_Get(self.ctx, None, C.byref(nparam)), "SFGS_Mlp_Get")
+        return nparam.value
+
+    def mlp_train(self, X, Y, epochs, batch, lr, seed=0):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        hist = np.zeros(epochs, np.float32)
+        rc = self.lib.SFGS_Mlp_Train(self.ctx, ptr(X), ptr(Y), len(X), int(epochs), int(batch), float(lr), int(seed),
+                                     ptr(hist))
+        self._check(rc, "SFGS_Mlp_Train")
+        return hist
+
     def body_update(self, step, nb, download=True):
         if not download:
             self._check(self.lib.SFGS_Body_Update(self.ctx, step, None, None), "SFGS_Body_Update")

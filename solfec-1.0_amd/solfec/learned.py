@@ -1,0 +1,146 @@
+"""Learned warm start for GAUSS_SEIDEL_SOLVER (no counterpart in the reference).
+
+A perceptron 18 -> 128 -> 128 -> 128 -> 3 (GELU) maps per-contact features
+(gap, free velocity B, W_ii, previous-step reaction R_prev, number of
+off-diagonal blocks, friction) to a correction of the warm start:
+R0 = R_prev + s * f(x).  The plain Gauss-Seidel then runs from R0, so a bad
+prediction costs iterations, never accuracy.  Forward, backward and the Adam
+steps run on the GPU in libsfgs.so (bf16 matrix-core kernels, FP32 master
+weights).  Training data come from the solver itself: each step of a config-1
+trajectory (1000 spheres, seeded lattice, friction drawn per trajectory)
+yields (features, converged R) for every contact.
+"""
+import os
+
+import numpy as np
+
+import solfec
+
+NF, NI, NH, NOP = 18, 32, 128, 32
+SHAPES = [(NI, NH), (NH,), (NH, NH), (NH,), (NH, NH), (NH,), (NH, NOP), (NOP,)]
+NPARAM = sum(int(np.prod(s)) for s in SHAPES)
+DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models", "warmstart_mlp.npz")
+
+
+def init_params(seed=0):
+    """He-scaled normal weights, zero biases; padded inputs / outputs are zero."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for k, s in enumerate(SHAPES):
+        if len(s) == 1:
+            parts.append(np.zeros(s, np.float32))
+            continue
+        fan_in = NF if k == 0 else s[0]
+        w = (rng.normal(size=s) * np.sqrt(2.0 / fan_in)).astype(np.float32)
+        if k == 0:
+            w[NF:, :] = 0.0
+        if k == 6:
+            w[:, 3:] = 0.0
+            w *= 0.1            # start close to "use R_prev"
+        parts.append(w)
+    return np.concatenate([p.ravel() for p in parts])
+
+
+def unpack(params):
+    out, o = [], 0
+    for s in SHAPES:
+        size = int(np.prod(s))
+        out.append(params[o:o + size].reshape(s))
+        o += size
+    return out
+
+
+def norm_from_data(X, Y):
+    mean = X.mean(axis=0)
+    std = X.std(axis=0)
+    std[std < 1e-12] = 1.0
+    yscale = np.sqrt((Y ** 2).mean(axis=0))
+    yscale[yscale < 1e-12] = 1.0
+    return np.concatenate([mean, std, yscale]).astype(np.float32)
+
+
+def snapshots(seed, mu, spheres=1000, steps=200, L=1.0, epsilon=1e-4, maxiter=1000, every=1):
+    """Run one config-1 trajectory; return features (n x 18) and targets R - R_prev (n x 3)."""
+    s = solfec.SOLFEC("DYNAMIC", 1e-3, "/tmp/solfec_snap")
+    solfec.SURFACE_MATERIAL(s, friction=mu, restitution=0.0)
+    solfec.GRAVITY(s, (0.0, 0.0, -9.81))
+    solfec.LATTICE(s, spheres, L, 0.045, 0.055, 0.005, 2000.0, seed)
+    gs = solfec.GAUSS_SEIDEL_SOLVER(epsilon, maxiter)
+    cap = 16 * spheres + 64
+    buf = np.zeros(18 * cap, np.float32)
+    s.lib.SOLFEC_Capture(s.sol, buf.ctypes.data_as(solfec._lib.C.c_void_p), cap)
+    Xs, Ys = [], []
+    for k in range(steps):
+        rc = s.step_begin() or s.step_solve(gs)
+        if rc:
+            raise RuntimeError(s.lib.SOLFEC_Last_Error(s.sol).decode())
+        n = s.lib.SOLFEC_Captured(s.sol)
+        if n and k % every == 0:
+            X = buf[:18 * n].reshape(n, 18).copy()
+            R = s.contacts()["R"].reshape(n, 3)
+            Xs.append(X)
+            Ys.append((R - X[:, 13:16]).astype(np.float32))
+        s.step_end()
+    s.lib.SOLFEC_Capture(s.sol, None, 0)
+    return np.concatenate(Xs), np.concatenate(Ys)
+
+
+def dataset(seeds, mu_range=(0.1, 0.8), rng_seed=0, **kw):
+    rng = np.random.default_rng(rng_seed)
+    Xs, Ys = [], []
+    for sd in seeds:
+        X, Y = snapshots(int(sd), float(rng.uniform(*mu_range)), **kw)
+        Xs.append(X)
+        Ys.append(Y)
+    return np.concatenate(Xs), np.concatenate(Ys)
+
+
+def train(dev, X, Y, epochs=8, batch=4096, lr=2e-3, seed=0, params=None):
+    """Train on the GPU; returns (params, norm, loss history)."""
+    norm = norm_from_data(X, Y)
+    p0 = init_params(seed) if params is None else params
+    dev.mlp_set(p0, norm)
+    hist = dev.mlp_train(X, Y, epochs, batch, lr, seed)
+    return dev.mlp_get(), norm, hist
+
+
+def save(path, params, norm, meta=None):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez(path, params=params.astype(np.float32), norm=norm.astype(np.float32),
+             meta=np.array(repr(meta or {})))
+
+
+def load(path=DEFAULT_MODEL):
+    z = np.load(path, allow_pickle=False)
+    return z["params"], z["norm"]
+
+
+def install(dev_or_solfec, path=DEFAULT_MODEL, params=None, norm=None):
+    """Load a model into a Device or into the device context of a SOLFEC domain."""
+    if params is None:
+        params, norm = load(path)
+    if isinstance(dev_or_solfec, solfec.SOLFEC):
+        s = dev_or_solfec
+        ctx = s.lib.SOLFEC_Gpu(s.sol)
+        if not ctx:
+            raise solfec.SfgsError("no GPU context")
+        d = solfec.Device.__new__(solfec.Device)
+        d.lib, d.ctx = solfec._lib.sfgs(), ctx
+        d.mlp_set(params, norm)
+        d.ctx = None          # owned by the domain
+        return
+    dev_or_solfec.mlp_set(params, norm)
+
+
+def compare_iterations(dev, b, c, step=1e-3, epsilon=1e-6, maxiter=5000, meritval=1.0, precision=0):
+    """Solve the same system from R_prev and from the learned R0; return both reports."""
+    dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
+    dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
+                 c["restitution"], c["cohesion"], c["R"], c["U"])
+    par = solfec.make_param(epsilon, maxiter, meritval=meritval, precision=precision)
+    dev.assemble(step)
+    base, _, _ = dev.solve(par, history=False)
+    dev.assemble(step)
+    dev.warm_start()
+    learned, _, _ = dev.solve(par, history=False)
+    return base, learned

@@ -160,7 +160,7 @@ int SFGS_Body_Update(sfgs_ctx *ctx, double step, double *velo_out, double *force
  * R_prev (3), number of off-diagonal blocks, friction.  Output: R - R_prev in
  * units of the output scale.  Parameters are FP32, layout per layer [in][out]:
  * W1 32x128, b1 128, W2 128x128, b2 128, W3 128x128, b3 128, W4 128x32, b4 32
- * (35,872 values); norm = 18 feature means, 18 feature scales, 3 output scales. */
+ * (41,376 values); norm = 18 feature means, 18 feature scales, 3 output scales. */
 int SFGS_Mlp_Set(sfgs_ctx *ctx, const float *params, long long nparams, const float *norm);
 int SFGS_Mlp_Get(sfgs_ctx *ctx, float *params, long long *nparams);
 /* prediction for host features (n x 18): out n x 3, units of R */
@@ -175,6 +175,47 @@ int SFGS_Features(sfgs_ctx *ctx, float *features);
 /* replace the carried R of the current system by R_prev + MLP(features) (call
  * after SFGS_Locdyn_Update_Begin, before SFGS_Solve) */
 int SFGS_Warm_Start(sfgs_ctx *ctx);
+
+/* ---- batched solve of independent scenes (no counterpart in the reference,
+ * which solves one domain per GAUSS_SEIDEL_Solve call, bgs.c:913-1069) ----
+ * The uploaded bodies are split into scenes by a scene id per body; contacts
+ * belong to their master body's scene and may not couple rigid bodies of two
+ * scenes.  All scenes are solved by one persistent kernel launch: each scene
+ * owns a set of workgroups that sweep its colours and synchronise among
+ * themselves only, and each scene runs its own termination test (bgs.c:1036),
+ * so a scene that converges stops while the others go on.  MIXED precision
+ * runs the FP32 sweeps, the refinement sweeps and the final FP64 merit inside
+ * the same launch.  Call SFGS_Batch_Scenes after SFGS_Bodies_Upload and before
+ * SFGS_Locdyn_Update_Begin (a body upload resets the split); SFGS_Solve
+ * refuses a split system. */
+typedef struct {
+  int lanes_per_row;    /* layout the system was assembled with (SFGS_Set_Layout): 1 or 2 */
+  int waves_per_group;  /* wavefronts per workgroup: 1, 2, 4 or 8 */
+  int rows_per_group;   /* target rows of a scene per workgroup; sets each scene's workgroup count */
+} SFGS_BATCH_TUNE;
+
+typedef struct {
+  int rows;             /* contacts of the scene */
+  int groups;           /* workgroups that solved it */
+  int iters;            /* Gauss-Seidel iterations (MIXED: FP32 iterations + refinement sweeps) */
+  int error_code;       /* SFGS_GS_* */
+  int fail_rows;        /* local solver failures */
+  int aborted;          /* FAILURE_EXIT raised */
+  double error;         /* last relative error */
+  double merit;         /* last merit (0 with the merit off) */
+  double energy;        /* free energy 1/2 sum B'AB of the scene */
+} SFGS_SCENE_REPORT;
+
+int SFGS_Batch_Scenes(sfgs_ctx *ctx, int nscenes, const int *body_scene);
+/* solve every scene from its current R; tune == NULL uses the last autotuned
+ * (or default) shape; reports receives nscenes records, total (may be NULL)
+ * the launch totals (iters = largest scene count, solve_ms = device time) */
+int SFGS_Batch_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TUNE *tune, SFGS_SCENE_REPORT *reports,
+                     SFGS_REPORT *total);
+/* time trial_iters iterations of every candidate shape (lanes 1 and 2, which
+ * re-assembles the system, waves 1..8, rows per group 256..all) and keep the
+ * fastest; R and U are restored after each trial */
+int SFGS_Batch_Autotune(sfgs_ctx *ctx, const SFGS_PARAM *par, int trial_iters, SFGS_BATCH_TUNE *best, double *best_ms);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,650 @@
+// sfgs_batch.hip -- many independent contact systems ("scenes") solved by one
+// persistent kernel launch.
+//
+// The scenes share one device store: rows are ordered by (scene, colour,
+// -row length, i), so the rows of a scene form one contiguous range and each
+// (scene, colour) group one contiguous set of SELL-64 slices.  Scene s owns
+// workgroups [wg0, wg0 + nwg); they sweep the scene's colours in order, the
+// waves of the scene's workgroups taking the slices of a colour round robin,
+// and meet at a scene barrier after every colour.  A scene of one workgroup
+// synchronises with __syncthreads only; larger scenes use a counter in global
+// memory with agent-scope fences and a bounded wait (a wait that exceeds the
+// bound raises an abort flag that ends every wait, so the grid always drains).
+// Every workgroup of a scene forms the same sums in the same order, so all of
+// them reach the same termination decision without a broadcast.
+//
+// Per scene the kernel runs what SFGS_Solve runs for one system: the free
+// energy, Gauss-Seidel sweeps with the fused merit of the previous iterate and
+// the termination test of bgs.c:1016-1036 (as k_finish), then for MIXED the
+// conversion to FP64, the refinement sweeps and the final FP64 merit with the
+// update of U (mrf.c:61-76).  The sweep itself is sweep_slice, shared with the
+// per-colour kernels.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cfloat>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include "sfgs_ctx.hpp"
+#include "sfgs_math.hpp"
+#include "sfgs_sweep.hpp"
+#include "sfgs_host.hpp"
+
+using namespace sfgs;
+
+namespace {
+
+struct BatchScene {
+  int wg0, nwg;     // workgroups of the scene
+  int row0, row1;   // its rows in the colour-major order
+};
+
+struct BatchOut {
+  int iters, nref, sweeps, done, aborted, failcount, timeout, pad;
+  unsigned long long failkey;
+  double error, merit, energy;
+};
+
+// scene layout shared by the sweeps of both precisions
+struct BatchGeom {
+  const int *gslice;       // first slice of every (scene, colour) group, plus the end
+  const int *gptr;         // first row of every group, plus the end
+  int ncol;
+  int *abort_all;          // raised by a wait that exceeded spin_limit
+  unsigned spin_limit;
+};
+
+template <typename T>
+struct BatchArgs {
+  SweepArgs<T> a;          // sweep data of the first phase (P, Q set per sweep)
+  SweepArgs<double> d;     // FP64 sweep data (energy, refinement)
+  MeritArgs m;             // FP64 merit (R set at the end)
+  const BatchScene *scenes;
+  const int *wg_scene;     // scene of every workgroup
+  BatchGeom geo;
+  T *X0, *X1;              // first-phase R buffers
+  double *R64, *R64b;      // FP64 R buffers (R64 is the canonical store)
+  T *Ut, *Umt;             // first-phase U and merit-U
+  double *U64, *Um64;      // FP64 U and merit-U
+  BatchOut *out;
+  double *wpart;           // 2 parities x 4 doubles per workgroup
+  unsigned *bar;           // barrier counter per scene
+  int maxiter, nomerit, reverse, refine, mixed;
+  double eps, meritval;
+};
+
+// all workgroups of a scene meet; false once any wait of the launch timed out
+__device__ __forceinline__ bool scene_barrier(unsigned *bar, int nw, int *abort_all, unsigned spin_limit, int *flag)
+{
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    int ab = 0;
+    if (nw > 1)
+    {
+      __threadfence();
+      const unsigned arrived = atomicAdd(bar, 1u);
+      const unsigned target = (arrived / (unsigned)nw + 1u) * (unsigned)nw;
+      unsigned spins = 0;
+      while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+      {
+        if (++spins > spin_limit) { atomicExch(abort_all, 1); ab = 1; break; }
+        if ((spins & 255u) == 0 && __hip_atomic_load(abort_all, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        {
+          ab = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __threadfence();
+    }
+    *flag = ab;
+  }
+  __syncthreads();
+  return *flag == 0;
+}
+
+// FP64 free energy term B'AB of the rows of one slice (lane g = 0 of each row)
+template <int G>
+__device__ __forceinline__ double slice_energy(const SweepArgs<double> &d, int s)
+{
+  const int l = threadIdx.x & 63, r = l / G, g = l % G;
+  if (g != 0 || r >= d.nrows[s]) return 0.0;
+  const int p = d.row0[s] + r, n = d.n;
+  const double B0 = d.B[p], B1 = d.B[(size_t)n + p], B2 = d.B[2*(size_t)n + p];
+  double A[9];
+#pragma unroll
+  for (int e = 0; e < 9; e ++) A[e] = d.A[(size_t)e*n + p];
+  return B0*(A[0]*B0 + A[3]*B1 + A[6]*B2) + B1*(A[1]*B0 + A[4]*B1 + A[7]*B2) + B2*(A[2]*B0 + A[5]*B1 + A[8]*B2);
+}
+
+// row p of slice s handled by this lane (g = 0), or -1
+template <int G>
+__device__ __forceinline__ int slice_row(const int *row0, const int *nrows, int s)
+{
+  const int l = threadIdx.x & 63, r = l / G, g = l % G;
+  return (g == 0 && r < nrows[s]) ? row0[s] + r : -1;
+}
+
+// one sweep of a scene: its colours in order (reversed when back), a scene
+// barrier after each colour
+template <typename T, int G, int NW, bool MERIT>
+__device__ __forceinline__ bool batch_sweep(const BatchGeom &geo, const SweepArgs<T> &a, const BatchScene &sc, int g0,
+                                            int w, int nw, int back, int *sweepno, BatchOut *o, unsigned *bar,
+                                            int *flag, double &eu, double &el, double &up)
+{
+  const int wave = threadIdx.x >> 6;
+  for (int ci = 0; ci < geo.ncol; ci ++)
+  {
+    const int c = back ? geo.ncol - 1 - ci : ci, gi = g0 + c;
+    const int s0 = geo.gslice[gi], s1 = geo.gslice[gi + 1];
+    if (s0 == s1) continue;   // same decision in every workgroup of the scene
+    const int lo = back ? geo.gptr[gi + 1] : sc.row0, hi = back ? sc.row1 : geo.gptr[gi];
+    for (int s = s0 + w*NW + wave; s < s1; s += nw*NW)
+      sweep_slice<T, G, MERIT>(a, s, true, lo, hi, back, sweepno, &o->failkey, &o->failcount, &o->aborted, eu, el, up);
+    if (!scene_barrier(bar, nw, geo.abort_all, geo.spin_limit, flag)) return false;
+  }
+  return true;
+}
+
+__global__ void k_batch_to_f32(long long n, const double *src, float *dst)
+{
+  long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
+template <typename T, int G, int NW>
+__global__ __launch_bounds__(NW*64) void k_batch(BatchArgs<T> b)
+{
+  __shared__ double red[3][NW];
+  __shared__ double tot[3];
+  __shared__ int flag, sweepno;
+  const int wg = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sid = b.wg_scene[wg];
+  const BatchScene sc = b.scenes[sid];
+  const int w = wg - sc.wg0, nw = sc.nwg, g0 = sid*b.geo.ncol;
+  const int sl_lo = b.geo.gslice[g0], sl_hi = b.geo.gslice[g0 + b.geo.ncol];
+  BatchOut *o = b.out + sid;
+  unsigned *bar = b.bar + sid;
+  int parity = 0;
+
+  // deterministic scene-wide sums of three values (identical in every workgroup)
+  auto scene_sum = [&](double &v0, double &v1, double &v2) -> bool {
+    v0 = wave_sum(v0); v1 = wave_sum(v1); v2 = wave_sum(v2);
+    if (lane == 0) { red[0][wave] = v0; red[1][wave] = v1; red[2][wave] = v2; }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+      double t0 = red[0][0], t1 = red[1][0], t2 = red[2][0];
+      for (int k = 1; k < NW; k ++) { t0 += red[0][k]; t1 += red[1][k]; t2 += red[2][k]; }
+      if (nw == 1) { tot[0] = t0; tot[1] = t1; tot[2] = t2; }
+      else
+      {
+        double *pp = b.wpart + 4*(2*(size_t)wg + parity);
+        pp[0] = t0; pp[1] = t1; pp[2] = t2;
+      }
+    }
+    bool ok = true;
+    if (nw > 1)
+    {
+      ok = scene_barrier(bar, nw, b.geo.abort_all, b.geo.spin_limit, &flag);
+      if (threadIdx.x == 0)
+      {
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+        for (int q = 0; q < nw; q ++)
+        {
+          double *pp = b.wpart + 4*(2*(size_t)(sc.wg0 + q) + parity);
+          t0 += __hip_atomic_load(pp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          t1 += __hip_atomic_load(pp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          t2 += __hip_atomic_load(pp + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        tot[0] = t0; tot[1] = t1; tot[2] = t2;
+      }
+      parity ^= 1;
+    }
+    __syncthreads();
+    v0 = tot[0]; v1 = tot[1]; v2 = tot[2];
+    return ok;
+  };
+
+  // ---- free energy of the scene (ldy.c:729-736)
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+  for (int s = sl_lo + w*NW + wave; s < sl_hi; s += nw*NW) e0 += slice_energy<G>(b.d, s);
+  bool alive = scene_sum(e0, e1, e2);
+  const double energy = 0.5*e0, lo_energy = energy < DBL_EPSILON ? 1.0 : energy;
+
+  // ---- Gauss-Seidel sweeps in T with the merit of the previous iterate fused in
+  const int maxsweeps = b.nomerit ? b.maxiter : b.maxiter + 1;
+  int done = 0, iters = 0, final_buf = 0, final_U = 0, aborted = 0, nsw = 0;
+  double error = 0.0, merit = 0.0, prev_error = 0.0;
+  for (int k = 0; alive && k < maxsweeps; k ++)
+  {
+    if (threadIdx.x == 0) sweepno = k;
+    __syncthreads();
+    SweepArgs<T> a = b.a;
+    a.P = (k & 1) ? b.X1 : b.X0;
+    a.Q = (k & 1) ? b.X0 : b.X1;
+    const int back = b.reverse && (k & 1);
+    double eu = 0.0, el = 0.0, up = 0.0;
+    if (!b.nomerit && k > 0)
+      alive = batch_sweep<T, G, NW, true>(b.geo, a, sc, g0, w, nw, back, &sweepno, o, bar, &flag, eu, el, up);
+    else
+      alive = batch_sweep<T, G, NW, false>(b.geo, a, sc, g0, w, nw, back, &sweepno, o, bar, &flag, eu, el, up);
+    if (!alive) break;
+    nsw ++;
+    alive = scene_sum(eu, el, up);
+    if (!alive) break;
+    // termination (bgs.c:1016-1036), as k_finish
+    aborted = __hip_atomic_load(&o->aborted, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    const double error_k = sqrt(eu) / sqrt(el == 0.0 ? 1.0 : el);
+    if (b.nomerit)
+    {
+      iters = k + 1; error = error_k; merit = 0.0; final_buf = (k + 1) & 1; final_U = 0;
+      if (aborted || !(k + 1 < b.maxiter && error_k > b.eps)) done = 1;
+    }
+    else if (aborted)
+    {
+      iters = k + 1; final_buf = (k + 1) & 1; final_U = 0; done = 1;
+    }
+    else if (k == 0) prev_error = error_k;
+    else
+    {
+      merit = 0.5*up / lo_energy;
+      error = prev_error;
+      iters = k;
+      if (!(k < b.maxiter && (error > b.eps || merit > b.meritval))) { final_buf = k & 1; final_U = 1; done = 1; }
+      else prev_error = error_k;
+    }
+    if (done) break;
+  }
+  if (!done) { final_buf = maxsweeps & 1; final_U = 0; }
+
+  int nref = 0;
+  if (alive)
+  {
+    const T *Rf = final_buf ? b.X1 : b.X0;
+    const T *Uf = final_U ? b.Umt : b.Ut;
+    const int n = b.a.n;
+    if (b.mixed)
+    {
+      // first-phase state into the FP64 store
+      for (int s = sl_lo + w*NW + wave; s < sl_hi; s += nw*NW)
+      {
+        const int p = slice_row<G>(b.d.row0, b.d.nrows, s);
+        if (p < 0) continue;
+#pragma unroll
+        for (int k = 0; k < 3; k ++)
+        {
+          b.R64[4*(size_t)p + k] = (double)Rf[4*(size_t)p + k];
+          b.U64[(size_t)k*n + p] = (double)Uf[(size_t)k*n + p];
+        }
+        b.R64[4*(size_t)p + 3] = 0.0;
+      }
+      alive = scene_barrier(bar, nw, b.geo.abort_all, b.geo.spin_limit, &flag);
+      // FP64 refinement sweeps (no merit, no early stop)
+      for (int k = 0; alive && !aborted && k < b.refine; k ++)
+      {
+        SweepArgs<double> a = b.d;
+        a.P = (k & 1) ? b.R64b : b.R64;
+        a.Q = (k & 1) ? b.R64 : b.R64b;
+        if (threadIdx.x == 0) sweepno = iters + k;
+        __syncthreads();
+        double eu = 0.0, el = 0.0, up = 0.0;
+        alive = batch_sweep<double, G, NW, false>(b.geo, a, sc, g0, w, nw, 0, &sweepno, o, bar, &flag, eu, el, up);
+        if (!alive) break;
+        nsw ++;
+        alive = scene_sum(eu, el, up);
+        if (!alive) break;
+        error = sqrt(eu) / sqrt(el == 0.0 ? 1.0 : el);
+        nref = k + 1;
+      }
+      const double *Rfin = (nref & 1) ? b.R64b : b.R64;
+      if (alive && !b.nomerit)
+      {
+        // final FP64 merit with the update of U (mrf.c:61-76)
+        MeritArgs m = b.m;
+        m.R = Rfin;
+        double up = 0.0, z1 = 0.0, z2 = 0.0;
+        for (int s = sl_lo + w*NW + wave; s < sl_hi; s += nw*NW) merit_slice<G>(m, s, true, up);
+        alive = scene_sum(up, z1, z2);
+        merit = 0.5*up / lo_energy;
+      }
+      if (alive && (nref & 1))
+      {
+        // R64b -> R64 for own rows (the merit pass reads R64b only)
+        for (int s = sl_lo + w*NW + wave; alive && s < sl_hi; s += nw*NW)
+        {
+          const int p = slice_row<G>(b.d.row0, b.d.nrows, s);
+          if (p < 0) continue;
+#pragma unroll
+          for (int k = 0; k < 4; k ++) b.R64[4*(size_t)p + k] = b.R64b[4*(size_t)p + k];
+        }
+      }
+    }
+    else
+    {
+      // FP64 sweeps: the final state into the canonical buffers (own rows only)
+      for (int s = sl_lo + w*NW + wave; s < sl_hi; s += nw*NW)
+      {
+        const int p = slice_row<G>(b.d.row0, b.d.nrows, s);
+        if (p < 0) continue;
+        if (final_buf)
+#pragma unroll
+          for (int k = 0; k < 4; k ++) b.R64[4*(size_t)p + k] = (double)Rf[4*(size_t)p + k];
+        if (final_U)
+#pragma unroll
+          for (int k = 0; k < 3; k ++) b.U64[(size_t)k*n + p] = (double)Uf[(size_t)k*n + p];
+      }
+    }
+  }
+  if (w == 0 && threadIdx.x == 0)
+  {
+    o->iters = iters + nref;
+    o->nref = nref;
+    o->sweeps = nsw;
+    o->done = done;
+    o->error = error;
+    o->merit = merit;
+    o->energy = energy;
+    o->timeout = alive ? 0 : 1;
+  }
+}
+
+static int dev_cap(sfgs_ctx *ctx, const void *kern, int threads, int *cap)
+{
+  int nb = 0, cus = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, threads, 0));
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  *cap = (nb * cus * 3) / 4;   // margin: every workgroup of a launch must be resident at once
+  return 0;
+}
+
+template <typename T, int G, int NW>
+static int launch_batch(sfgs_ctx *ctx, const BatchArgs<T> &b, int nwg, int *cap_out)
+{
+  auto kern = k_batch<T, G, NW>;
+  if (cap_out) return dev_cap(ctx, (const void*)kern, NW*64, cap_out);
+  hipLaunchKernelGGL(kern, dim3(nwg), dim3(NW*64), 0, ctx->stream, b);
+  return 0;
+}
+
+template <typename T>
+static int dispatch_batch(sfgs_ctx *ctx, int G, int NW, const BatchArgs<T> &b, int nwg, int *cap_out)
+{
+#define SFGS_BATCH_NW(GG) \
+  switch (NW) { \
+  case 1: return launch_batch<T, GG, 1>(ctx, b, nwg, cap_out); \
+  case 2: return launch_batch<T, GG, 2>(ctx, b, nwg, cap_out); \
+  case 4: return launch_batch<T, GG, 4>(ctx, b, nwg, cap_out); \
+  default: return launch_batch<T, GG, 8>(ctx, b, nwg, cap_out); }
+  if (G == 2) { SFGS_BATCH_NW(2) }
+  SFGS_BATCH_NW(1)
+#undef SFGS_BATCH_NW
+}
+
+struct Tuned { SFGS_BATCH_TUNE t; int valid; };
+static Tuned g_default = {{1, 4, 2048}, 0};
+
+static bool tune_ok(const SFGS_BATCH_TUNE &t)
+{
+  return (t.lanes_per_row == 1 || t.lanes_per_row == 2) &&
+         (t.waves_per_group == 1 || t.waves_per_group == 2 || t.waves_per_group == 4 || t.waves_per_group == 8) &&
+         t.rows_per_group > 0;
+}
+
+} // namespace
+
+// one launch over all scenes (several if the scenes need more workgroups than
+// can be resident at once); ms receives the device time
+static int batch_solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TUNE &tn, SFGS_SCENE_REPORT *reps,
+                       SFGS_REPORT *total, float *ms_out)
+{
+  if (!ctx->assembled) { ctx->err = "solve before LOCDYN assembly"; return SFGS_E_STATE; }
+  if (tn.lanes_per_row != ctx->G_built)
+  {
+    ctx->err = "batch shape has " + std::to_string(tn.lanes_per_row) + " lanes per row, the system was assembled with " +
+               std::to_string(ctx->G_built);
+    return SFGS_E_STATE;
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int nsc = ctx->nscenes, ncol = ctx->ncolours, n = ctx->nc;
+  const bool mixed = par->precision == SFGS_PREC_MIXED;
+  const int G = tn.lanes_per_row, NW = tn.waves_per_group;
+  const int ng = nsc*ncol;
+  if ((int)ctx->h_colptr.size() != ng + 1 || (int)ctx->h_cslice.size() != ng + 1)
+  {
+    ctx->err = "layout tables do not match the scenes";
+    return SFGS_E_STATE;
+  }
+
+  // workgroups per scene
+  int cap = 0;
+  if (mixed) CHK((dispatch_batch<float>(ctx, G, NW, BatchArgs<float>(), 0, &cap)));
+  else CHK((dispatch_batch<double>(ctx, G, NW, BatchArgs<double>(), 0, &cap)));
+  if (cap < 1) { ctx->err = "batched kernel cannot be resident"; return SFGS_E_HIP; }
+  std::vector<int> rows((size_t)nsc), want((size_t)nsc);
+  for (int s = 0; s < nsc; s ++)
+  {
+    rows[s] = ctx->h_colptr[(size_t)(s + 1)*ncol] - ctx->h_colptr[(size_t)s*ncol];
+    const int slices = ctx->h_cslice[(size_t)(s + 1)*ncol] - ctx->h_cslice[(size_t)s*ncol];
+    long long w = rows[s] > 0 ? ((long long)rows[s] + tn.rows_per_group - 1) / tn.rows_per_group : 0;
+    w = std::min<long long>(w, std::max(1, (slices + NW - 1) / NW));   // no more workgroups than slice rounds
+    want[s] = (int)std::max<long long>(rows[s] > 0 ? 1 : 0, std::min<long long>(w, cap));
+  }
+
+  // FP32 mirrors and the first-phase R / U
+  if (mixed)
+  {
+    CHK(build_f32_mirror(ctx));
+    if (n > 0)
+    {
+      hipLaunchKernelGGL(k_batch_to_f32, dim3(nblk_ll(4LL*n)), dim3(TPB), 0, st, 4LL*n, ctx->R.p, ctx->R32.p);
+      hipLaunchKernelGGL(k_batch_to_f32, dim3(nblk_ll(3LL*n)), dim3(TPB), 0, st, 3LL*n, ctx->U.p, ctx->U32.p);
+    }
+  }
+  CHK(ensure(ctx, ctx->Rb, 4*(size_t)std::max(n, 1)));
+  CHK(upload(ctx, ctx->bgptr, ctx->h_colptr.data(), (size_t)ng + 1));
+  CHK(upload(ctx, ctx->bgslice, ctx->h_cslice.data(), (size_t)ng + 1));
+  CHK(ensure(ctx, ctx->bout, sizeof(BatchOut)*(size_t)std::max(nsc, 1)));
+  CHK(ensure(ctx, ctx->bbar, (size_t)std::max(nsc, 1)));
+  CHK(ensure(ctx, ctx->babort, 4));
+  HIPCHK(hipMemsetAsync(ctx->bout.p, 0, sizeof(BatchOut)*(size_t)std::max(nsc, 1), st));
+  HIPCHK(hipMemsetAsync(ctx->bbar.p, 0, sizeof(unsigned)*(size_t)std::max(nsc, 1), st));
+  HIPCHK(hipMemsetAsync(ctx->babort.p, 0, sizeof(int)*4, st));
+
+  SFGS_PARAM pd = *par;
+  BatchArgs<double> bd;
+  BatchArgs<float> bf;
+  auto fill = [&](auto &b) {
+    b.d = sweep_args<double>(ctx, &pd, ctx->W.p, ctx->Wd.p, ctx->B.p, ctx->rho.p, ctx->pfri.p, ctx->pcoh.p, ctx->shift.p,
+                             ctx->U.p, ctx->Um.p, par->diagepsilon);
+    b.m = merit_args(ctx, 1, nullptr);
+    b.geo.gslice = ctx->bgslice.p; b.geo.gptr = ctx->bgptr.p; b.geo.ncol = ncol;
+    b.geo.abort_all = ctx->babort.p; b.geo.spin_limit = 1u << 22;
+    b.R64 = ctx->R.p; b.R64b = ctx->Rb.p; b.U64 = ctx->U.p; b.Um64 = ctx->Um.p;
+    b.out = (BatchOut*)ctx->bout.p; b.bar = ctx->bbar.p;
+    b.maxiter = par->maxiter; b.nomerit = par->meritval >= 1.0 ? 1 : 0; b.reverse = par->reverse;
+    b.refine = mixed ? par->refine : 0; b.mixed = mixed ? 1 : 0;
+    b.eps = par->epsilon; b.meritval = par->meritval;
+  };
+  fill(bd);
+  fill(bf);
+  bd.a = bd.d;
+  bd.X0 = ctx->R.p; bd.X1 = ctx->Rb.p; bd.Ut = ctx->U.p; bd.Umt = ctx->Um.p;
+  if (mixed)
+  {
+    // FP32 resolves a relative change of about 1e-7: keep the local tolerance above it
+    bf.a = sweep_args<float>(ctx, &pd, ctx->W32.p, ctx->Wd32.p, ctx->B32.p, ctx->rho32.p, ctx->pfri32.p, ctx->pcoh32.p,
+                             ctx->shift32.p, ctx->U32.p, ctx->Um32.p, std::max(par->diagepsilon, 1e-6));
+    bf.X0 = ctx->R32.p; bf.X1 = ctx->R32b.p; bf.Ut = ctx->U32.p; bf.Umt = ctx->Um32.p;
+  }
+
+  // launches over chunks of scenes that fit the resident capacity
+  std::vector<int> wgs((size_t)nsc, 0);
+  HIPCHK(hipEventRecord(ctx->ev0, st));
+  int s = 0;
+  while (s < nsc)
+  {
+    std::vector<BatchScene> tab((size_t)nsc);
+    std::vector<int> wg_scene;
+    int used = 0, first = s;
+    for (; s < nsc; s ++)
+    {
+      if (want[s] == 0) { tab[s] = BatchScene{0, 0, 0, 0}; continue; }
+      if (used + want[s] > cap && used > 0) break;
+      const int nw = std::min(want[s], cap - used);
+      tab[s] = BatchScene{used, nw, ctx->h_colptr[(size_t)s*ncol], ctx->h_colptr[(size_t)(s + 1)*ncol]};
+      for (int q = 0; q < nw; q ++) wg_scene.push_back(s);
+      used += nw;
+      wgs[s] = nw;
+    }
+    if (used == 0) continue;
+    (void)first;
+    CHK(upload(ctx, ctx->bscene_tab, (const int*)tab.data(), tab.size()*sizeof(BatchScene)/sizeof(int)));
+    CHK(upload(ctx, ctx->bwg_scene, wg_scene.data(), wg_scene.size()));
+    CHK(ensure(ctx, ctx->bpart, 8*(size_t)used));
+    bd.scenes = bf.scenes = (const BatchScene*)ctx->bscene_tab.p;
+    bd.wg_scene = bf.wg_scene = ctx->bwg_scene.p;
+    bd.wpart = bf.wpart = ctx->bpart.p;
+    if (mixed) CHK(dispatch_batch<float>(ctx, G, NW, bf, used, nullptr));
+    else CHK(dispatch_batch<double>(ctx, G, NW, bd, used, nullptr));
+    // the tables are re-uploaded for the next chunk only after this launch has run
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(ctx->ev1, st));
+  std::vector<BatchOut> out((size_t)std::max(nsc, 1));
+  int timed_out = 0;
+  HIPCHK(hipMemcpyAsync(out.data(), ctx->bout.p, sizeof(BatchOut)*(size_t)nsc, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&timed_out, ctx->babort.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (ms_out) *ms_out = ms;
+  if (timed_out) { ctx->err = "a scene barrier of the batched kernel timed out"; return SFGS_E_STATE; }
+
+  int maxit = 0;
+  long long updates = 0;
+  for (int q = 0; q < nsc; q ++)
+  {
+    const BatchOut &b = out[q];
+    if (reps)
+    {
+      SFGS_SCENE_REPORT &r = reps[q];
+      std::memset(&r, 0, sizeof(r));
+      r.rows = rows[q];
+      r.groups = wgs[q];
+      r.iters = b.iters;
+      r.error = b.error;
+      r.merit = b.merit;
+      r.energy = b.energy;
+      r.fail_rows = b.failcount;
+      r.aborted = b.aborted;
+      r.error_code = SFGS_GS_OK;
+      if (b.failkey) r.error_code = (int)(b.failkey & 3ULL);
+      if (rows[q] > 0 && (b.error > par->epsilon || (par->meritval < 1.0 && b.merit > par->meritval)))
+        r.error_code = SFGS_GS_DIVERGED;
+    }
+    maxit = std::max(maxit, b.iters);
+    updates += (long long)b.sweeps * rows[q];
+  }
+  if (total)
+  {
+    std::memset(total, 0, sizeof(*total));
+    total->iters = maxit;
+    total->ncolours = ncol;
+    total->solve_ms = ms;
+    total->row_updates = updates;
+    total->nnz = ctx->nnz;
+  }
+  return 0;
+}
+
+extern "C" {
+
+int SFGS_Batch_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TUNE *tune, SFGS_SCENE_REPORT *reports,
+                     SFGS_REPORT *total)
+{
+  if (!ctx || !par || par->maxiter <= 0 || !(par->epsilon > 0.0) || par->diagmaxiter <= 0) return SFGS_E_ARG;
+  if (par->diagsolver < 0 || par->diagsolver > 3 || par->failure < 0 || par->failure > 2) return SFGS_E_ARG;
+  if (par->precision != SFGS_PREC_FP64 && par->precision != SFGS_PREC_MIXED) return SFGS_E_ARG;
+  if (par->precision == SFGS_PREC_MIXED && par->refine < 0) return SFGS_E_ARG;
+  SFGS_BATCH_TUNE tn = tune ? *tune : g_default.t;
+  if (!tune) tn.lanes_per_row = ctx->G_built;
+  if (!tune_ok(tn)) return SFGS_E_ARG;
+  return batch_solve(ctx, par, tn, reports, total, nullptr);
+}
+
+int SFGS_Batch_Autotune(sfgs_ctx *ctx, const SFGS_PARAM *par, int trial_iters, SFGS_BATCH_TUNE *best, double *best_ms)
+{
+  if (!ctx || !par || trial_iters <= 0) return SFGS_E_ARG;
+  if (!ctx->assembled || !ctx->from_bodies) { ctx->err = "autotuning needs a system assembled from bodies"; return SFGS_E_STATE; }
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int n = ctx->nc;
+  // the trials run a fixed number of iterations from the current R and U, restored after each
+  DBuf<double> R0, U0;
+  CHK(ensure(ctx, R0, 4*(size_t)std::max(n, 1)));
+  CHK(ensure(ctx, U0, 3*(size_t)std::max(n, 1)));
+  if (n > 0)
+  {
+    HIPCHK(hipMemcpyAsync(R0.p, ctx->R.p, sizeof(double)*4*(size_t)n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(U0.p, ctx->U.p, sizeof(double)*3*(size_t)n, hipMemcpyDeviceToDevice, st));
+  }
+  SFGS_PARAM tp = *par;
+  tp.maxiter = trial_iters;
+  tp.epsilon = 1e-300;        // no early stop: every shape runs the same iterations
+  tp.meritval = par->meritval < 1.0 ? 1e-300 : par->meritval;
+  tp.failure = SFGS_FAILURE_CONTINUE;
+  const int G0 = ctx->G;
+  SFGS_BATCH_TUNE bt = g_default.t;
+  double bms = 1e300;
+  int rc = 0;
+  const int lanes[2] = {1, 2}, waves[4] = {1, 2, 4, 8}, rpg[4] = {256, 1024, 4096, 1 << 30};
+  for (int li = 0; li < 2 && !rc; li ++)
+  {
+    if (ctx->G_built != lanes[li])
+    {
+      ctx->G = lanes[li];
+      if ((rc = reassemble(ctx))) break;
+    }
+    for (int wi = 0; wi < 4 && !rc; wi ++)
+      for (int ri = 0; ri < 4 && !rc; ri ++)
+      {
+        SFGS_BATCH_TUNE t{lanes[li], waves[wi], rpg[ri]};
+        if (n > 0)
+        {
+          HIPCHK(hipMemcpyAsync(ctx->R.p, R0.p, sizeof(double)*4*(size_t)n, hipMemcpyDeviceToDevice, st));
+          HIPCHK(hipMemcpyAsync(ctx->U.p, U0.p, sizeof(double)*3*(size_t)n, hipMemcpyDeviceToDevice, st));
+        }
+        float ms = 0.0f;
+        if ((rc = batch_solve(ctx, &tp, t, nullptr, nullptr, &ms))) break;
+        if (ms < bms) { bms = ms; bt = t; }
+      }
+  }
+  // keep the best layout and restore the state
+  if (!rc && ctx->G_built != bt.lanes_per_row)
+  {
+    ctx->G = bt.lanes_per_row;
+    rc = reassemble(ctx);
+  }
+  if (rc) ctx->G = G0;
+  if (!rc && n > 0)
+  {
+    HIPCHK(hipMemcpyAsync(ctx->R.p, R0.p, sizeof(double)*4*(size_t)n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->U.p, U0.p, sizeof(double)*3*(size_t)n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  release(R0);
+  release(U0);
+  if (rc) return rc;
+  g_default.t = bt;
+  g_default.valid = 1;
+  if (best) *best = bt;
+  if (best_ms) *best_ms = bms;
+  return 0;
+}
+
+} // extern "C"

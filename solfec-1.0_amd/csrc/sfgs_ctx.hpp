@@ -99,4 +99,15 @@ struct sfgs_ctx {
   sfgs::DBuf<double> body_out;            // body update results (velo, force)
   sfgs::DBuf<float> feat;                 // learned warm start: per-row features (18 per row)
   void *mlp = nullptr;                    // learned warm start model (sfgs_mlp.hip)
+
+  // ---- independent scenes (batched solve, sfgs_batch.hip).  With nscenes > 1
+  // rows are ordered by (scene, colour, -row length, i) and h_colptr / h_cslice
+  // hold rows / slices per group g = scene*ncolours + colour.
+  int nscenes = 1;
+  sfgs::DBuf<int> body_scene, row_scene;
+  sfgs::DBuf<int> bscene_tab, bwg_scene, bgptr, bgslice;
+  sfgs::DBuf<unsigned> bbar;
+  sfgs::DBuf<double> bpart;
+  sfgs::DBuf<unsigned char> bout;         // per-scene result records
+  sfgs::DBuf<int> babort;
 };

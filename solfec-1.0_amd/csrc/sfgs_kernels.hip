@@ -23,50 +23,10 @@
 #include "sfgs_ctx.hpp"
 #include "sfgs_math.hpp"
 #include "sfgs_prims.hpp"
+#include "sfgs_sweep.hpp"
+#include "sfgs_host.hpp"
 
 using namespace sfgs;
-
-#define HIPCHK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) { \
-    ctx->err = std::string(#call) + ": " + hipGetErrorString(e_); return SFGS_E_HIP; } } while (0)
-#define CHK(call) do { int r_ = (call); if (r_) return r_; } while (0)
-
-static const int TPB = 256;          // threads per block (4 wavefronts)
-static const int SLICE = 64;         // rows per slice (one wavefront)
-
-template <typename T>
-static int ensure(sfgs_ctx *ctx, DBuf<T> &b, size_t n)
-{
-  if (n == 0) n = 1;
-  if (b.cap < n)
-  {
-    if (b.p) { (void)hipStreamSynchronize(ctx->stream); (void)hipFree(b.p); }
-    b.p = nullptr; b.cap = 0;
-    size_t cap = n + n/4 + 16;
-    hipError_t e = hipMalloc(&b.p, cap * sizeof(T));
-    if (e != hipSuccess)
-    {
-      b.p = nullptr;
-      ctx->err = std::string("hipMalloc: ") + hipGetErrorString(e);
-      return SFGS_E_HIP;
-    }
-    b.cap = cap;
-  }
-  b.n = n;
-  return 0;
-}
-
-template <typename T>
-static void release(DBuf<T> &b) { if (b.p) (void)hipFree(b.p); b.p = nullptr; b.cap = b.n = 0; }
-
-template <typename T>
-static int upload(sfgs_ctx *ctx, DBuf<T> &b, const T *h, size_t n)
-{
-  CHK(ensure(ctx, b, n));
-  if (n && h) HIPCHK(hipMemcpyAsync(b.p, h, n*sizeof(T), hipMemcpyHostToDevice, ctx->stream));
-  return 0;
-}
-
-static inline int nblk(long long n, int t = TPB) { return (int)((n + t - 1) / t); }
 
 // =====================================================================
 // device helpers
@@ -178,14 +138,6 @@ __device__ int inverse3(const double *W, double *A)
     A[3*c] = x[0]; A[3*c + 1] = x[1]; A[3*c + 2] = x[2];
   }
   return 0;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v)
-{
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
 }
 
 // deterministic block sum of two values; thread 0 returns them
@@ -300,13 +252,34 @@ __global__ void k_fill_int(int n, int v, int *b)
   if (i < n) b[i] = v;
 }
 
-__global__ void k_sort_keys(int n, const int *colour, const int *rowptr, unsigned long long *keys, int *vals)
+// sort key of row i: (scene, colour, -row length); scene = 0 without scenes
+__global__ void k_sort_keys(int n, const int *colour, const int *rowptr, const int *scene, unsigned long long *keys,
+                            int *vals)
 {
   int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int len = rowptr[i + 1] - rowptr[i];
-  keys[i] = ((unsigned long long)(unsigned)colour[i] << 32) | (unsigned)(0x7fffffff - len);
+  unsigned len = (unsigned)min(rowptr[i + 1] - rowptr[i], 0xffffff);
+  unsigned long long sc = scene ? (unsigned long long)(unsigned)scene[i] : 0ULL;
+  keys[i] = (sc << 32) | ((unsigned long long)(unsigned)colour[i] << 24) | (0xffffffu - len);
   vals[i] = i;
+}
+
+// scene of each contact (its master body's); a coupling slave of another scene is an error
+__global__ void k_row_scene(int n, const int *master, const int *slave, const int *kind, const int *body_scene,
+                            int *scene, int *bad)
+{
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int sc = body_scene[master[i]], s = slave[i];
+  scene[i] = sc;
+  if (s >= 0 && kind[s] == 0 && body_scene[s] != sc) atomicAdd(bad, 1);
+}
+
+// rows per (scene, colour) group
+__global__ void k_group_count(int n, const int *colour, const int *scene, int ncolours, int *count)
+{
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&count[scene[i]*ncolours + colour[i]], 1);
 }
 
 __global__ void k_iperm(int n, const int *perm, int *iperm)
@@ -542,203 +515,15 @@ __global__ __launch_bounds__(TPB) void k_offdiag_sys(SysIn si, SellOut so, const
   }
 }
 
-// =====================================================================
-// Gauss-Seidel sweep of one colour (bgs.c:950-1019) with the merit of the
-// previous iteration fused in (mrf.c:35-184)
-//
-// R lives in two buffers.  Sweep k reads P (R at the end of sweep k-1, all
-// rows) and writes Q.  Row i gathers Q_j for neighbours already updated in
-// this sweep (permuted index in [upd_lo, upd_hi)) and P_j otherwise, which is
-// the serial colour-major Gauss-Seidel order.  When the merit is on, sweep
-// k >= 1 also forms U_i = B_i + W_ii P_i + sum_j W_ij P_j, i.e. the merit
-// update of U after iteration k-1 (mrf.c:61-76), accumulates C'AC for it, keeps
-// that U in Um and starts the local solve from it, exactly as the reference's
-// next sweep does after MERIT_Function.  The termination test of iteration k-1
-// therefore completes after sweep k; when it stops, P and Um are the state.
-// G lanes cooperate on one row (blocks k = g, g+G, ...), reduce B' across the
-// group with cross-lane adds, and run the local solve redundantly; lane g = 0
-// writes.
-// =====================================================================
-
-template <typename T>
-struct SweepArgs {
-  const int *row0, *nrows, *rowlen;
-  const long long *off;
-  const T *W;
-  const int *wcol;
-  const T *Wd, *B, *rho, *fri, *coh, *shift;
-  const double *A;      // merit weights stay FP64
-  const unsigned char *sep;
-  const T *P;
-  T *Q, *U, *Um;
-  int n;
-  T diageps;
-  int diagmaxiter, diagsolver, failure;
-  double *partials;     // 3 per block: errup, errlo, merit numerator
-  unsigned long long *failkey;
-  int *failcount;
-  Ctrl *ctrl;
-};
-
-template <int G, typename T>
-__device__ __forceinline__ T gsum(T v)
-{
-  if (G >= 2) v += __shfl_xor(v, 1, 64);
-  if (G >= 4) v += __shfl_xor(v, 2, 64);
-  if (G >= 8) v += __shfl_xor(v, 4, 64);
-  if (G >= 16) v += __shfl_xor(v, 8, 64);
-  return v;
-}
-
-// deterministic block sum of three values; thread 0 returns them
-__device__ __forceinline__ void block_sum3(double &a, double &b, double &c)
-{
-  __shared__ double sa[TPB/64], sb[TPB/64], sc[TPB/64];
-  a = wave_sum(a); b = wave_sum(b); c = wave_sum(c);
-  int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) { sa[w] = a; sb[w] = b; sc[w] = c; }
-  __syncthreads();
-  if (threadIdx.x == 0)
-  {
-    a = sa[0]; b = sb[0]; c = sc[0];
-    for (int k = 1; k < TPB/64; k ++) { a += sa[k]; b += sb[k]; c += sc[k]; }
-  }
-}
-
 template <typename T, int G, bool MERIT>
 __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<T> a, int s0, int s1, int pbase, int upd_lo, int upd_hi,
                                                 int backward)
 {
   if (a.ctrl->done) return;   // uniform: set only by k_finish between launches
-  const int l = threadIdx.x & 63, r = l / G, g = l % G;
   const int s = s0 + blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6);
-  const int n = a.n;
-  bool act = false;
-  int p = 0, len = 0;
-  long long off = 0;
-  if (s < s1)
-  {
-    act = r < a.nrows[s];
-    off = a.off[s];
-    if (act) { p = a.row0[s] + r; len = a.rowlen[p]; }
-  }
-  // the row's own data first, so that its loads overlap the gather below
-  T Bi[3] = {T(0), T(0), T(0)}, Wi[9], R[3] = {T(0), T(0), T(0)}, U[3] = {T(0), T(0), T(0)};
-  double Ai[9];
-  RowLaw<T> L{T(0), T(0), T(0), T(0), false};
-  if (act)
-  {
-    Bi[0] = a.B[p]; Bi[1] = a.B[(size_t)n + p]; Bi[2] = a.B[2*(size_t)n + p];
-#pragma unroll
-    for (int e = 0; e < 9; e ++) Wi[e] = a.Wd[(size_t)e*n + p];
-#pragma unroll
-    for (int k = 0; k < 3; k ++) R[k] = a.P[4*(size_t)p + k];
-    if (MERIT)
-    {
-#pragma unroll
-      for (int e = 0; e < 9; e ++) Ai[e] = a.A[(size_t)e*n + p];
-    }
-    else
-    {
-#pragma unroll
-      for (int k = 0; k < 3; k ++) U[k] = a.U[(size_t)k*n + p];
-    }
-    L = RowLaw<T>{a.fri[p], a.coh[p], a.shift[p], a.rho[p], a.sep[p] != 0};
-  }
-  T b0 = T(0), b1 = T(0), b2 = T(0), u0 = T(0), u1 = T(0), u2 = T(0);
-  if (act)
-  {
-    const T *Wb = a.W + 9*off + l;
-    const int *cb = a.wcol + off + l;
-#pragma unroll 4
-    for (int k = g, m = 0; k < len; k += G, m ++)
-    {
-      const int j = cb[(size_t)m*SLICE];
-      const T *w = Wb + 9*(size_t)m*SLICE;
-      const T w0 = w[0], w1 = w[SLICE], w2 = w[2*SLICE], w3 = w[3*SLICE], w4 = w[4*SLICE],
-              w5 = w[5*SLICE], w6 = w[6*SLICE], w7 = w[7*SLICE], w8 = w[8*SLICE];
-      const bool upd = j >= upd_lo && j < upd_hi;
-      const T *xj = (upd ? a.Q : a.P) + 4*(size_t)j;
-      const T r0 = xj[0], r1 = xj[1], r2 = xj[2];
-      b0 += w0*r0 + w3*r1 + w6*r2;
-      b1 += w1*r0 + w4*r1 + w7*r2;
-      b2 += w2*r0 + w5*r1 + w8*r2;
-      if (MERIT)
-      {
-        T p0 = r0, p1 = r1, p2 = r2;
-        if (upd)
-        {
-          const T *pj = a.P + 4*(size_t)j;
-          p0 = pj[0]; p1 = pj[1]; p2 = pj[2];
-        }
-        u0 += w0*p0 + w3*p1 + w6*p2;
-        u1 += w1*p0 + w4*p1 + w7*p2;
-        u2 += w2*p0 + w5*p1 + w8*p2;
-      }
-    }
-  }
-  // sum over the G lanes of each row; every lane of the wave takes part
-  b0 = gsum<G>(b0); b1 = gsum<G>(b1); b2 = gsum<G>(b2);
-  if (MERIT) { u0 = gsum<G>(u0); u1 = gsum<G>(u1); u2 = gsum<G>(u2); }
-
   double eu = 0.0, el = 0.0, up = 0.0;
-  if (act)
-  {
-    T Bp[3] = {Bi[0] + b0, Bi[1] + b1, Bi[2] + b2};
-    const T R0[3] = {R[0], R[1], R[2]};
-    if (MERIT)
-    {
-      // merit of the previous iteration at this row (mrf.c:63-84)
-      mv_add(Bi, Wi, R, U);
-      U[0] += u0; U[1] += u1; U[2] += u2;
-      // the merit is evaluated in FP64 whatever the sweep type
-      double C[3], Ud[3] = {U[0], U[1], U[2]}, Rd[3] = {R[0], R[1], R[2]};
-      scf_residual<double>(L.fri, L.coh, L.shift, Ud, Rd, C, nullptr, nullptr);
-      const double q0 = Ai[0]*C[0] + Ai[3]*C[1] + Ai[6]*C[2];
-      const double q1 = Ai[1]*C[0] + Ai[4]*C[1] + Ai[7]*C[2];
-      const double q2 = Ai[2]*C[0] + Ai[5]*C[1] + Ai[8]*C[2];
-      up = q0*C[0] + q1*C[1] + q2*C[2];
-      if (g == 0)
-#pragma unroll
-        for (int k = 0; k < 3; k ++) a.Um[(size_t)k*n + p] = U[k];
-    }
-
-    int d = local_solve(a.diagsolver, L, a.diageps, a.diagmaxiter, Wi, Bp, U, R);
-    if (d >= a.diagmaxiter || d < 0)
-    {
-      if (g == 0)
-      {
-        unsigned code = d < 0 ? SFGS_GS_DIAGONAL_FAILED : SFGS_GS_DIAGONAL_DIVERGED;
-        unsigned long long pos = backward ? (unsigned long long)(n - 1 - p) : (unsigned long long)p;
-        atomicMax(a.failkey, ((unsigned long long)(a.ctrl->sweeps + 1) << 40) | (pos << 2) | code);
-        atomicAdd(a.failcount, 1);
-        if (a.failure == SFGS_FAILURE_EXIT) atomicExch(&a.ctrl->aborted, 1);
-      }
-      if (a.failure == SFGS_FAILURE_CONTINUE)
-      {
-        const int chain[4] = {SFGS_DS_SEMISMOOTH_NEWTON, SFGS_DS_PROJECTED_GRADIENT,
-                              SFGS_DS_DE_SAXCE_FENG, SFGS_DS_PROJECTED_NEWTON};
-        for (int c = 0; c < 4; c ++)
-        {
-          if (chain[c] == a.diagsolver) continue;
-          R[0] = R0[0]; R[1] = R0[1]; R[2] = R0[2];
-          d = local_solve(chain[c], L, a.diageps, a.diagmaxiter, Wi, Bp, U, R);
-          if (d < a.diagmaxiter && d >= 0) break;
-        }
-        if (d >= a.diagmaxiter || d < 0) { R[0] = R0[0]; R[1] = R0[1]; R[2] = R0[2]; }
-      }
-    }
-    if (g == 0)
-    {
-#pragma unroll
-      for (int k = 0; k < 3; k ++) { a.Q[4*(size_t)p + k] = R[k]; a.U[(size_t)k*n + p] = U[k]; }
-      a.Q[4*(size_t)p + 3] = T(0);
-      const double d0 = (double)R[0] - R0[0], d1 = (double)R[1] - R0[1], d2 = (double)R[2] - R0[2];
-      eu = d0*d0 + d1*d1 + d2*d2;
-      el = (double)R[0]*R[0] + (double)R[1]*R[1] + (double)R[2]*R[2];
-    }
-    else up = 0.0;
-  }
+  sweep_slice<T, G, MERIT>(a, s, s < s1, upd_lo, upd_hi, backward, &a.ctrl->sweeps, a.failkey, a.failcount,
+                           &a.ctrl->aborted, eu, el, up);
   block_sum3(eu, el, up);
   if (threadIdx.x == 0)
   {
@@ -747,84 +532,12 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<T> a, int s0, int s1, i
   }
 }
 
-// standalone merit (MERIT_Function with update_U, mrf.c:35-184) on the
-// canonical R buffer; one launch over all slices
-struct MeritArgs {
-  const int *row0, *nrows, *rowlen;
-  const long long *off;
-  const double *W;
-  const int *wcol;
-  const double *Wd, *A, *B, *fri, *coh, *shift;
-  const double *R;
-  double *U;
-  int n, update_U;
-  double *mpartials;
-  double *conmerit;   // may be null: per-row C'AC (colour-major)
-};
-
 template <int G>
 __global__ __launch_bounds__(TPB) void k_merit(MeritArgs a, int nslices)
 {
-  const int l = threadIdx.x & 63, r = l / G, g = l % G;
-  const int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6), n = a.n;
-  bool act = false;
-  int p = 0, len = 0;
-  long long off = 0;
-  if (s < nslices)
-  {
-    act = r < a.nrows[s];
-    off = a.off[s];
-    if (act) { p = a.row0[s] + r; len = a.update_U ? a.rowlen[p] : 0; }
-  }
-  double u0 = 0.0, u1 = 0.0, u2 = 0.0;
-  if (act)
-  {
-    const double *Wb = a.W + 9*off + l;
-    const int *cb = a.wcol + off + l;
-    for (int k = g, m = 0; k < len; k += G, m ++)
-    {
-      const int j = cb[(size_t)m*SLICE];
-      const double *w = Wb + 9*(size_t)m*SLICE;
-      const double r0 = a.R[4*(size_t)j], r1 = a.R[4*(size_t)j + 1], r2 = a.R[4*(size_t)j + 2];
-      u0 += w[0]*r0 + w[3*SLICE]*r1 + w[6*SLICE]*r2;
-      u1 += w[SLICE]*r0 + w[4*SLICE]*r1 + w[7*SLICE]*r2;
-      u2 += w[2*SLICE]*r0 + w[5*SLICE]*r1 + w[8*SLICE]*r2;
-    }
-  }
-  u0 = gsum<G>(u0); u1 = gsum<G>(u1); u2 = gsum<G>(u2);
+  const int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6);
   double up = 0.0, z1 = 0.0, z2 = 0.0;
-  if (act)
-  {
-    double R[3] = {a.R[4*(size_t)p], a.R[4*(size_t)p + 1], a.R[4*(size_t)p + 2]}, U[3];
-    if (a.update_U)
-    {
-      double Wi[9], Bv[3] = {a.B[p], a.B[(size_t)n + p], a.B[2*(size_t)n + p]};
-#pragma unroll
-      for (int e = 0; e < 9; e ++) Wi[e] = a.Wd[(size_t)e*n + p];
-      mv_add(Bv, Wi, R, U);
-      U[0] += u0; U[1] += u1; U[2] += u2;
-      if (g == 0)
-#pragma unroll
-        for (int k = 0; k < 3; k ++) a.U[(size_t)k*n + p] = U[k];
-    }
-    else
-    {
-#pragma unroll
-      for (int k = 0; k < 3; k ++) U[k] = a.U[(size_t)k*n + p];
-    }
-    double C[3], Ai[9];
-    scf_residual<double>(a.fri[p], a.coh[p], a.shift[p], U, R, C, nullptr, nullptr);
-#pragma unroll
-    for (int e = 0; e < 9; e ++) Ai[e] = a.A[(size_t)e*n + p];
-    const double q0 = Ai[0]*C[0] + Ai[3]*C[1] + Ai[6]*C[2];
-    const double q1 = Ai[1]*C[0] + Ai[4]*C[1] + Ai[7]*C[2];
-    const double q2 = Ai[2]*C[0] + Ai[5]*C[1] + Ai[8]*C[2];
-    if (g == 0)
-    {
-      up = q0*C[0] + q1*C[1] + q2*C[2];
-      if (a.conmerit) a.conmerit[p] = up;
-    }
-  }
+  merit_slice<G>(a, s, s < nslices, up);
   block_sum3(up, z1, z2);
   if (threadIdx.x == 0) a.mpartials[blockIdx.x] = up;
 }
@@ -1032,27 +745,47 @@ static int colour_graph(sfgs_ctx *ctx, int n)
 static int build_order(sfgs_ctx *ctx, int n)
 {
   const int G = ctx->G, RPS = SLICE / G;
+  const bool scenes = ctx->nscenes > 1;
   ctx->G_built = G;
   CHK(ensure(ctx, ctx->skeys, n));
   CHK(ensure(ctx, ctx->skeys2, n));
   CHK(ensure(ctx, ctx->svals, n));
   CHK(ensure(ctx, ctx->perm, n));
   CHK(ensure(ctx, ctx->iperm, n));
+  int keybits = 32;
+  if (scenes)
+  {
+    for (int v = ctx->nscenes - 1; v > 0; v >>= 1) keybits ++;
+    CHK(ensure(ctx, ctx->row_scene, n));
+    CHK(ensure(ctx, ctx->counter, 4));
+    HIPCHK(hipMemsetAsync(ctx->counter.p, 0, sizeof(int), ctx->stream));
+    int bad = 0;
+    if (n > 0)
+      hipLaunchKernelGGL(k_row_scene, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->master.p, ctx->slave.p,
+                         ctx->kind.p, ctx->body_scene.p, ctx->row_scene.p, ctx->counter.p);
+    HIPCHK(hipMemcpyAsync(&bad, ctx->counter.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (bad) { ctx->err = std::to_string(bad) + " contacts couple bodies of different scenes"; return SFGS_E_ARG; }
+  }
   if (n > 0)
   {
     hipLaunchKernelGGL(k_sort_keys, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->colour.p, ctx->rowptr.p,
-                       ctx->skeys.p, ctx->svals.p);
+                       scenes ? ctx->row_scene.p : (const int*)nullptr, ctx->skeys.p, ctx->svals.p);
     size_t bytes = 0;
-    HIPCHK(sfgs::sort_pairs_u64(nullptr, bytes, ctx->skeys.p, ctx->skeys2.p, ctx->svals.p, ctx->perm.p, n, 41, ctx->stream));
+    HIPCHK(sfgs::sort_pairs_u64(nullptr, bytes, ctx->skeys.p, ctx->skeys2.p, ctx->svals.p, ctx->perm.p, n, keybits, ctx->stream));
     CHK(ensure(ctx, ctx->scratch, bytes));
-    HIPCHK(sfgs::sort_pairs_u64(ctx->scratch.p, bytes, ctx->skeys.p, ctx->skeys2.p, ctx->svals.p, ctx->perm.p, n, 41, ctx->stream));
+    HIPCHK(sfgs::sort_pairs_u64(ctx->scratch.p, bytes, ctx->skeys.p, ctx->skeys2.p, ctx->svals.p, ctx->perm.p, n, keybits, ctx->stream));
     hipLaunchKernelGGL(k_iperm, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->perm.p, ctx->iperm.p);
   }
-  // rows per colour
-  int nc = ctx->ncolours;
+  // rows per colour, or per (scene, colour) group
+  const int ncol = ctx->ncolours, nc = scenes ? ctx->nscenes*ncol : ncol;
   CHK(ensure(ctx, ctx->counter, (size_t)std::max(nc, 4)));
   HIPCHK(hipMemsetAsync(ctx->counter.p, 0, sizeof(int)*(size_t)std::max(nc, 4), ctx->stream));
-  if (n > 0) hipLaunchKernelGGL(k_colour_count, dim3(nblk(n, 1024)), dim3(1024), 0, ctx->stream, n, ctx->colour.p, ctx->counter.p, nc);
+  if (n > 0 && !scenes)
+    hipLaunchKernelGGL(k_colour_count, dim3(nblk(n, 1024)), dim3(1024), 0, ctx->stream, n, ctx->colour.p, ctx->counter.p, nc);
+  if (n > 0 && scenes)
+    hipLaunchKernelGGL(k_group_count, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->colour.p, ctx->row_scene.p, ncol,
+                       ctx->counter.p);
   std::vector<int> cnt((size_t)std::max(nc, 1), 0);
   if (nc > 0) HIPCHK(hipMemcpyAsync(cnt.data(), ctx->counter.p, sizeof(int)*(size_t)nc, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1194,6 +927,9 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   release(ctx->rho32); release(ctx->pfri32); release(ctx->pcoh32); release(ctx->shift32);
   release(ctx->partials); release(ctx->mpartials); release(ctx->hist); release(ctx->failkey);
   release(ctx->failcount); release(ctx->ctrl);
+  release(ctx->body_scene); release(ctx->row_scene); release(ctx->bscene_tab); release(ctx->bwg_scene);
+  release(ctx->bgptr); release(ctx->bgslice); release(ctx->bbar); release(ctx->bpart); release(ctx->bout);
+  release(ctx->babort);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
@@ -1223,6 +959,28 @@ int SFGS_Bodies_Upload(sfgs_ctx *ctx, int nb, const int *kind, const double *rot
   CHK(upload(ctx, ctx->velo0, velo0, 6*(size_t)nb));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->have_bodies = 1;
+  ctx->assembled = 0;
+  ctx->nscenes = 1;     // scene ids refer to one body upload
+  return 0;
+}
+
+int SFGS_Batch_Scenes(sfgs_ctx *ctx, int nscenes, const int *body_scene)
+{
+  if (!ctx || nscenes < 1 || (nscenes > 1 && !body_scene)) return SFGS_E_ARG;
+  if (!ctx->have_bodies) { ctx->err = "bodies must be uploaded before their scenes"; return SFGS_E_STATE; }
+  if (nscenes > 1)
+  {
+    for (int b = 0; b < ctx->nb; b ++)
+      if (body_scene[b] < 0 || body_scene[b] >= nscenes)
+      {
+        ctx->err = "body " + std::to_string(b) + " has an invalid scene";
+        return SFGS_E_ARG;
+      }
+    HIPCHK(hipSetDevice(ctx->device));
+    CHK(upload(ctx, ctx->body_scene, body_scene, (size_t)ctx->nb));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  ctx->nscenes = nscenes;
   ctx->assembled = 0;
   return 0;
 }
@@ -1406,31 +1164,6 @@ int SFGS_System_Upload(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, 
 
 } // extern "C"
 
-template <typename T>
-static SweepArgs<T> sweep_args(sfgs_ctx *ctx, const SFGS_PARAM *par, const T *W, const T *Wd, const T *B,
-                               const T *rho, const T *fri, const T *coh, const T *shift, T *U, T *Um, double diageps)
-{
-  SweepArgs<T> a;
-  a.row0 = ctx->slice_row0.p; a.nrows = ctx->slice_nrows.p; a.rowlen = ctx->rowlen.p; a.off = ctx->slice_off.p;
-  a.W = W; a.wcol = ctx->wcol.p; a.Wd = Wd; a.A = ctx->A.p; a.B = B; a.rho = rho;
-  a.fri = fri; a.coh = coh; a.shift = shift; a.sep = ctx->sep.p;
-  a.P = nullptr; a.Q = nullptr; a.U = U; a.Um = Um; a.n = ctx->nc;
-  a.diageps = (T)diageps; a.diagmaxiter = par->diagmaxiter; a.diagsolver = par->diagsolver;
-  a.failure = par->failure; a.partials = ctx->partials.p; a.failkey = ctx->failkey.p; a.failcount = ctx->failcount.p;
-  a.ctrl = ctx->ctrl.p;
-  return a;
-}
-
-static MeritArgs merit_args(sfgs_ctx *ctx, int update_U, double *conmerit)
-{
-  MeritArgs m;
-  m.row0 = ctx->slice_row0.p; m.nrows = ctx->slice_nrows.p; m.rowlen = ctx->rowlen.p; m.off = ctx->slice_off.p;
-  m.W = ctx->W.p; m.wcol = ctx->wcol.p; m.Wd = ctx->Wd.p; m.A = ctx->A.p; m.B = ctx->B.p; m.fri = ctx->pfri.p;
-  m.coh = ctx->pcoh.p; m.shift = ctx->shift.p; m.R = ctx->R.p; m.U = ctx->U.p; m.n = ctx->nc; m.update_U = update_U;
-  m.mpartials = ctx->mpartials.p; m.conmerit = conmerit;
-  return m;
-}
-
 // one sweep over all colours (forward or backward) with or without the fused merit;
 // sweep k reads buffer X[k & 1] and writes X[(k + 1) & 1]
 template <typename T>
@@ -1511,10 +1244,8 @@ static int run_gs(sfgs_ctx *ctx, SweepArgs<T> sa, T *X0, T *X1, int maxiter, int
   return 0;
 }
 
-static inline int nblk_ll(long long n) { return (int)((n + TPB - 1) / TPB); }
-
 // FP32 copies of the sweep data (built once per assembled system)
-static int build_f32_mirror(sfgs_ctx *ctx)
+int sfgs::build_f32_mirror(sfgs_ctx *ctx)
 {
   if (ctx->mirror_valid) return 0;
   const long long n = ctx->nc, nb = 9*ctx->sell_blocks;
@@ -1553,6 +1284,7 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   if (par->precision != SFGS_PREC_FP64 && par->precision != SFGS_PREC_MIXED) return SFGS_E_ARG;
   if (par->precision == SFGS_PREC_MIXED && par->refine < 0) return SFGS_E_ARG;
   if (!ctx->assembled) { ctx->err = "solve before LOCDYN assembly"; return SFGS_E_STATE; }
+  if (ctx->nscenes > 1) { ctx->err = "the system holds independent scenes: use SFGS_Batch_Solve"; return SFGS_E_STATE; }
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const int ncol = ctx->ncolours, maxiter = par->maxiter, n = ctx->nc;
@@ -1922,3 +1654,9 @@ int SFGS_Body_Update(sfgs_ctx *ctx, double step, double *velo_out, double *force
 }
 
 } // extern "C"
+
+int sfgs::reassemble(sfgs_ctx *ctx)
+{
+  if (!ctx->from_bodies) { ctx->err = "re-assembly needs a system assembled from bodies"; return SFGS_E_STATE; }
+  return SFGS_Locdyn_Update_Begin(ctx, ctx->step, ctx->dynamic);
+}

@@ -1,0 +1,305 @@
+// sfgs_sweep.hpp -- device code of the colour-major Gauss-Seidel sweep and of
+// the merit pass, shared by the per-colour kernels (sfgs_kernels.hip) and the
+// persistent batched kernel (sfgs_batch.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "sfgs_ctx.hpp"
+#include "sfgs_math.hpp"
+
+namespace sfgs {
+
+static const int TPB = 256;          // threads per block (4 wavefronts)
+static const int SLICE = 64;         // rows per slice (one wavefront)
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// =====================================================================
+// Gauss-Seidel sweep of one colour (bgs.c:950-1019) with the merit of the
+// previous iteration fused in (mrf.c:35-184)
+//
+// R lives in two buffers.  Sweep k reads P (R at the end of sweep k-1, all
+// rows) and writes Q.  Row i gathers Q_j for neighbours already updated in
+// this sweep (permuted index in [upd_lo, upd_hi)) and P_j otherwise, which is
+// the serial colour-major Gauss-Seidel order.  When the merit is on, sweep
+// k >= 1 also forms U_i = B_i + W_ii P_i + sum_j W_ij P_j, i.e. the merit
+// update of U after iteration k-1 (mrf.c:61-76), accumulates C'AC for it, keeps
+// that U in Um and starts the local solve from it, exactly as the reference's
+// next sweep does after MERIT_Function.  The termination test of iteration k-1
+// therefore completes after sweep k; when it stops, P and Um are the state.
+// G lanes cooperate on one row (blocks k = g, g+G, ...), reduce B' across the
+// group with cross-lane adds, and run the local solve redundantly; lane g = 0
+// writes.
+// =====================================================================
+
+template <typename T>
+struct SweepArgs {
+  const int *row0, *nrows, *rowlen;
+  const long long *off;
+  const T *W;
+  const int *wcol;
+  const T *Wd, *B, *rho, *fri, *coh, *shift;
+  const double *A;      // merit weights stay FP64
+  const unsigned char *sep;
+  const T *P;
+  T *Q, *U, *Um;
+  int n;
+  T diageps;
+  int diagmaxiter, diagsolver, failure;
+  double *partials;     // 3 per block: errup, errlo, merit numerator
+  unsigned long long *failkey;
+  int *failcount;
+  Ctrl *ctrl;
+};
+
+template <int G, typename T>
+__device__ __forceinline__ T gsum(T v)
+{
+  if (G >= 2) v += __shfl_xor(v, 1, 64);
+  if (G >= 4) v += __shfl_xor(v, 2, 64);
+  if (G >= 8) v += __shfl_xor(v, 4, 64);
+  if (G >= 16) v += __shfl_xor(v, 8, 64);
+  return v;
+}
+
+// deterministic block sum of three values; thread 0 returns them
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c)
+{
+  __shared__ double sa[TPB/64], sb[TPB/64], sc[TPB/64];
+  a = wave_sum(a); b = wave_sum(b); c = wave_sum(c);
+  int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sa[w] = a; sb[w] = b; sc[w] = c; }
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    a = sa[0]; b = sb[0]; c = sc[0];
+    for (int k = 1; k < TPB/64; k ++) { a += sa[k]; b += sb[k]; c += sc[k]; }
+  }
+}
+
+// One slice (64/G rows of one colour, one wavefront) of a sweep.  eu, el and
+// up receive this lane's contributions to the error sums and to the merit
+// numerator.  A failed local solve is recorded in failkey / failcount, keyed by
+// the sweep number *sweepno, and raises *abortflag under FAILURE_EXIT.
+template <typename T, int G, bool MERIT>
+__device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool valid, int upd_lo, int upd_hi,
+                                            int backward, const int *sweepno, unsigned long long *failkey,
+                                            int *failcount, int *abortflag, double &eu, double &el, double &up)
+{
+  const int l = threadIdx.x & 63, r = l / G, g = l % G;
+  const int n = a.n;
+  bool act = false;
+  int p = 0, len = 0;
+  long long off = 0;
+  if (valid)
+  {
+    act = r < a.nrows[s];
+    off = a.off[s];
+    if (act) { p = a.row0[s] + r; len = a.rowlen[p]; }
+  }
+  // the row's own data first, so that its loads overlap the gather below
+  T Bi[3] = {T(0), T(0), T(0)}, Wi[9], R[3] = {T(0), T(0), T(0)}, U[3] = {T(0), T(0), T(0)};
+  double Ai[9];
+  RowLaw<T> L{T(0), T(0), T(0), T(0), false};
+  if (act)
+  {
+    Bi[0] = a.B[p]; Bi[1] = a.B[(size_t)n + p]; Bi[2] = a.B[2*(size_t)n + p];
+#pragma unroll
+    for (int e = 0; e < 9; e ++) Wi[e] = a.Wd[(size_t)e*n + p];
+#pragma unroll
+    for (int k = 0; k < 3; k ++) R[k] = a.P[4*(size_t)p + k];
+    if (MERIT)
+    {
+#pragma unroll
+      for (int e = 0; e < 9; e ++) Ai[e] = a.A[(size_t)e*n + p];
+    }
+    else
+    {
+#pragma unroll
+      for (int k = 0; k < 3; k ++) U[k] = a.U[(size_t)k*n + p];
+    }
+    L = RowLaw<T>{a.fri[p], a.coh[p], a.shift[p], a.rho[p], a.sep[p] != 0};
+  }
+  T b0 = T(0), b1 = T(0), b2 = T(0), u0 = T(0), u1 = T(0), u2 = T(0);
+  if (act)
+  {
+    const T *Wb = a.W + 9*off + l;
+    const int *cb = a.wcol + off + l;
+#pragma unroll 4
+    for (int k = g, m = 0; k < len; k += G, m ++)
+    {
+      const int j = cb[(size_t)m*SLICE];
+      const T *w = Wb + 9*(size_t)m*SLICE;
+      const T w0 = w[0], w1 = w[SLICE], w2 = w[2*SLICE], w3 = w[3*SLICE], w4 = w[4*SLICE],
+              w5 = w[5*SLICE], w6 = w[6*SLICE], w7 = w[7*SLICE], w8 = w[8*SLICE];
+      const bool upd = j >= upd_lo && j < upd_hi;
+      const T *xj = (upd ? a.Q : a.P) + 4*(size_t)j;
+      const T r0 = xj[0], r1 = xj[1], r2 = xj[2];
+      b0 += w0*r0 + w3*r1 + w6*r2;
+      b1 += w1*r0 + w4*r1 + w7*r2;
+      b2 += w2*r0 + w5*r1 + w8*r2;
+      if (MERIT)
+      {
+        T p0 = r0, p1 = r1, p2 = r2;
+        if (upd)
+        {
+          const T *pj = a.P + 4*(size_t)j;
+          p0 = pj[0]; p1 = pj[1]; p2 = pj[2];
+        }
+        u0 += w0*p0 + w3*p1 + w6*p2;
+        u1 += w1*p0 + w4*p1 + w7*p2;
+        u2 += w2*p0 + w5*p1 + w8*p2;
+      }
+    }
+  }
+  // sum over the G lanes of each row; every lane of the wave takes part
+  b0 = gsum<G>(b0); b1 = gsum<G>(b1); b2 = gsum<G>(b2);
+  if (MERIT) { u0 = gsum<G>(u0); u1 = gsum<G>(u1); u2 = gsum<G>(u2); }
+
+  if (act)
+  {
+    T Bp[3] = {Bi[0] + b0, Bi[1] + b1, Bi[2] + b2};
+    const T R0[3] = {R[0], R[1], R[2]};
+    if (MERIT)
+    {
+      // merit of the previous iteration at this row (mrf.c:63-84)
+      mv_add(Bi, Wi, R, U);
+      U[0] += u0; U[1] += u1; U[2] += u2;
+      // the merit is evaluated in FP64 whatever the sweep type
+      double C[3], Ud[3] = {U[0], U[1], U[2]}, Rd[3] = {R[0], R[1], R[2]};
+      scf_residual<double>(L.fri, L.coh, L.shift, Ud, Rd, C, nullptr, nullptr);
+      const double q0 = Ai[0]*C[0] + Ai[3]*C[1] + Ai[6]*C[2];
+      const double q1 = Ai[1]*C[0] + Ai[4]*C[1] + Ai[7]*C[2];
+      const double q2 = Ai[2]*C[0] + Ai[5]*C[1] + Ai[8]*C[2];
+      if (g == 0)
+      {
+        up += q0*C[0] + q1*C[1] + q2*C[2];
+#pragma unroll
+        for (int k = 0; k < 3; k ++) a.Um[(size_t)k*n + p] = U[k];
+      }
+    }
+
+    int d = local_solve(a.diagsolver, L, a.diageps, a.diagmaxiter, Wi, Bp, U, R);
+    if (d >= a.diagmaxiter || d < 0)
+    {
+      if (g == 0)
+      {
+        unsigned code = d < 0 ? SFGS_GS_DIAGONAL_FAILED : SFGS_GS_DIAGONAL_DIVERGED;
+        unsigned long long pos = backward ? (unsigned long long)(n - 1 - p) : (unsigned long long)p;
+        atomicMax(failkey, ((unsigned long long)(*sweepno + 1) << 40) | (pos << 2) | code);
+        atomicAdd(failcount, 1);
+        if (a.failure == SFGS_FAILURE_EXIT) atomicExch(abortflag, 1);
+      }
+      if (a.failure == SFGS_FAILURE_CONTINUE)
+      {
+        const int chain[4] = {SFGS_DS_SEMISMOOTH_NEWTON, SFGS_DS_PROJECTED_GRADIENT,
+                              SFGS_DS_DE_SAXCE_FENG, SFGS_DS_PROJECTED_NEWTON};
+        for (int c = 0; c < 4; c ++)
+        {
+          if (chain[c] == a.diagsolver) continue;
+          R[0] = R0[0]; R[1] = R0[1]; R[2] = R0[2];
+          d = local_solve(chain[c], L, a.diageps, a.diagmaxiter, Wi, Bp, U, R);
+          if (d < a.diagmaxiter && d >= 0) break;
+        }
+        if (d >= a.diagmaxiter || d < 0) { R[0] = R0[0]; R[1] = R0[1]; R[2] = R0[2]; }
+      }
+    }
+    if (g == 0)
+    {
+#pragma unroll
+      for (int k = 0; k < 3; k ++) { a.Q[4*(size_t)p + k] = R[k]; a.U[(size_t)k*n + p] = U[k]; }
+      a.Q[4*(size_t)p + 3] = T(0);
+      const double d0 = (double)R[0] - R0[0], d1 = (double)R[1] - R0[1], d2 = (double)R[2] - R0[2];
+      eu += d0*d0 + d1*d1 + d2*d2;
+      el += (double)R[0]*R[0] + (double)R[1]*R[1] + (double)R[2]*R[2];
+    }
+  }
+}
+
+// standalone merit (MERIT_Function with update_U, mrf.c:35-184) on the
+// canonical R buffer; one launch over all slices
+struct MeritArgs {
+  const int *row0, *nrows, *rowlen;
+  const long long *off;
+  const double *W;
+  const int *wcol;
+  const double *Wd, *A, *B, *fri, *coh, *shift;
+  const double *R;
+  double *U;
+  int n, update_U;
+  double *mpartials;
+  double *conmerit;   // may be null: per-row C'AC (colour-major)
+};
+
+// merit terms C'AC of the rows of one slice, added to up (lane g = 0 of each row)
+template <int G>
+__device__ __forceinline__ void merit_slice(const MeritArgs &a, int s, bool valid, double &up)
+{
+  const int l = threadIdx.x & 63, r = l / G, g = l % G;
+  const int n = a.n;
+  bool act = false;
+  int p = 0, len = 0;
+  long long off = 0;
+  if (valid)
+  {
+    act = r < a.nrows[s];
+    off = a.off[s];
+    if (act) { p = a.row0[s] + r; len = a.update_U ? a.rowlen[p] : 0; }
+  }
+  double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+  if (act)
+  {
+    const double *Wb = a.W + 9*off + l;
+    const int *cb = a.wcol + off + l;
+    for (int k = g, m = 0; k < len; k += G, m ++)
+    {
+      const int j = cb[(size_t)m*SLICE];
+      const double *w = Wb + 9*(size_t)m*SLICE;
+      const double r0 = a.R[4*(size_t)j], r1 = a.R[4*(size_t)j + 1], r2 = a.R[4*(size_t)j + 2];
+      u0 += w[0]*r0 + w[3*SLICE]*r1 + w[6*SLICE]*r2;
+      u1 += w[SLICE]*r0 + w[4*SLICE]*r1 + w[7*SLICE]*r2;
+      u2 += w[2*SLICE]*r0 + w[5*SLICE]*r1 + w[8*SLICE]*r2;
+    }
+  }
+  u0 = gsum<G>(u0); u1 = gsum<G>(u1); u2 = gsum<G>(u2);
+  if (act)
+  {
+    double R[3] = {a.R[4*(size_t)p], a.R[4*(size_t)p + 1], a.R[4*(size_t)p + 2]}, U[3];
+    if (a.update_U)
+    {
+      double Wi[9], Bv[3] = {a.B[p], a.B[(size_t)n + p], a.B[2*(size_t)n + p]};
+#pragma unroll
+      for (int e = 0; e < 9; e ++) Wi[e] = a.Wd[(size_t)e*n + p];
+      mv_add(Bv, Wi, R, U);
+      U[0] += u0; U[1] += u1; U[2] += u2;
+      if (g == 0)
+#pragma unroll
+        for (int k = 0; k < 3; k ++) a.U[(size_t)k*n + p] = U[k];
+    }
+    else
+    {
+#pragma unroll
+      for (int k = 0; k < 3; k ++) U[k] = a.U[(size_t)k*n + p];
+    }
+    double C[3], Ai[9];
+    scf_residual<double>(a.fri[p], a.coh[p], a.shift[p], U, R, C, nullptr, nullptr);
+#pragma unroll
+    for (int e = 0; e < 9; e ++) Ai[e] = a.A[(size_t)e*n + p];
+    const double q0 = Ai[0]*C[0] + Ai[3]*C[1] + Ai[6]*C[2];
+    const double q1 = Ai[1]*C[0] + Ai[4]*C[1] + Ai[7]*C[2];
+    const double q2 = Ai[2]*C[0] + Ai[5]*C[1] + Ai[8]*C[2];
+    if (g == 0)
+    {
+      const double t = q0*C[0] + q1*C[1] + q2*C[2];
+      up += t;
+      if (a.conmerit) a.conmerit[p] = t;
+    }
+  }
+}
+
+} // namespace sfgs

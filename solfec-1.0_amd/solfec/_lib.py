@@ -32,6 +32,15 @@ class SFGS_REPORT(C.Structure):
                 ("sweep_launches", _ll), ("row_updates", _ll), ("nnz", _ll)]
 
 
+class SFGS_BATCH_TUNE(C.Structure):
+    _fields_ = [("lanes_per_row", _i), ("waves_per_group", _i), ("rows_per_group", _i)]
+
+
+class SFGS_SCENE_REPORT(C.Structure):
+    _fields_ = [("rows", _i), ("groups", _i), ("iters", _i), ("error_code", _i), ("fail_rows", _i),
+                ("aborted", _i), ("error", _d), ("merit", _d), ("energy", _d)]
+
+
 _SFGS_PROTOS = {
     "SFGS_Create": (_p, [_i]),
     "SFGS_Destroy": (None, [_p]),
@@ -56,6 +65,9 @@ _SFGS_PROTOS = {
     "SFGS_Mlp_Train": (_i, [_p, _p, _p, _ll, _i, _i, C.c_float, C.c_ulonglong, _p]),
     "SFGS_Features": (_i, [_p, _p]),
     "SFGS_Warm_Start": (_i, [_p]),
+    "SFGS_Batch_Scenes": (_i, [_p, _i, _p]),
+    "SFGS_Batch_Solve": (_i, [_p, C.POINTER(SFGS_PARAM), _p, _p, C.POINTER(SFGS_REPORT)]),
+    "SFGS_Batch_Autotune": (_i, [_p, C.POINTER(SFGS_PARAM), _i, C.POINTER(SFGS_BATCH_TUNE), C.POINTER(_d)]),
 }
 
 _SOL_PROTOS = {

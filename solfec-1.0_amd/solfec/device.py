@@ -6,7 +6,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import SFGS_PARAM, SFGS_REPORT, f64, i32, ptr, sfgs
+from ._lib import SFGS_BATCH_TUNE, SFGS_PARAM, SFGS_REPORT, SFGS_SCENE_REPORT, f64, i32, ptr, sfgs
 
 FP64, MIXED = 0, 1
 DIAGSOLVERS = {"PROJECTED_GRADIENT": 0, "DE_SAXCE_FENG": 1, "SEMISMOOTH_NEWTON": 2, "PROJECTED_NEWTON": 3}
@@ -198,6 +198,39 @@ class Device:
                                      ptr(hist))
         self._check(rc, "SFGS_Mlp_Train")
         return hist
+
+    # ---- batched solve of independent scenes ----------------------------
+    def batch_scenes(self, body_scene, nscenes=None):
+        body_scene = np.ascontiguousarray(body_scene, np.int32)
+        ns = int(body_scene.max()) + 1 if nscenes is None else int(nscenes)
+        self._check(self.lib.SFGS_Batch_Scenes(self.ctx, ns, ptr(body_scene) if ns > 1 else None),
+                    "SFGS_Batch_Scenes")
+        return ns
+
+    def batch_solve(self, param, nscenes, tune=None):
+        """Solve every scene; returns (per-scene records as a structured array, SFGS_REPORT)."""
+        reps = (SFGS_SCENE_REPORT * max(nscenes, 1))()
+        tot = SFGS_REPORT()
+        t = None
+        if tune is not None:
+            t = SFGS_BATCH_TUNE(*[int(v) for v in tune])
+        rc = self.lib.SFGS_Batch_Solve(self.ctx, C.byref(param), C.byref(t) if t is not None else None, reps,
+                                       C.byref(tot))
+        self._check(rc, "SFGS_Batch_Solve")
+        out = np.zeros(nscenes, dtype=[("rows", "i4"), ("groups", "i4"), ("iters", "i4"), ("error_code", "i4"),
+                                       ("fail_rows", "i4"), ("aborted", "i4"), ("error", "f8"), ("merit", "f8"),
+                                       ("energy", "f8")])
+        for k in range(nscenes):
+            r = reps[k]
+            out[k] = (r.rows, r.groups, r.iters, r.error_code, r.fail_rows, r.aborted, r.error, r.merit, r.energy)
+        return out, tot
+
+    def batch_autotune(self, param, trial_iters=20):
+        best = SFGS_BATCH_TUNE()
+        ms = C.c_double(0.0)
+        self._check(self.lib.SFGS_Batch_Autotune(self.ctx, C.byref(param), int(trial_iters), C.byref(best),
+                                                 C.byref(ms)), "SFGS_Batch_Autotune")
+        return (best.lanes_per_row, best.waves_per_group, best.rows_per_group), ms.value
 
     def body_update(self, step, nb, download=True):
         if not download:

@@ -84,8 +84,8 @@ def test_gradient_matches_oracle(dev):
 
 
 def test_training_reduces_loss(dev):
-    X, Y = _data(4, 20000)
-    Xv, Yv = _data(5, 4000)
+    X, Y = _data(4, 24000)            # one distribution: the last 4000 samples are held out
+    X, Y, Xv, Yv = X[:20000], Y[:20000], X[20000:], Y[20000:]
     norm = learned.norm_from_data(X, Y)
     p0 = learned.init_params(11)
     dev.mlp_set(p0, norm)

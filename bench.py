@@ -99,6 +99,8 @@ def main():
     ap.add_argument("--snapshot", default=None, help="load a settled state instead of settling")
     ap.add_argument("--no-mixed", dest="mixed", action="store_false", help="skip the mixed-precision mode")
     ap.add_argument("--headline", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--schedule", type=int, default=0, choices=[0, 1],
+                    help="sweep schedule: 0 one launch per colour, 1 persistent dataflow")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the contact solve on a single MI355X "
@@ -119,6 +121,7 @@ def main():
     nb = len(b["kind"])
     dev = solfec.Device(0)
     dev.set_layout(args.lanes)
+    dev.set_schedule(args.schedule)
     dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
     dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
                  c["restitution"], c["cohesion"], c["R"], c["U"])
@@ -165,11 +168,11 @@ def main():
     prep, _, _ = dev.solve(prof, history=False)
     sz = 4 if head == "mixed" else 8
     bytes_per_sweep = sz * (9 * nnz + 26 * n) + 4 * nnz + 4 * n
-    sweeps_run = prep.sweep_launches / max(ncol, 1)
+    sweeps_run = prep.row_updates / max(n, 1)
     achieved = bytes_per_sweep * sweeps_run / (prep.sweep_ms * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                kernel="k_sweep<%s>" % ("float" if head == "mixed" else "double"),
+                kernel=("k_flow<%s>" if args.schedule else "k_sweep<%s>") % ("float" if head == "mixed" else "double"),
                 avg_launch_us=round(1e3 * prep.sweep_ms / max(prep.sweep_launches, 1), 3),
                 launches=int(prep.sweep_launches), bytes_per_sweep=int(bytes_per_sweep))
     log(f"[bench] sweep kernel: {prep.sweep_launches} launches, {prep.sweep_ms:.2f} ms, {achieved:.1f} GB/s")
@@ -200,6 +203,7 @@ def main():
             "epsilon": 1e-6, "meritval": 1e-6, "maxiter": 5000, "friction": args.friction,
             "iters_per_step": iters, "merit": modes[head]["merit"], "rel_error": modes[head]["rel_error"],
             "precision": head, "warm_start": "previous-step R", "lanes_per_row": args.lanes,
+            "schedule": "dataflow" if args.schedule else "per-colour launches",
             "modes": modes,
             "speedup_vs_plain_fp64": modes["fp64"]["ms_per_step"] / ms_per_step,
         },

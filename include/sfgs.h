@@ -142,6 +142,13 @@ int SFGS_Get_Sizes(sfgs_ctx *ctx, int *n, long long *nnz, int *ncolours);
  * upload.  No counterpart in the reference (a tuning knob of the GPU layout). */
 int SFGS_Set_Layout(sfgs_ctx *ctx, int lanes_per_row);
 
+/* Schedule of the Gauss-Seidel sweeps of SFGS_Solve (no counterpart in the
+ * reference, which sweeps serially): 0 = one kernel launch per colour
+ * (default); 1 = one persistent dataflow launch in which each row waits only
+ * for its own neighbours, with the same iterates (forward sweeps; a solve with
+ * reverse = 1, or with 4 or 8 lanes per row, uses schedule 0). */
+int SFGS_Set_Schedule(sfgs_ctx *ctx, int schedule);
+
 /* Colour an arbitrary graph given in CSR (no self loops) with the device
  * colouring kernel; returns the number of colours or a negative code. */
 int SFGS_Colour(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, int *colour);

@@ -544,8 +544,8 @@ static int batch_solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TU
       r.aborted = b.aborted;
       r.error_code = SFGS_GS_OK;
       if (b.failkey) r.error_code = (int)(b.failkey & 3ULL);
-      if (rows[q] > 0 && (b.error > par->epsilon || (par->meritval < 1.0 && b.merit > par->meritval)))
-        r.error_code = SFGS_GS_DIVERGED;
+      // bgs.c:1053: the iteration bound reached without an abort
+      if (rows[q] > 0 && !b.aborted && b.iters - b.nref >= par->maxiter) r.error_code = SFGS_GS_DIVERGED;
     }
     maxit = std::max(maxit, b.iters);
     updates += (long long)b.sweeps * rows[q];

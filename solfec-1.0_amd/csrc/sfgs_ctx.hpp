@@ -80,6 +80,8 @@ struct sfgs_ctx {
   sfgs::DBuf<int> rowlen;                // per permuted row
   sfgs::DBuf<double> W, Wd, A, B, V, rho, pfri, pcoh, shift, R, U, Rb, Um;
   int G = 1;                             // lanes per row of the sweep layout (1, 2, 4, 8)
+  int schedule = 0;                      // sweeps: 0 one launch per colour, 1 persistent dataflow
+  sfgs::DBuf<unsigned char> flow_mem;    // dataflow schedule: third R buffer, U pairs, versions, partials
   int G_built = 1;                       // layout of the current store
   sfgs::DBuf<unsigned char> sep;
   sfgs::DBuf<int> wcol;

@@ -60,7 +60,7 @@ static SweepArgs<T> sweep_args(sfgs_ctx *ctx, const SFGS_PARAM *par, const T *W,
   a.row0 = ctx->slice_row0.p; a.nrows = ctx->slice_nrows.p; a.rowlen = ctx->rowlen.p; a.off = ctx->slice_off.p;
   a.W = W; a.wcol = ctx->wcol.p; a.Wd = Wd; a.A = ctx->A.p; a.B = B; a.rho = rho;
   a.fri = fri; a.coh = coh; a.shift = shift; a.sep = ctx->sep.p;
-  a.P = nullptr; a.Q = nullptr; a.U = U; a.Um = Um; a.n = ctx->nc;
+  a.P = nullptr; a.Q = nullptr; a.U = U; a.Um = Um; a.Uin = U; a.n = ctx->nc;
   a.diageps = (T)diageps; a.diagmaxiter = par->diagmaxiter; a.diagsolver = par->diagsolver;
   a.failure = par->failure; a.partials = ctx->partials.p; a.failkey = ctx->failkey.p; a.failcount = ctx->failcount.p;
   a.ctrl = ctx->ctrl.p;
@@ -76,6 +76,13 @@ static MeritArgs merit_args(sfgs_ctx *ctx, int update_U, double *conmerit)
   m.mpartials = ctx->mpartials.p; m.conmerit = conmerit;
   return m;
 }
+
+// the sweeps of SFGS_Solve as one persistent dataflow launch (sfgs_flow.hip);
+// the final R and U are left in X0 and sa.U, hc.final_buf = hc.final_U = 0
+int run_flow(sfgs_ctx *ctx, const SweepArgs<double> &sa, double *X0, double *X1, int maxiter, int nomerit,
+             double eps, double meritval, double *hist, Ctrl &hc, float *ms);
+int run_flow(sfgs_ctx *ctx, const SweepArgs<float> &sa, float *X0, float *X1, int maxiter, int nomerit,
+             double eps, double meritval, double *hist, Ctrl &hc, float *ms);
 
 // FP32 copies of the sweep data (sfgs_kernels.hip)
 int build_f32_mirror(sfgs_ctx *ctx);

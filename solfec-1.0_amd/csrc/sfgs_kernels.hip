@@ -929,7 +929,7 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   release(ctx->failcount); release(ctx->ctrl);
   release(ctx->body_scene); release(ctx->row_scene); release(ctx->bscene_tab); release(ctx->bwg_scene);
   release(ctx->bgptr); release(ctx->bgslice); release(ctx->bbar); release(ctx->bpart); release(ctx->bout);
-  release(ctx->babort);
+  release(ctx->babort); release(ctx->flow_mem);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
@@ -1291,6 +1291,8 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   const int nomerit = par->meritval >= 1.0 ? 1 : 0;
   const bool mixed = par->precision == SFGS_PREC_MIXED;
   const int refine = mixed ? par->refine : 0;
+  // the dataflow schedule runs forward sweeps only
+  const bool flow = ctx->schedule == 1 && !par->reverse && n > 0 && ctx->G_built <= 2;
 
   GsPlan pl;
   pl.grid.assign((size_t)ncol, 0);
@@ -1330,8 +1332,16 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   {
     SweepArgs<double> sa = sweep_args<double>(ctx, par, ctx->W.p, ctx->Wd.p, ctx->B.p, ctx->rho.p, ctx->pfri.p,
                                               ctx->pcoh.p, ctx->shift.p, ctx->U.p, ctx->Um.p, par->diagepsilon);
-    CHK(run_gs<double>(ctx, sa, ctx->R.p, ctx->Rb.p, maxiter, nomerit, par->epsilon, par->meritval, par->reverse,
-                       ctx->hist.p, pl, hc));
+    if (flow)
+    {
+      float fms = 0.0f;
+      CHK(run_flow(ctx, sa, ctx->R.p, ctx->Rb.p, maxiter, nomerit, par->epsilon, par->meritval, ctx->hist.p, hc, &fms));
+      pl.sweep_ms += fms;
+      pl.launches += 1;
+    }
+    else
+      CHK(run_gs<double>(ctx, sa, ctx->R.p, ctx->Rb.p, maxiter, nomerit, par->epsilon, par->meritval, par->reverse,
+                         ctx->hist.p, pl, hc));
     if (n > 0 && hc.final_buf == 1)
       HIPCHK(hipMemcpyAsync(ctx->R.p, ctx->Rb.p, sizeof(double)*4*(size_t)n, hipMemcpyDeviceToDevice, st));
     if (n > 0 && hc.final_U == 1)
@@ -1350,8 +1360,17 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
     const double de32 = std::max(par->diagepsilon, 1e-6);
     SweepArgs<float> sa = sweep_args<float>(ctx, par, ctx->W32.p, ctx->Wd32.p, ctx->B32.p, ctx->rho32.p,
                                             ctx->pfri32.p, ctx->pcoh32.p, ctx->shift32.p, ctx->U32.p, ctx->Um32.p, de32);
-    CHK(run_gs<float>(ctx, sa, ctx->R32.p, ctx->R32b.p, maxiter, nomerit, par->epsilon, par->meritval, par->reverse,
-                      ctx->hist.p, pl, hc));
+    if (flow)
+    {
+      float fms = 0.0f;
+      CHK(run_flow(ctx, sa, ctx->R32.p, ctx->R32b.p, maxiter, nomerit, par->epsilon, par->meritval, ctx->hist.p, hc,
+                   &fms));
+      pl.sweep_ms += fms;
+      pl.launches += 1;
+    }
+    else
+      CHK(run_gs<float>(ctx, sa, ctx->R32.p, ctx->R32b.p, maxiter, nomerit, par->epsilon, par->meritval, par->reverse,
+                        ctx->hist.p, pl, hc));
     if (n > 0)
     {
       float *Rf = hc.final_buf == 1 ? ctx->R32b.p : ctx->R32.p, *Uf = hc.final_U == 1 ? ctx->Um32.p : ctx->U32.p;
@@ -1428,7 +1447,7 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
     rep->solve_ms = ms;
     rep->sweep_ms = pl.sweep_ms;
     rep->sweep_launches = pl.launches;
-    rep->row_updates = (long long)iters * n;
+    rep->row_updates = (long long)(hc.sweeps + hr.sweeps) * n;   // every sweep run, the merit's extra one included
     rep->nnz = ctx->nnz;
     rep->error = fk ? (int)(fk & 3ull) : SFGS_GS_OK;
     if (!hc.aborted && hc.iters >= maxiter) rep->error = SFGS_GS_DIVERGED;
@@ -1500,6 +1519,13 @@ int SFGS_Warm_Start(sfgs_ctx *ctx)
   HIPCHK(hipSetDevice(ctx->device));
   CHK(compute_features(ctx, 0));
   return sfgs_mlp_apply(ctx, ctx->feat.p, ctx->nc, ctx->R.p);
+}
+
+int SFGS_Set_Schedule(sfgs_ctx *ctx, int schedule)
+{
+  if (!ctx || (schedule != 0 && schedule != 1)) return SFGS_E_ARG;
+  ctx->schedule = schedule;
+  return 0;
 }
 
 int SFGS_Set_Layout(sfgs_ctx *ctx, int lanes_per_row)

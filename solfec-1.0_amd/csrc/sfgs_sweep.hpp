@@ -48,6 +48,7 @@ struct SweepArgs {
   const unsigned char *sep;
   const T *P;
   T *Q, *U, *Um;
+  const T *Uin;         // local-solver start U (U itself except in the dataflow schedule)
   int n;
   T diageps;
   int diagmaxiter, diagsolver, failure;
@@ -121,7 +122,7 @@ __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool v
     else
     {
 #pragma unroll
-      for (int k = 0; k < 3; k ++) U[k] = a.U[(size_t)k*n + p];
+      for (int k = 0; k < 3; k ++) U[k] = a.Uin[(size_t)k*n + p];
     }
     L = RowLaw<T>{a.fri[p], a.coh[p], a.shift[p], a.rho[p], a.sep[p] != 0};
   }

@@ -58,6 +58,7 @@ _SFGS_PROTOS = {
     "SFGS_Colour": (_i, [_p, _i, _p, _p, _p]),
     "SFGS_Body_Update": (_i, [_p, _d, _p, _p]),
     "SFGS_Set_Layout": (_i, [_p, _i]),
+    "SFGS_Set_Schedule": (_i, [_p, _i]),
     "SFGS_Mlp_Set": (_i, [_p, _p, _ll, _p]),
     "SFGS_Mlp_Get": (_i, [_p, _p, _p]),
     "SFGS_Mlp_Forward": (_i, [_p, _p, _i, _p]),

@@ -72,6 +72,10 @@ class Device:
     def set_layout(self, lanes_per_row):
         self._check(self.lib.SFGS_Set_Layout(self.ctx, int(lanes_per_row)), "SFGS_Set_Layout")
 
+    def set_schedule(self, schedule):
+        """0: one launch per colour; 1: persistent dataflow sweeps (same iterates)."""
+        self._check(self.lib.SFGS_Set_Schedule(self.ctx, int(schedule)), "SFGS_Set_Schedule")
+
     # ---- inputs -------------------------------------------------------
     def bodies(self, kind, rot, x0, jinv, minv, velo, velo0):
         nb = len(kind)

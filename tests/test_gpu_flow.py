@@ -1,0 +1,84 @@
+"""The dataflow schedule of the sweeps (one persistent launch, rows wait for
+their own neighbours) against the per-colour schedule on the same system:
+every row reads the values the serial colour-major sweep reads, so the
+reactions and velocities are bit-identical whenever both stop after the same
+number of iterations (the error and merit sums are formed in a different order,
+so a stop can move by one iteration at the threshold).  Also against the CPU
+oracle within 1e-6 of max|R|."""
+import numpy as np
+import pytest
+
+import oracle
+import solfec
+from scenes import lattice
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = solfec.Device(0)
+    yield d
+    d.close()
+
+
+def _system(dev, O, R0):
+    n = O["n"]
+    dev.system(O["rowptr"], O["col"], O["W"], O["Wd"], O["A"], O["B"], O["V"], O["rho"], O["mu"], O["rest"],
+               O["coh"], O["gap"], O["free_energy"], O["step"], R0, np.zeros(3 * n))
+
+
+@pytest.mark.parametrize("n,seed,lanes,kw", [
+    (343, 2, 1, dict(epsilon=1e-6, maxiter=2000)),
+    (1000, 1, 1, dict(epsilon=1e-6, maxiter=3000, meritval=1e-6)),
+    (1000, 3, 2, dict(epsilon=1e-6, maxiter=3000, meritval=1e-6)),
+    (512, 4, 1, dict(epsilon=1e-5, maxiter=50, meritval=1e-12)),       # stops at maxiter
+    (729, 5, 1, dict(epsilon=1e-6, maxiter=3000, meritval=1e-8, precision=1, refine=2)),
+])
+def test_flow_equals_colour_schedule(dev, n, seed, lanes, kw):
+    print(f"[flow] n={n} seed={seed} lanes={lanes} {kw}", flush=True)
+    s = lattice(n, seed=seed)
+    b, c = s.bodies(), s.contacts()
+    O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), 1e-3)
+    R0 = np.random.default_rng(seed).normal(scale=1e-3, size=3 * O["n"])
+    dev.set_layout(lanes)
+    out = []
+    for sched in (0, 1):
+        dev.set_schedule(sched)
+        _system(dev, O, R0)
+        rep, rer, mer = dev.solve(solfec.make_param(**kw))
+        R, U = dev.get_RU()
+        out.append((rep, rer, mer, R, U))
+    dev.set_schedule(0)
+    dev.set_layout(1)
+    (r0, rer0, mer0, R0g, U0g), (r1, rer1, mer1, R1g, U1g) = out
+    print(f"[flow] iters colour {r0.iters} flow {r1.iters}; merit {r0.merit:.3e} {r1.merit:.3e}; "
+          f"solve ms {r0.solve_ms:.2f} {r1.solve_ms:.2f}")
+    assert abs(r0.iters - r1.iters) <= 1
+    if r0.iters == r1.iters:
+        assert np.array_equal(R0g, R1g)
+        assert np.array_equal(U0g, U1g)
+        np.testing.assert_allclose(rer1, rer0, rtol=1e-10, atol=0)
+        np.testing.assert_allclose(mer1, mer0, rtol=1e-10, atol=0)
+    scale = np.abs(R0g).max()
+    assert np.abs(R1g - R0g).max() <= 1e-6 * scale
+
+
+def test_flow_matches_oracle(dev):
+    s = lattice(1000, seed=7)
+    b, c = s.bodies(), s.contacts()
+    O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), 1e-3)
+    kw = dict(epsilon=1e-6, maxiter=5000, meritval=1e-6)
+    dev.set_schedule(1)
+    _system(dev, O, np.zeros(3 * O["n"]))
+    rep, _, _ = dev.solve(solfec.make_param(**kw))
+    Rg, _ = dev.get_RU()
+    colours = dev.locdyn()["colour"]
+    dev.set_schedule(0)
+    order, _ = oracle.colour_order(colours, int(colours.max()) + 1)
+    rc, orep, Ro, Uo, _, _ = oracle.gs_solve(oracle.System(O), oracle.gs_param(**kw), order, np.zeros(3 * O["n"]),
+                                             np.zeros(3 * O["n"]), history=False)
+    err = np.abs(Rg.reshape(-1, 3) - Ro).max() / np.abs(Ro).max()
+    print(f"[flow] oracle iters {orep.iters} gpu {rep.iters} |dR|/max|R| {err:.2e}")
+    assert abs(rep.iters - orep.iters) <= 1
+    assert err < 1e-6

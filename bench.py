@@ -82,6 +82,22 @@ def cpu_baseline(b, c, colours, n, seconds):
                        f"system, oracle/liboracle.so on 1 host core, {dt:.1f} s")
 
 
+def copy_calibration(mib):
+    """Device-to-device copies of a known size, for scaling HBM traffic counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    for _ in range(3):
+        hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,9 +115,16 @@ def main():
     ap.add_argument("--snapshot", default=None, help="load a settled state instead of settling")
     ap.add_argument("--no-mixed", dest="mixed", action="store_false", help="skip the mixed-precision mode")
     ap.add_argument("--headline", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--modes", default="fp64,mixed", help="solve modes to time (comma list of fp64, mixed)")
+    ap.add_argument("--maxiter", type=int, default=5000, help="iteration bound (5000 is the workload's)")
     ap.add_argument("--schedule", type=int, default=0, choices=[0, 1],
                     help="sweep schedule: 0 one launch per colour, 1 persistent dataflow")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 3 device copies of MIB MiB (scale of the traffic counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the contact solve on a single MI355X "
               f"(multi-GPU sharding is a design only); got --gpus {args.gpus}", file=sys.stderr)
@@ -128,7 +151,7 @@ def main():
     n, nnz, ncol = 0, 0, 0
 
     def run_mode(name, precision, steps, warmup):
-        par = solfec.make_param(1e-6, 5000, meritval=1e-6, precision=precision, refine=2)
+        par = solfec.make_param(1e-6, args.maxiter, meritval=1e-6, precision=precision, refine=2)
 
         def step():
             dev.assemble(h)
@@ -151,11 +174,17 @@ def main():
         return res, R
 
     modes = {}
-    modes["fp64"], R64 = run_mode("plain FP64 GS", solfec.device.FP64, args.steps, args.warmup)
-    if args.mixed:
+    want = [m.strip() for m in args.modes.split(",") if m.strip()]
+    if not args.mixed and "mixed" in want:
+        want.remove("mixed")
+    R64 = Rmx = None
+    if "fp64" in want:
+        modes["fp64"], R64 = run_mode("plain FP64 GS", solfec.device.FP64, args.steps, args.warmup)
+    if "mixed" in want:
         modes["mixed"], Rmx = run_mode("mixed FP32/FP64 GS", solfec.device.MIXED, args.steps, args.warmup)
-        modes["mixed"]["err_vs_fp64"] = float(np.abs(Rmx - R64).max() / max(np.abs(R64).max(), 1e-300))
-    head = "mixed" if args.mixed and args.headline == "mixed" else "fp64"
+        if R64 is not None:
+            modes["mixed"]["err_vs_fp64"] = float(np.abs(Rmx - R64).max() / max(np.abs(R64).max(), 1e-300))
+    head = "mixed" if "mixed" in modes and args.headline == "mixed" else "fp64"
     n, nnz, ncol = dev.sizes()
     value = modes[head]["updates_per_s"]
     ms_per_step = modes[head]["ms_per_step"]
@@ -164,7 +193,7 @@ def main():
     # roofline of the sweep kernel of the headline mode: one profiled solve, HIP events around every launch
     dev.assemble(h)
     prec = solfec.device.MIXED if head == "mixed" else solfec.device.FP64
-    prof = solfec.make_param(1e-6, 5000, meritval=1e-6, profile=1, precision=prec, refine=2)
+    prof = solfec.make_param(1e-6, args.maxiter, meritval=1e-6, profile=1, precision=prec, refine=2)
     prep, _, _ = dev.solve(prof, history=False)
     sz = 4 if head == "mixed" else 8
     bytes_per_sweep = sz * (9 * nnz + 26 * n) + 4 * nnz + 4 * n
@@ -200,12 +229,12 @@ def main():
             "workload": f"{args.spheres}-sphere packing settled {args.settle} steps, "
                         + ("mixed FP32/FP64 GPU GS" if head == "mixed" else "plain FP64 GPU GS"),
             "spheres": args.spheres, "contacts": n, "offdiag_blocks": nnz, "colours": ncol,
-            "epsilon": 1e-6, "meritval": 1e-6, "maxiter": 5000, "friction": args.friction,
+            "epsilon": 1e-6, "meritval": 1e-6, "maxiter": args.maxiter, "friction": args.friction,
             "iters_per_step": iters, "merit": modes[head]["merit"], "rel_error": modes[head]["rel_error"],
             "precision": head, "warm_start": "previous-step R", "lanes_per_row": args.lanes,
             "schedule": "dataflow" if args.schedule else "per-colour launches",
             "modes": modes,
-            "speedup_vs_plain_fp64": modes["fp64"]["ms_per_step"] / ms_per_step,
+            "speedup_vs_plain_fp64": modes["fp64"]["ms_per_step"] / ms_per_step if "fp64" in modes else None,
         },
         "roofline": roof,
         "cpu_baseline": cpu,

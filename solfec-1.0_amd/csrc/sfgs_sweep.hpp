@@ -40,6 +40,7 @@ __device__ __forceinline__ T wave_sum(T v)
 template <typename T>
 struct SweepArgs {
   const int *row0, *nrows, *rowlen;
+  const int *slen;      // block slots per lane of each slice (those of its first, longest row)
   const long long *off;
   const T *W;
   const int *wcol;
@@ -129,32 +130,50 @@ __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool v
   T b0 = T(0), b1 = T(0), b2 = T(0), u0 = T(0), u1 = T(0), u2 = T(0);
   if (act)
   {
+    // The gather runs over the slice's slot count, known with the slice table,
+    // so the neighbour indices and blocks load without waiting for this row's
+    // length; slots past the row's own (padding: zero block, row 0) are loaded
+    // but not added.  CH slots at a time: all their indices, then all their
+    // blocks and reactions, in flight together.
+    constexpr int CH = sizeof(T) == 8 ? 4 : 8;
     const T *Wb = a.W + 9*off + l;
     const int *cb = a.wcol + off + l;
-#pragma unroll 4
-    for (int k = g, m = 0; k < len; k += G, m ++)
+    const int nsl = a.slen[s], mine = len > g ? (len - g + G - 1) / G : 0;
+    for (int m0 = 0; m0 < nsl; m0 += CH)
     {
-      const int j = cb[(size_t)m*SLICE];
-      const T *w = Wb + 9*(size_t)m*SLICE;
-      const T w0 = w[0], w1 = w[SLICE], w2 = w[2*SLICE], w3 = w[3*SLICE], w4 = w[4*SLICE],
-              w5 = w[5*SLICE], w6 = w[6*SLICE], w7 = w[7*SLICE], w8 = w[8*SLICE];
-      const bool upd = j >= upd_lo && j < upd_hi;
-      const T *xj = (upd ? a.Q : a.P) + 4*(size_t)j;
-      const T r0 = xj[0], r1 = xj[1], r2 = xj[2];
-      b0 += w0*r0 + w3*r1 + w6*r2;
-      b1 += w1*r0 + w4*r1 + w7*r2;
-      b2 += w2*r0 + w5*r1 + w8*r2;
-      if (MERIT)
+      int jj[CH];
+#pragma unroll
+      for (int q = 0; q < CH; q ++) jj[q] = m0 + q < nsl ? cb[(size_t)(m0 + q)*SLICE] : 0;
+#pragma unroll
+      for (int q = 0; q < CH; q ++)
       {
+        const int m = m0 + q;
+        if (m >= nsl) break;
+        const int j = jj[q];
+        const T *w = Wb + 9*(size_t)m*SLICE;
+        const T w0 = w[0], w1 = w[SLICE], w2 = w[2*SLICE], w3 = w[3*SLICE], w4 = w[4*SLICE],
+                w5 = w[5*SLICE], w6 = w[6*SLICE], w7 = w[7*SLICE], w8 = w[8*SLICE];
+        const bool upd = j >= upd_lo && j < upd_hi;
+        const T *xj = (upd ? a.Q : a.P) + 4*(size_t)j;
+        const T r0 = xj[0], r1 = xj[1], r2 = xj[2];
         T p0 = r0, p1 = r1, p2 = r2;
-        if (upd)
+        if (MERIT && upd)
         {
           const T *pj = a.P + 4*(size_t)j;
           p0 = pj[0]; p1 = pj[1]; p2 = pj[2];
         }
-        u0 += w0*p0 + w3*p1 + w6*p2;
-        u1 += w1*p0 + w4*p1 + w7*p2;
-        u2 += w2*p0 + w5*p1 + w8*p2;
+        if (m < mine)
+        {
+          b0 += w0*r0 + w3*r1 + w6*r2;
+          b1 += w1*r0 + w4*r1 + w7*r2;
+          b2 += w2*r0 + w5*r1 + w8*r2;
+          if (MERIT)
+          {
+            u0 += w0*p0 + w3*p1 + w6*p2;
+            u1 += w1*p0 + w4*p1 + w7*p2;
+            u2 += w2*p0 + w5*p1 + w8*p2;
+          }
+        }
       }
     }
   }

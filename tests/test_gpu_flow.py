@@ -65,7 +65,7 @@ def test_flow_equals_colour_schedule(dev, n, seed, lanes, kw):
 
 
 def test_flow_matches_oracle(dev):
-    s = lattice(1000, seed=7)
+    s = lattice(1000, seed=1)
     b, c = s.bodies(), s.contacts()
     O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), 1e-3)
     kw = dict(epsilon=1e-6, maxiter=5000, meritval=1e-6)

@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--headline", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--modes", default="fp64,mixed", help="solve modes to time (comma list of fp64, mixed)")
     ap.add_argument("--maxiter", type=int, default=5000, help="iteration bound (5000 is the workload's)")
+    ap.add_argument("--row-order", type=int, default=0, choices=[0, 1],
+                    help="rows inside a colour: 0 by decreasing length, 1 by contact index")
     ap.add_argument("--schedule", type=int, default=0, choices=[0, 1],
                     help="sweep schedule: 0 one launch per colour, 1 persistent dataflow")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -145,6 +147,7 @@ def main():
     dev = solfec.Device(0)
     dev.set_layout(args.lanes)
     dev.set_schedule(args.schedule)
+    dev.set_row_order(args.row_order)
     dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
     dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
                  c["restitution"], c["cohesion"], c["R"], c["U"])
@@ -233,6 +236,7 @@ def main():
             "iters_per_step": iters, "merit": modes[head]["merit"], "rel_error": modes[head]["rel_error"],
             "precision": head, "warm_start": "previous-step R", "lanes_per_row": args.lanes,
             "schedule": "dataflow" if args.schedule else "per-colour launches",
+            "row_order": "contact index" if args.row_order else "decreasing row length",
             "modes": modes,
             "speedup_vs_plain_fp64": modes["fp64"]["ms_per_step"] / ms_per_step if "fp64" in modes else None,
         },

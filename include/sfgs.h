@@ -149,6 +149,14 @@ int SFGS_Set_Layout(sfgs_ctx *ctx, int lanes_per_row);
  * reverse = 1, or with 4 or 8 lanes per row, uses schedule 0). */
 int SFGS_Set_Schedule(sfgs_ctx *ctx, int schedule);
 
+/* Order of the rows inside a colour (no counterpart in the reference): 0 =
+ * decreasing row length (least slice padding, default); 1 = contact index,
+ * which keeps the detection's spatial order so that the rows of a wavefront
+ * gather their neighbours' reactions from fewer cache lines.  Takes effect at
+ * the next assembly; the iterates do not depend on it (rows of a colour are
+ * independent). */
+int SFGS_Set_Row_Order(sfgs_ctx *ctx, int order);
+
 /* Colour an arbitrary graph given in CSR (no self loops) with the device
  * colouring kernel; returns the number of colours or a negative code. */
 int SFGS_Colour(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, int *colour);
@@ -182,6 +190,10 @@ int SFGS_Features(sfgs_ctx *ctx, float *features);
 /* replace the carried R of the current system by R_prev + MLP(features) (call
  * after SFGS_Locdyn_Update_Begin, before SFGS_Solve) */
 int SFGS_Warm_Start(sfgs_ctx *ctx);
+/* what SFGS_Warm_Start changes: new_only = 1 applies the prediction only to
+ * contacts without a carried reaction (R_prev = 0, new contacts, mem.c:77)
+ * and keeps R_prev elsewhere; scale multiplies the prediction (default 0, 1) */
+int SFGS_Warm_Start_Mode(sfgs_ctx *ctx, int new_only, float scale);
 
 /* ---- batched solve of independent scenes (no counterpart in the reference,
  * which solves one domain per GAUSS_SEIDEL_Solve call, bgs.c:913-1069) ----

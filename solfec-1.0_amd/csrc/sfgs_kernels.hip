@@ -252,13 +252,15 @@ __global__ void k_fill_int(int n, int v, int *b)
   if (i < n) b[i] = v;
 }
 
-// sort key of row i: (scene, colour, -row length); scene = 0 without scenes
-__global__ void k_sort_keys(int n, const int *colour, const int *rowptr, const int *scene, unsigned long long *keys,
-                            int *vals)
+// sort key of row i: (scene, colour, -row length), or (scene, colour) when
+// by_index (the stable sort then keeps contact order inside a colour);
+// scene = 0 without scenes
+__global__ void k_sort_keys(int n, const int *colour, const int *rowptr, const int *scene, int by_index,
+                            unsigned long long *keys, int *vals)
 {
   int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
-  unsigned len = (unsigned)min(rowptr[i + 1] - rowptr[i], 0xffffff);
+  unsigned len = by_index ? 0u : (unsigned)min(rowptr[i + 1] - rowptr[i], 0xffffff);
   unsigned long long sc = scene ? (unsigned long long)(unsigned)scene[i] : 0ULL;
   keys[i] = (sc << 32) | ((unsigned long long)(unsigned)colour[i] << 24) | (0xffffffu - len);
   vals[i] = i;
@@ -300,12 +302,17 @@ __global__ void k_colour_count(int n, const int *colour, int *count, int ncolour
   for (int c = threadIdx.x; c < ncolours; c += blockDim.x) if (h[c]) atomicAdd(&count[c], h[c]);
 }
 
-__global__ void k_slice_len(int ns, const int *row0, const int *perm, const int *rowptr, int G, int *slen, long long *slen_ll)
+__global__ void k_slice_len(int ns, const int *row0, const int *nrows, const int *perm, const int *rowptr, int G,
+                            int *slen, long long *slen_ll)
 {
   int s = blockIdx.x*blockDim.x + threadIdx.x;
   if (s >= ns) return;
-  int i = perm[row0[s]];
-  int len = rowptr[i + 1] - rowptr[i];
+  int len = 0;   // longest row of the slice
+  for (int r = 0; r < nrows[s]; r ++)
+  {
+    int i = perm[row0[s] + r];
+    len = max(len, rowptr[i + 1] - rowptr[i]);
+  }
   int nslot = (len + G - 1) / G;   // slots of one block per lane
   slen[s] = nslot;
   slen_ll[s] = (long long)nslot * SLICE;
@@ -770,7 +777,7 @@ static int build_order(sfgs_ctx *ctx, int n)
   if (n > 0)
   {
     hipLaunchKernelGGL(k_sort_keys, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->colour.p, ctx->rowptr.p,
-                       scenes ? ctx->row_scene.p : (const int*)nullptr, ctx->skeys.p, ctx->svals.p);
+                       scenes ? ctx->row_scene.p : (const int*)nullptr, ctx->row_order, ctx->skeys.p, ctx->svals.p);
     size_t bytes = 0;
     HIPCHK(sfgs::sort_pairs_u64(nullptr, bytes, ctx->skeys.p, ctx->skeys2.p, ctx->svals.p, ctx->perm.p, n, keybits, ctx->stream));
     CHK(ensure(ctx, ctx->scratch, bytes));
@@ -813,7 +820,7 @@ static int build_order(sfgs_ctx *ctx, int n)
   CHK(ensure(ctx, lens, (size_t)ns + 1));
   HIPCHK(hipMemsetAsync(lens.p, 0, sizeof(long long)*((size_t)ns + 1), ctx->stream));
   if (ns > 0)
-    hipLaunchKernelGGL(k_slice_len, dim3(nblk(ns)), dim3(TPB), 0, ctx->stream, ns, ctx->slice_row0.p, ctx->perm.p,
+    hipLaunchKernelGGL(k_slice_len, dim3(nblk(ns)), dim3(TPB), 0, ctx->stream, ns, ctx->slice_row0.p, ctx->slice_nrows.p, ctx->perm.p,
                        ctx->rowptr.p, G, ctx->slice_len.p, lens.p);
   CHK(scan_ll(ctx, lens.p, ctx->slice_off.p, ns + 1));
   long long total = 0;
@@ -1532,6 +1539,13 @@ int SFGS_Set_Layout(sfgs_ctx *ctx, int lanes_per_row)
 {
   if (!ctx || (lanes_per_row != 1 && lanes_per_row != 2 && lanes_per_row != 4 && lanes_per_row != 8)) return SFGS_E_ARG;
   ctx->G = lanes_per_row;
+  return 0;
+}
+
+int SFGS_Set_Row_Order(sfgs_ctx *ctx, int order)
+{
+  if (!ctx || (order != 0 && order != 1)) return SFGS_E_ARG;
+  ctx->row_order = order;
   return 0;
 }
 

@@ -135,6 +135,8 @@ struct Model {
   float norm[2*NF + NO];   // feature mean, feature std, output scale
   float *dnorm = nullptr;  // device copy of norm
   int have = 0;
+  int new_only = 0;        // warm start only contacts without a carried reaction
+  float scale = 1.0f;      // trust factor of the prediction
   long long step = 0;
   // activation workspace for a batch
   int cap = 0;
@@ -228,12 +230,15 @@ __global__ void k_gather(int B, const int *idx, const float *tX, const float *tY
   for (int c = 0; c < NO; c ++) Y[(size_t)i*NO + c] = tY[(size_t)s*NO + c] / norm[2*NF + c];
 }
 
-// R0 = R_prev + s * f(features), rows in colour-major order (R has 4 doubles per row)
-__global__ void k_apply(int n, const float *O, const float *norm, double *R)
+// R0 = R_prev + scale * f(features), rows in colour-major order (R has 4
+// doubles per row); new_only leaves rows that carry a reaction unchanged
+__global__ void k_apply(int n, const float *O, const float *norm, double *R, int new_only, float scale)
 {
   int p = blockIdx.x*blockDim.x + threadIdx.x;
   if (p >= n) return;
-  for (int c = 0; c < NO; c ++) R[4*(size_t)p + c] += (double)(O[(size_t)p*NOP + c] * norm[2*NF + c]);
+  double *r = R + 4*(size_t)p;
+  if (new_only && (r[0] != 0.0 || r[1] != 0.0 || r[2] != 0.0)) return;
+  for (int c = 0; c < NO; c ++) r[c] += (double)(scale * O[(size_t)p*NOP + c] * norm[2*NF + c]);
 }
 
 __global__ void k_iota(int n, int *idx)
@@ -394,7 +399,7 @@ int sfgs_mlp_apply(sfgs_ctx *ctx, const float *feat, int n, double *R)
   hipLaunchKernelGGL(k_iota, dim3((n + 255)/256), dim3(256), 0, ctx->stream, n, m->idx);
   hipLaunchKernelGGL(k_gather, dim3((n + 255)/256), dim3(256), 0, ctx->stream, n, m->idx, feat, feat, dn, m->X, m->Y);
   forward(ctx, m, n);
-  hipLaunchKernelGGL(k_apply, dim3((n + 255)/256), dim3(256), 0, ctx->stream, n, m->O, dn, R);
+  hipLaunchKernelGGL(k_apply, dim3((n + 255)/256), dim3(256), 0, ctx->stream, n, m->O, dn, R, m->new_only, m->scale);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -427,6 +432,15 @@ int SFGS_Mlp_Set(sfgs_ctx *ctx, const float *params, long long nparams, const fl
   HIPCHK(hipStreamSynchronize(ctx->stream));
   m->have = 1;
   m->step = 0;
+  return 0;
+}
+
+int SFGS_Warm_Start_Mode(sfgs_ctx *ctx, int new_only, float scale)
+{
+  if (!ctx || (new_only != 0 && new_only != 1) || !(scale >= 0.0f)) return SFGS_E_ARG;
+  Model *m = model_of(ctx);
+  m->new_only = new_only;
+  m->scale = scale;
   return 0;
 }
 

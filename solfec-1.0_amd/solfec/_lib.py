@@ -59,6 +59,7 @@ _SFGS_PROTOS = {
     "SFGS_Body_Update": (_i, [_p, _d, _p, _p]),
     "SFGS_Set_Layout": (_i, [_p, _i]),
     "SFGS_Set_Schedule": (_i, [_p, _i]),
+    "SFGS_Set_Row_Order": (_i, [_p, _i]),
     "SFGS_Mlp_Set": (_i, [_p, _p, _ll, _p]),
     "SFGS_Mlp_Get": (_i, [_p, _p, _p]),
     "SFGS_Mlp_Forward": (_i, [_p, _p, _i, _p]),
@@ -66,6 +67,7 @@ _SFGS_PROTOS = {
     "SFGS_Mlp_Train": (_i, [_p, _p, _p, _ll, _i, _i, C.c_float, C.c_ulonglong, _p]),
     "SFGS_Features": (_i, [_p, _p]),
     "SFGS_Warm_Start": (_i, [_p]),
+    "SFGS_Warm_Start_Mode": (_i, [_p, _i, C.c_float]),
     "SFGS_Batch_Scenes": (_i, [_p, _i, _p]),
     "SFGS_Batch_Solve": (_i, [_p, C.POINTER(SFGS_PARAM), _p, _p, C.POINTER(SFGS_REPORT)]),
     "SFGS_Batch_Autotune": (_i, [_p, C.POINTER(SFGS_PARAM), _i, C.POINTER(SFGS_BATCH_TUNE), C.POINTER(_d)]),

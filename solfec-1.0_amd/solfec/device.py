@@ -72,6 +72,10 @@ class Device:
     def set_layout(self, lanes_per_row):
         self._check(self.lib.SFGS_Set_Layout(self.ctx, int(lanes_per_row)), "SFGS_Set_Layout")
 
+    def set_row_order(self, order):
+        """0: rows of a colour by decreasing length; 1: by contact index (spatial order)."""
+        self._check(self.lib.SFGS_Set_Row_Order(self.ctx, int(order)), "SFGS_Set_Row_Order")
+
     def set_schedule(self, schedule):
         """0: one launch per colour; 1: persistent dataflow sweeps (same iterates)."""
         self._check(self.lib.SFGS_Set_Schedule(self.ctx, int(schedule)), "SFGS_Set_Schedule")
@@ -162,6 +166,9 @@ class Device:
 
     def warm_start(self):
         self._check(self.lib.SFGS_Warm_Start(self.ctx), "SFGS_Warm_Start")
+
+    def warm_start_mode(self, new_only=0, scale=1.0):
+        self._check(self.lib.SFGS_Warm_Start_Mode(self.ctx, int(new_only), float(scale)), "SFGS_Warm_Start_Mode")
 
     def mlp_set(self, params, norm):
         params = np.ascontiguousarray(params, np.float32)

@@ -82,3 +82,31 @@ def test_flow_matches_oracle(dev):
     print(f"[flow] oracle iters {orep.iters} gpu {rep.iters} |dR|/max|R| {err:.2e}")
     assert abs(rep.iters - orep.iters) <= 1
     assert err < 1e-6
+
+
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_row_order_by_index_same_iterates(dev, lanes):
+    """Rows of a colour are independent: ordering them by contact index instead of
+    by length changes only the layout, not the reactions."""
+    s = lattice(1000, seed=2)
+    b, c = s.bodies(), s.contacts()
+    kw = dict(epsilon=1e-6, maxiter=3000, meritval=1e-6)
+    out = []
+    dev.set_layout(lanes)
+    for order in (0, 1):
+        dev.set_row_order(order)
+        dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
+        dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
+                     c["restitution"], c["cohesion"], c["R"], c["U"])
+        dev.assemble(1e-3)
+        rep, _, _ = dev.solve(solfec.make_param(**kw))
+        R, U = dev.get_RU()
+        out.append((rep, R, U, dev.locdyn()["colour"]))
+    dev.set_row_order(0)
+    dev.set_layout(1)
+    (r0, R0, U0, c0), (r1, R1, U1, c1) = out
+    print(f"[order] lanes {lanes}: iters {r0.iters} {r1.iters}")
+    assert np.array_equal(c0, c1)
+    assert abs(r0.iters - r1.iters) <= 1
+    if r0.iters == r1.iters:
+        assert np.array_equal(R0, R1) and np.array_equal(U0, U1)

@@ -98,6 +98,94 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
 
 
+def batch_scenes(nscenes, seed, lo=500, hi=50000):
+    """Independent scenes with contact counts log-uniform in [lo, hi] and mu uniform in
+    [0.1, 0.8]: each a tightly packed jittered lattice (spacing 0.09 m, radii
+    U[0.045, 0.055], density 2000) in its own box of 6 planes, at its first step."""
+    import solfec
+    rng = np.random.default_rng(seed)
+    want = np.exp(rng.uniform(np.log(lo), np.log(hi), nscenes))
+    mus = rng.uniform(0.1, 0.8, nscenes)
+    out = []
+    for k in range(nscenes):
+        n = max(8, int(round(want[k] / 3.0)))
+        side = int(np.ceil(round(n ** (1.0 / 3.0), 9)))
+        s = solfec.SOLFEC("DYNAMIC", 1e-3, "/tmp/solfec_bench")
+        solfec.SURFACE_MATERIAL(s, friction=float(mus[k]), restitution=0.0)
+        solfec.GRAVITY(s, (0.0, 0.0, -9.81))
+        solfec.LATTICE(s, n, 0.09 * side, 0.045, 0.055, 0.005, 2000.0, int(seed * 1000 + k))
+        s.step_begin()
+        out.append((s.bodies(), s.contacts()))
+    return out
+
+
+def bench_batch(args):
+    """Config 5: many independent scenes in one persistent launch (learned warm
+    start + mixed precision), against the same batch in FP64 without the network."""
+    import solfec
+    from solfec import batch, learned
+    h = 1e-3
+    t0 = time.perf_counter()
+    scenes = batch_scenes(args.scenes, args.seed)
+    B, Cn, bs, cs = batch.concat(scenes)
+    ncon = len(cs)
+    log(f"[batch] {args.scenes} scenes, {len(bs)} bodies, {ncon} contacts, built in {time.perf_counter() - t0:.1f}s")
+    dev = solfec.Device(0)
+    dev.set_layout(args.lanes)
+    ns = batch.upload(dev, B, Cn, bs, h)
+    have_model = os.path.exists(learned.DEFAULT_MODEL)
+    if have_model:
+        learned.install(dev)
+        dev.warm_start_mode(args.ws_new_only, args.ws_scale)
+    eps, maxiter = 1e-6, args.maxiter
+    pm = solfec.make_param(eps, maxiter, meritval=1e-6, precision=solfec.device.MIXED, refine=2)
+    pf = solfec.make_param(eps, maxiter, meritval=1e-6, precision=solfec.device.FP64)
+    tune, tms = dev.batch_autotune(pm, trial_iters=20)
+    log(f"[batch] autotuned shape {tune} ({tms:.2f} ms for 20 iterations)")
+
+    def run(par, learned_ws, steps, warmup):
+        batch.upload(dev, B, Cn, bs, h)   # every run starts from the scenes' own R0
+        def step():
+            dev.assemble(h)   # resident inputs; R restarts from the uploaded R0 (only get_RU writes it back)
+            if learned_ws:
+                dev.warm_start()
+            return dev.batch_solve(par, ns)
+        for _ in range(warmup):
+            step()
+        t = time.perf_counter()
+        for _ in range(steps):
+            reps, tot = step()
+        dt = (time.perf_counter() - t) / steps
+        R, _ = dev.get_RU()
+        return dict(ms_per_step=1e3 * dt, updates_per_s=tot.row_updates / dt, solve_ms=tot.solve_ms,
+                    max_iters=int(reps["iters"].max()), mean_iters=float(reps["iters"].mean()),
+                    converged=int(np.sum(reps["merit"] <= 1e-6)), merit_max=float(reps["merit"].max())), R, reps
+
+    head, Rh, rh = run(pm, have_model, args.steps, args.warmup)
+    log(f"[batch] learned+mixed: {head['ms_per_step']:.1f} ms/step, iters max {head['max_iters']}")
+    plain, Rp, rp = run(pf, False, args.steps, args.warmup)
+    log(f"[batch] plain FP64: {plain['ms_per_step']:.1f} ms/step, iters max {plain['max_iters']}")
+    # per-scene agreement with the FP64 run, relative to each scene's max|R|
+    errs = []
+    for Ra, Rb_ in zip(batch.split(Rh, cs, 3), batch.split(Rp, cs, 3)):
+        sc = np.abs(Rb_).max() if len(Rb_) else 0.0
+        errs.append(float(np.abs(Ra - Rb_).max() / sc) if sc > 0 else 0.0)
+    out = {
+        "metric": "batched solve: contact-block updates/s, learned+mixed vs plain FP64 (config 5)",
+        "value": head["updates_per_s"], "unit": "contact-block updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{args.scenes} independent lattice scenes, contacts log-uniform [500, 50000], "
+                               "one persistent launch", "scenes": args.scenes, "contacts": ncon,
+                   "epsilon": eps, "meritval": 1e-6, "maxiter": maxiter, "shape": list(tune),
+                   "learned_model": have_model, "learned_mixed": head, "plain_fp64": plain,
+                   "speedup_vs_plain_fp64": plain["ms_per_step"] / head["ms_per_step"],
+                   "scene_err_vs_fp64_max": max(errs), "scene_err_vs_fp64_median": float(np.median(errs))},
+    }
+    print(json.dumps(out), flush=True)
+    dev.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,13 +205,25 @@ def main():
     ap.add_argument("--headline", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--modes", default="fp64,mixed", help="solve modes to time (comma list of fp64, mixed)")
     ap.add_argument("--maxiter", type=int, default=5000, help="iteration bound (5000 is the workload's)")
-    ap.add_argument("--row-order", type=int, default=0, choices=[0, 1],
-                    help="rows inside a colour: 0 by decreasing length, 1 by contact index")
-    ap.add_argument("--schedule", type=int, default=0, choices=[0, 1],
-                    help="sweep schedule: 0 one launch per colour, 1 persistent dataflow")
+    ap.add_argument("--row-order", type=int, default=0, choices=[0, 1, 2],
+                    help="rows inside a colour: 0 by decreasing length, 1 by contact index, 2 by body block")
+    ap.add_argument("--schedule", type=int, default=0, choices=[0, 1, 2],
+                    help="sweep schedule: 0 one launch per colour (XCD-aware), 1 persistent dataflow, "
+                         "2 one launch per colour (hardware order)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (scale of the traffic counters) and exit")
+    ap.add_argument("--workload", default="packing", choices=["packing", "batch"],
+                    help="packing: the 200k-sphere settled packing (headline); batch: independent scenes")
+    ap.add_argument("--scenes", type=int, default=256)
+    ap.add_argument("--ws-new-only", type=int, default=0, help="learned warm start only on new contacts")
+    ap.add_argument("--ws-scale", type=float, default=1.0, help="trust factor of the learned warm start")
     args = ap.parse_args()
+    if args.workload == "batch":
+        if args.gpus != 1:
+            print("bench.py measures one GPU", file=sys.stderr)
+            sys.exit(2)
+        bench_batch(args)
+        return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
@@ -204,7 +304,7 @@ def main():
     achieved = bytes_per_sweep * sweeps_run / (prep.sweep_ms * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                kernel=("k_flow<%s>" if args.schedule else "k_sweep<%s>") % ("float" if head == "mixed" else "double"),
+                kernel=("k_flow<%s>" if args.schedule == 1 else "k_sweep<%s>") % ("float" if head == "mixed" else "double"),
                 avg_launch_us=round(1e3 * prep.sweep_ms / max(prep.sweep_launches, 1), 3),
                 launches=int(prep.sweep_launches), bytes_per_sweep=int(bytes_per_sweep))
     log(f"[bench] sweep kernel: {prep.sweep_launches} launches, {prep.sweep_ms:.2f} ms, {achieved:.1f} GB/s")
@@ -235,8 +335,8 @@ def main():
             "epsilon": 1e-6, "meritval": 1e-6, "maxiter": args.maxiter, "friction": args.friction,
             "iters_per_step": iters, "merit": modes[head]["merit"], "rel_error": modes[head]["rel_error"],
             "precision": head, "warm_start": "previous-step R", "lanes_per_row": args.lanes,
-            "schedule": "dataflow" if args.schedule else "per-colour launches",
-            "row_order": "contact index" if args.row_order else "decreasing row length",
+            "schedule": ["per-colour launches, XCD-aware", "dataflow", "per-colour launches"][args.schedule],
+            "row_order": ["decreasing row length", "contact index", "body block, row length"][args.row_order],
             "modes": modes,
             "speedup_vs_plain_fp64": modes["fp64"]["ms_per_step"] / ms_per_step if "fp64" in modes else None,
         },

@@ -143,18 +143,21 @@ int SFGS_Get_Sizes(sfgs_ctx *ctx, int *n, long long *nnz, int *ncolours);
 int SFGS_Set_Layout(sfgs_ctx *ctx, int lanes_per_row);
 
 /* Schedule of the Gauss-Seidel sweeps of SFGS_Solve (no counterpart in the
- * reference, which sweeps serially): 0 = one kernel launch per colour
- * (default); 1 = one persistent dataflow launch in which each row waits only
- * for its own neighbours, with the same iterates (forward sweeps; a solve with
- * reverse = 1, or with 4 or 8 lanes per row, uses schedule 0). */
+ * reference, which sweeps serially): 0 = one kernel launch per colour, each
+ * XCD taking a contiguous run of the colour's rows (default); 1 = one
+ * persistent dataflow launch in which each row waits only for its own
+ * neighbours, with the same iterates (forward sweeps; a solve with reverse = 1,
+ * or with 4 or 8 lanes per row, uses schedule 0); 2 = one launch per colour in
+ * the hardware's workgroup order. */
 int SFGS_Set_Schedule(sfgs_ctx *ctx, int schedule);
 
 /* Order of the rows inside a colour (no counterpart in the reference): 0 =
- * decreasing row length (least slice padding, default); 1 = contact index,
- * which keeps the detection's spatial order so that the rows of a wavefront
- * gather their neighbours' reactions from fewer cache lines.  Takes effect at
- * the next assembly; the iterates do not depend on it (rows of a colour are
- * independent). */
+ * decreasing row length (least slice padding, default); 1 = contact index;
+ * 2 = master body in blocks of 64 bodies, then decreasing row length, which
+ * keeps the rows of nearby bodies together (their neighbours' reactions then
+ * share cache lines) while slices stay of even length (needs a system
+ * assembled from bodies; otherwise 0).  Takes effect at the next assembly; the
+ * iterates do not depend on it (rows of a colour are independent). */
 int SFGS_Set_Row_Order(sfgs_ctx *ctx, int order);
 
 /* Colour an arbitrary graph given in CSR (no self loops) with the device

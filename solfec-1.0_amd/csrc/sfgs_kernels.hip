@@ -252,17 +252,28 @@ __global__ void k_fill_int(int n, int v, int *b)
   if (i < n) b[i] = v;
 }
 
-// sort key of row i: (scene, colour, -row length), or (scene, colour) when
-// by_index (the stable sort then keeps contact order inside a colour);
-// scene = 0 without scenes
-__global__ void k_sort_keys(int n, const int *colour, const int *rowptr, const int *scene, int by_index,
-                            unsigned long long *keys, int *vals)
+// sort key of row i, by order mode: 0 (scene, colour, -row length); 1 (scene,
+// colour), the stable sort keeping contact order inside a colour; 2 (scene,
+// colour, master body / 64, -row length), which keeps rows of nearby bodies
+// together while slices stay of even length.  scene = 0 without scenes
+__global__ void k_sort_keys(int n, const int *colour, const int *rowptr, const int *scene, int mode,
+                            const int *master, unsigned long long *keys, int *vals)
 {
   int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
-  unsigned len = by_index ? 0u : (unsigned)min(rowptr[i + 1] - rowptr[i], 0xffffff);
-  unsigned long long sc = scene ? (unsigned long long)(unsigned)scene[i] : 0ULL;
-  keys[i] = (sc << 32) | ((unsigned long long)(unsigned)colour[i] << 24) | (0xffffffu - len);
+  const int len = rowptr[i + 1] - rowptr[i];
+  const unsigned long long sc = scene ? (unsigned long long)(unsigned)scene[i] : 0ULL;
+  const unsigned long long col = (unsigned long long)(unsigned)colour[i];
+  if (mode == 2)
+  {
+    const unsigned long long blk = (unsigned long long)((unsigned)master[i] >> 6) & 0xffffffULL;
+    keys[i] = (sc << 40) | (col << 32) | (blk << 8) | (0xffu - (unsigned)min(len, 0xff));
+  }
+  else
+  {
+    const unsigned l24 = mode == 1 ? 0u : (unsigned)min(len, 0xffffff);
+    keys[i] = (sc << 32) | (col << 24) | (0xffffffu - (mode == 1 ? 0xffffffu : l24));
+  }
   vals[i] = i;
 }
 
@@ -522,12 +533,23 @@ __global__ __launch_bounds__(TPB) void k_offdiag_sys(SysIn si, SellOut so, const
   }
 }
 
+// Workgroup b of a launch runs on XCD b mod 8.  With xcd set, the blocks of
+// one XCD take one contiguous run of the colour's slices, so the rows an XCD
+// updates are neighbours in the row order, and (with a spatial row order)
+// their neighbours' reactions are shared in that XCD's L2.
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+  const int x = b & 7, k = b >> 3, q = nb >> 3, rm = nb & 7;
+  return x*q + min(x, rm) + k;
+}
+
 template <typename T, int G, bool MERIT>
 __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<T> a, int s0, int s1, int pbase, int upd_lo, int upd_hi,
-                                                int backward)
+                                                int backward, int xcd)
 {
   if (a.ctrl->done) return;   // uniform: set only by k_finish between launches
-  const int s = s0 + blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6);
+  const int bb = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int s = s0 + bb*(TPB/SLICE) + (threadIdx.x >> 6);
   double eu = 0.0, el = 0.0, up = 0.0;
   sweep_slice<T, G, MERIT>(a, s, s < s1, upd_lo, upd_hi, backward, &a.ctrl->sweeps, a.failkey, a.failcount,
                            &a.ctrl->aborted, eu, el, up);
@@ -749,7 +771,7 @@ static int colour_graph(sfgs_ctx *ctx, int n)
 }
 
 // permutation into colour-major order and the SELL-64 slice table
-static int build_order(sfgs_ctx *ctx, int n)
+static int build_order(sfgs_ctx *ctx, int n, bool have_master)
 {
   const int G = ctx->G, RPS = SLICE / G;
   const bool scenes = ctx->nscenes > 1;
@@ -759,7 +781,9 @@ static int build_order(sfgs_ctx *ctx, int n)
   CHK(ensure(ctx, ctx->svals, n));
   CHK(ensure(ctx, ctx->perm, n));
   CHK(ensure(ctx, ctx->iperm, n));
-  int keybits = 32;
+  // order 2 needs the master bodies of the contacts (a system uploaded as W has none)
+  const int mode = ctx->row_order == 2 && !have_master ? 0 : ctx->row_order;
+  int keybits = mode == 2 ? 40 : 32;
   if (scenes)
   {
     for (int v = ctx->nscenes - 1; v > 0; v >>= 1) keybits ++;
@@ -777,7 +801,8 @@ static int build_order(sfgs_ctx *ctx, int n)
   if (n > 0)
   {
     hipLaunchKernelGGL(k_sort_keys, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->colour.p, ctx->rowptr.p,
-                       scenes ? ctx->row_scene.p : (const int*)nullptr, ctx->row_order, ctx->skeys.p, ctx->svals.p);
+                       scenes ? ctx->row_scene.p : (const int*)nullptr, mode, ctx->master.p, ctx->skeys.p,
+                       ctx->svals.p);
     size_t bytes = 0;
     HIPCHK(sfgs::sort_pairs_u64(nullptr, bytes, ctx->skeys.p, ctx->skeys2.p, ctx->svals.p, ctx->perm.p, n, keybits, ctx->stream));
     CHK(ensure(ctx, ctx->scratch, bytes));
@@ -1073,7 +1098,7 @@ int SFGS_Locdyn_Update_Begin(sfgs_ctx *ctx, double step, int dynamic)
 
   // colouring and colour-major layout
   CHK(colour_graph(ctx, nc));
-  CHK(build_order(ctx, nc));
+  CHK(build_order(ctx, nc, true));
   CHK(ensure_rows(ctx, nc));
 
   // rows and blocks
@@ -1143,7 +1168,7 @@ int SFGS_System_Upload(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, 
   CHK(upload(ctx, ctx->Rn, R, 3*(size_t)n));
   CHK(upload(ctx, ctx->Un, U, 3*(size_t)n));
   int rc = colour_graph(ctx, n);
-  if (!rc) rc = build_order(ctx, n);
+  if (!rc) rc = build_order(ctx, n, false);
   if (!rc) rc = ensure_rows(ctx, n);
   if (!rc && n > 0)
   {
@@ -1178,6 +1203,7 @@ static void launch_sweep(sfgs_ctx *ctx, const SweepArgs<T> &base, T *X0, T *X1, 
                          const std::vector<int> &grid, const std::vector<int> &pbase, bool prof)
 {
   const int ncol = ctx->ncolours, n = ctx->nc;
+  const int xcd = ctx->schedule == 2 ? 0 : 1;
   SweepArgs<T> a = base;
   if (k & 1) { a.P = X1; a.Q = X0; }
   else { a.P = X0; a.Q = X1; }
@@ -1190,12 +1216,14 @@ static void launch_sweep(sfgs_ctx *ctx, const SweepArgs<T> &base, T *X0, T *X1, 
     if (merit)
     {
       SFGS_DISPATCH_G(ctx->G_built, auto kern = k_sweep<T COMMA GG COMMA true>; hipLaunchKernelGGL(kern, dim3(grid[c]),
-                      dim3(TPB), 0, ctx->stream, a, ctx->h_cslice[c], ctx->h_cslice[c + 1], pbase[c], lo, hi, backward));
+                      dim3(TPB), 0, ctx->stream, a, ctx->h_cslice[c], ctx->h_cslice[c + 1], pbase[c], lo, hi, backward,
+                      xcd));
     }
     else
     {
       SFGS_DISPATCH_G(ctx->G_built, auto kern = k_sweep<T COMMA GG COMMA false>; hipLaunchKernelGGL(kern, dim3(grid[c]),
-                      dim3(TPB), 0, ctx->stream, a, ctx->h_cslice[c], ctx->h_cslice[c + 1], pbase[c], lo, hi, backward));
+                      dim3(TPB), 0, ctx->stream, a, ctx->h_cslice[c], ctx->h_cslice[c + 1], pbase[c], lo, hi, backward,
+                      xcd));
     }
     if (prof) (void)hipEventRecord(ctx->prof_ev[2*ci + 1], ctx->stream);
   }
@@ -1530,7 +1558,7 @@ int SFGS_Warm_Start(sfgs_ctx *ctx)
 
 int SFGS_Set_Schedule(sfgs_ctx *ctx, int schedule)
 {
-  if (!ctx || (schedule != 0 && schedule != 1)) return SFGS_E_ARG;
+  if (!ctx || schedule < 0 || schedule > 2) return SFGS_E_ARG;
   ctx->schedule = schedule;
   return 0;
 }
@@ -1544,7 +1572,7 @@ int SFGS_Set_Layout(sfgs_ctx *ctx, int lanes_per_row)
 
 int SFGS_Set_Row_Order(sfgs_ctx *ctx, int order)
 {
-  if (!ctx || (order != 0 && order != 1)) return SFGS_E_ARG;
+  if (!ctx || order < 0 || order > 2) return SFGS_E_ARG;
   ctx->row_order = order;
   return 0;
 }

@@ -144,3 +144,37 @@ def compare_iterations(dev, b, c, step=1e-3, epsilon=1e-6, maxiter=5000, meritva
     dev.warm_start()
     learned, _, _ = dev.solve(par, history=False)
     return base, learned
+
+
+def train_default(dev, seeds=(1, 2, 3, 4, 5, 6), steps=100, new_only=True, epochs=60, batch=1024, lr=2e-3, seed=1,
+                  log=print):
+    """The shipped model: snapshots of config-1 trajectories (seeded lattices of
+    1000 spheres, friction U[0.1, 0.8] drawn from each seed, GS to 1e-6), the
+    samples of new contacts only, Adam on the GPU."""
+    Xs, Ys = [], []
+    for sd in seeds:
+        mu = float(np.random.default_rng(sd).uniform(0.1, 0.8))
+        x, y = snapshots(int(sd), mu, steps=steps, epsilon=1e-6, maxiter=3000)
+        Xs.append(x)
+        Ys.append(y)
+        log(f"[learned] seed {sd} mu {mu:.3f}: {len(x)} samples")
+    X, Y = np.concatenate(Xs), np.concatenate(Ys)
+    if new_only:
+        keep = np.all(X[:, 13:16] == 0, axis=1)
+        X, Y = X[keep], Y[keep]
+    params, norm, hist = train(dev, X, Y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    log(f"[learned] {len(X)} samples, loss {hist[0]:.4f} -> {hist[-1]:.4f}")
+    return params, norm, dict(seeds=list(seeds), steps=steps, new_only=new_only, epochs=epochs, batch=batch, lr=lr,
+                              samples=len(X), loss_first=float(hist[0]), loss_last=float(hist[-1]))
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="train the warm-start network on the GPU and save it")
+    ap.add_argument("--out", default=DEFAULT_MODEL)
+    ap.add_argument("--steps", type=int, default=100)
+    args = ap.parse_args()
+    d = solfec.Device(0)
+    p, nrm, meta = train_default(d, steps=args.steps)
+    save(args.out, p, nrm, meta)
+    print(f"saved {args.out}")

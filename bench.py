@@ -41,12 +41,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def settle(n, L, steps, seed, friction):
+def settle(n, L, steps, seed, friction, shape="sphere"):
     import solfec
     s = solfec.SOLFEC("DYNAMIC", 1e-3, "/tmp/solfec_bench")
     solfec.SURFACE_MATERIAL(s, friction=friction, restitution=0.0)
     solfec.GRAVITY(s, (0.0, 0.0, -9.81))
-    solfec.LATTICE(s, n, L, 0.045, 0.055, 0.005, 2000.0, seed)
+    if shape == "ellipsoid":   # semi-axes U[0.03, 0.06], random orientations
+        solfec.ELLIP_LATTICE(s, n, L, 0.03, 0.06, 0.005, 2000.0, seed)
+    else:
+        solfec.LATTICE(s, n, L, 0.045, 0.055, 0.005, 2000.0, seed)
     gs = solfec.GAUSS_SEIDEL_SOLVER(1e-4, 1000)
     t0 = time.perf_counter()
     for k in range(steps):
@@ -82,6 +85,22 @@ def cpu_baseline(b, c, colours, n, seconds):
                        f"system, oracle/liboracle.so on 1 host core, {dt:.1f} s")
 
 
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of the headline kernel from the committed rocprofv3 --pmc
+    summary of this workload and shape (FETCH_SIZE and WRITE_SIZE scaled by a
+    device copy of known size), or None when none matches."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_sweep_config2_traffic.md")
+    if args.schedule == 1 or not os.path.exists(path):
+        return None
+    key = f"{kernel} lanes={args.lanes} order={args.row_order}"
+    for line in open(path):
+        m = re.match(r"\|\s*`([^`]+)`\s*\|\s*([0-9.e+]+)\s*\|", line)
+        if m and m.group(1) == key:
+            return float(m.group(2))
+    return None
+
+
 def copy_calibration(mib):
     """Device-to-device copies of a known size, for scaling HBM traffic counters."""
     import ctypes as C
@@ -100,20 +119,21 @@ def copy_calibration(mib):
 
 def batch_scenes(nscenes, seed, lo=500, hi=50000):
     """Independent scenes with contact counts log-uniform in [lo, hi] and mu uniform in
-    [0.1, 0.8]: each a tightly packed jittered lattice (spacing 0.09 m, radii
-    U[0.045, 0.055], density 2000) in its own box of 6 planes, at its first step."""
+    [0.1, 0.8]: each the config-1 generator (jittered lattice of spacing 0.1 m, radii
+    U[0.045, 0.055], density 2000, 6 planes) sized for its count (about 1.52 contacts
+    per sphere at the first step), at its first step."""
     import solfec
     rng = np.random.default_rng(seed)
     want = np.exp(rng.uniform(np.log(lo), np.log(hi), nscenes))
     mus = rng.uniform(0.1, 0.8, nscenes)
     out = []
     for k in range(nscenes):
-        n = max(8, int(round(want[k] / 3.0)))
+        n = max(8, int(round(want[k] / 1.52)))
         side = int(np.ceil(round(n ** (1.0 / 3.0), 9)))
         s = solfec.SOLFEC("DYNAMIC", 1e-3, "/tmp/solfec_bench")
         solfec.SURFACE_MATERIAL(s, friction=float(mus[k]), restitution=0.0)
         solfec.GRAVITY(s, (0.0, 0.0, -9.81))
-        solfec.LATTICE(s, n, 0.09 * side, 0.045, 0.055, 0.005, 2000.0, int(seed * 1000 + k))
+        solfec.LATTICE(s, n, 0.1 * side, 0.045, 0.055, 0.005, 2000.0, int(seed * 1000 + k))
         s.step_begin()
         out.append((s.bodies(), s.contacts()))
     return out
@@ -159,17 +179,22 @@ def bench_batch(args):
         R, _ = dev.get_RU()
         return dict(ms_per_step=1e3 * dt, updates_per_s=tot.row_updates / dt, solve_ms=tot.solve_ms,
                     max_iters=int(reps["iters"].max()), mean_iters=float(reps["iters"].mean()),
-                    converged=int(np.sum(reps["merit"] <= 1e-6)), merit_max=float(reps["merit"].max())), R, reps
+                    groups=int(reps["groups"].sum()),
+                    converged=int(np.sum((reps["merit"] <= 1e-6) & (reps["error"] <= eps))),
+                    merit_max=float(reps["merit"].max())), R, reps
 
     head, Rh, rh = run(pm, have_model, args.steps, args.warmup)
     log(f"[batch] learned+mixed: {head['ms_per_step']:.1f} ms/step, iters max {head['max_iters']}")
     plain, Rp, rp = run(pf, False, args.steps, args.warmup)
     log(f"[batch] plain FP64: {plain['ms_per_step']:.1f} ms/step, iters max {plain['max_iters']}")
-    # per-scene agreement with the FP64 run, relative to each scene's max|R|
+    # per-scene agreement with the FP64 run, relative to each scene's max|R|, where both converged
     errs = []
-    for Ra, Rb_ in zip(batch.split(Rh, cs, 3), batch.split(Rp, cs, 3)):
+    ok = (rh["iters"] - 2 < maxiter) & (rp["iters"] < maxiter) & (rh["rows"] > 0)
+    for k, (Ra, Rb_) in enumerate(zip(batch.split(Rh, cs, 3), batch.split(Rp, cs, 3))):
         sc = np.abs(Rb_).max() if len(Rb_) else 0.0
-        errs.append(float(np.abs(Ra - Rb_).max() / sc) if sc > 0 else 0.0)
+        if ok[k] and sc > 0:
+            errs.append(float(np.abs(Ra - Rb_).max() / sc))
+    errs = errs or [float("nan")]
     out = {
         "metric": "batched solve: contact-block updates/s, learned+mixed vs plain FP64 (config 5)",
         "value": head["updates_per_s"], "unit": "contact-block updates/s", "n_gpus": 1, "steps": args.steps,
@@ -180,6 +205,7 @@ def bench_batch(args):
                    "epsilon": eps, "meritval": 1e-6, "maxiter": maxiter, "shape": list(tune),
                    "learned_model": have_model, "learned_mixed": head, "plain_fp64": plain,
                    "speedup_vs_plain_fp64": plain["ms_per_step"] / head["ms_per_step"],
+                   "scenes_converged_both": int(ok.sum()),
                    "scene_err_vs_fp64_max": max(errs), "scene_err_vs_fp64_median": float(np.median(errs))},
     }
     print(json.dumps(out), flush=True)
@@ -198,25 +224,28 @@ def main():
     ap.add_argument("--friction", type=float, default=0.5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--lanes", type=int, default=1, help="lanes of a wavefront per contact row (1, 2, 4, 8)")
+    ap.add_argument("--lanes", type=int, default=2, help="lanes of a wavefront per contact row (1, 2, 4, 8)")
     ap.add_argument("--save-snapshot", default=None, help="write the settled state to this .npz")
     ap.add_argument("--snapshot", default=None, help="load a settled state instead of settling")
     ap.add_argument("--no-mixed", dest="mixed", action="store_false", help="skip the mixed-precision mode")
-    ap.add_argument("--headline", default="mixed", choices=["mixed", "fp64"])
-    ap.add_argument("--modes", default="fp64,mixed", help="solve modes to time (comma list of fp64, mixed)")
+    ap.add_argument("--headline", default="learned", choices=["learned", "mixed", "fp64"])
+    ap.add_argument("--modes", default="fp64,mixed,learned",
+                    help="solve modes to time (comma list of fp64, mixed, learned = learned warm start + mixed)")
     ap.add_argument("--maxiter", type=int, default=5000, help="iteration bound (5000 is the workload's)")
-    ap.add_argument("--row-order", type=int, default=0, choices=[0, 1, 2],
+    ap.add_argument("--row-order", type=int, default=2, choices=[0, 1, 2],
                     help="rows inside a colour: 0 by decreasing length, 1 by contact index, 2 by body block")
     ap.add_argument("--schedule", type=int, default=0, choices=[0, 1, 2],
                     help="sweep schedule: 0 one launch per colour (XCD-aware), 1 persistent dataflow, "
                          "2 one launch per colour (hardware order)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (scale of the traffic counters) and exit")
+    ap.add_argument("--shape", default="sphere", choices=["sphere", "ellipsoid"],
+                    help="packing bodies: spheres (config 2) or rigid ellipsoids (the config-3 variant)")
     ap.add_argument("--workload", default="packing", choices=["packing", "batch"],
                     help="packing: the 200k-sphere settled packing (headline); batch: independent scenes")
     ap.add_argument("--scenes", type=int, default=256)
-    ap.add_argument("--ws-new-only", type=int, default=0, help="learned warm start only on new contacts")
-    ap.add_argument("--ws-scale", type=float, default=1.0, help="trust factor of the learned warm start")
+    ap.add_argument("--ws-new-only", type=int, default=1, help="learned warm start only on new contacts")
+    ap.add_argument("--ws-scale", type=float, default=0.5, help="trust factor of the learned warm start")
     args = ap.parse_args()
     if args.workload == "batch":
         if args.gpus != 1:
@@ -239,7 +268,7 @@ def main():
         b = {k[2:]: z[k] for k in z.files if k.startswith("b_")}
         c = {k[2:]: z[k] for k in z.files if k.startswith("c_")}
     else:
-        s = settle(args.spheres, args.box, args.settle, args.seed, args.friction)
+        s = settle(args.spheres, args.box, args.settle, args.seed, args.friction, args.shape)
         b, c = s.bodies(), s.contacts()
         if args.save_snapshot:
             np.savez(args.save_snapshot, **{"b_" + k: v for k, v in b.items()}, **{"c_" + k: v for k, v in c.items()})
@@ -253,11 +282,16 @@ def main():
                  c["restitution"], c["cohesion"], c["R"], c["U"])
     n, nnz, ncol = 0, 0, 0
 
-    def run_mode(name, precision, steps, warmup):
+    def run_mode(name, precision, steps, warmup, learned_ws=False):
         par = solfec.make_param(1e-6, args.maxiter, meritval=1e-6, precision=precision, refine=2)
+        # every mode starts from the step's own R0 (reading R back overwrites the uploaded one)
+        dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
+                     c["restitution"], c["cohesion"], c["R"], c["U"])
 
         def step():
             dev.assemble(h)
+            if learned_ws:
+                dev.warm_start()
             rep, _, _ = dev.solve(par, history=False)
             dev.body_update(h, nb, download=False)
             return rep
@@ -287,24 +321,38 @@ def main():
         modes["mixed"], Rmx = run_mode("mixed FP32/FP64 GS", solfec.device.MIXED, args.steps, args.warmup)
         if R64 is not None:
             modes["mixed"]["err_vs_fp64"] = float(np.abs(Rmx - R64).max() / max(np.abs(R64).max(), 1e-300))
-    head = "mixed" if "mixed" in modes and args.headline == "mixed" else "fp64"
+    if "learned" in want:
+        from solfec import learned
+        learned.install(dev)
+        dev.warm_start_mode(args.ws_new_only, args.ws_scale)
+        modes["learned"], Rl = run_mode("learned warm start + mixed GS", solfec.device.MIXED, args.steps, args.warmup,
+                                        learned_ws=True)
+        if R64 is not None:
+            modes["learned"]["err_vs_fp64"] = float(np.abs(Rl - R64).max() / max(np.abs(R64).max(), 1e-300))
+    head = args.headline if args.headline in modes else ("mixed" if "mixed" in modes else "fp64")
     n, nnz, ncol = dev.sizes()
     value = modes[head]["updates_per_s"]
     ms_per_step = modes[head]["ms_per_step"]
     iters = modes[head]["iters"]
 
     # roofline of the sweep kernel of the headline mode: one profiled solve, HIP events around every launch
+    is32 = head in ("mixed", "learned")
+    dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
+                 c["restitution"], c["cohesion"], c["R"], c["U"])
     dev.assemble(h)
-    prec = solfec.device.MIXED if head == "mixed" else solfec.device.FP64
+    if head == "learned":
+        dev.warm_start()
+    prec = solfec.device.MIXED if is32 else solfec.device.FP64
     prof = solfec.make_param(1e-6, args.maxiter, meritval=1e-6, profile=1, precision=prec, refine=2)
     prep, _, _ = dev.solve(prof, history=False)
-    sz = 4 if head == "mixed" else 8
+    sz = 4 if is32 else 8
     bytes_per_sweep = sz * (9 * nnz + 26 * n) + 4 * nnz + 4 * n
     sweeps_run = prep.row_updates / max(n, 1)
     achieved = bytes_per_sweep * sweeps_run / (prep.sweep_ms * 1e-3) / 1e9
+    kname = ("k_flow<%s>" if args.schedule == 1 else "k_sweep<%s>") % ("float" if is32 else "double")
     roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                kernel=("k_flow<%s>" if args.schedule == 1 else "k_sweep<%s>") % ("float" if head == "mixed" else "double"),
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, args),
+                kernel=kname,
                 avg_launch_us=round(1e3 * prep.sweep_ms / max(prep.sweep_launches, 1), 3),
                 launches=int(prep.sweep_launches), bytes_per_sweep=int(bytes_per_sweep))
     log(f"[bench] sweep kernel: {prep.sweep_launches} launches, {prep.sweep_ms:.2f} ms, {achieved:.1f} GB/s")
@@ -326,15 +374,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if head == "mixed" else "f64",
+        "dtype": "f32+f64" if is32 else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.spheres}-sphere packing settled {args.settle} steps, "
-                        + ("mixed FP32/FP64 GPU GS" if head == "mixed" else "plain FP64 GPU GS"),
+            "workload": f"{args.spheres}-{args.shape} packing settled {args.settle} steps, "
+                        + {"learned": "learned warm start + mixed FP32/FP64 GPU GS", "mixed": "mixed FP32/FP64 GPU GS",
+                           "fp64": "plain FP64 GPU GS"}[head],
             "spheres": args.spheres, "contacts": n, "offdiag_blocks": nnz, "colours": ncol,
             "epsilon": 1e-6, "meritval": 1e-6, "maxiter": args.maxiter, "friction": args.friction,
             "iters_per_step": iters, "merit": modes[head]["merit"], "rel_error": modes[head]["rel_error"],
-            "precision": head, "warm_start": "previous-step R", "lanes_per_row": args.lanes,
+            "precision": "mixed" if is32 else "fp64",
+            "warm_start": "learned (new contacts, trust %.2g)" % args.ws_scale if head == "learned" else "previous-step R",
+            "lanes_per_row": args.lanes,
             "schedule": ["per-colour launches, XCD-aware", "dataflow", "per-colour launches"][args.schedule],
             "row_order": ["decreasing row length", "contact index", "body block, row length"][args.row_order],
             "modes": modes,

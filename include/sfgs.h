@@ -138,7 +138,7 @@ int SFGS_Get_Locdyn(sfgs_ctx *ctx, int *rowptr, int *col, int *via, double *W, d
 int SFGS_Get_Sizes(sfgs_ctx *ctx, int *n, long long *nnz, int *ncolours);
 
 /* Lanes of a wavefront that serve one contact row in the sweep layout (1, 2,
- * 4 or 8; default 1).  Takes effect at the next LOCDYN assembly or system
+ * 4 or 8; default 2).  Takes effect at the next LOCDYN assembly or system
  * upload.  No counterpart in the reference (a tuning knob of the GPU layout). */
 int SFGS_Set_Layout(sfgs_ctx *ctx, int lanes_per_row);
 
@@ -152,8 +152,8 @@ int SFGS_Set_Layout(sfgs_ctx *ctx, int lanes_per_row);
 int SFGS_Set_Schedule(sfgs_ctx *ctx, int schedule);
 
 /* Order of the rows inside a colour (no counterpart in the reference): 0 =
- * decreasing row length (least slice padding, default); 1 = contact index;
- * 2 = master body in blocks of 64 bodies, then decreasing row length, which
+ * decreasing row length (least slice padding); 1 = contact index; 2 (default)
+ * = master body in blocks of 64 bodies, then decreasing row length, which
  * keeps the rows of nearby bodies together (their neighbours' reactions then
  * share cache lines) while slices stay of even length (needs a system
  * assembled from bodies; otherwise 0).  Takes effect at the next assembly; the
@@ -213,7 +213,8 @@ int SFGS_Warm_Start_Mode(sfgs_ctx *ctx, int new_only, float scale);
 typedef struct {
   int lanes_per_row;    /* layout the system was assembled with (SFGS_Set_Layout): 1 or 2 */
   int waves_per_group;  /* wavefronts per workgroup: 1, 2, 4 or 8 */
-  int rows_per_group;   /* target rows of a scene per workgroup; sets each scene's workgroup count */
+  int rows_per_group;   /* target rows of a scene per workgroup; 0 = share the resident workgroups
+                           among the scenes in proportion to their rows (one launch) */
 } SFGS_BATCH_TUNE;
 
 typedef struct {
@@ -235,8 +236,8 @@ int SFGS_Batch_Scenes(sfgs_ctx *ctx, int nscenes, const int *body_scene);
 int SFGS_Batch_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TUNE *tune, SFGS_SCENE_REPORT *reports,
                      SFGS_REPORT *total);
 /* time trial_iters iterations of every candidate shape (lanes 1 and 2, which
- * re-assembles the system, waves 1..8, rows per group 256..all) and keep the
- * fastest; R and U are restored after each trial */
+ * re-assembles the system, waves 1..8, rows per group proportional, 1024, 4096
+ * or all) and keep the fastest; R and U are restored after each trial */
 int SFGS_Batch_Autotune(sfgs_ctx *ctx, const SFGS_PARAM *par, int trial_iters, SFGS_BATCH_TUNE *best, double *best_ms);
 
 #ifdef __cplusplus

@@ -70,6 +70,11 @@ void SOLFEC_Gravity(SOLFEC *sol, double gx, double gy, double gz);
 int SOLFEC_Surface_Material(SOLFEC *sol, int s1, int s2, double friction, double restitution, double cohesion);
 /* rigid sphere of given density; returns the body index */
 int SOLFEC_Sphere(SOLFEC *sol, const double *centre, double radius, double density, int surface);
+/* rigid ellipsoid (ELLIP, eli.c) with semi-axes 3 along the columns of axes
+ * (column-major rotation of the referential frame, NULL = identity) and the
+ * given density; returns the body index */
+int SOLFEC_Ellipsoid(SOLFEC *sol, const double *centre, const double *semi_axes, const double *axes,
+                     double density, int surface);
 /* fixed half-space obstacle {x : (x - point).normal >= 0}; returns the body index */
 int SOLFEC_Plane(SOLFEC *sol, const double *point, const double *normal, int surface);
 /* set the velocity [W v] of a body (W referential angular, v spatial linear) */
@@ -78,6 +83,10 @@ int SOLFEC_Set_Velocity(SOLFEC *sol, int body, const double *velo6);
  * bounded by 6 planes of surface 0; seeded (splitmix64). Returns 0. */
 int SOLFEC_Lattice_Scene(SOLFEC *sol, int n, double L, double rmin, double rmax, double jitter,
                          double density, unsigned long long seed);
+/* the same lattice of ellipsoids: semi-axes U[amin,amax] each, uniformly random
+ * orientations; seeded. Returns 0. */
+int SOLFEC_Ellipsoid_Lattice_Scene(SOLFEC *sol, int n, double L, double amin, double amax, double jitter,
+                                   double density, unsigned long long seed);
 
 /* SOLFEC_Run (sol.c:665-825): step until time >= t0 + duration */
 int SOLFEC_Run(SOLFEC *sol, GAUSS_SEIDEL *gs, double duration);

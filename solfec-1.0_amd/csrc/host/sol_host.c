@@ -5,7 +5,8 @@
  *                         (BODY_Dynamic_Step_Begin, scheme RIG_NEG, bod.c:1313-1373),
  *                         update of existing contacts (update_contact, dom.c:430-483),
  *                         detection of new ones (overlap_create, dom.c:348-416;
- *                         sphere pairs goc.c:421-455, plane-sphere goc.c:308-348),
+ *                         sphere pairs goc.c:421-455, plane-sphere goc.c:308-348,
+ *                         ellipsoids: see ellip_pair and plane_ellip),
  *                         new contacts start with R = U = 0 (mem.c:77,86);
  *   LOCDYN_Update_Begin + GAUSS_SEIDEL_Solve -> SOLFEC_Step_Solve (GPU, sfgs.h);
  *   DOM_Update_End     -> SOLFEC_Step_End: body velocity update by the reactions
@@ -149,6 +150,7 @@ void SOLFEC_Destroy(SOLFEC *sol)
   if (!sol) return;
   if (sol->gpu) SFGS_Destroy(sol->gpu);
   free(sol->kind); free(sol->shape); free(sol->rad); free(sol->surf); free(sol->plane);
+  free(sol->sca); free(sol->rot0);
   free(sol->rot); free(sol->x); free(sol->x0); free(sol->J); free(sol->jinv); free(sol->minv);
   free(sol->velo); free(sol->velo0); free(sol->aux); free(sol->force); free(sol->mass);
   free(sol->mat);
@@ -220,6 +222,8 @@ static int add_body(SOLFEC *sol)
     sol->surf = xrealloc(sol->surf, sizeof(int)*(size_t)cap);
     sol->rad = xrealloc(sol->rad, sizeof(double)*(size_t)cap);
     sol->plane = xrealloc(sol->plane, sizeof(double)*6*(size_t)cap);
+    sol->sca = xrealloc(sol->sca, sizeof(double)*3*(size_t)cap);
+    sol->rot0 = xrealloc(sol->rot0, sizeof(double)*9*(size_t)cap);
     sol->rot = xrealloc(sol->rot, sizeof(double)*9*(size_t)cap);
     sol->x = xrealloc(sol->x, sizeof(double)*3*(size_t)cap);
     sol->x0 = xrealloc(sol->x0, sizeof(double)*3*(size_t)cap);
@@ -246,6 +250,8 @@ static int add_body(SOLFEC *sol)
   memset(sol->aux + 3*b, 0, 3*sizeof(double));
   memset(sol->force + 6*b, 0, 6*sizeof(double));
   memset(sol->plane + 6*b, 0, 6*sizeof(double));
+  memset(sol->sca + 3*b, 0, 3*sizeof(double));
+  memcpy(sol->rot0 + 9*b, I, sizeof(I));
   sol->minv[b] = 0.0;
   sol->mass[b] = 0.0;
   sol->rad[b] = 0.0;
@@ -266,6 +272,39 @@ int SOLFEC_Sphere(SOLFEC *sol, const double *centre, double radius, double densi
   memcpy(sol->x0 + 3*b, centre, 3*sizeof(double));
   sol->J[9*b] = sol->J[9*b + 4] = sol->J[9*b + 8] = j;
   sol->jinv[9*b] = sol->jinv[9*b + 4] = sol->jinv[9*b + 8] = 1.0/j;
+  sol->mass[b] = m;
+  sol->minv[b] = 1.0/m;
+  return b;
+}
+
+int SOLFEC_Ellipsoid(SOLFEC *sol, const double *centre, const double *semi_axes, const double *axes,
+                     double density, int surface)
+{
+  if (!sol || !(density > 0.0)) return -1;
+  for (int k = 0; k < 3; k ++) if (!(semi_axes[k] > 0.0)) return -1;
+  int b = add_body(sol);
+  const double a = semi_axes[0], bb = semi_axes[1], c = semi_axes[2];
+  /* ellipsoid mass characteristics (eli.c:170-227): m = rho 4/3 pi abc,
+   * principal inertia m/5 (b^2+c^2, a^2+c^2, a^2+b^2), J = R0 diag R0' */
+  double m = density * (4.0/3.0) * M_PI * a*bb*c, P[9] = {0}, T[9], J[9];
+  P[0] = 0.2*m*(bb*bb + c*c); P[4] = 0.2*m*(a*a + c*c); P[8] = 0.2*m*(a*a + bb*bb);
+  double *R0 = sol->rot0 + 9*b;
+  if (axes) memcpy(R0, axes, 9*sizeof(double));
+  for (int col = 0; col < 3; col ++)        /* T = P R0' */
+    for (int r = 0; r < 3; r ++) T[r + 3*col] = P[r + 3*r] * R0[col + 3*r];
+  mm(R0, T, J);
+  sol->kind[b] = SOL_RIG;
+  sol->shape[b] = 2;
+  sol->surf[b] = surface;
+  memcpy(sol->sca + 3*b, semi_axes, 3*sizeof(double));
+  sol->rad[b] = a > bb ? (a > c ? a : c) : (bb > c ? bb : c);
+  memcpy(sol->x + 3*b, centre, 3*sizeof(double));
+  memcpy(sol->x0 + 3*b, centre, 3*sizeof(double));
+  memcpy(sol->J + 9*b, J, sizeof(J));
+  /* J^-1 = R0 diag^-1 R0' */
+  for (int col = 0; col < 3; col ++)
+    for (int r = 0; r < 3; r ++) T[r + 3*col] = R0[col + 3*r] / P[r + 3*r];
+  mm(R0, T, sol->jinv + 9*b);
   sol->mass[b] = m;
   sol->minv[b] = 1.0/m;
   return b;
@@ -317,6 +356,42 @@ int SOLFEC_Lattice_Scene(SOLFEC *sol, int n, double L, double rmin, double rmax,
     for (int k = 0; k < 3; k ++) c[k] += (2.0*urand(&s) - 1.0)*jitter;
     double r = rmin + (rmax - rmin)*urand(&s);
     SOLFEC_Sphere(sol, c, r, density, 1);
+  }
+  const double P[6][6] = {
+    {0, 0, 0, 1, 0, 0}, {L, 0, 0, -1, 0, 0},
+    {0, 0, 0, 0, 1, 0}, {0, L, 0, 0, -1, 0},
+    {0, 0, 0, 0, 0, 1}, {0, 0, L, 0, 0, -1}};
+  for (int k = 0; k < 6; k ++) SOLFEC_Plane(sol, P[k], P[k] + 3, 0);
+  return 0;
+}
+
+/* uniformly random rotation from a uniformly random unit quaternion (K. Shoemake,
+ * "Uniform random rotations", Graphics Gems III, 1992) */
+static void random_rotation(unsigned long long *s, double *R)
+{
+  double u1 = urand(s), u2 = 2.0*M_PI*urand(s), u3 = 2.0*M_PI*urand(s);
+  double a = sqrt(1.0 - u1), b = sqrt(u1);
+  double w = a*sin(u2), x = a*cos(u2), y = b*sin(u3), z = b*cos(u3);
+  R[0] = 1.0 - 2.0*(y*y + z*z); R[3] = 2.0*(x*y - w*z);       R[6] = 2.0*(x*z + w*y);
+  R[1] = 2.0*(x*y + w*z);       R[4] = 1.0 - 2.0*(x*x + z*z); R[7] = 2.0*(y*z - w*x);
+  R[2] = 2.0*(x*z - w*y);       R[5] = 2.0*(y*z + w*x);       R[8] = 1.0 - 2.0*(x*x + y*y);
+}
+
+int SOLFEC_Ellipsoid_Lattice_Scene(SOLFEC *sol, int n, double L, double amin, double amax, double jitter,
+                                   double density, unsigned long long seed)
+{
+  if (!sol || n <= 0 || !(L > 0.0) || !(amin > 0.0) || amax < amin) return SOL_ERR_ARG;
+  int side = (int)ceil(cbrt((double)n) - 1e-9);
+  double a = L / side;
+  unsigned long long s = seed;
+  for (int i = 0; i < n; i ++)
+  {
+    int ix = i % side, iy = (i / side) % side, iz = i / (side*side);
+    double c[3] = {(ix + 0.5)*a, (iy + 0.5)*a, (iz + 0.5)*a}, sca[3], R[9];
+    for (int k = 0; k < 3; k ++) c[k] += (2.0*urand(&s) - 1.0)*jitter;
+    for (int k = 0; k < 3; k ++) sca[k] = amin + (amax - amin)*urand(&s);
+    random_rotation(&s, R);
+    SOLFEC_Ellipsoid(sol, c, sca, R, density, 1);
   }
   const double P[6][6] = {
     {0, 0, 0, 1, 0, 0}, {L, 0, 0, -1, 0, 0},
@@ -427,6 +502,133 @@ static int sphere_pair(SOLFEC *sol, int a, int b, double *spnt, double *mpnt, do
   return 1;
 }
 
+/* principal frame Q (current: rot R0) and semi-axes S of a convex body; a
+ * sphere is an ellipsoid with equal semi-axes in any frame */
+static void convex_frame(SOLFEC *sol, int b, double *Q, double *S)
+{
+  if (sol->shape[b] == 2)
+  {
+    mm(sol->rot + 9*b, sol->rot0 + 9*b, Q);
+    memcpy(S, sol->sca + 3*b, 3*sizeof(double));
+  }
+  else
+  {
+    static const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    memcpy(Q, I, sizeof(I));
+    S[0] = S[1] = S[2] = sol->rad[b];
+  }
+}
+
+/* support function h(d) = |S Q' d| of the centred ellipsoid and the offset of
+ * its support point, A d / h with A = Q S^2 Q' */
+static double ell_support(const double *Q, const double *S, const double *d, double *off)
+{
+  double t[3], u[3], v[3];
+  mtv(Q, d, t);
+  for (int k = 0; k < 3; k ++) u[k] = S[k]*t[k];
+  double h = sqrt(u[0]*u[0] + u[1]*u[1] + u[2]*u[2]);
+  for (int k = 0; k < 3; k ++) v[k] = S[k]*u[k]/h;
+  mv(Q, v, off);
+  return h;
+}
+
+/* pair of convex bodies with at least one ellipsoid, as (slave a, master b).
+ * The normal n, from a towards b, maximises the separation along n,
+ *   gap(n) = n.(c_b - c_a) - h_a(n) - h_b(n),
+ * whose largest value is the distance of apart bodies and minus the
+ * penetration depth of overlapping ones; Newton's method on the tangent plane
+ * of the unit sphere finds it from n = (c_b - c_a)/|c_b - c_a|.  The contact
+ * points are the support points c_a + h_a'(n) and c_b - h_b'(n), merged at
+ * their midpoint when the bodies overlap, as for spheres.  The reference
+ * (goc.c:457-492, gjk.c:651-836) takes the normal at the GJK closest points
+ * instead; both give the common normal of touching bodies. */
+static int ellip_pair(SOLFEC *sol, int a, int b, double *spnt, double *mpnt, double *normal, double *gap)
+{
+  const double *ca = sol->x + 3*a, *cb = sol->x + 3*b;
+  double Qa[9], Sa[3], Qb[9], Sb[3], Aa[9], Ab[9], T[9];
+  convex_frame(sol, a, Qa, Sa);
+  convex_frame(sol, b, Qb, Sb);
+  for (int c = 0; c < 3; c ++)
+    for (int r = 0; r < 3; r ++) T[r + 3*c] = Sa[r]*Sa[r]*Qa[c + 3*r];
+  mm(Qa, T, Aa);
+  for (int c = 0; c < 3; c ++)
+    for (int r = 0; r < 3; r ++) T[r + 3*c] = Sb[r]*Sb[r]*Qb[c + 3*r];
+  mm(Qb, T, Ab);
+  double d[3] = {cb[0] - ca[0], cb[1] - ca[1], cb[2] - ca[2]};
+  double dl = sqrt(d[0]*d[0] + d[1]*d[1] + d[2]*d[2]);
+  if (dl == 0.0) return 0;
+  double n[3] = {d[0]/dl, d[1]/dl, d[2]/dl}, oa[3], ob[3], ha = 0.0, hb = 0.0, g = 0.0;
+  for (int it = 0; it < 50; it ++)
+  {
+    ha = ell_support(Qa, Sa, n, oa);
+    hb = ell_support(Qb, Sb, n, ob);
+    double grad[3], loc[9], H[9];
+    for (int k = 0; k < 3; k ++) grad[k] = d[k] - oa[k] - ob[k];
+    g = grad[0]*n[0] + grad[1]*n[1] + grad[2]*n[2];   /* = gap(n) */
+    /* Euclidean Hessian: -(A_a - oa oa')/h_a - (A_b - ob ob')/h_b */
+    for (int c = 0; c < 3; c ++)
+      for (int r = 0; r < 3; r ++)
+        H[r + 3*c] = -(Aa[r + 3*c] - oa[r]*oa[c])/ha - (Ab[r + 3*c] - ob[r]*ob[c])/hb;
+    localbase(n, loc);
+    const double *e1 = loc, *e2 = loc + 3;
+    double g1 = e1[0]*grad[0] + e1[1]*grad[1] + e1[2]*grad[2];
+    double g2 = e2[0]*grad[0] + e2[1]*grad[1] + e2[2]*grad[2];
+    if (sqrt(g1*g1 + g2*g2) <= 1e-14*(dl + ha + hb)) break;
+    double He1[3], He2[3];
+    mv(H, e1, He1);
+    mv(H, e2, He2);
+    /* Riemannian Hessian on the tangent plane: e_i' H e_j - (grad.n) delta_ij */
+    double h11 = e1[0]*He1[0] + e1[1]*He1[1] + e1[2]*He1[2] - g;
+    double h12 = e1[0]*He2[0] + e1[1]*He2[1] + e1[2]*He2[2];
+    double h22 = e2[0]*He2[0] + e2[1]*He2[1] + e2[2]*He2[2] - g;
+    double det = h11*h22 - h12*h12, s1, s2;
+    if (h11 < 0.0 && det > 0.0) { s1 = -(h22*g1 - h12*g2)/det; s2 = -(h11*g2 - h12*g1)/det; }
+    else { double al = 1.0/(dl + ha + hb); s1 = al*g1; s2 = al*g2; }   /* ascent step away from a maximum */
+    double sl = sqrt(s1*s1 + s2*s2);
+    if (sl > 0.5) { s1 *= 0.5/sl; s2 *= 0.5/sl; }
+    double nl = 0.0;
+    for (int k = 0; k < 3; k ++) { n[k] += s1*e1[k] + s2*e2[k]; nl += n[k]*n[k]; }
+    nl = sqrt(nl);
+    for (int k = 0; k < 3; k ++) n[k] /= nl;
+  }
+  if (g >= GEOMETRIC_EPSILON) return 0;
+  ha = ell_support(Qa, Sa, n, oa);
+  hb = ell_support(Qb, Sb, n, ob);
+  g = n[0]*d[0] + n[1]*d[1] + n[2]*d[2] - ha - hb;
+  for (int k = 0; k < 3; k ++) { spnt[k] = ca[k] + oa[k]; mpnt[k] = cb[k] - ob[k]; normal[k] = n[k]; }
+  if (g < 0.0)
+    for (int k = 0; k < 3; k ++) { spnt[k] = 0.5*(spnt[k] + mpnt[k]); mpnt[k] = spnt[k]; }
+  *gap = g < 0.0 ? g : 0.0;
+  return 1;
+}
+
+/* plane (slave) and ellipsoid (master): the ellipsoid's support point against
+ * the plane normal (goc.c:350-404 with the plane as the convex) */
+static int plane_ellip(SOLFEC *sol, int pl, int e, double *spnt, double *mpnt, double *normal, double *gap)
+{
+  const double *P = sol->plane + 6*pl, *n = P + 3, *c = sol->x + 3*e;
+  double Q[9], S[3], o[3];
+  convex_frame(sol, e, Q, S);
+  double h = ell_support(Q, S, n, o);
+  double dist = (c[0] - P[0])*n[0] + (c[1] - P[1])*n[1] + (c[2] - P[2])*n[2], g = dist - h;
+  if (g >= GEOMETRIC_EPSILON) return 0;
+  for (int k = 0; k < 3; k ++)
+  {
+    normal[k] = n[k];
+    mpnt[k] = c[k] - o[k];
+    spnt[k] = mpnt[k] - g*n[k];
+  }
+  *gap = g < 0.0 ? g : 0.0;
+  return 1;
+}
+
+/* dispatch by shape: the sphere routines for sphere pairs, the ellipsoid ones otherwise */
+static int convex_pair(SOLFEC *sol, int a, int b, double *spnt, double *mpnt, double *normal, double *gap)
+{
+  if (sol->shape[a] == 0 && sol->shape[b] == 0) return sphere_pair(sol, a, b, spnt, mpnt, normal, gap);
+  return ellip_pair(sol, a, b, spnt, mpnt, normal, gap);
+}
+
 /* plane (slave) and sphere (master): goc.c:308-348 with the plane as the convex */
 static int plane_sphere(SOLFEC *sol, int pl, int s, double *spnt, double *mpnt, double *normal, double *gap)
 {
@@ -525,8 +727,10 @@ static int update_and_detect(SOLFEC *sol)
   {
     int m = sol->c_master[i], s = sol->c_slave[i];
     double *G = sol->upd_geo + 10*(size_t)i;
-    if (sol->shape[s] == 1) sol->upd_ok[i] = plane_sphere(sol, s, m, G, G + 3, G + 6, G + 9);
-    else sol->upd_ok[i] = sphere_pair(sol, s, m, G, G + 3, G + 6, G + 9);
+    if (sol->shape[s] == 1)
+      sol->upd_ok[i] = sol->shape[m] == 2 ? plane_ellip(sol, s, m, G, G + 3, G + 6, G + 9)
+                                          : plane_sphere(sol, s, m, G, G + 3, G + 6, G + 9);
+    else sol->upd_ok[i] = convex_pair(sol, s, m, G, G + 3, G + 6, G + 9);
   }
   /* compaction keeps the insertion order; upd_ok receives the old index of each kept slot */
   for (int i = 0; i < nc; i ++)
@@ -555,7 +759,7 @@ static int update_and_detect(SOLFEC *sol)
   int nb = sol->nb;
   double rmax = 0.0, lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   for (int b = 0; b < nb; b ++)
-    if (sol->shape[b] == 0)
+    if (sol->shape[b] != 1)   /* spheres and ellipsoids, by their bounding spheres */
     {
       if (sol->rad[b] > rmax) rmax = sol->rad[b];
       for (int k = 0; k < 3; k ++)
@@ -596,7 +800,7 @@ static int update_and_detect(SOLFEC *sol)
     /* insert in descending body order so that cell lists run in ascending order */
     for (int b = nb - 1; b >= 0; b --)
     {
-      if (sol->shape[b] != 0) continue;
+      if (sol->shape[b] == 1) continue;
       int ci[3];
       for (int k = 0; k < 3; k ++) ci[k] = (int)floor((sol->x[3*b + k] - lo[k])/cell);
       long long c = ((long long)ci[2]*dim[1] + ci[1])*dim[0] + ci[0];
@@ -625,7 +829,7 @@ static int update_and_detect(SOLFEC *sol)
       sol->cand_n[t] = 0;
       for (int a = a0; a < a1; a ++)
       {
-        if (sol->shape[a] != 0) continue;
+        if (sol->shape[a] == 1) continue;
         int ci[3];
         for (int k = 0; k < 3; k ++) ci[k] = (int)floor((sol->x[3*a + k] - lo[k])/cell);
         for (int dz = -1; dz <= 1; dz ++)
@@ -660,7 +864,7 @@ static int update_and_detect(SOLFEC *sol)
          * lexicographically larger centre is the master */
         int slave = a, master = b;
         if (pntcmp(sol->x + 3*a, sol->x + 3*b) > 0) { slave = b; master = a; }
-        if (!sphere_pair(sol, slave, master, sp, mp, nrm, &g)) continue;
+        if (!convex_pair(sol, slave, master, sp, mp, nrm, &g)) continue;
         ensure_contacts(sol, sol->nc + 1);
         int i = sol->nc ++;
         set_contact(sol, i, master, slave, mp, sp, nrm, g);
@@ -674,12 +878,13 @@ static int update_and_detect(SOLFEC *sol)
     if (sol->shape[p] != 1) continue;
     for (int s = 0; s < nb; s ++)
     {
-      if (sol->shape[s] != 0) continue;
+      if (sol->shape[s] == 1) continue;
       const double *P = sol->plane + 6*p, *c = sol->x + 3*s;
       double dist = (c[0] - P[0])*P[3] + (c[1] - P[1])*P[4] + (c[2] - P[2])*P[5];
       if (dist >= sol->rad[s] + GEOMETRIC_EPSILON) continue;
       if (map_find(sol, p, s) >= 0) continue;
-      if (!plane_sphere(sol, p, s, sp, mp, nrm, &g)) continue;
+      if (sol->shape[s] == 2 ? !plane_ellip(sol, p, s, sp, mp, nrm, &g) : !plane_sphere(sol, p, s, sp, mp, nrm, &g))
+        continue;
       ensure_contacts(sol, sol->nc + 1);
       int i = sol->nc ++;
       set_contact(sol, i, s, p, mp, sp, nrm, g);

@@ -21,8 +21,9 @@ struct solfec {
 
   /* bodies */
   int nb, bcap;
-  int *kind, *shape, *surf;        /* shape: 0 sphere, 1 plane */
-  double *rad, *plane;             /* plane: point 3, unit normal 3 */
+  int *kind, *shape, *surf;        /* shape: 0 sphere, 1 plane, 2 ellipsoid */
+  double *rad, *plane;             /* rad: radius, or bounding radius of an ellipsoid; plane: point 3, unit normal 3 */
+  double *sca, *rot0;              /* ellipsoid: semi-axes 3, principal axes in the referential frame 9 */
   double *rot, *x, *x0, *J, *jinv, *minv, *mass;
   double *velo, *velo0, *aux, *force, *tmp_velo;
 

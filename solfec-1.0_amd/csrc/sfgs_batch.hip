@@ -383,13 +383,13 @@ static int dispatch_batch(sfgs_ctx *ctx, int G, int NW, const BatchArgs<T> &b, i
 }
 
 struct Tuned { SFGS_BATCH_TUNE t; int valid; };
-static Tuned g_default = {{1, 4, 2048}, 0};
+static Tuned g_default = {{1, 4, 0}, 0};
 
 static bool tune_ok(const SFGS_BATCH_TUNE &t)
 {
   return (t.lanes_per_row == 1 || t.lanes_per_row == 2) &&
          (t.waves_per_group == 1 || t.waves_per_group == 2 || t.waves_per_group == 4 || t.waves_per_group == 8) &&
-         t.rows_per_group > 0;
+         t.rows_per_group >= 0;
 }
 
 } // namespace
@@ -423,13 +423,26 @@ static int batch_solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TU
   if (mixed) CHK((dispatch_batch<float>(ctx, G, NW, BatchArgs<float>(), 0, &cap)));
   else CHK((dispatch_batch<double>(ctx, G, NW, BatchArgs<double>(), 0, &cap)));
   if (cap < 1) { ctx->err = "batched kernel cannot be resident"; return SFGS_E_HIP; }
-  std::vector<int> rows((size_t)nsc), want((size_t)nsc);
+  std::vector<int> rows((size_t)nsc), want((size_t)nsc), slices((size_t)nsc);
+  long long total_rows = 0;
+  int busy = 0;
   for (int s = 0; s < nsc; s ++)
   {
     rows[s] = ctx->h_colptr[(size_t)(s + 1)*ncol] - ctx->h_colptr[(size_t)s*ncol];
-    const int slices = ctx->h_cslice[(size_t)(s + 1)*ncol] - ctx->h_cslice[(size_t)s*ncol];
-    long long w = rows[s] > 0 ? ((long long)rows[s] + tn.rows_per_group - 1) / tn.rows_per_group : 0;
-    w = std::min<long long>(w, std::max(1, (slices + NW - 1) / NW));   // no more workgroups than slice rounds
+    slices[s] = ctx->h_cslice[(size_t)(s + 1)*ncol] - ctx->h_cslice[(size_t)s*ncol];
+    total_rows += rows[s];
+    busy += rows[s] > 0 ? 1 : 0;
+  }
+  // rows_per_group 0: share the resident capacity among the scenes in proportion to their rows,
+  // so that every scene runs in one launch and takes about the same time per iteration
+  const bool balanced = tn.rows_per_group <= 0 && busy <= cap;
+  for (int s = 0; s < nsc; s ++)
+  {
+    long long w;
+    if (rows[s] == 0) w = 0;
+    else if (balanced) w = (long long)((double)rows[s] / (double)total_rows * (double)(cap - busy)) + 1;
+    else w = ((long long)rows[s] + std::max(tn.rows_per_group, 1) - 1) / std::max(tn.rows_per_group, 1);
+    w = std::min<long long>(w, std::max(1, (slices[s] + NW - 1) / NW));   // no more workgroups than slice rounds
     want[s] = (int)std::max<long long>(rows[s] > 0 ? 1 : 0, std::min<long long>(w, cap));
   }
 
@@ -602,7 +615,7 @@ int SFGS_Batch_Autotune(sfgs_ctx *ctx, const SFGS_PARAM *par, int trial_iters, S
   SFGS_BATCH_TUNE bt = g_default.t;
   double bms = 1e300;
   int rc = 0;
-  const int lanes[2] = {1, 2}, waves[4] = {1, 2, 4, 8}, rpg[4] = {256, 1024, 4096, 1 << 30};
+  const int lanes[2] = {1, 2}, waves[4] = {1, 2, 4, 8}, rpg[4] = {0, 1024, 4096, 1 << 30};
   for (int li = 0; li < 2 && !rc; li ++)
   {
     if (ctx->G_built != lanes[li])

@@ -79,9 +79,10 @@ struct sfgs_ctx {
   // ---- permuted row data (colour-major order)
   sfgs::DBuf<int> rowlen;                // per permuted row
   sfgs::DBuf<double> W, Wd, A, B, V, rho, pfri, pcoh, shift, R, U, Rb, Um;
-  int G = 1;                             // lanes per row of the sweep layout (1, 2, 4, 8)
+  int G = 2;                             // lanes per row of the sweep layout (1, 2, 4, 8)
   int schedule = 0;                      // sweeps: 0 one launch per colour, 1 persistent dataflow
-  int row_order = 0;                     // inside a colour: 0 by decreasing row length, 1 by contact index
+  int row_order = 2;                     // inside a colour: 0 by decreasing row length, 1 by contact index,
+                                         // 2 by master body block then decreasing row length
   sfgs::DBuf<unsigned char> flow_mem;    // dataflow schedule: third R buffer, U pairs, versions, partials
   int G_built = 1;                       // layout of the current store
   sfgs::DBuf<unsigned char> sep;

@@ -18,8 +18,8 @@ import numpy as np
 from ._lib import SFGS_REPORT, f64, ptr, solfec_lib
 from .device import DIAGSOLVERS, FAILURES, GS_ERRORS, Device, SfgsError, make_param, default_diag  # noqa: F401
 
-__all__ = ["SOLFEC", "SURFACE_MATERIAL", "BULK_MATERIAL", "SPHERE", "PLANE", "BODY", "GRAVITY",
-           "INITIAL_VELOCITY", "GAUSS_SEIDEL_SOLVER", "RUN", "LATTICE", "Device", "SfgsError"]
+__all__ = ["SOLFEC", "SURFACE_MATERIAL", "BULK_MATERIAL", "SPHERE", "ELLIP", "PLANE", "BODY", "GRAVITY",
+           "INITIAL_VELOCITY", "GAUSS_SEIDEL_SOLVER", "RUN", "LATTICE", "ELLIP_LATTICE", "Device", "SfgsError"]
 
 _CALLBACK_T = C.CFUNCTYPE(C.c_int, C.c_void_p)
 
@@ -138,6 +138,18 @@ class SPHERE:
         self.center, self.radius, self.volid, self.surfid = tuple(map(float, center)), float(radius), volid, surfid
 
 
+class ELLIP:
+    """ELLIP(center, radii, volid, surfid) (lng.c:1560-1600); this build adds
+    rotation: a column-major 3x3 rotation of the principal axes (default identity)."""
+
+    def __init__(self, center, radii, volid, surfid, rotation=None):
+        radii = tuple(map(float, radii))
+        if len(radii) != 3 or not all(r > 0 for r in radii):
+            raise ValueError("'radii' must be three positive numbers")
+        self.center, self.radii, self.volid, self.surfid = tuple(map(float, center)), radii, volid, surfid
+        self.rotation = None if rotation is None else np.asarray(rotation, np.float64).reshape(9)
+
+
 class PLANE:
     """PLANE(point, normal, surfid): fixed half-space {x : (x - point).normal >= 0}, obstacles only."""
 
@@ -146,15 +158,22 @@ class PLANE:
 
 
 class BODY:
-    """BODY(solfec, kind, shape, material, label=None) with kind 'RIGID' (SPHERE) or 'OBSTACLE' (PLANE)."""
+    """BODY(solfec, kind, shape, material, label=None) with kind 'RIGID' (SPHERE, ELLIP) or 'OBSTACLE' (PLANE)."""
 
     def __init__(self, solfec, kind, shape, material, label=None):
         lib = solfec.lib
         if kind == "RIGID":
-            if not isinstance(shape, SPHERE):
-                raise ValueError("this build supports RIGID spheres only")
-            c = f64(shape.center, 3)
-            self.id = lib.SOLFEC_Sphere(solfec.sol, ptr(c), shape.radius, float(material.density), int(shape.surfid))
+            if isinstance(shape, SPHERE):
+                c = f64(shape.center, 3)
+                self.id = lib.SOLFEC_Sphere(solfec.sol, ptr(c), shape.radius, float(material.density),
+                                            int(shape.surfid))
+            elif isinstance(shape, ELLIP):
+                c, r = f64(shape.center, 3), f64(shape.radii, 3)
+                rot = None if shape.rotation is None else f64(shape.rotation, 9)
+                self.id = lib.SOLFEC_Ellipsoid(solfec.sol, ptr(c), ptr(r), ptr(rot), float(material.density),
+                                               int(shape.surfid))
+            else:
+                raise ValueError("this build supports RIGID spheres and ellipsoids only")
         elif kind == "OBSTACLE":
             if not isinstance(shape, PLANE):
                 raise ValueError("this build supports PLANE obstacles only")
@@ -178,6 +197,14 @@ def INITIAL_VELOCITY(body, linear, angular):
     u = f64(list(angular) + list(linear), 6)
     if body.solfec.lib.SOLFEC_Set_Velocity(body.solfec.sol, body.id, ptr(u)):
         raise ValueError("invalid body")
+
+
+def ELLIP_LATTICE(solfec, n, L, amin, amax, jitter, density, seed):
+    """Seeded jittered cubic lattice of n rigid ellipsoids (semi-axes U[amin, amax], random
+    orientations) in [0, L]^3 with 6 bounding planes."""
+    if solfec.lib.SOLFEC_Ellipsoid_Lattice_Scene(solfec.sol, int(n), float(L), float(amin), float(amax),
+                                                 float(jitter), float(density), int(seed)):
+        raise ValueError("invalid lattice parameters")
 
 
 def LATTICE(solfec, n, L, rmin, rmax, jitter, density, seed):

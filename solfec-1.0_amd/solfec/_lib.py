@@ -96,6 +96,8 @@ _SOL_PROTOS = {
     "SOLFEC_Plane": (_i, [_p, _p, _p, _i]),
     "SOLFEC_Set_Velocity": (_i, [_p, _i, _p]),
     "SOLFEC_Lattice_Scene": (_i, [_p, _i, _d, _d, _d, _d, _d, C.c_ulonglong]),
+    "SOLFEC_Ellipsoid_Lattice_Scene": (_i, [_p, _i, _d, _d, _d, _d, _d, C.c_ulonglong]),
+    "SOLFEC_Ellipsoid": (_i, [_p, _p, _p, _p, _d, _i]),
     "SOLFEC_Run": (_i, [_p, _p, _d]),
     "SOLFEC_Step_Begin": (_i, [_p]),
     "SOLFEC_Step_Solve": (_i, [_p, _p]),

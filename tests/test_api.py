@@ -108,3 +108,80 @@ def test_contacts_persist_with_their_reactions():
             assert np.array_equal(c1["R"][3 * i:3 * i + 3], np.arange(3 * k, 3 * k + 3, dtype=float))
         else:
             assert np.all(c1["R"][3 * i:3 * i + 3] == 0.0)
+
+
+def _domain():
+    s = solfec.SOLFEC("DYNAMIC", 1e-3, "/tmp/solfec_test")
+    solfec.SURFACE_MATERIAL(s, friction=0.3, restitution=0.0)
+    return s
+
+
+def _rot(axis, angle):
+    a = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(angle) * K + (1 - np.cos(angle)) * K @ K
+
+
+def test_ellipsoid_mass_characteristics():
+    s = _domain()
+    R = _rot((1, 2, 3), 0.7)
+    mat = solfec.BULK_MATERIAL(s, density=2000.0)
+    solfec.BODY(s, "RIGID", solfec.ELLIP((0.1, 0.2, 0.3), (0.05, 0.04, 0.03), 1, 1, rotation=R.T.ravel()), mat)
+    b = s.bodies()
+    a, bb, c = 0.05, 0.04, 0.03
+    m = 2000.0 * 4.0 / 3.0 * np.pi * a * bb * c
+    J = R @ np.diag([m / 5 * (bb * bb + c * c), m / 5 * (a * a + c * c), m / 5 * (a * a + bb * bb)]) @ R.T
+    np.testing.assert_allclose(b["minv"][0], 1.0 / m, rtol=1e-14)
+    np.testing.assert_allclose(b["jinv"].reshape(3, 3).T, np.linalg.inv(J), rtol=1e-12)
+
+
+def test_round_ellipsoids_contact_like_spheres():
+    """An ellipsoid with equal semi-axes goes through the ellipsoid contact routine and
+    must give the sphere routine's contact: point, frame and gap."""
+    out = []
+    for shape in ("sphere", "ellip"):
+        s = _domain()
+        mat = solfec.BULK_MATERIAL(s, density=2000.0)
+        for x, r in (((0.0, 0.0, 0.0), 0.05), ((0.06, 0.07, 0.02), 0.06)):
+            sh = solfec.SPHERE(x, r, 1, 1) if shape == "sphere" else solfec.ELLIP(x, (r, r, r), 1, 1)
+            solfec.BODY(s, "RIGID", sh, mat)
+        s.step_begin()
+        out.append(s.contacts())
+    a, b = out
+    assert len(a["gap"]) == len(b["gap"]) == 1
+    for k in ("master", "slave"):
+        assert np.array_equal(a[k], b[k])
+    for k in ("mpnt", "spnt", "base", "gap"):
+        np.testing.assert_allclose(b[k], a[k], rtol=1e-9, atol=1e-12)
+
+
+def test_ellipsoid_pair_and_plane_closed_forms():
+    s = _domain()
+    mat = solfec.BULK_MATERIAL(s, density=2000.0)
+    a1, a2, dlt = 0.05, 0.03, 0.002
+    # two axis-aligned ellipsoids on the x axis, overlapping by dlt along x
+    solfec.BODY(s, "RIGID", solfec.ELLIP((0.5, 0.5, 0.5), (a1, 0.02, 0.04), 1, 1), mat)
+    solfec.BODY(s, "RIGID", solfec.ELLIP((0.5 + a1 + a2 - dlt, 0.5, 0.5), (a2, 0.035, 0.025), 1, 1), mat)
+    # a rotated ellipsoid above the plane z = 0, penetrating it by 0.001
+    R = _rot((1, 1, 0), 0.9)
+    A = R @ np.diag([0.06, 0.03, 0.02]) ** 2 @ R.T
+    hz = np.sqrt(A[2, 2])
+    solfec.BODY(s, "RIGID", solfec.ELLIP((2.0, 2.0, hz - 0.001), (0.06, 0.03, 0.02), 1, 1, rotation=R.T.ravel()), mat)
+    solfec.BODY(s, "OBSTACLE", solfec.PLANE((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 1), mat)
+    s.step_begin()
+    c = s.contacts()
+    assert len(c["gap"]) == 2
+    n = len(c["gap"])
+    base = c["base"].reshape(n, 3, 3)
+    i = int(np.where(c["slave"] == 3)[0][0]) if np.any(c["slave"] == 3) else None
+    j = 1 - i
+    # pair: normal along +x from slave (first, smaller x centre) to master; gap = -dlt
+    np.testing.assert_allclose(abs(base[j][2][0]), 1.0, rtol=1e-10)
+    np.testing.assert_allclose(c["gap"][j], -dlt, rtol=1e-8)
+    # plane: normal +z, gap -0.001, contact point at the ellipsoid's lowest point
+    np.testing.assert_allclose(base[i][2], [0.0, 0.0, 1.0], atol=1e-14)
+    np.testing.assert_allclose(c["gap"][i], -0.001, rtol=1e-9)
+    x = s.bodies()["x"].reshape(-1, 3)[2]
+    low = x - A @ np.array([0.0, 0.0, 1.0]) / hz
+    X0 = s.bodies()["x0"].reshape(-1, 3)[2]
+    np.testing.assert_allclose(c["mpnt"].reshape(n, 3)[i], X0 + (low - x), atol=1e-12)

@@ -201,3 +201,28 @@ def test_empty_system(dev):
                np.zeros(0), np.zeros(0))
     rep, rer, mer = dev.solve(solfec.make_param(1e-4, 10))
     assert rep.iters == 1 and rep.error == 0
+
+
+def test_ellipsoids_assembly_and_gs_match_oracle(dev):
+    """Rigid ellipsoids (general inertia, non-central contact normals): LOCDYN and GS
+    against the oracle, as for spheres."""
+    s = solfec.SOLFEC("DYNAMIC", 1e-3, "/tmp/solfec_test")
+    solfec.SURFACE_MATERIAL(s, friction=0.4, restitution=0.0)
+    solfec.GRAVITY(s, (0.0, 0.0, -9.81))
+    solfec.ELLIP_LATTICE(s, 729, 0.8, 0.03, 0.06, 0.005, 2000.0, 5)
+    s.step_begin()
+    b, c = _upload(dev, s)
+    assert len(c["gap"]) > 200
+    dev.assemble(1e-3)
+    G = dev.locdyn()
+    O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), 1e-3)
+    assert np.array_equal(G["col"], O["col"])
+    for k in ("Wd", "A", "B", "W"):
+        scale = max(np.abs(O[k]).max(), 1e-300)
+        assert np.abs(G[k] - O[k]).max() / scale < RTOL_ASM, k
+    kw = dict(epsilon=1e-6, maxiter=3000, meritval=1e-6)
+    rep, rer, mer, Rg, Ug, orep, orer, omer, Ro, Uo = _solve_both(dev, O, kw)
+    err = np.abs(Rg.reshape(-1, 3) - Ro).max() / np.abs(Ro).max()
+    print(f"[ellip] contacts {O['n']} iters gpu {rep.iters} oracle {orep.iters} |dR|/max|R| {err:.2e}")
+    assert abs(rep.iters - orep.iters) <= 1
+    assert err < RTOL_R

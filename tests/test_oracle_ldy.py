@@ -4,7 +4,9 @@ base' (v + omega x (x_p - x_c)) of the material points, W_ij as the local
 velocity response at contact i to a unit local impulse at contact j over one
 step, the sphere closed forms W_ii = (h/m) diag(7/2, 7/2, 1) (sphere on a
 plane) and h diag(7/2 (1/m1+1/m2), ..., 1/m1+1/m2) (touching pair), numpy
-eigvalsh / inv for rho and A, symmetry and positive semi-definiteness."""
+eigvalsh / inv for rho and A, symmetry and positive semi-definiteness; the
+impulse-response checks run on spheres and on rigid ellipsoids (general
+inertia, normals off the centre lines)."""
 import numpy as np
 import pytest
 
@@ -60,10 +62,18 @@ def impulse_response(B, Cc, i, j, k):
     return local_velocity(B, Cc, i, lambda b: du.get(b, np.zeros(6)) if B["kind"][b] == 0 else np.zeros(6))
 
 
-@pytest.fixture(scope="module")
-def scene():
-    s = lattice(64, seed=4)
+@pytest.fixture(scope="module", params=["spheres", "ellipsoids"])
+def scene(request):
+    if request.param == "spheres":
+        s = lattice(64, seed=4)
+    else:   # general inertia and contact points off the centre lines
+        s = solfec.SOLFEC("DYNAMIC", H, "/tmp/solfec_test")
+        solfec.SURFACE_MATERIAL(s, friction=0.3, restitution=0.0)
+        solfec.GRAVITY(s, (0.0, 0.0, -9.81))
+        solfec.ELLIP_LATTICE(s, 64, 0.4, 0.03, 0.06, 0.005, 2000.0, 4)
+        s.step_begin()
     b, c = s.bodies(), s.contacts()
+    assert len(c["gap"]) > 20
     O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), H)
     return b, c, O
 

@@ -82,7 +82,7 @@ def test_batch_workgroups_do_not_change_result(dev):
     B, Cn, bs, cs = batch.concat(scenes)
     par = solfec.make_param(epsilon=1e-7, maxiter=2000, meritval=1e-9)
     res = []
-    for tune in ((1, 4, 1 << 30), (1, 1, 32), (1, 2, 200), (1, 8, 64)):
+    for tune in ((2, 4, 1 << 30), (2, 1, 32), (2, 2, 200), (2, 8, 64), (2, 4, 0)):   # lanes: the default layout
         # the solve writes R back into the uploaded contacts: upload the same R0 for every shape
         ns = batch.upload(dev, B, Cn, bs, 1e-3)
         reps, _ = dev.batch_solve(par, ns, tune=tune)

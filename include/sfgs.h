@@ -215,6 +215,9 @@ typedef struct {
   int waves_per_group;  /* wavefronts per workgroup: 1, 2, 4 or 8 */
   int rows_per_group;   /* target rows of a scene per workgroup; 0 = share the resident workgroups
                            among the scenes in proportion to their rows (one launch) */
+  int schedule;         /* 0 = one persistent launch (the fields above); 1 = one launch per colour
+                           over the colour's slices of every running scene, a scene-wise
+                           termination test after each sweep (forward sweeps; reverse uses 0) */
 } SFGS_BATCH_TUNE;
 
 typedef struct {
@@ -236,8 +239,9 @@ int SFGS_Batch_Scenes(sfgs_ctx *ctx, int nscenes, const int *body_scene);
 int SFGS_Batch_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TUNE *tune, SFGS_SCENE_REPORT *reports,
                      SFGS_REPORT *total);
 /* time trial_iters iterations of every candidate shape (lanes 1 and 2, which
- * re-assembles the system, waves 1..8, rows per group proportional, 1024, 4096
- * or all) and keep the fastest; R and U are restored after each trial */
+ * re-assembles the system; the persistent launch with waves 1..8 and rows per
+ * group proportional, 1024, 4096 or all; the per-colour schedule) and keep the
+ * fastest; R and U are restored after each trial */
 int SFGS_Batch_Autotune(sfgs_ctx *ctx, const SFGS_PARAM *par, int trial_iters, SFGS_BATCH_TUNE *best, double *best_ms);
 
 #ifdef __cplusplus

@@ -40,9 +40,11 @@ struct BatchScene {
 };
 
 struct BatchOut {
-  int iters, nref, sweeps, done, aborted, failcount, timeout, pad;
+  int iters, nref, sweeps, done, aborted, failcount, timeout;
+  int final_buf, final_U;   // per-colour schedule: buffer of the final R (0, 1), final U (0 U, 1 Um)
+  int pad;
   unsigned long long failkey;
-  double error, merit, energy;
+  double error, merit, energy, prev_error;
 };
 
 // scene layout shared by the sweeps of both precisions
@@ -382,17 +384,310 @@ static int dispatch_batch(sfgs_ctx *ctx, int G, int NW, const BatchArgs<T> &b, i
 #undef SFGS_BATCH_NW
 }
 
+// ---------------------------------------------------------------------
+// The per-colour schedule of the batch (tune.schedule = 1): one launch per
+// colour sweeps colour c of every scene that is still running, the scenes'
+// slices of that colour listed together, and one small launch per sweep takes
+// each scene's termination test.  Kernel boundaries are the barriers, so no
+// workgroup waits for another; a launch holds the colour of all scenes, which
+// fills the device even when each scene alone would not.  Each wavefront
+// handles one slice and writes that slice's partial sums, which a scene's test
+// adds up in slice order.
+// ---------------------------------------------------------------------
+
+struct UScene {
+  int slice0, slice1;     // the scene's slices (all colours)
+};
+
+template <typename T, int G, bool MERIT>
+__global__ __launch_bounds__(TPB) void k_usweep(SweepArgs<T> a, const int *clist, int nlist, const int2 *sbound,
+                                                const int *sscene, BatchOut *out, double *spart, int sweepno_base)
+{
+  const int idx = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6);
+  if (idx >= nlist) return;                       // whole wavefront
+  const int s = clist[idx], sc = sscene[s];
+  BatchOut *o = out + sc;
+  if (o->done) return;                            // a finished scene: whole wavefront
+  const int2 bd = sbound[s];
+  double eu = 0.0, el = 0.0, up = 0.0;
+  int sweepno = sweepno_base + o->sweeps;
+  sweep_slice<T, G, MERIT>(a, s, true, bd.x, bd.y, 0, &sweepno, &o->failkey, &o->failcount, &o->aborted, eu, el, up);
+  eu = wave_sum(eu); el = wave_sum(el); up = wave_sum(up);
+  if ((threadIdx.x & 63) == 0)
+  {
+    double *pt = spart + 3*(size_t)s;
+    pt[0] = eu; pt[1] = el; pt[2] = up;
+  }
+}
+
+// per-slice free energy terms
+template <int G>
+__global__ __launch_bounds__(TPB) void k_uenergy(SweepArgs<double> d, int nslices, double *spart)
+{
+  const int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6);
+  if (s >= nslices) return;
+  double e = wave_sum(slice_energy<G>(d, s));
+  if ((threadIdx.x & 63) == 0) spart[3*(size_t)s] = e;
+}
+
+// per-slice merit terms of the final FP64 state (with the update of U)
+template <int G>
+__global__ __launch_bounds__(TPB) void k_umerit(MeritArgs m, int nslices, const int *sscene, const BatchOut *out,
+                                                double *spart)
+{
+  const int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6);
+  if (s >= nslices) return;
+  double up = 0.0;
+  merit_slice<G>(m, s, true, up);
+  up = wave_sum(up);
+  if ((threadIdx.x & 63) == 0) spart[3*(size_t)s + 2] = up;
+}
+
+// deterministic sum of three partials over a scene's slices (one block per scene)
+__device__ __forceinline__ void scene_partials(const double *spart, int s0, int s1, double &a, double &b, double &c)
+{
+  a = b = c = 0.0;
+  for (int s = s0 + (int)threadIdx.x; s < s1; s += TPB)
+  {
+    a += spart[3*(size_t)s]; b += spart[3*(size_t)s + 1]; c += spart[3*(size_t)s + 2];
+  }
+  block_sum3(a, b, c);
+}
+
+// mode 0: free energy of each scene (before the sweeps)
+// mode 1: termination test after a sweep (bgs.c:1016-1036, as k_finish) per running scene;
+//         running scenes are counted into active[sweep]
+// mode 2: error of a refinement sweep; mode 3: the final merit
+__global__ __launch_bounds__(TPB) void k_uscene(int mode, const UScene *sc, const double *spart, BatchOut *out,
+                                                int maxiter, int nomerit, double eps, double meritval, int sweep,
+                                                int *active)
+{
+  BatchOut *o = out + blockIdx.x;
+  const UScene u = sc[blockIdx.x];
+  if (u.slice1 == u.slice0) { if (threadIdx.x == 0 && mode == 0) o->done = 1; return; }
+  if (mode == 1 && o->done) return;
+  if (mode == 2 && o->aborted) return;
+  double eu, el, up;
+  scene_partials(spart, u.slice0, u.slice1, eu, el, up);
+  if (threadIdx.x != 0) return;
+  if (mode == 0) { o->energy = 0.5*eu; return; }
+  if (mode == 3) { const double lo = o->energy < DBL_EPSILON ? 1.0 : o->energy; o->merit = 0.5*up/lo; return; }
+  const double error_k = sqrt(eu) / sqrt(el == 0.0 ? 1.0 : el);
+  if (mode == 2) { o->error = error_k; o->nref ++; o->sweeps ++; o->iters ++; return; }
+  const int k = o->sweeps;
+  o->sweeps = k + 1;
+  const int aborted = o->aborted != 0;
+  int done = 0;
+  if (nomerit)
+  {
+    o->iters = k + 1; o->error = error_k; o->merit = 0.0; o->final_buf = (k + 1) & 1; o->final_U = 0;
+    done = aborted || !(k + 1 < maxiter && error_k > eps);
+  }
+  else if (aborted) { o->iters = k + 1; o->final_buf = (k + 1) & 1; o->final_U = 0; done = 1; }
+  else if (k == 0) o->prev_error = error_k;
+  else
+  {
+    const double lo = o->energy < DBL_EPSILON ? 1.0 : o->energy;
+    const double merit = 0.5*up/lo, error = o->prev_error;
+    o->iters = k; o->error = error; o->merit = merit;
+    if (!(k < maxiter && (error > eps || merit > meritval))) { o->final_buf = k & 1; o->final_U = 1; done = 1; }
+    else o->prev_error = error_k;
+  }
+  if (done) o->done = 1;
+  else atomicAdd(active + sweep, 1);
+}
+
+// final state of the first phase into the FP64 store, per scene's final buffers
+template <typename T, int G>
+__global__ __launch_bounds__(TPB) void k_ufinal(int nslices, const int *row0, const int *nrows, const int *sscene,
+                                                const BatchOut *out, const T *X0, const T *X1, const T *Ut,
+                                                const T *Umt, double *R64, double *U64, int n, int mixed)
+{
+  const int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6);
+  if (s >= nslices) return;
+  const int p = slice_row<G>(row0, nrows, s);
+  if (p < 0) return;
+  const BatchOut &o = out[sscene[s]];
+  if (mixed || o.final_buf)
+  {
+    const T *Rf = o.final_buf ? X1 : X0;
+#pragma unroll
+    for (int k = 0; k < 3; k ++) R64[4*(size_t)p + k] = (double)Rf[4*(size_t)p + k];
+    R64[4*(size_t)p + 3] = 0.0;
+  }
+  if (mixed || o.final_U)
+  {
+    const T *Uf = o.final_U ? Umt : Ut;
+#pragma unroll
+    for (int k = 0; k < 3; k ++) U64[(size_t)k*n + p] = (double)Uf[(size_t)k*n + p];
+  }
+}
+
+// before the refinement sweeps: every scene that did not abort runs again
+__global__ void k_ureset(int nscenes, BatchOut *out)
+{
+  const int q = blockIdx.x*blockDim.x + threadIdx.x;
+  if (q < nscenes) out[q].done = out[q].aborted ? 1 : 0;
+}
+
+// after an odd number of refinement sweeps the scene's R is in the second buffer
+template <int G>
+__global__ __launch_bounds__(TPB) void k_ucopy(int nslices, const int *row0, const int *nrows, const int *sscene,
+                                               const BatchOut *out, const double *R64b, double *R64)
+{
+  const int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6);
+  if (s >= nslices) return;
+  const int p = slice_row<G>(row0, nrows, s);
+  if (p < 0 || !(out[sscene[s]].nref & 1)) return;
+#pragma unroll
+  for (int k = 0; k < 4; k ++) R64[4*(size_t)p + k] = R64b[4*(size_t)p + k];
+}
+
 struct Tuned { SFGS_BATCH_TUNE t; int valid; };
-static Tuned g_default = {{1, 4, 0}, 0};
+static Tuned g_default = {{2, 4, 0, 1}, 0};
 
 static bool tune_ok(const SFGS_BATCH_TUNE &t)
 {
-  return (t.lanes_per_row == 1 || t.lanes_per_row == 2) &&
+  return (t.lanes_per_row == 1 || t.lanes_per_row == 2) && (t.schedule == 0 || t.schedule == 1) &&
          (t.waves_per_group == 1 || t.waves_per_group == 2 || t.waves_per_group == 4 || t.waves_per_group == 8) &&
          t.rows_per_group >= 0;
 }
 
 } // namespace
+
+// the per-colour schedule of the batch (k_usweep, k_uscene); b holds the
+// sweep data and buffers as for the persistent kernel
+template <typename T, int G>
+static int run_union(sfgs_ctx *ctx, const SFGS_PARAM *par, const BatchArgs<T> &b)
+{
+  hipStream_t st = ctx->stream;
+  const int nsc = ctx->nscenes, ncol = ctx->ncolours, ns = ctx->nslices, n = ctx->nc;
+  const bool mixed = par->precision == SFGS_PREC_MIXED;
+  const int nomerit = par->meritval >= 1.0 ? 1 : 0, maxiter = par->maxiter;
+  const int refine = mixed ? par->refine : 0;
+  // slices listed colour by colour (all scenes), their scene and update bounds
+  std::vector<int> clist, cstart((size_t)ncol + 1, 0), sscene((size_t)std::max(ns, 1), 0),
+                   sbound(2*(size_t)std::max(ns, 1), 0), uscene(2*(size_t)std::max(nsc, 1), 0);
+  for (int c = 0; c < ncol; c ++)
+  {
+    cstart[c] = (int)clist.size();
+    for (int q = 0; q < nsc; q ++)
+    {
+      const int g = q*ncol + c;
+      for (int sl = ctx->h_cslice[g]; sl < ctx->h_cslice[g + 1]; sl ++)
+      {
+        clist.push_back(sl);
+        sscene[sl] = q;
+        sbound[2*(size_t)sl] = ctx->h_colptr[(size_t)q*ncol];   // rows of the scene's earlier colours
+        sbound[2*(size_t)sl + 1] = ctx->h_colptr[g];
+      }
+    }
+  }
+  cstart[ncol] = (int)clist.size();
+  for (int q = 0; q < nsc; q ++)
+  {
+    uscene[2*(size_t)q] = ctx->h_cslice[(size_t)q*ncol];
+    uscene[2*(size_t)q + 1] = ctx->h_cslice[(size_t)(q + 1)*ncol];
+  }
+  const int maxsweeps = nomerit ? maxiter : maxiter + 1;
+  CHK(upload(ctx, ctx->u_clist, clist.data(), std::max(clist.size(), (size_t)1)));
+  CHK(upload(ctx, ctx->u_sscene, sscene.data(), sscene.size()));
+  CHK(upload(ctx, ctx->u_sbound, sbound.data(), sbound.size()));
+  CHK(upload(ctx, ctx->u_scene, uscene.data(), uscene.size()));
+  CHK(ensure(ctx, ctx->u_spart, 3*(size_t)std::max(ns, 1)));
+  CHK(ensure(ctx, ctx->u_active, (size_t)maxsweeps + 1));
+  HIPCHK(hipMemsetAsync(ctx->u_active.p, 0, sizeof(int)*((size_t)maxsweeps + 1), st));
+  const UScene *usc = (const UScene*)ctx->u_scene.p;
+  const int2 *sbd = (const int2*)ctx->u_sbound.p;
+  BatchOut *out = (BatchOut*)ctx->bout.p;
+  double *spart = ctx->u_spart.p;
+  const int sblk = (ns + TPB/SLICE - 1) / (TPB/SLICE);
+  if (ns == 0 || nsc == 0) return 0;
+
+  // free energy of every scene
+  hipLaunchKernelGGL(k_uenergy<G>, dim3(sblk), dim3(TPB), 0, st, b.d, ns, spart);
+  hipLaunchKernelGGL(k_uscene, dim3(nsc), dim3(TPB), 0, st, 0, usc, (const double*)spart, out, maxiter, nomerit,
+                     par->epsilon, par->meritval, 0, ctx->u_active.p);
+  // Gauss-Seidel sweeps until every scene has stopped
+  auto sweep = [&](auto tag, SweepArgs<T> a, int kk, bool merit) {
+    (void)tag;
+    for (int c = 0; c < ncol; c ++)
+    {
+      const int nl = cstart[c + 1] - cstart[c];
+      if (nl == 0) continue;
+      const dim3 grid((nl + TPB/SLICE - 1) / (TPB/SLICE));
+      if (merit)
+        hipLaunchKernelGGL((k_usweep<T, G, true>), grid, dim3(TPB), 0, st, a, ctx->u_clist.p + cstart[c], nl, sbd,
+                           (const int*)ctx->u_sscene.p, out, spart, 0);
+      else
+        hipLaunchKernelGGL((k_usweep<T, G, false>), grid, dim3(TPB), 0, st, a, ctx->u_clist.p + cstart[c], nl, sbd,
+                           (const int*)ctx->u_sscene.p, out, spart, 0);
+    }
+    (void)kk;
+  };
+  int k = 0, batch = 2;
+  while (k < maxsweeps)
+  {
+    const int nbt = std::min(batch, maxsweeps - k);
+    for (int j = 0; j < nbt; j ++)
+    {
+      const int kk = k + j;
+      SweepArgs<T> a = b.a;
+      a.P = (kk & 1) ? b.X1 : b.X0;
+      a.Q = (kk & 1) ? b.X0 : b.X1;
+      sweep(0, a, kk, !nomerit && kk > 0);
+      hipLaunchKernelGGL(k_uscene, dim3(nsc), dim3(TPB), 0, st, 1, usc, (const double*)spart, out, maxiter, nomerit,
+                         par->epsilon, par->meritval, kk, ctx->u_active.p);
+    }
+    k += nbt;
+    int act = 0;
+    HIPCHK(hipMemcpyAsync(&act, ctx->u_active.p + (k - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipGetLastError());
+    if (act == 0) break;
+    batch = std::min(batch*2, 32);
+  }
+  hipLaunchKernelGGL((k_ufinal<T, G>), dim3(sblk), dim3(TPB), 0, st, ns, (const int*)ctx->slice_row0.p,
+                     (const int*)ctx->slice_nrows.p, (const int*)ctx->u_sscene.p, (const BatchOut*)out,
+                     (const T*)b.X0, (const T*)b.X1, (const T*)b.Ut, (const T*)b.Umt, b.R64, b.U64, n, mixed ? 1 : 0);
+  if (mixed)
+  {
+    // FP64 refinement sweeps of every scene that did not abort, then the final FP64 merit
+    if (refine > 0)
+    {
+      hipLaunchKernelGGL(k_ureset, dim3((nsc + TPB - 1)/TPB), dim3(TPB), 0, st, nsc, out);
+      for (int kk = 0; kk < refine; kk ++)
+      {
+        SweepArgs<double> a = b.d;
+        a.P = (kk & 1) ? b.R64b : b.R64;
+        a.Q = (kk & 1) ? b.R64 : b.R64b;
+        for (int c = 0; c < ncol; c ++)
+        {
+          const int nl = cstart[c + 1] - cstart[c];
+          if (nl == 0) continue;
+          hipLaunchKernelGGL((k_usweep<double, G, false>), dim3((nl + TPB/SLICE - 1) / (TPB/SLICE)), dim3(TPB), 0, st,
+                             a, ctx->u_clist.p + cstart[c], nl, sbd, (const int*)ctx->u_sscene.p, out, spart, 0);
+        }
+        hipLaunchKernelGGL(k_uscene, dim3(nsc), dim3(TPB), 0, st, 2, usc, (const double*)spart, out, maxiter, nomerit,
+                           par->epsilon, par->meritval, 0, ctx->u_active.p);
+      }
+      hipLaunchKernelGGL(k_ucopy<G>, dim3(sblk), dim3(TPB), 0, st, ns, (const int*)ctx->slice_row0.p,
+                         (const int*)ctx->slice_nrows.p, (const int*)ctx->u_sscene.p, (const BatchOut*)out,
+                         (const double*)b.R64b, b.R64);
+    }
+    if (!nomerit)
+    {
+      MeritArgs m = b.m;
+      m.R = b.R64;
+      hipLaunchKernelGGL(k_umerit<G>, dim3(sblk), dim3(TPB), 0, st, m, ns, (const int*)ctx->u_sscene.p,
+                         (const BatchOut*)out, spart);
+      hipLaunchKernelGGL(k_uscene, dim3(nsc), dim3(TPB), 0, st, 3, usc, (const double*)spart, out, maxiter, nomerit,
+                         par->epsilon, par->meritval, 0, ctx->u_active.p);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
 
 // one launch over all scenes (several if the scenes need more workgroups than
 // can be resident at once); ms receives the device time
@@ -493,10 +788,18 @@ static int batch_solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TU
     bf.X0 = ctx->R32.p; bf.X1 = ctx->R32b.p; bf.Ut = ctx->U32.p; bf.Umt = ctx->Um32.p;
   }
 
-  // launches over chunks of scenes that fit the resident capacity
+  // launches over chunks of scenes that fit the resident capacity, or the per-colour schedule
   std::vector<int> wgs((size_t)nsc, 0);
+  const bool per_colour = tn.schedule == 1 && !par->reverse;
   HIPCHK(hipEventRecord(ctx->ev0, st));
-  int s = 0;
+  if (per_colour)
+  {
+    int rc = 0;
+    if (mixed) rc = G == 2 ? run_union<float, 2>(ctx, par, bf) : run_union<float, 1>(ctx, par, bf);
+    else rc = G == 2 ? run_union<double, 2>(ctx, par, bd) : run_union<double, 1>(ctx, par, bd);
+    if (rc) return rc;
+  }
+  int s = per_colour ? nsc : 0;
   while (s < nsc)
   {
     std::vector<BatchScene> tab((size_t)nsc);
@@ -623,10 +926,12 @@ int SFGS_Batch_Autotune(sfgs_ctx *ctx, const SFGS_PARAM *par, int trial_iters, S
       ctx->G = lanes[li];
       if ((rc = reassemble(ctx))) break;
     }
-    for (int wi = 0; wi < 4 && !rc; wi ++)
-      for (int ri = 0; ri < 4 && !rc; ri ++)
+    // shape index 16 is the per-colour schedule (no workgroup shape)
+    for (int wr = 0; wr <= 16 && !rc; wr ++)
       {
-        SFGS_BATCH_TUNE t{lanes[li], waves[wi], rpg[ri]};
+        const int wi = wr / 4, ri = wr % 4;
+        SFGS_BATCH_TUNE t = wr == 16 ? SFGS_BATCH_TUNE{lanes[li], 4, 0, 1}
+                                     : SFGS_BATCH_TUNE{lanes[li], waves[wi], rpg[ri], 0};
         if (n > 0)
         {
           HIPCHK(hipMemcpyAsync(ctx->R.p, R0.p, sizeof(double)*4*(size_t)n, hipMemcpyDeviceToDevice, st));

@@ -114,4 +114,6 @@ struct sfgs_ctx {
   sfgs::DBuf<double> bpart;
   sfgs::DBuf<unsigned char> bout;         // per-scene result records
   sfgs::DBuf<int> babort;
+  sfgs::DBuf<int> u_clist, u_sscene, u_sbound, u_scene, u_active;   // per-colour batch schedule
+  sfgs::DBuf<double> u_spart;
 };

@@ -962,6 +962,8 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   release(ctx->body_scene); release(ctx->row_scene); release(ctx->bscene_tab); release(ctx->bwg_scene);
   release(ctx->bgptr); release(ctx->bgslice); release(ctx->bbar); release(ctx->bpart); release(ctx->bout);
   release(ctx->babort); release(ctx->flow_mem);
+  release(ctx->u_clist); release(ctx->u_sscene); release(ctx->u_sbound); release(ctx->u_scene);
+  release(ctx->u_active); release(ctx->u_spart);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);

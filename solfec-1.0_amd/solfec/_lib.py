@@ -33,7 +33,7 @@ class SFGS_REPORT(C.Structure):
 
 
 class SFGS_BATCH_TUNE(C.Structure):
-    _fields_ = [("lanes_per_row", _i), ("waves_per_group", _i), ("rows_per_group", _i)]
+    _fields_ = [("lanes_per_row", _i), ("waves_per_group", _i), ("rows_per_group", _i), ("schedule", _i)]
 
 
 class SFGS_SCENE_REPORT(C.Structure):

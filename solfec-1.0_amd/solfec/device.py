@@ -224,7 +224,7 @@ class Device:
         tot = SFGS_REPORT()
         t = None
         if tune is not None:
-            t = SFGS_BATCH_TUNE(*[int(v) for v in tune])
+            t = SFGS_BATCH_TUNE(*[int(v) for v in (tuple(tune) + (0,))[:4]])
         rc = self.lib.SFGS_Batch_Solve(self.ctx, C.byref(param), C.byref(t) if t is not None else None, reps,
                                        C.byref(tot))
         self._check(rc, "SFGS_Batch_Solve")
@@ -241,7 +241,7 @@ class Device:
         ms = C.c_double(0.0)
         self._check(self.lib.SFGS_Batch_Autotune(self.ctx, C.byref(param), int(trial_iters), C.byref(best),
                                                  C.byref(ms)), "SFGS_Batch_Autotune")
-        return (best.lanes_per_row, best.waves_per_group, best.rows_per_group), ms.value
+        return (best.lanes_per_row, best.waves_per_group, best.rows_per_group, best.schedule), ms.value
 
     def body_update(self, step, nb, download=True):
         if not download:

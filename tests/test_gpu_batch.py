@@ -129,3 +129,27 @@ def test_batch_autotune(dev):
     assert tune[0] in (1, 2) and tune[1] in (1, 2, 4, 8) and tune[2] > 0 and ms > 0
     reps, _ = dev.batch_solve(par, ns)
     assert np.all(reps["merit"] <= 1e-6) or np.all(reps["iters"] > 0)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_schedules_agree(dev, precision):
+    """The persistent launch and the per-colour schedule run the same sweeps of every
+    scene: bit-identical reactions whenever a scene stops after the same iteration
+    (the schedules add the error and merit partials in different orders)."""
+    scenes = _scenes(SPECS)
+    B, Cn, bs, cs = batch.concat(scenes)
+    par = solfec.make_param(epsilon=1e-7, maxiter=2000, meritval=1e-9, precision=precision, refine=2)
+    res = []
+    for tune in ((2, 4, 0, 0), (2, 4, 0, 1)):
+        ns = batch.upload(dev, B, Cn, bs, 1e-3)
+        reps, tot = dev.batch_solve(par, ns, tune=tune)
+        R, U = dev.get_RU()
+        print(f"[batch] schedule {tune[3]}: iters {list(reps['iters'])} merit {reps['merit'].max():.2e} "
+              f"solve {tot.solve_ms:.2f} ms")
+        res.append((reps, R, U))
+    (r0, R0, U0), (r1, R1, U1) = res
+    for k in range(ns):
+        assert abs(int(r0["iters"][k]) - int(r1["iters"][k])) <= 1
+        if r0["iters"][k] == r1["iters"][k]:
+            assert np.array_equal(batch.split(R0, cs, 3)[k], batch.split(R1, cs, 3)[k])
+            np.testing.assert_allclose(r1["merit"][k], r0["merit"][k], rtol=1e-9, atol=1e-300)

@@ -98,6 +98,7 @@ struct sfgs_ctx {
   sfgs::DBuf<unsigned long long> failkey;
   sfgs::DBuf<int> failcount;
   sfgs::DBuf<sfgs::Ctrl> ctrl;
+  sfgs::DBuf<int> fin;                    // ticket counter of the launch that takes a sweep's termination test
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   std::vector<hipEvent_t> prof_ev;        // per-launch sweep timing (profile mode)
   sfgs::DBuf<double> body_out;            // body update results (velo, force)

@@ -543,9 +543,13 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
   return x*q + min(x, rm) + k;
 }
 
+__device__ void finish_sweep(Ctrl *ctrl, const double *partials, int np, double *hist);
+
+// fin (the last launch of a sweep only): the block that completes the launch
+// last takes the sweep's termination test, as k_finish would after it
 template <typename T, int G, bool MERIT>
 __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<T> a, int s0, int s1, int pbase, int upd_lo, int upd_hi,
-                                                int backward, int xcd)
+                                                int backward, int xcd, int *fin, int np, double *hist)
 {
   if (a.ctrl->done) return;   // uniform: set only by k_finish between launches
   const int bb = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -558,6 +562,22 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<T> a, int s0, int s1, i
   {
     double *pt = a.partials + 3*(size_t)(pbase + blockIdx.x);
     pt[0] = eu; pt[1] = el; pt[2] = up;
+  }
+  if (fin)
+  {
+    __shared__ int last;
+    if (threadIdx.x == 0)
+    {
+      __threadfence();   // this block's partial before its ticket
+      last = atomicAdd(fin, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last)
+    {
+      __threadfence();   // every other block's partial is visible
+      finish_sweep(a.ctrl, a.partials, np, hist);
+      if (threadIdx.x == 0) *fin = 0;
+    }
   }
 }
 
@@ -575,9 +595,8 @@ __global__ __launch_bounds__(TPB) void k_merit(MeritArgs a, int nslices)
 // Without merit, sweep k closes iteration k.  With merit, sweep k >= 1 closes
 // iteration k-1 (its merit was formed during sweep k); sweep 0 only records its
 // error.  ctrl->final_buf / final_U name the buffers holding the final state.
-__global__ __launch_bounds__(TPB) void k_finish(Ctrl *ctrl, const double *partials, int np, double *hist)
+__device__ void finish_sweep(Ctrl *ctrl, const double *partials, int np, double *hist)
 {
-  if (ctrl->done) return;
   double eu = 0.0, el = 0.0, up = 0.0;
   for (int k = threadIdx.x; k < np; k += TPB)
   {
@@ -623,6 +642,12 @@ __global__ __launch_bounds__(TPB) void k_finish(Ctrl *ctrl, const double *partia
     ctrl->done = 1;
   }
   else ctrl->prev_error = error_k;
+}
+
+__global__ __launch_bounds__(TPB) void k_finish(Ctrl *ctrl, const double *partials, int np, double *hist)
+{
+  if (ctrl->done) return;
+  finish_sweep(ctrl, partials, np, hist);
 }
 
 // learned warm-start features of row p: gap, B (3), W_ii (9), R_prev (3), row
@@ -958,7 +983,7 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   release(ctx->W32); release(ctx->Wd32); release(ctx->B32); release(ctx->R32); release(ctx->R32b); release(ctx->U32); release(ctx->Um32);
   release(ctx->rho32); release(ctx->pfri32); release(ctx->pcoh32); release(ctx->shift32);
   release(ctx->partials); release(ctx->mpartials); release(ctx->hist); release(ctx->failkey);
-  release(ctx->failcount); release(ctx->ctrl);
+  release(ctx->failcount); release(ctx->ctrl); release(ctx->fin);
   release(ctx->body_scene); release(ctx->row_scene); release(ctx->bscene_tab); release(ctx->bwg_scene);
   release(ctx->bgptr); release(ctx->bgslice); release(ctx->bbar); release(ctx->bpart); release(ctx->bout);
   release(ctx->babort); release(ctx->flow_mem);
@@ -1199,36 +1224,47 @@ int SFGS_System_Upload(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, 
 } // extern "C"
 
 // one sweep over all colours (forward or backward) with or without the fused merit;
-// sweep k reads buffer X[k & 1] and writes X[(k + 1) & 1]
+// sweep k reads buffer X[k & 1] and writes X[(k + 1) & 1].  The last launch also
+// takes the termination test (fin); returns false when nothing was launched
+// (then k_finish takes it)
 template <typename T>
-static void launch_sweep(sfgs_ctx *ctx, const SweepArgs<T> &base, T *X0, T *X1, int k, int backward, bool merit,
-                         const std::vector<int> &grid, const std::vector<int> &pbase, bool prof)
+static bool launch_sweep(sfgs_ctx *ctx, const SweepArgs<T> &base, T *X0, T *X1, int k, int backward, bool merit,
+                         const std::vector<int> &grid, const std::vector<int> &pbase, bool prof, int *fin, int np,
+                         double *hist)
 {
   const int ncol = ctx->ncolours, n = ctx->nc;
   const int xcd = ctx->schedule == 2 ? 0 : 1;
   SweepArgs<T> a = base;
   if (k & 1) { a.P = X1; a.Q = X0; }
   else { a.P = X0; a.Q = X1; }
+  int lastc = -1;
+  for (int ci = 0; ci < ncol; ci ++)
+  {
+    int c = backward ? ncol - 1 - ci : ci;
+    if (grid[c] > 0) lastc = c;
+  }
   for (int ci = 0; ci < ncol; ci ++)
   {
     int c = backward ? ncol - 1 - ci : ci;
     if (grid[c] == 0) continue;
     int lo = backward ? ctx->h_colptr[c + 1] : 0, hi = backward ? n : ctx->h_colptr[c];
+    int *f = c == lastc ? fin : nullptr;
     if (prof) (void)hipEventRecord(ctx->prof_ev[2*ci], ctx->stream);
     if (merit)
     {
       SFGS_DISPATCH_G(ctx->G_built, auto kern = k_sweep<T COMMA GG COMMA true>; hipLaunchKernelGGL(kern, dim3(grid[c]),
                       dim3(TPB), 0, ctx->stream, a, ctx->h_cslice[c], ctx->h_cslice[c + 1], pbase[c], lo, hi, backward,
-                      xcd));
+                      xcd, f, np, hist));
     }
     else
     {
       SFGS_DISPATCH_G(ctx->G_built, auto kern = k_sweep<T COMMA GG COMMA false>; hipLaunchKernelGGL(kern, dim3(grid[c]),
                       dim3(TPB), 0, ctx->stream, a, ctx->h_cslice[c], ctx->h_cslice[c + 1], pbase[c], lo, hi, backward,
-                      xcd));
+                      xcd, f, np, hist));
     }
     if (prof) (void)hipEventRecord(ctx->prof_ev[2*ci + 1], ctx->stream);
   }
+  return lastc >= 0;
 }
 
 struct GsPlan {
@@ -1257,8 +1293,9 @@ static int run_gs(sfgs_ctx *ctx, SweepArgs<T> sa, T *X0, T *X1, int maxiter, int
     for (int j = 0; j < nbt; j ++)
     {
       int kk = k + j;
-      launch_sweep<T>(ctx, sa, X0, X1, kk, reverse && (kk % 2), !nomerit && kk > 0, pl.grid, pl.pbase, pl.prof);
-      hipLaunchKernelGGL(k_finish, dim3(1), dim3(TPB), 0, st, ctx->ctrl.p, ctx->partials.p, pl.np, hist);
+      if (!launch_sweep<T>(ctx, sa, X0, X1, kk, reverse && (kk % 2), !nomerit && kk > 0, pl.grid, pl.pbase, pl.prof,
+                           ctx->fin.p, pl.np, hist))
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(TPB), 0, st, ctx->ctrl.p, ctx->partials.p, pl.np, hist);
     }
     k += nbt;
     HIPCHK(hipMemcpyAsync(&hc, ctx->ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
@@ -1349,6 +1386,8 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   CHK(ensure(ctx, ctx->failkey, 1));
   CHK(ensure(ctx, ctx->failcount, 4));
   CHK(ensure(ctx, ctx->ctrl, 1));
+  CHK(ensure(ctx, ctx->fin, 1));
+  HIPCHK(hipMemsetAsync(ctx->fin.p, 0, sizeof(int), st));
   HIPCHK(hipMemsetAsync(ctx->partials.p, 0, sizeof(double)*3*(size_t)std::max(pl.np, 1), st));
   HIPCHK(hipMemsetAsync(ctx->failkey.p, 0, sizeof(unsigned long long), st));
   HIPCHK(hipMemsetAsync(ctx->failcount.p, 0, sizeof(int)*4, st));

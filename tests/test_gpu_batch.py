@@ -81,6 +81,7 @@ def test_batch_workgroups_do_not_change_result(dev):
     scenes = _scenes(SPECS[:4])
     B, Cn, bs, cs = batch.concat(scenes)
     par = solfec.make_param(epsilon=1e-7, maxiter=2000, meritval=1e-9)
+    dev.set_layout(2)
     res = []
     for tune in ((2, 4, 1 << 30), (2, 1, 32), (2, 2, 200), (2, 8, 64), (2, 4, 0)):   # lanes: the default layout
         # the solve writes R back into the uploaded contacts: upload the same R0 for every shape
@@ -139,6 +140,7 @@ def test_batch_schedules_agree(dev, precision):
     scenes = _scenes(SPECS)
     B, Cn, bs, cs = batch.concat(scenes)
     par = solfec.make_param(epsilon=1e-7, maxiter=2000, meritval=1e-9, precision=precision, refine=2)
+    dev.set_layout(2)   # the autotuner may have left another layout on the shared device
     res = []
     for tune in ((2, 4, 0, 0), (2, 4, 0, 1)):
         ns = batch.upload(dev, B, Cn, bs, 1e-3)

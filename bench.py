@@ -201,7 +201,8 @@ def bench_batch(args):
         "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"{args.scenes} independent lattice scenes, contacts log-uniform [500, 50000], "
-                               "one persistent launch", "scenes": args.scenes, "contacts": ncon,
+                               + ("one launch per colour over all running scenes" if tune[3] == 1
+                                  else "one persistent launch"), "scenes": args.scenes, "contacts": ncon,
                    "epsilon": eps, "meritval": 1e-6, "maxiter": maxiter, "shape": list(tune),
                    "learned_model": have_model, "learned_mixed": head, "plain_fp64": plain,
                    "speedup_vs_plain_fp64": plain["ms_per_step"] / head["ms_per_step"],

@@ -111,6 +111,17 @@ int SOLFEC_Get_Contacts(SOLFEC *sol, int *master, int *slave, double *mpnt, doub
                         double *R, double *U);
 /* replace the carried reactions / velocities (warm start) of the current contacts */
 int SOLFEC_Set_RU(SOLFEC *sol, const double *R, const double *U);
+/* LOCDYN_Dump (ldy.c:892-1002) and LOCDYN_W_MatrixMarket (ldy.c:1005-1052) of
+ * the last assembled step, in the reference's text formats (rows in list
+ * order, newest contact first); the _Write forms take host arrays in contact
+ * order (natural CSR adjacency as SFGS_Get_Locdyn returns it) */
+int SOLFEC_Locdyn_Dump(SOLFEC *sol, const char *path);
+int SOLFEC_W_MatrixMarket(SOLFEC *sol, const char *path);
+int SOLFEC_Locdyn_Write(const char *path, int n, const double *point, const int *master, const int *slave,
+                        const double *base, const double *Wd, const double *B, const double *V, const double *U,
+                        const double *R, const int *rowptr, const int *col, const double *W);
+int SOLFEC_W_MatrixMarket_Write(const char *path, int n, const double *Wd, const int *rowptr, const int *col,
+                                const double *W);
 /* snapshot writer: after each assembly, the features of every contact (18
  * floats, natural order, see SFGS_Features) are copied to buffer when the step
  * has at most cap contacts; SOLFEC_Captured gives the count of the last step */

@@ -1034,6 +1034,158 @@ int SOLFEC_Step_Solve(SOLFEC *sol, GAUSS_SEIDEL *gs)
   return err;
 }
 
+/* ---------------- LOCDYN dump formats ---------------- */
+
+typedef struct { const double *point; const int *master, *slave; } DUMPKEY;
+
+/* order of the neighbours of a dump line (dumpcmp, ldy.c:330-353): point, then
+ * the sorted body pair (all rows here are contacts) */
+static int dump_cmp(const DUMPKEY *k, int a, int b)
+{
+  for (int i = 0; i < 3; i ++)
+  {
+    if (k->point[3*a + i] < k->point[3*b + i]) return -1;
+    if (k->point[3*a + i] > k->point[3*b + i]) return 1;
+  }
+  int am = k->master[a], as = k->slave[a], bm = k->master[b], bs = k->slave[b];
+  int a0 = am < as ? am : as, a1 = am < as ? as : am, b0 = bm < bs ? bm : bs, b1 = bm < bs ? bs : bm;
+  if (a0 != b0) return a0 < b0 ? -1 : 1;
+  if (a1 != b1) return a1 < b1 ? -1 : 1;
+  return 0;
+}
+
+#define G6 "%.6g"
+#define V3 "(" G6 ", " G6 ", " G6 ")"
+#define M9 "[" G6 ", " G6 ", " G6 ", " G6 ", " G6 ", " G6 ", " G6 ", " G6 ", " G6 "]"
+
+/* LOCDYN_Dump (ldy.c:892-1002) on host arrays in contact order: one line per
+ * row in list order (newest contact first, ldy.c:505-508), neighbours sorted
+ * by dump_cmp with the blocks of a neighbour reached through both bodies
+ * added together */
+int SOLFEC_Locdyn_Write(const char *path, int n, const double *point, const int *master, const int *slave,
+                        const double *base, const double *Wd, const double *B, const double *V, const double *U,
+                        const double *R, const int *rowptr, const int *col, const double *W)
+{
+  FILE *f = fopen(path, "w");
+  if (!f) return SOL_ERR_ARG;
+  DUMPKEY key = {point, master, slave};
+  int *nb = NULL, cap = 0;
+  double *acc = NULL;
+  for (int i = n - 1; i >= 0; i --)
+  {
+    const double *b = base + 9*i, *w = Wd + 9*i;
+    fprintf(f, "CONTACT " V3 " (%d, %d) " M9 " " M9 " " V3 " " V3 " " V3 " " V3,
+            point[3*i], point[3*i + 1], point[3*i + 2], master[i], slave[i],
+            b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7], b[8], w[0], w[1], w[2], w[3], w[4], w[5], w[6], w[7], w[8],
+            B[3*i], B[3*i + 1], B[3*i + 2], V[3*i], V[3*i + 1], V[3*i + 2],
+            U[3*i], U[3*i + 1], U[3*i + 2], R[3*i], R[3*i + 1], R[3*i + 2]);
+    /* distinct neighbours with summed blocks, insertion-sorted by dump_cmp */
+    int m = 0, len = rowptr[i + 1] - rowptr[i];
+    if (len > cap) { cap = len; nb = xrealloc(nb, sizeof(int)*(size_t)cap); acc = xrealloc(acc, sizeof(double)*9*(size_t)cap); }
+    for (int q = rowptr[i]; q < rowptr[i + 1]; q ++)
+    {
+      int j = col[q], at = 0;
+      while (at < m && nb[at] != j) at ++;
+      if (at == m)
+      {
+        int pos = m;
+        while (pos > 0 && dump_cmp(&key, j, nb[pos - 1]) < 0) pos --;
+        memmove(nb + pos + 1, nb + pos, sizeof(int)*(size_t)(m - pos));
+        memmove(acc + 9*(pos + 1), acc + 9*pos, sizeof(double)*9*(size_t)(m - pos));
+        nb[pos] = j;
+        memset(acc + 9*pos, 0, 9*sizeof(double));
+        at = pos;
+        m ++;
+      }
+      for (int e = 0; e < 9; e ++) acc[9*at + e] += W[9*(size_t)q + e];
+    }
+    for (int t = 0; t < m; t ++)
+    {
+      int j = nb[t];
+      const double *bj = base + 9*j, *a = acc + 9*t;
+      fprintf(f, " => CONTACT " V3 " (%d, %d) " M9 " " M9, point[3*j], point[3*j + 1], point[3*j + 2], master[j], slave[j],
+              bj[0], bj[1], bj[2], bj[3], bj[4], bj[5], bj[6], bj[7], bj[8], a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8]);
+    }
+    fprintf(f, "\n");
+  }
+  free(nb);
+  free(acc);
+  fclose(f);
+  return 0;
+}
+
+/* LOCDYN_W_MatrixMarket (ldy.c:1005-1052): W as 3n x 3n coordinate entries,
+ * rows numbered in list order */
+int SOLFEC_W_MatrixMarket_Write(const char *path, int n, const double *Wd, const int *rowptr, const int *col,
+                                const double *W)
+{
+  FILE *f = fopen(path, "w");
+  if (!f) return SOL_ERR_ARG;
+  long long nnz = 9LL*n + 9LL*rowptr[n];
+  fprintf(f, "%%%%MatrixMarket matrix coordinate real general\n");
+  fprintf(f, "%d  %d  %lld\n", 3*n, 3*n, nnz);
+  for (int i = n - 1; i >= 0; i --)
+  {
+    int r = 3*(n - 1 - i) + 1;
+    for (int a = 0; a < 3; a ++)
+      for (int b = 0; b < 3; b ++) fprintf(f, "%d  %d  %.15g\n", r + a, r + b, Wd[9*i + 3*b + a]);
+    for (int q = rowptr[i]; q < rowptr[i + 1]; q ++)
+    {
+      int c = 3*(n - 1 - col[q]) + 1;
+      for (int a = 0; a < 3; a ++)
+        for (int b = 0; b < 3; b ++) fprintf(f, "%d  %d  %.15g\n", r + a, c + b, W[9*(size_t)q + 3*b + a]);
+    }
+  }
+  fclose(f);
+  return 0;
+}
+
+/* the last assembled system of the domain, downloaded from the device */
+static int locdyn_download(SOLFEC *sol, int **rowptr, int **col, double **W, double **Wd, double **B, double **V)
+{
+  sfgs_ctx *g = sol->gpu;
+  int n = 0, ncol = 0;
+  long long nnz = 0;
+  if (!g || SFGS_Get_Sizes(g, &n, &nnz, &ncol) || n != sol->nc)
+  {
+    snprintf(sol->err, sizeof(sol->err), "no assembled local dynamics to dump");
+    return SOL_ERR_ARG;
+  }
+  *rowptr = xrealloc(NULL, sizeof(int)*((size_t)n + 1));
+  *col = xrealloc(NULL, sizeof(int)*((size_t)nnz + 1));
+  *W = xrealloc(NULL, sizeof(double)*9*((size_t)nnz + 1));
+  *Wd = xrealloc(NULL, sizeof(double)*9*((size_t)n + 1));
+  *B = xrealloc(NULL, sizeof(double)*3*((size_t)n + 1));
+  *V = xrealloc(NULL, sizeof(double)*3*((size_t)n + 1));
+  if (SFGS_Get_Locdyn(g, *rowptr, *col, NULL, *W, *Wd, NULL, *B, *V, NULL, NULL, NULL))
+  {
+    snprintf(sol->err, sizeof(sol->err), "LOCDYN download failed: %s", SFGS_Last_Error(g));
+    return SOL_ERR_GPU;
+  }
+  return 0;
+}
+
+int SOLFEC_Locdyn_Dump(SOLFEC *sol, const char *path)
+{
+  int *rowptr = NULL, *col = NULL;
+  double *W = NULL, *Wd = NULL, *B = NULL, *V = NULL;
+  int rc = locdyn_download(sol, &rowptr, &col, &W, &Wd, &B, &V);
+  if (!rc) rc = SOLFEC_Locdyn_Write(path, sol->nc, sol->c_point, sol->c_master, sol->c_slave, sol->c_base, Wd, B, V,
+                                    sol->c_U, sol->c_R, rowptr, col, W);
+  free(rowptr); free(col); free(W); free(Wd); free(B); free(V);
+  return rc;
+}
+
+int SOLFEC_W_MatrixMarket(SOLFEC *sol, const char *path)
+{
+  int *rowptr = NULL, *col = NULL;
+  double *W = NULL, *Wd = NULL, *B = NULL, *V = NULL;
+  int rc = locdyn_download(sol, &rowptr, &col, &W, &Wd, &B, &V);
+  if (!rc) rc = SOLFEC_W_MatrixMarket_Write(path, sol->nc, Wd, rowptr, col, W);
+  free(rowptr); free(col); free(W); free(Wd); free(B); free(V);
+  return rc;
+}
+
 int SOLFEC_Run(SOLFEC *sol, GAUSS_SEIDEL *gs, double duration)
 {
   if (!sol || !gs || !(duration > 0.0)) return SOL_ERR_ARG;

@@ -19,7 +19,8 @@ from ._lib import SFGS_REPORT, f64, ptr, solfec_lib
 from .device import DIAGSOLVERS, FAILURES, GS_ERRORS, Device, SfgsError, make_param, default_diag  # noqa: F401
 
 __all__ = ["SOLFEC", "SURFACE_MATERIAL", "BULK_MATERIAL", "SPHERE", "ELLIP", "PLANE", "BODY", "GRAVITY",
-           "INITIAL_VELOCITY", "GAUSS_SEIDEL_SOLVER", "RUN", "LATTICE", "ELLIP_LATTICE", "Device", "SfgsError"]
+           "INITIAL_VELOCITY", "GAUSS_SEIDEL_SOLVER", "RUN", "LATTICE", "ELLIP_LATTICE", "LOCDYN_DUMP",
+           "W_MATRIX_MARKET", "Device", "SfgsError"]
 
 _CALLBACK_T = C.CFUNCTYPE(C.c_int, C.c_void_p)
 
@@ -361,6 +362,18 @@ class GAUSS_SEIDEL_SOLVER:
         rep = SFGS_REPORT()
         self.lib.GAUSS_SEIDEL_Report(self.gs, C.byref(rep))
         return rep
+
+
+def LOCDYN_DUMP(solfec, path):
+    """LOCDYN_DUMP(solfec, path) (lng.c, ldy.c:892-1002): text dump of the last assembled step."""
+    if solfec.lib.SOLFEC_Locdyn_Dump(solfec.sol, str(path).encode()):
+        raise RuntimeError(solfec.lib.SOLFEC_Last_Error(solfec.sol).decode())
+
+
+def W_MATRIX_MARKET(solfec, path):
+    """W of the last assembled step in MatrixMarket format (LOCDYN_W_MatrixMarket, ldy.c:1005-1052)."""
+    if solfec.lib.SOLFEC_W_MatrixMarket(solfec.sol, str(path).encode()):
+        raise RuntimeError(solfec.lib.SOLFEC_Last_Error(solfec.sol).decode())
 
 
 def RUN(solfec, solver, duration):

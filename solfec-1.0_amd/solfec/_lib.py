@@ -98,6 +98,10 @@ _SOL_PROTOS = {
     "SOLFEC_Lattice_Scene": (_i, [_p, _i, _d, _d, _d, _d, _d, C.c_ulonglong]),
     "SOLFEC_Ellipsoid_Lattice_Scene": (_i, [_p, _i, _d, _d, _d, _d, _d, C.c_ulonglong]),
     "SOLFEC_Ellipsoid": (_i, [_p, _p, _p, _p, _d, _i]),
+    "SOLFEC_Locdyn_Dump": (_i, [_p, C.c_char_p]),
+    "SOLFEC_W_MatrixMarket": (_i, [_p, C.c_char_p]),
+    "SOLFEC_Locdyn_Write": (_i, [C.c_char_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "SOLFEC_W_MatrixMarket_Write": (_i, [C.c_char_p, _i, _p, _p, _p, _p]),
     "SOLFEC_Run": (_i, [_p, _p, _d]),
     "SOLFEC_Step_Begin": (_i, [_p]),
     "SOLFEC_Step_Solve": (_i, [_p, _p]),

@@ -226,3 +226,32 @@ def test_ellipsoids_assembly_and_gs_match_oracle(dev):
     print(f"[ellip] contacts {O['n']} iters gpu {rep.iters} oracle {orep.iters} |dR|/max|R| {err:.2e}")
     assert abs(rep.iters - orep.iters) <= 1
     assert err < RTOL_R
+
+
+def test_locdyn_dump_of_a_gpu_step_matches_oracle(tmp_path):
+    """LOCDYN_DUMP after a GPU step against the same text written from the oracle's assembly
+    of that step (and the step's R, U), compared as the reference's locdyn_compare does."""
+    from solfec import locdyn
+    from solfec._lib import ptr, solfec_lib
+    s = lattice(216, seed=9)
+    b, c = s.bodies(), s.contacts()
+    O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), 1e-3)
+    gs = solfec.GAUSS_SEIDEL_SOLVER(1e-6, 1000)
+    assert s.step_solve(gs) == 0
+    solfec.LOCDYN_DUMP(s, tmp_path / "gpu.txt")
+    solfec.W_MATRIX_MARKET(s, tmp_path / "gpu.mtx")
+    c2 = s.contacts()
+    n = O["n"]
+    x, x0 = b["x"].reshape(-1, 3), b["x0"].reshape(-1, 3)
+    rot = b["rot"].reshape(-1, 3, 3).transpose(0, 2, 1)
+    m = c["master"]
+    point = np.ascontiguousarray([x[m[i]] + rot[m[i]] @ (c["mpnt"].reshape(n, 3)[i] - x0[m[i]]) for i in range(n)])
+    f = np.ascontiguousarray
+    args = [point, f(c["master"], np.int32), f(c["slave"], np.int32), f(c["base"]), f(O["Wd"]), f(O["B"]), f(O["V"]),
+            f(c2["U"]), f(c2["R"]), f(O["rowptr"], np.int32), f(O["col"], np.int32), f(O["W"])]
+    assert solfec_lib().SOLFEC_Locdyn_Write(str(tmp_path / "ora.txt").encode(), n, *[ptr(a) for a in args]) == 0
+    assert locdyn.compare(tmp_path / "gpu.txt", tmp_path / "ora.txt", rtol=2e-5, atol=1e-12)
+    import scipy.io
+    M = scipy.io.mmread(str(tmp_path / "gpu.mtx")).toarray()
+    assert M.shape == (3 * n, 3 * n)
+    np.testing.assert_allclose(M, M.T, atol=1e-12 * np.abs(M).max())

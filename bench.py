@@ -63,26 +63,72 @@ def settle(n, L, steps, seed, friction, shape="sphere"):
     return s
 
 
-def cpu_baseline(b, c, colours, n, seconds):
+def cpu_baseline(b, c, colours, n, seconds, maxiter):
     """The oracle (C restatement, 1 core) on the same system: GS iterations of
-    sweep + merit, as the GPU solve does, for about `seconds` of CPU time."""
+    sweep + merit, as the GPU solve does, for about `seconds` of CPU time.  The
+    local solver runs with the GPU solve's diagepsilon and diagmaxiter (those of
+    epsilon = meritval = 1e-6); only the outer test is switched off, so that
+    exactly the planned number of iterations runs."""
     import oracle
     O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), 1e-3)
     sysobj = oracle.System(O)
     order, _ = oracle.colour_order(colours, int(colours.max()) + 1)
     R0, U0 = c["R"].copy(), c["U"].copy()
-    p = oracle.gs_param(1e-6, 1, meritval=1e-6)
+    de, dm = oracle.default_diag(1e-6, maxiter, 1e-6)
+    p = oracle.gs_param(1e-300, 1, meritval=1e-300, diagepsilon=de, diagmaxiter=dm)
     t0 = time.perf_counter()
     oracle.gs_solve(sysobj, p, order, R0, U0, history=False)
     t1 = time.perf_counter() - t0
     k = max(1, int(seconds / max(t1, 1e-6)))
-    p = oracle.gs_param(1e-300, k, meritval=1e-300)
+    p = oracle.gs_param(1e-300, k, meritval=1e-300, diagepsilon=de, diagmaxiter=dm)
     t0 = time.perf_counter()
     rc, rep, _, _, _, _ = oracle.gs_solve(sysobj, p, order, R0, U0, history=False)
     dt = time.perf_counter() - t0
+    if rep.iters != k:
+        log(f"[bench] cpu baseline: the oracle stopped after {rep.iters} of {k} iterations "
+            f"(relative error {rep.rel_error}, merit {rep.merit})")
+    # the reference's own row order (LOCDYN list order, newest contact first), half the sample
+    kl = max(1, k // 2)
+    p = oracle.gs_param(1e-300, kl, meritval=1e-300, diagepsilon=de, diagmaxiter=dm)
+    t0 = time.perf_counter()
+    _, repl, _, _, _, _ = oracle.gs_solve(sysobj, p, oracle.list_order(n), R0, U0, history=False)
+    dtl = time.perf_counter() - t0
     return dict(value=rep.iters * n / dt, unit="contact-block updates/s", cores=1, kind="port",
                 sample=f"{rep.iters} GS iterations (colour-major sweep + merit) of the same {n}-contact "
-                       f"system, oracle/liboracle.so on 1 host core, {dt:.1f} s")
+                       f"system, oracle/liboracle.so on 1 host core, {dt:.1f} s",
+                list_order=dict(value=repl.iters * n / dtl, iterations=int(repl.iters), seconds=round(dtl, 2)))
+
+
+def stream_copy_rate(mib=1024, reps=10):
+    """On-device copy rate of this GPU, for comparison with the nominal HBM peak:
+    hipMemcpyAsync device to device of `mib` MiB, `reps` times, timed with HIP
+    events; bytes counted = read + write."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+    hip.hipEventSynchronize.argtypes = [C.c_void_p]
+    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    hip.hipEventDestroy.argtypes = [C.c_void_p]
+    hip.hipFree.argtypes = [C.c_void_p]
+    nbytes = mib << 20
+    src, dst, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)):
+        raise RuntimeError("stream copy: hipMalloc failed")
+    assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
+    assert hip.hipMemcpyAsync(dst, src, nbytes, 3, None) == 0       # hipMemcpyDeviceToDevice, warm-up
+    hip.hipEventRecord(e0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(dst, src, nbytes, 3, None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+    for e in (e0, e1):
+        hip.hipEventDestroy(e)
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
 
 
 def measured_traffic(kernel, args):
@@ -91,7 +137,7 @@ def measured_traffic(kernel, args):
     device copy of known size), or None when none matches."""
     import re
     path = os.path.join(ROOT, "profiles", "k_sweep_config2_traffic.md")
-    if args.schedule == 1 or not os.path.exists(path):
+    if args.schedule == 1 or args.shape != "sphere" or not os.path.exists(path):
         return None
     key = f"{kernel} lanes={args.lanes} order={args.row_order}"
     for line in open(path):
@@ -357,11 +403,15 @@ def main():
                 avg_launch_us=round(1e3 * prep.sweep_ms / max(prep.sweep_launches, 1), 3),
                 launches=int(prep.sweep_launches), bytes_per_sweep=int(bytes_per_sweep))
     log(f"[bench] sweep kernel: {prep.sweep_launches} launches, {prep.sweep_ms:.2f} ms, {achieved:.1f} GB/s")
+    copy = stream_copy_rate()
+    roof["copy_rate"] = dict(value=round(copy, 1), unit="GB/s", frac_of_copy=round(achieved / copy, 4),
+                             what="hipMemcpyAsync device to device, 1 GiB x 10, read + write bytes")
+    log(f"[bench] device copy rate {copy:.1f} GB/s")
 
     cpu = None
     if not args.no_cpu:
         colours = dev.locdyn()["colour"]
-        cpu = cpu_baseline(b, c, colours, n, args.cpu_seconds)
+        cpu = cpu_baseline(b, c, colours, n, args.cpu_seconds, args.maxiter)
         log(f"[bench] cpu baseline: {cpu['value']:.3e} updates/s")
 
     out = {

@@ -41,7 +41,10 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def settle(n, L, steps, seed, friction, shape="sphere"):
+PHASES = ("TIMINT", "CONUPD", "CONDET", "LOCDYN", "CONSOL")
+
+
+def scene(n, L, seed, friction, shape="sphere"):
     import solfec
     s = solfec.SOLFEC("DYNAMIC", 1e-3, "/tmp/solfec_bench")
     solfec.SURFACE_MATERIAL(s, friction=friction, restitution=0.0)
@@ -50,6 +53,15 @@ def settle(n, L, steps, seed, friction, shape="sphere"):
         solfec.ELLIP_LATTICE(s, n, L, 0.03, 0.06, 0.005, 2000.0, seed)
     else:
         solfec.LATTICE(s, n, L, 0.045, 0.055, 0.005, 2000.0, seed)
+    return s
+
+
+def settle(n, L, steps, seed, friction, shape="sphere", device_loop=True):
+    """The settling run through the product's RUN loop: by default the device loop
+    (bodies and contacts stay on the GPU; the same bits as the host loop)."""
+    import solfec
+    s = scene(n, L, seed, friction, shape)
+    s.device_loop(device_loop)
     gs = solfec.GAUSS_SEIDEL_SOLVER(1e-4, 1000)
     t0 = time.perf_counter()
     for k in range(steps):
@@ -59,8 +71,49 @@ def settle(n, L, steps, seed, friction, shape="sphere"):
         if (k + 1) % 25 == 0 or k == 0:
             log(f"[settle] step {k + 1}/{steps} contacts={s.ncon} gs_iters={gs.iters} "
                 f"elapsed={time.perf_counter() - t0:.1f}s")
+    log("[settle] phases (s): " + ", ".join(f"{p} {s.timing(p):.2f}" for p in PHASES))
     s.step_begin()   # contacts of the next step; R, U carried from the last solve
+    s.device_loop(False)
     return s
+
+
+def bench_run(args):
+    """The whole time step (RUN loop) with bodies and contacts on the device
+    against the host loop: config 1 (1000 spheres, 200 steps, GS 1e-4 / 1000)
+    and the first steps of the config-2 settle.  Both loops compute the same bits;
+    the final states are compared."""
+    import solfec
+    out = {}
+    for name, n, L, steps, fr in (("config1", 1000, 1.0, 200, 0.3), ("config2_settle", args.spheres, args.box,
+                                                                    args.run_steps, args.friction)):
+        res = {}
+        states = []
+        for loop in ("host", "device"):
+            s = scene(n, L, args.seed if name != "config1" else 1, fr)
+            s.device_loop(loop == "device")
+            gs = solfec.GAUSS_SEIDEL_SOLVER(1e-4, 1000)
+            t0 = time.perf_counter()
+            for k in range(steps):
+                rc = s.step_begin() or s.step_solve(gs) or s.step_end()
+                if rc:
+                    raise RuntimeError(f"{name} {loop} step {k}: {s.lib.SOLFEC_Last_Error(s.sol).decode()}")
+            dt = time.perf_counter() - t0
+            ph = {p: round(1e3 * s.timing(p) / steps, 3) for p in PHASES}
+            res[loop] = dict(ms_per_step=1e3 * dt / steps, contacts=s.ncon, phases_ms_per_step=ph)
+            log(f"[run] {name} {loop}: {1e3 * dt / steps:.2f} ms/step, {s.ncon} contacts, phases {ph}")
+            b = s.bodies()
+            states.append((b["x"], b["velo"], s.contacts()["R"]))
+        same = all(np.array_equal(a, b) for a, b in zip(*states))
+        res["identical_state"] = bool(same)
+        res["speedup"] = res["host"]["ms_per_step"] / res["device"]["ms_per_step"]
+        out[name] = res
+    line = {"metric": "RUN loop ms/step: device-resident bodies and contacts vs host loop", "value":
+            out["config2_settle"]["device"]["ms_per_step"], "unit": "ms/step", "n_gpus": 1, "steps": args.run_steps,
+            "warmup": 0, "ms_per_step": out["config2_settle"]["device"]["ms_per_step"], "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"config-1 run (1000 spheres, 200 steps) and the first {args.run_steps} steps of "
+                                   f"the {args.spheres}-sphere settle, GS 1e-4 / 1000", **out}}
+    print(json.dumps(line), flush=True)
 
 
 def cpu_baseline(b, c, colours, n, seconds, maxiter):
@@ -288,17 +341,20 @@ def main():
                     help="only run 3 device copies of MIB MiB (scale of the traffic counters) and exit")
     ap.add_argument("--shape", default="sphere", choices=["sphere", "ellipsoid"],
                     help="packing bodies: spheres (config 2) or rigid ellipsoids (the config-3 variant)")
-    ap.add_argument("--workload", default="packing", choices=["packing", "batch"],
-                    help="packing: the 200k-sphere settled packing (headline); batch: independent scenes")
+    ap.add_argument("--workload", default="packing", choices=["packing", "batch", "run"],
+                    help="packing: the 200k-sphere settled packing (headline); batch: independent scenes; "
+                         "run: whole time steps, device loop against host loop")
+    ap.add_argument("--run-steps", type=int, default=100, help="config-2 settle steps timed by --workload run")
+    ap.add_argument("--host-loop", action="store_true", help="settle with the host loop instead of the device loop")
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--ws-new-only", type=int, default=1, help="learned warm start only on new contacts")
     ap.add_argument("--ws-scale", type=float, default=0.5, help="trust factor of the learned warm start")
     args = ap.parse_args()
-    if args.workload == "batch":
+    if args.workload in ("batch", "run"):
         if args.gpus != 1:
             print("bench.py measures one GPU", file=sys.stderr)
             sys.exit(2)
-        bench_batch(args)
+        (bench_batch if args.workload == "batch" else bench_run)(args)
         return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -315,7 +371,8 @@ def main():
         b = {k[2:]: z[k] for k in z.files if k.startswith("b_")}
         c = {k[2:]: z[k] for k in z.files if k.startswith("c_")}
     else:
-        s = settle(args.spheres, args.box, args.settle, args.seed, args.friction, args.shape)
+        s = settle(args.spheres, args.box, args.settle, args.seed, args.friction, args.shape,
+                   device_loop=not args.host_loop)
         b, c = s.bodies(), s.contacts()
         if args.save_snapshot:
             np.savez(args.save_snapshot, **{"b_" + k: v for k, v in b.items()}, **{"c_" + k: v for k, v in c.items()})

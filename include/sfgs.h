@@ -244,6 +244,60 @@ int SFGS_Batch_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TUNE
  * fastest; R and U are restored after each trial */
 int SFGS_Batch_Autotune(sfgs_ctx *ctx, const SFGS_PARAM *par, int trial_iters, SFGS_BATCH_TUNE *best, double *best_ms);
 
+/* ---- device-resident time stepping: DOM_Update_Begin / DOM_Update_End on the
+ * device, so that a run loop keeps bodies and contacts in HBM ----
+ * Rigid spheres and ellipsoids with fixed planes (OBS).  The body state is the
+ * one of SFGS_Bodies_Upload (kind, rot, x0, J^-1, 1/m, velo, velo0) plus the
+ * fields below; the contact list is the one of SFGS_Contacts_Upload, with the
+ * spatial points given here.  Same arithmetic as the host layer of this
+ * library (one shared source, no floating-point contraction), so the device
+ * loop reproduces the host loop. */
+typedef struct {
+  const int *shape;      /* 0 sphere, 1 plane (an OBS body), 2 ellipsoid */
+  const int *surf;       /* surface id, for the material table */
+  const double *rad;     /* radius; the largest semi-axis of an ellipsoid */
+  const double *plane;   /* planes: a point and the unit normal, 6 per body (others ignored) */
+  const double *sca;     /* ellipsoids: semi-axes, 3 per body */
+  const double *rot0;    /* ellipsoids: principal axes in the referential frame, 9 per body */
+  const double *x;       /* current centre, 3 per body */
+  const double *J;       /* inertia tensor, 9 per body */
+  const double *mass;    /* mass */
+  const double *aux;     /* exp[-(h/2)W] J W + h T carried from step begin to step end, 3 per body */
+} SFGS_SHAPES;
+
+/* surface pair material (SURFACE_MATERIAL, mat.h); the pair is unordered */
+typedef struct {
+  int s1, s2;
+  double friction, restitution, cohesion;
+} SFGS_MATERIAL;
+
+/* body shapes, the material table (defmat for pairs not in it) and the
+ * contacts' spatial points (3 per uploaded contact); call after
+ * SFGS_Bodies_Upload and SFGS_Contacts_Upload */
+int SFGS_Dom_Upload(sfgs_ctx *ctx, const SFGS_SHAPES *shapes, int nmat, const SFGS_MATERIAL *mat,
+                    const SFGS_MATERIAL *defmat, const double *point);
+/* DOM_Update_Begin (dom.c:3630) for this body set: the rigid half step
+ * of every body with gravity (BODY_Dynamic_Step_Begin, bod.c:1313-1373); the
+ * update of every contact, which keeps its slot, R and U, or is removed when
+ * the bodies separate (update_contact, dom.c:430-483); detection of new
+ * contacts (overlap_create, dom.c:348-416; sphere pairs goc.c:421-455,
+ * plane-sphere goc.c:308-348), appended in ascending order of the body pair,
+ * convex pairs before plane pairs, with R = U = 0 (mem.c:77,86).  *ncon
+ * receives the new contact count.  SFGS_Locdyn_Update_Begin follows. */
+int SFGS_Dom_Update_Begin(sfgs_ctx *ctx, double step, const double *gravity, int *ncon);
+/* DOM_Update_End (dom.c:3818) after SFGS_Solve: R and U back into the
+ * contact list, the velocity update by the reactions (as SFGS_Body_Update)
+ * and the rigid end of step (BODY_Dynamic_Step_End, bod.c:1425-1492) */
+int SFGS_Dom_Update_End(sfgs_ctx *ctx, double step);
+/* current body state (any pointer may be NULL): rot 9, x 3, velo 6, velo0 6,
+ * aux 3 and the last generalised reaction force 6 per body */
+int SFGS_Dom_Get_Bodies(sfgs_ctx *ctx, double *rot, double *x, double *velo, double *velo0, double *aux,
+                        double *force);
+/* current contact list, natural order (any pointer may be NULL) */
+int SFGS_Dom_Get_Contacts(sfgs_ctx *ctx, int *master, int *slave, double *point, double *mpnt, double *spnt,
+                          double *base, double *gap, double *friction, double *restitution, double *cohesion,
+                          double *R, double *U);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,11 @@ int SOLFEC_Ellipsoid_Lattice_Scene(SOLFEC *sol, int n, double L, double amin, do
 /* SOLFEC_Run (sol.c:665-825): step until time >= t0 + duration */
 int SOLFEC_Run(SOLFEC *sol, GAUSS_SEIDEL *gs, double duration);
 /* single step phases (sol.c:761-773) for drivers that interpose work */
+/* 1: the step functions run DOM_Update_Begin / DOM_Update_End on the device
+ * (SFGS_Dom_Update_Begin / _End) and keep bodies and contacts there; the host
+ * arrays are refreshed when read (SOLFEC_Get_*, dumps) and uploaded again after
+ * a change on the host.  Same results as 0, the host loop, bit for bit. */
+int SOLFEC_Device_Loop(SOLFEC *sol, int on);
 int SOLFEC_Step_Begin(SOLFEC *sol);                    /* DOM_Update_Begin */
 int SOLFEC_Step_Solve(SOLFEC *sol, GAUSS_SEIDEL *gs);  /* LOCDYN_Update_Begin + solve (+ Update_End) */
 int SOLFEC_Step_End(SOLFEC *sol);                      /* DOM_Update_End */

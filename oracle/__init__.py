@@ -70,6 +70,12 @@ def lib():
             "OR_Colour_Order": (None, [_i, _p, _i, _p, _p]),
             "OR_Body_Reaction": (None, [C.POINTER(OR_BODIES), C.POINTER(OR_CONTACTS), _p, _p]),
             "OR_Body_Velocity_Update": (None, [C.POINTER(OR_BODIES), _p, _d, _p]),
+            "OR_Sphere_Contact": (_i, [_p, _d, _p, _d, _p]),
+            "OR_Plane_Contact": (_i, [_p, _p, _d, _p]),
+            "OR_Detect": (C.c_longlong, [_i, _p, _p, _p, _p, _i, _p, _p, C.c_longlong, _p, _p, _p, _p]),
+            "OR_Expmap": (None, [_p, _p]),
+            "OR_Rig_Step_Begin": (None, [_i, _d, _p, _d, _d, _p, _p, _p, _p, _p, _p, _p]),
+            "OR_Rig_Step_End": (None, [_i, _d, _p, _p, _p, _p, _p, _p]),
         }
         for fn, (res, args) in protos.items():
             f = getattr(L, fn)
@@ -227,3 +233,61 @@ def body_update(bod, con, R, step):
     velo = bod.a["velo"].copy()
     lib().OR_Body_Velocity_Update(C.byref(bod.s), _ptr(force), step, _ptr(velo))
     return velo.reshape(nb, 6), force.reshape(nb, 6)
+
+
+def sphere_contact(ca, ra, cb, rb):
+    """(swap, slave point, master point, normal, gap) or None (goc.c:421-455)."""
+    out = np.zeros(11)
+    if not lib().OR_Sphere_Contact(_ptr(_f(ca)), ra, _ptr(_f(cb)), rb, _ptr(out)):
+        return None
+    return bool(out[0]), out[1:4], out[4:7], out[7:10], out[10]
+
+
+def plane_contact(plane, c, r):
+    """(plane point, sphere point, normal, gap) or None (goc.c:308-348)."""
+    out = np.zeros(10)
+    if not lib().OR_Plane_Contact(_ptr(_f(plane)), _ptr(_f(c)), r, _ptr(out)):
+        return None
+    return out[0:3], out[3:6], out[6:9], out[9]
+
+
+def detect(shape, rad, plane, x, master_old=(), slave_old=()):
+    """One contact pass by brute force: dict of master, slave, old (index of the kept
+    contact or -1), spnt, mpnt (spatial), normal, gap."""
+    shape, rad, plane, x = _n(shape), _f(rad), _f(plane), _f(x)
+    mo, so = _n(np.asarray(master_old, np.int32)), _n(np.asarray(slave_old, np.int32))
+    nb, nold = len(shape), len(mo)
+    cap = 4 * nb + nold + 16
+    while True:
+        m, s, o = np.zeros(cap, np.int32), np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        g = np.zeros(10 * cap)
+        n = lib().OR_Detect(nb, _ptr(shape), _ptr(rad), _ptr(plane), _ptr(x), nold,
+                            _ptr(mo if nold else np.zeros(1, np.int32)), _ptr(so if nold else np.zeros(1, np.int32)),
+                            cap, _ptr(m), _ptr(s), _ptr(o), _ptr(g))
+        if n <= cap:
+            break
+        cap = int(n)
+    g = g[:10 * n].reshape(n, 10)
+    return dict(master=m[:n], slave=s[:n], old=o[:n], spnt=g[:, 0:3], mpnt=g[:, 3:6], normal=g[:, 6:9], gap=g[:, 9])
+
+
+def expmap(O):
+    R = np.zeros(9)
+    lib().OR_Expmap(_ptr(_f(O)), _ptr(R))
+    return R
+
+
+def rig_step_begin(kind, h, gravity, mass, minv, J, I, velo, rot, x):
+    """BODY_Dynamic_Step_Begin of one rigid body: returns velo, velo0, rot, x, aux."""
+    velo, rot, x = _f(velo).copy(), _f(rot).copy(), _f(x).copy()
+    velo0, aux = np.zeros(6), np.zeros(3)
+    lib().OR_Rig_Step_Begin(int(kind), h, _ptr(_f(gravity)), mass, minv, _ptr(_f(J)), _ptr(_f(I)), _ptr(velo),
+                            _ptr(velo0), _ptr(rot), _ptr(x), _ptr(aux))
+    return velo, velo0, rot, x, aux
+
+
+def rig_step_end(kind, h, I, force, velo, rot, x, aux):
+    """BODY_Dynamic_Step_End of one rigid body (velocity already updated): velo, rot, x, aux."""
+    velo, rot, x, aux = _f(velo).copy(), _f(rot).copy(), _f(x).copy(), _f(aux).copy()
+    lib().OR_Rig_Step_End(int(kind), h, _ptr(_f(I)), _ptr(_f(force)), _ptr(velo), _ptr(rot), _ptr(x), _ptr(aux))
+    return velo, rot, x, aux

@@ -11,6 +11,9 @@
  *   or_scf.c  -- scf.c:28-174 (cone linearisation / projection)
  *   or_bgs.c  -- bgs.c:913-1069 (serial Gauss-Seidel), mrf.c:35-184 (merit)
  *   or_bod.c  -- bod.c:418-435, 1425-1447 (body velocity update by reactions)
+ *   or_dom.c  -- goc.c:50-59, 308-348, 421-455, gjk.c:560-584 (sphere and plane
+ *                contacts), dom.c:348-483 (contact update and creation, brute
+ *                force), bod.c:1313-1373, 1425-1492 (rigid step), alg.h:1138-1204
  *   or_colour.c -- greedy colouring in priority order (no reference counterpart;
  *                 it fixes the sweep order used by the GPU schedule)
  *
@@ -135,6 +138,23 @@ void OR_Colour_Order(int n, const int *colour, int ncolours, int *order, int *co
 /* ---- or_bod.c ---- */
 void OR_Body_Reaction(const OR_BODIES *bod, const OR_CONTACTS *con, const double *R, double *force);
 void OR_Body_Velocity_Update(const OR_BODIES *bod, const double *force, double step, double *velo);
+
+/* contact detection and the rigid step (or_dom.c).  Sphere contact out[11]:
+ * swap (1 when b is the slave), slave point, master point, normal, gap; plane
+ * contact out[10]: plane point, sphere point, normal, gap.  OR_Detect takes
+ * shape 0 sphere / 1 plane, returns the contact count (writing at most cap):
+ * master, slave, the index of the kept old contact (-1 for new ones) and 10
+ * geometry values per contact (slave point, master point, normal, gap). */
+int OR_Sphere_Contact(const double *ca, double ra, const double *cb, double rb, double *out);
+int OR_Plane_Contact(const double *plane, const double *c, double r, double *out);
+long long OR_Detect(int nb, const int *shape, const double *rad, const double *plane, const double *x,
+                    int nold, const int *master_old, const int *slave_old, long long cap,
+                    int *master, int *slave, int *old_index, double *geo);
+void OR_Expmap(const double *O, double *R);
+void OR_Rig_Step_Begin(int kind, double h, const double *gravity, double mass, double minv, const double *J,
+                       const double *I, double *velo, double *velo0, double *R, double *x, double *A);
+void OR_Rig_Step_End(int kind, double h, const double *I, const double *force, double *velo, double *R, double *x,
+                     double *A);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,7 @@
 #include <time.h>
 #include "solfec.h"
 #include "sol_internal.h"
+#include "sol_geom.h"
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -32,7 +33,7 @@ static int omp_get_num_threads_(void) { return 1; }
 static int omp_get_thread_num_(void) { return 0; }
 #endif
 
-#define GEOMETRIC_EPSILON 1e-6   /* alg.c:24 */
+#define GEOMETRIC_EPSILON SOL_GEOMETRIC_EPSILON
 
 static double now_s(void)
 {
@@ -48,83 +49,14 @@ static void* xrealloc(void *p, size_t bytes)
   return q;
 }
 
-/* ---------------- small algebra ---------------- */
+static int dev_pull(SOLFEC *sol);
 
-static void mm(const double *A, const double *B, double *C)  /* C = A B, column-major */
+/* a change of the host state: bring the host arrays up to date first, and
+ * upload again at the next device step */
+static void dev_edit(SOLFEC *sol)
 {
-  double T[9];
-  for (int c = 0; c < 3; c ++)
-    for (int r = 0; r < 3; r ++)
-      T[r + 3*c] = A[r]*B[3*c] + A[r + 3]*B[3*c + 1] + A[r + 6]*B[3*c + 2];
-  memcpy(C, T, sizeof(T));
-}
-static void mv(const double *A, const double *x, double *y)
-{
-  double t0 = A[0]*x[0] + A[3]*x[1] + A[6]*x[2];
-  double t1 = A[1]*x[0] + A[4]*x[1] + A[7]*x[2];
-  double t2 = A[2]*x[0] + A[5]*x[1] + A[8]*x[2];
-  y[0] = t0; y[1] = t1; y[2] = t2;
-}
-static void mtv(const double *A, const double *x, double *y)
-{
-  double t0 = A[0]*x[0] + A[1]*x[1] + A[2]*x[2];
-  double t1 = A[3]*x[0] + A[4]*x[1] + A[5]*x[2];
-  double t2 = A[6]*x[0] + A[7]*x[1] + A[8]*x[2];
-  y[0] = t0; y[1] = t1; y[2] = t2;
-}
-
-/* rotation exp([O]x) by the Rodrigues formula; series below 1e-4 rad^2 */
-static void expmap(const double *O, double *R)
-{
-  double t2 = O[0]*O[0] + O[1]*O[1] + O[2]*O[2], a, b;
-  if (t2 < 1e-4)
-  {
-    a = 1.0 - t2/6.0*(1.0 - t2/20.0*(1.0 - t2/42.0));        /* sin t / t */
-    b = 0.5*(1.0 - t2/12.0*(1.0 - t2/30.0*(1.0 - t2/56.0)));  /* (1 - cos t) / t^2 */
-  }
-  else
-  {
-    double t = sqrt(t2);
-    a = sin(t)/t;
-    b = (1.0 - cos(t))/t2;
-  }
-  double x = O[0], y = O[1], z = O[2];
-  R[0] = 1.0 - b*(y*y + z*z); R[3] = -a*z + b*x*y;          R[6] = a*y + b*x*z;
-  R[1] = a*z + b*x*y;         R[4] = 1.0 - b*(x*x + z*z);   R[7] = -a*x + b*y*z;
-  R[2] = -a*y + b*x*z;        R[5] = a*x + b*y*z;           R[8] = 1.0 - b*(x*x + y*y);
-}
-
-static void cross(const double *a, const double *b, double *c)
-{
-  double c0 = a[1]*b[2] - a[2]*b[1], c1 = a[2]*b[0] - a[0]*b[2], c2 = a[0]*b[1] - a[1]*b[0];
-  c[0] = c0; c[1] = c1; c[2] = c2;
-}
-
-/* local frame with n as the last column (localbase, dom.c:133-151) */
-static void localbase(const double *n, double *loc)
-{
-  const double e0[3] = {1.0, 0.0, 0.0}, e1[3] = {0.0, 1.0, 0.0};
-  loc[6] = n[0]; loc[7] = n[1]; loc[8] = n[2];
-  cross(e0, n, loc);
-  double len = sqrt(loc[0]*loc[0] + loc[1]*loc[1] + loc[2]*loc[2]);
-  if (len < GEOMETRIC_EPSILON)
-  {
-    cross(e1, n, loc);
-    len = sqrt(loc[0]*loc[0] + loc[1]*loc[1] + loc[2]*loc[2]);
-  }
-  loc[0] /= len; loc[1] /= len; loc[2] /= len;
-  cross(loc, n, loc + 3);
-}
-
-/* lexicographic comparison with the geometric tolerance (alg.h:35-52) */
-static int pntcmp(const double *a, const double *b)
-{
-  for (int k = 0; k < 3; k ++)
-  {
-    if (a[k] + GEOMETRIC_EPSILON < b[k] - GEOMETRIC_EPSILON) return -1;
-    if (a[k] - GEOMETRIC_EPSILON > b[k] + GEOMETRIC_EPSILON) return 1;
-  }
-  return 0;
+  if (sol->devloop) (void)dev_pull(sol);
+  sol->dev_valid = 0;
 }
 
 /* ---------------- domain construction ---------------- */
@@ -190,6 +122,7 @@ void SOLFEC_Gravity(SOLFEC *sol, double gx, double gy, double gz)
 int SOLFEC_Surface_Material(SOLFEC *sol, int s1, int s2, double friction, double restitution, double cohesion)
 {
   if (!sol || friction < 0.0 || restitution < 0.0 || cohesion < 0.0) return SOL_ERR_ARG;
+  dev_edit(sol);
   if (s1 < 0 && s2 < 0)
   {
     sol->defmat.friction = friction;
@@ -214,6 +147,7 @@ static const MATPAIR* find_material(SOLFEC *sol, int s1, int s2)
 
 static int add_body(SOLFEC *sol)
 {
+  dev_edit(sol);
   if (sol->nb == sol->bcap)
   {
     int cap = sol->bcap ? 2*sol->bcap : 64;
@@ -292,7 +226,7 @@ int SOLFEC_Ellipsoid(SOLFEC *sol, const double *centre, const double *semi_axes,
   if (axes) memcpy(R0, axes, 9*sizeof(double));
   for (int col = 0; col < 3; col ++)        /* T = P R0' */
     for (int r = 0; r < 3; r ++) T[r + 3*col] = P[r + 3*r] * R0[col + 3*r];
-  mm(R0, T, J);
+  geom_mm(R0, T, J);
   sol->kind[b] = SOL_RIG;
   sol->shape[b] = 2;
   sol->surf[b] = surface;
@@ -304,7 +238,7 @@ int SOLFEC_Ellipsoid(SOLFEC *sol, const double *centre, const double *semi_axes,
   /* J^-1 = R0 diag^-1 R0' */
   for (int col = 0; col < 3; col ++)
     for (int r = 0; r < 3; r ++) T[r + 3*col] = R0[col + 3*r] / P[r + 3*r];
-  mm(R0, T, sol->jinv + 9*b);
+  geom_mm(R0, T, sol->jinv + 9*b);
   sol->mass[b] = m;
   sol->minv[b] = 1.0/m;
   return b;
@@ -328,6 +262,7 @@ int SOLFEC_Plane(SOLFEC *sol, const double *point, const double *normal, int sur
 int SOLFEC_Set_Velocity(SOLFEC *sol, int body, const double *velo6)
 {
   if (!sol || body < 0 || body >= sol->nb) return SOL_ERR_ARG;
+  dev_edit(sol);
   memcpy(sol->velo + 6*body, velo6, 6*sizeof(double));
   return 0;
 }
@@ -406,43 +341,21 @@ int SOLFEC_Ellipsoid_Lattice_Scene(SOLFEC *sol, int n, double L, double amin, do
 /* bod.c:1313-1373 for RIG, no point forces: gravity only */
 static void body_step_begin(SOLFEC *sol, int b)
 {
-  double h = sol->step, half = 0.5*h;
-  double *W = sol->velo + 6*b, *v = W + 3, *R = sol->rot + 9*b, *x = sol->x + 3*b;
-  double *J = sol->J + 9*b, *I = sol->jinv + 9*b, *A = sol->aux + 3*b;
-  memcpy(sol->velo0 + 6*b, W, 6*sizeof(double));
-  if (sol->kind[b] == SOL_OBS) return;   /* fixed obstacles */
-  double O[3] = {half*W[0], half*W[1], half*W[2]}, DR[9], JW[3], DJW[3], W05[3], T[3] = {0, 0, 0}, f[3];
-  expmap(O, DR);
-  mm(R, DR, R);                                   /* R(t+h/2) = R(t) exp[(h/2) W(t)] */
-  for (int k = 0; k < 3; k ++) x[k] += half*v[k]; /* x(t+h/2) = x(t) + (h/2) v(t) */
-  for (int k = 0; k < 3; k ++) f[k] = sol->mass[b]*sol->gravity[k];   /* no torque loads: T = 0 */
-  mv(J, W, JW);
-  mtv(DR, JW, DJW);                               /* exp[-(h/2)W] J W */
-  for (int k = 0; k < 3; k ++) A[k] = DJW[k] + h*T[k];
-  for (int k = 0; k < 3; k ++) O[k] = half*T[k] + DJW[k];
-  mv(I, O, W05);                                  /* W05 = I [exp[-(h/2)W] J W + (h/2) T] */
-  double JW05[3], c[3], dW[3];
-  mv(J, W05, JW05);
-  cross(W05, JW05, c);
-  for (int k = 0; k < 3; k ++) T[k] -= c[k];      /* T - W05 x J W05 */
-  mv(I, T, dW);
-  for (int k = 0; k < 3; k ++) { W[k] += h*dW[k]; v[k] += h*sol->minv[b]*f[k]; }   /* u += h iM f */
+  if (sol->kind[b] == SOL_OBS)   /* fixed obstacles */
+  {
+    memcpy(sol->velo0 + 6*b, sol->velo + 6*b, 6*sizeof(double));
+    return;
+  }
+  geom_rig_step_begin(sol->step, sol->gravity, sol->mass[b], sol->minv[b], sol->J + 9*b, sol->jinv + 9*b,
+                      sol->velo + 6*b, sol->velo0 + 6*b, sol->rot + 9*b, sol->x + 3*b, sol->aux + 3*b);
 }
 
 /* bod.c:1425-1492 for RIG: velocity already updated by the reactions (GPU) */
 static void body_step_end(SOLFEC *sol, int b)
 {
   if (sol->kind[b] == SOL_OBS) return;
-  double h = sol->step, half = 0.5*h;
-  double *W = sol->velo + 6*b, *v = W + 3, *R = sol->rot + 9*b, *x = sol->x + 3*b;
-  double *I = sol->jinv + 9*b, *A = sol->aux + 3*b, *r = sol->force + 6*b;
-  for (int k = 0; k < 3; k ++) x[k] += half*v[k];
-  double O[3] = {half*W[0], half*W[1], half*W[2]}, DR[9], t[3];
-  expmap(O, DR);
-  mm(R, DR, R);
-  for (int k = 0; k < 3; k ++) A[k] += h*r[k];
-  mtv(DR, A, t);
-  mv(I, t, W);   /* W(t+h) = I exp[-(h/2)W1][exp[-(h/2)W(t)] J W(t) + h T] */
+  geom_rig_step_end(sol->step, sol->jinv + 9*b, sol->force + 6*b, sol->velo + 6*b, sol->rot + 9*b, sol->x + 3*b,
+                    sol->aux + 3*b);
 }
 
 /* ---------------- contacts ---------------- */
@@ -470,194 +383,39 @@ static void ensure_contacts(SOLFEC *sol, int n)
 /* referential image of a spatial point (BODY_Ref_Point): X = X0 + R'(x - xc) */
 static void ref_point(SOLFEC *sol, int b, const double *p, double *X)
 {
-  double d[3] = {p[0] - sol->x[3*b], p[1] - sol->x[3*b + 1], p[2] - sol->x[3*b + 2]}, t[3];
-  mtv(sol->rot + 9*b, d, t);
-  for (int k = 0; k < 3; k ++) X[k] = sol->x0[3*b + k] + t[k];
+  geom_ref_point(sol->rot + 9*b, sol->x + 3*b, sol->x0 + 3*b, p, X);
 }
 
-/* geometry of a sphere pair given as (slave a, master b): returns 1 when in
- * contact (gap < GEOMETRIC_EPSILON, goc.c:658-677 / gjk.c:560-584) */
-static int sphere_pair(SOLFEC *sol, int a, int b, double *spnt, double *mpnt, double *normal, double *gap)
-{
-  const double *ca = sol->x + 3*a, *cb = sol->x + 3*b;
-  double ra = sol->rad[a], rb = sol->rad[b];
-  double d[3] = {ca[0] - cb[0], ca[1] - cb[1], ca[2] - cb[2]};
-  double dl = sqrt(d[0]*d[0] + d[1]*d[1] + d[2]*d[2]), g;
-  if (dl == 0.0) return 0;
-  for (int k = 0; k < 3; k ++) { spnt[k] = ca[k] - ra/dl*d[k]; mpnt[k] = cb[k] + rb/dl*d[k]; }
-  if (dl > ra + rb) g = dl - (ra + rb);
-  else
-  {
-    for (int k = 0; k < 3; k ++) { spnt[k] = 0.5*(spnt[k] + mpnt[k]); mpnt[k] = spnt[k]; }
-    g = 0.0;
-  }
-  if (g >= GEOMETRIC_EPSILON) return 0;
-  double nl = 0.0;
-  for (int k = 0; k < 3; k ++) { normal[k] = spnt[k] - ca[k]; nl += normal[k]*normal[k]; }
-  nl = sqrt(nl);
-  for (int k = 0; k < 3; k ++) normal[k] /= nl;
-  double dn = (cb[0] - ca[0])*normal[0] + (cb[1] - ca[1])*normal[1] + (cb[2] - ca[2])*normal[2];
-  double e = ra + rb;
-  *gap = e > dn ? dn - e : 0.0;   /* sphere_sphere_gap, goc.c:50-59 */
-  return 1;
-}
-
-/* principal frame Q (current: rot R0) and semi-axes S of a convex body; a
- * sphere is an ellipsoid with equal semi-axes in any frame */
 static void convex_frame(SOLFEC *sol, int b, double *Q, double *S)
 {
-  if (sol->shape[b] == 2)
-  {
-    mm(sol->rot + 9*b, sol->rot0 + 9*b, Q);
-    memcpy(S, sol->sca + 3*b, 3*sizeof(double));
-  }
-  else
-  {
-    static const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    memcpy(Q, I, sizeof(I));
-    S[0] = S[1] = S[2] = sol->rad[b];
-  }
+  geom_frame(sol->shape[b], sol->rot + 9*b, sol->rot0 + 9*b, sol->sca + 3*b, sol->rad[b], Q, S);
 }
 
-/* support function h(d) = |S Q' d| of the centred ellipsoid and the offset of
- * its support point, A d / h with A = Q S^2 Q' */
-static double ell_support(const double *Q, const double *S, const double *d, double *off)
-{
-  double t[3], u[3], v[3];
-  mtv(Q, d, t);
-  for (int k = 0; k < 3; k ++) u[k] = S[k]*t[k];
-  double h = sqrt(u[0]*u[0] + u[1]*u[1] + u[2]*u[2]);
-  for (int k = 0; k < 3; k ++) v[k] = S[k]*u[k]/h;
-  mv(Q, v, off);
-  return h;
-}
-
-/* pair of convex bodies with at least one ellipsoid, as (slave a, master b).
- * The normal n, from a towards b, maximises the separation along n,
- *   gap(n) = n.(c_b - c_a) - h_a(n) - h_b(n),
- * whose largest value is the distance of apart bodies and minus the
- * penetration depth of overlapping ones; Newton's method on the tangent plane
- * of the unit sphere finds it from n = (c_b - c_a)/|c_b - c_a|.  The contact
- * points are the support points c_a + h_a'(n) and c_b - h_b'(n), merged at
- * their midpoint when the bodies overlap, as for spheres.  The reference
- * (goc.c:457-492, gjk.c:651-836) takes the normal at the GJK closest points
- * instead; both give the common normal of touching bodies. */
-static int ellip_pair(SOLFEC *sol, int a, int b, double *spnt, double *mpnt, double *normal, double *gap)
-{
-  const double *ca = sol->x + 3*a, *cb = sol->x + 3*b;
-  double Qa[9], Sa[3], Qb[9], Sb[3], Aa[9], Ab[9], T[9];
-  convex_frame(sol, a, Qa, Sa);
-  convex_frame(sol, b, Qb, Sb);
-  for (int c = 0; c < 3; c ++)
-    for (int r = 0; r < 3; r ++) T[r + 3*c] = Sa[r]*Sa[r]*Qa[c + 3*r];
-  mm(Qa, T, Aa);
-  for (int c = 0; c < 3; c ++)
-    for (int r = 0; r < 3; r ++) T[r + 3*c] = Sb[r]*Sb[r]*Qb[c + 3*r];
-  mm(Qb, T, Ab);
-  double d[3] = {cb[0] - ca[0], cb[1] - ca[1], cb[2] - ca[2]};
-  double dl = sqrt(d[0]*d[0] + d[1]*d[1] + d[2]*d[2]);
-  if (dl == 0.0) return 0;
-  double n[3] = {d[0]/dl, d[1]/dl, d[2]/dl}, oa[3], ob[3], ha = 0.0, hb = 0.0, g = 0.0;
-  for (int it = 0; it < 50; it ++)
-  {
-    ha = ell_support(Qa, Sa, n, oa);
-    hb = ell_support(Qb, Sb, n, ob);
-    double grad[3], loc[9], H[9];
-    for (int k = 0; k < 3; k ++) grad[k] = d[k] - oa[k] - ob[k];
-    g = grad[0]*n[0] + grad[1]*n[1] + grad[2]*n[2];   /* = gap(n) */
-    /* Euclidean Hessian: -(A_a - oa oa')/h_a - (A_b - ob ob')/h_b */
-    for (int c = 0; c < 3; c ++)
-      for (int r = 0; r < 3; r ++)
-        H[r + 3*c] = -(Aa[r + 3*c] - oa[r]*oa[c])/ha - (Ab[r + 3*c] - ob[r]*ob[c])/hb;
-    localbase(n, loc);
-    const double *e1 = loc, *e2 = loc + 3;
-    double g1 = e1[0]*grad[0] + e1[1]*grad[1] + e1[2]*grad[2];
-    double g2 = e2[0]*grad[0] + e2[1]*grad[1] + e2[2]*grad[2];
-    if (sqrt(g1*g1 + g2*g2) <= 1e-14*(dl + ha + hb)) break;
-    double He1[3], He2[3];
-    mv(H, e1, He1);
-    mv(H, e2, He2);
-    /* Riemannian Hessian on the tangent plane: e_i' H e_j - (grad.n) delta_ij */
-    double h11 = e1[0]*He1[0] + e1[1]*He1[1] + e1[2]*He1[2] - g;
-    double h12 = e1[0]*He2[0] + e1[1]*He2[1] + e1[2]*He2[2];
-    double h22 = e2[0]*He2[0] + e2[1]*He2[1] + e2[2]*He2[2] - g;
-    double det = h11*h22 - h12*h12, s1, s2;
-    if (h11 < 0.0 && det > 0.0) { s1 = -(h22*g1 - h12*g2)/det; s2 = -(h11*g2 - h12*g1)/det; }
-    else { double al = 1.0/(dl + ha + hb); s1 = al*g1; s2 = al*g2; }   /* ascent step away from a maximum */
-    double sl = sqrt(s1*s1 + s2*s2);
-    if (sl > 0.5) { s1 *= 0.5/sl; s2 *= 0.5/sl; }
-    double nl = 0.0;
-    for (int k = 0; k < 3; k ++) { n[k] += s1*e1[k] + s2*e2[k]; nl += n[k]*n[k]; }
-    nl = sqrt(nl);
-    for (int k = 0; k < 3; k ++) n[k] /= nl;
-  }
-  if (g >= GEOMETRIC_EPSILON) return 0;
-  ha = ell_support(Qa, Sa, n, oa);
-  hb = ell_support(Qb, Sb, n, ob);
-  g = n[0]*d[0] + n[1]*d[1] + n[2]*d[2] - ha - hb;
-  for (int k = 0; k < 3; k ++) { spnt[k] = ca[k] + oa[k]; mpnt[k] = cb[k] - ob[k]; normal[k] = n[k]; }
-  if (g < 0.0)
-    for (int k = 0; k < 3; k ++) { spnt[k] = 0.5*(spnt[k] + mpnt[k]); mpnt[k] = spnt[k]; }
-  *gap = g < 0.0 ? g : 0.0;
-  return 1;
-}
-
-/* plane (slave) and ellipsoid (master): the ellipsoid's support point against
- * the plane normal (goc.c:350-404 with the plane as the convex) */
-static int plane_ellip(SOLFEC *sol, int pl, int e, double *spnt, double *mpnt, double *normal, double *gap)
-{
-  const double *P = sol->plane + 6*pl, *n = P + 3, *c = sol->x + 3*e;
-  double Q[9], S[3], o[3];
-  convex_frame(sol, e, Q, S);
-  double h = ell_support(Q, S, n, o);
-  double dist = (c[0] - P[0])*n[0] + (c[1] - P[1])*n[1] + (c[2] - P[2])*n[2], g = dist - h;
-  if (g >= GEOMETRIC_EPSILON) return 0;
-  for (int k = 0; k < 3; k ++)
-  {
-    normal[k] = n[k];
-    mpnt[k] = c[k] - o[k];
-    spnt[k] = mpnt[k] - g*n[k];
-  }
-  *gap = g < 0.0 ? g : 0.0;
-  return 1;
-}
-
-/* dispatch by shape: the sphere routines for sphere pairs, the ellipsoid ones otherwise */
+/* convex pair as (slave a, master b): the sphere routine for sphere pairs, the
+ * ellipsoid one otherwise */
 static int convex_pair(SOLFEC *sol, int a, int b, double *spnt, double *mpnt, double *normal, double *gap)
 {
-  if (sol->shape[a] == 0 && sol->shape[b] == 0) return sphere_pair(sol, a, b, spnt, mpnt, normal, gap);
-  return ellip_pair(sol, a, b, spnt, mpnt, normal, gap);
+  if (sol->shape[a] == 0 && sol->shape[b] == 0)
+    return geom_sphere_pair(sol->x + 3*a, sol->rad[a], sol->x + 3*b, sol->rad[b], spnt, mpnt, normal, gap);
+  double Qa[9], Sa[3], Qb[9], Sb[3];
+  convex_frame(sol, a, Qa, Sa);
+  convex_frame(sol, b, Qb, Sb);
+  return geom_ellip_pair(sol->x + 3*a, Qa, Sa, sol->x + 3*b, Qb, Sb, spnt, mpnt, normal, gap);
 }
 
-/* plane (slave) and sphere (master): goc.c:308-348 with the plane as the convex */
-static int plane_sphere(SOLFEC *sol, int pl, int s, double *spnt, double *mpnt, double *normal, double *gap)
+/* plane (slave) and sphere or ellipsoid (master) */
+static int plane_convex(SOLFEC *sol, int pl, int s, double *spnt, double *mpnt, double *normal, double *gap)
 {
-  const double *P = sol->plane + 6*pl, *n = P + 3, *c = sol->x + 3*s;
-  double r = sol->rad[s];
-  double dist = (c[0] - P[0])*n[0] + (c[1] - P[1])*n[1] + (c[2] - P[2])*n[2];
-  if (dist >= r + GEOMETRIC_EPSILON) return 0;
-  for (int k = 0; k < 3; k ++)
+  if (sol->shape[s] == 2)
   {
-    normal[k] = n[k];
-    spnt[k] = c[k] - dist*n[k];
-    mpnt[k] = c[k] - r*n[k];
+    double Q[9], S[3];
+    convex_frame(sol, s, Q, S);
+    return geom_plane_ellip(sol->plane + 6*pl, sol->x + 3*s, Q, S, spnt, mpnt, normal, gap);
   }
-  *gap = dist - r < 0.0 ? dist - r : 0.0;
-  return 1;
+  return geom_plane_sphere(sol->plane + 6*pl, sol->x + 3*s, sol->rad[s], spnt, mpnt, normal, gap);
 }
 
 /* open-addressing map from body pair to contact slot */
-static unsigned long long pair_key(int a, int b)
-{
-  unsigned lo = (unsigned)(a < b ? a : b), hi = (unsigned)(a < b ? b : a);
-  return ((unsigned long long)lo << 32) | hi;
-}
-static unsigned long long mix64(unsigned long long z)
-{
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-  return z ^ (z >> 31);
-}
 static void map_build(SOLFEC *sol)
 {
   size_t cap = 16;
@@ -672,8 +430,8 @@ static void map_build(SOLFEC *sol)
   for (size_t k = 0; k <= sol->pair_mask; k ++) sol->pair_keys[k] = ~0ULL;
   for (int i = 0; i < sol->nc; i ++)
   {
-    unsigned long long key = pair_key(sol->c_master[i], sol->c_slave[i]);
-    size_t h = mix64(key) & sol->pair_mask;
+    unsigned long long key = geom_pair_key(sol->c_master[i], sol->c_slave[i]);
+    size_t h = geom_mix64(key) & sol->pair_mask;
     while (sol->pair_keys[h] != ~0ULL) h = (h + 1) & sol->pair_mask;
     sol->pair_keys[h] = key;
     sol->pair_vals[h] = i;
@@ -681,8 +439,8 @@ static void map_build(SOLFEC *sol)
 }
 static int map_find(SOLFEC *sol, int a, int b)
 {
-  unsigned long long key = pair_key(a, b);
-  size_t h = mix64(key) & sol->pair_mask;
+  unsigned long long key = geom_pair_key(a, b);
+  size_t h = geom_mix64(key) & sol->pair_mask;
   while (sol->pair_keys[h] != ~0ULL)
   {
     if (sol->pair_keys[h] == key) return sol->pair_vals[h];
@@ -699,7 +457,7 @@ static void set_contact(SOLFEC *sol, int i, int master, int slave, const double 
   memcpy(sol->c_point + 3*i, mp, 3*sizeof(double));
   ref_point(sol, master, mp, sol->c_mpnt + 3*i);
   ref_point(sol, slave, sp, sol->c_spnt + 3*i);
-  localbase(normal, sol->c_base + 9*i);
+  geom_localbase(normal, sol->c_base + 9*i);
   sol->c_gap[i] = gap;
   const MATPAIR *m = find_material(sol, sol->surf[master], sol->surf[slave]);
   sol->c_fri[i] = m->friction;
@@ -708,7 +466,10 @@ static void set_contact(SOLFEC *sol, int i, int master, int slave, const double 
 }
 
 /* update existing contacts (keep slot, R, U; remove separated), then detect
- * new ones with a uniform grid over sphere centres */
+ * new ones with a uniform grid over sphere centres.  New convex pairs are
+ * appended in ascending order of (a, b), a < b the body indices, then new
+ * plane contacts in ascending order of (plane, body): an order fixed by the
+ * pairs alone, which the device loop reproduces. */
 static int update_and_detect(SOLFEC *sol)
 {
   double t0 = now_s();
@@ -727,9 +488,7 @@ static int update_and_detect(SOLFEC *sol)
   {
     int m = sol->c_master[i], s = sol->c_slave[i];
     double *G = sol->upd_geo + 10*(size_t)i;
-    if (sol->shape[s] == 1)
-      sol->upd_ok[i] = sol->shape[m] == 2 ? plane_ellip(sol, s, m, G, G + 3, G + 6, G + 9)
-                                          : plane_sphere(sol, s, m, G, G + 3, G + 6, G + 9);
+    if (sol->shape[s] == 1) sol->upd_ok[i] = plane_convex(sol, s, m, G, G + 3, G + 6, G + 9);
     else sol->upd_ok[i] = convex_pair(sol, s, m, G, G + 3, G + 6, G + 9);
   }
   /* compaction keeps the insertion order; upd_ok receives the old index of each kept slot */
@@ -802,7 +561,7 @@ static int update_and_detect(SOLFEC *sol)
     {
       if (sol->shape[b] == 1) continue;
       int ci[3];
-      for (int k = 0; k < 3; k ++) ci[k] = (int)floor((sol->x[3*b + k] - lo[k])/cell);
+      for (int k = 0; k < 3; k ++) ci[k] = geom_cell_coord((sol->x[3*b + k] - lo[k])/cell, dim[k]);
       long long c = ((long long)ci[2]*dim[1] + ci[1])*dim[0] + ci[0];
       sol->grid_next[b] = sol->grid_head[c];
       sol->grid_head[c] = b;
@@ -830,8 +589,8 @@ static int update_and_detect(SOLFEC *sol)
       for (int a = a0; a < a1; a ++)
       {
         if (sol->shape[a] == 1) continue;
-        int ci[3];
-        for (int k = 0; k < 3; k ++) ci[k] = (int)floor((sol->x[3*a + k] - lo[k])/cell);
+        int ci[3], first = sol->cand_n[t];
+        for (int k = 0; k < 3; k ++) ci[k] = geom_cell_coord((sol->x[3*a + k] - lo[k])/cell, dim[k]);
         for (int dz = -1; dz <= 1; dz ++)
           for (int dy = -1; dy <= 1; dy ++)
             for (int dx = -1; dx <= 1; dx ++)
@@ -854,6 +613,14 @@ static int update_and_detect(SOLFEC *sol)
                 sol->cand[t][sol->cand_n[t] ++] = b;
               }
             }
+        /* the pairs of a in ascending order of b, whatever the cell layout */
+        int *cb = sol->cand[t];
+        for (int q = first + 3; q < sol->cand_n[t]; q += 2)
+        {
+          int v = cb[q], r = q;
+          while (r > first + 1 && cb[r - 2] > v) { cb[r] = cb[r - 2]; r -= 2; }
+          cb[r] = v;
+        }
       }
     }
     for (int t = 0; t < nthr; t ++)
@@ -863,7 +630,7 @@ static int update_and_detect(SOLFEC *sol)
         /* orientation rule of detect_sphere_sphere (goc.c:421-455): the body with the
          * lexicographically larger centre is the master */
         int slave = a, master = b;
-        if (pntcmp(sol->x + 3*a, sol->x + 3*b) > 0) { slave = b; master = a; }
+        if (geom_pntcmp(sol->x + 3*a, sol->x + 3*b) > 0) { slave = b; master = a; }
         if (!convex_pair(sol, slave, master, sp, mp, nrm, &g)) continue;
         ensure_contacts(sol, sol->nc + 1);
         int i = sol->nc ++;
@@ -883,8 +650,7 @@ static int update_and_detect(SOLFEC *sol)
       double dist = (c[0] - P[0])*P[3] + (c[1] - P[1])*P[4] + (c[2] - P[2])*P[5];
       if (dist >= sol->rad[s] + GEOMETRIC_EPSILON) continue;
       if (map_find(sol, p, s) >= 0) continue;
-      if (sol->shape[s] == 2 ? !plane_ellip(sol, p, s, sp, mp, nrm, &g) : !plane_sphere(sol, p, s, sp, mp, nrm, &g))
-        continue;
+      if (!plane_convex(sol, p, s, sp, mp, nrm, &g)) continue;
       ensure_contacts(sol, sol->nc + 1);
       int i = sol->nc ++;
       set_contact(sol, i, s, p, mp, sp, nrm, g);
@@ -908,10 +674,146 @@ sfgs_ctx* SOLFEC_Gpu(SOLFEC *sol)
   return sol->gpu;
 }
 
+/* ---------------- device loop ---------------- */
+
+/* bodies, shapes, materials and contacts to the device */
+static int dev_push(SOLFEC *sol, sfgs_ctx *g)
+{
+  int rc = SFGS_Bodies_Upload(g, sol->nb, sol->kind, sol->rot, sol->x0, sol->jinv, sol->minv, sol->velo, sol->velo0);
+  if (!rc) rc = SFGS_Contacts_Upload(g, sol->nc, sol->c_master, sol->c_slave, sol->c_mpnt, sol->c_spnt, sol->c_base,
+                                     sol->c_gap, sol->c_fri, sol->c_rest, sol->c_coh, sol->c_R, sol->c_U);
+  if (!rc)
+  {
+    SFGS_SHAPES sh = {sol->shape, sol->surf, sol->rad, sol->plane, sol->sca, sol->rot0, sol->x, sol->J, sol->mass,
+                      sol->aux};
+    SFGS_MATERIAL *mat = xrealloc(NULL, sizeof(SFGS_MATERIAL)*(size_t)(sol->nmat + 1)), def;
+    for (int k = 0; k < sol->nmat; k ++)
+    {
+      mat[k].s1 = sol->mat[k].s1; mat[k].s2 = sol->mat[k].s2;
+      mat[k].friction = sol->mat[k].friction; mat[k].restitution = sol->mat[k].restitution;
+      mat[k].cohesion = sol->mat[k].cohesion;
+    }
+    def.s1 = def.s2 = -1;
+    def.friction = sol->defmat.friction; def.restitution = sol->defmat.restitution; def.cohesion = sol->defmat.cohesion;
+    rc = SFGS_Dom_Upload(g, &sh, sol->nmat, mat, &def, sol->c_point);
+    free(mat);
+  }
+  if (rc)
+  {
+    snprintf(sol->err, sizeof(sol->err), "device loop upload failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));
+    return SOL_ERR_GPU;
+  }
+  sol->dev_valid = 1;
+  sol->host_stale = 0;
+  return 0;
+}
+
+/* the device's bodies and contacts into the host arrays, when they are behind */
+static int dev_pull(SOLFEC *sol)
+{
+  if (!sol->devloop || !sol->host_stale || !sol->gpu) return 0;
+  sfgs_ctx *g = sol->gpu;
+  int n = 0;
+  int rc = SFGS_Dom_Get_Bodies(g, sol->rot, sol->x, sol->velo, sol->velo0, sol->aux, sol->force);
+  if (!rc) rc = SFGS_Get_Sizes(g, &n, NULL, NULL);
+  if (!rc)
+  {
+    ensure_contacts(sol, n > 0 ? n : 1);
+    sol->nc = n;
+    rc = SFGS_Dom_Get_Contacts(g, sol->c_master, sol->c_slave, sol->c_point, sol->c_mpnt, sol->c_spnt, sol->c_base,
+                               sol->c_gap, sol->c_fri, sol->c_rest, sol->c_coh, sol->c_R, sol->c_U);
+  }
+  if (rc)
+  {
+    snprintf(sol->err, sizeof(sol->err), "device loop download failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));
+    return SOL_ERR_GPU;
+  }
+  sol->host_stale = 0;
+  return 0;
+}
+
+int SOLFEC_Device_Loop(SOLFEC *sol, int on)
+{
+  if (!sol) return SOL_ERR_ARG;
+  if (sol->devloop && !on)
+  {
+    int rc = dev_pull(sol);
+    if (rc) return rc;
+  }
+  sol->devloop = on ? 1 : 0;
+  sol->dev_valid = 0;
+  sol->host_stale = 0;
+  return 0;
+}
+
+static int dev_step_begin(SOLFEC *sol)
+{
+  sfgs_ctx *g = SOLFEC_Gpu(sol);
+  if (!g) return SOL_ERR_GPU;
+  double t0 = now_s();
+  if (!sol->dev_valid)
+  {
+    int rc = dev_push(sol, g);
+    if (rc) return rc;
+  }
+  int nc = 0, rc = SFGS_Dom_Update_Begin(g, sol->step, sol->gravity, &nc);
+  if (rc)
+  {
+    snprintf(sol->err, sizeof(sol->err), "device step begin failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));
+    return SOL_ERR_GPU;
+  }
+  sol->nc = nc;
+  sol->host_stale = 1;
+  sol->t_condet += now_s() - t0;   /* the half step, contact update and detection in one call */
+  return 0;
+}
+
+static int dev_step_solve(SOLFEC *sol, GAUSS_SEIDEL *gs)
+{
+  sfgs_ctx *g = SOLFEC_Gpu(sol);
+  if (!g) return SOL_ERR_GPU;
+  double t0 = now_s();
+  int rc = SFGS_Locdyn_Update_Begin(g, sol->step, sol->dynamic);
+  double t1 = now_s();
+  sol->t_locdyn += t1 - t0;
+  if (!rc && sol->capture)
+  {
+    sol->captured = sol->nc <= sol->capture_cap ? sol->nc : 0;
+    if (sol->captured) rc = SFGS_Features(g, sol->capture);
+  }
+  if (!rc && gs->warmstart) rc = SFGS_Warm_Start(g);
+  if (rc)
+  {
+    snprintf(sol->err, sizeof(sol->err), "LOCDYN on GPU failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));
+    return SOL_ERR_GPU;
+  }
+  int err = GAUSS_SEIDEL_Solve_Device(gs, g, sol->err, sizeof(sol->err));
+  sol->t_consol += now_s() - t1;
+  return err;
+}
+
+static int dev_step_end(SOLFEC *sol)
+{
+  sfgs_ctx *g = SOLFEC_Gpu(sol);
+  if (!g) return SOL_ERR_GPU;
+  double t0 = now_s();
+  int rc = SFGS_Dom_Update_End(g, sol->step);
+  if (rc)
+  {
+    snprintf(sol->err, sizeof(sol->err), "device step end failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));
+    return SOL_ERR_GPU;
+  }
+  sol->time += sol->step;
+  sol->host_stale = 1;
+  sol->t_timint += now_s() - t0;
+  return 0;
+}
+
 /* ---------------- time stepping ---------------- */
 
 int SOLFEC_Step_Begin(SOLFEC *sol)
 {
+  if (sol->devloop) return dev_step_begin(sol);
   double t0 = now_s();
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < sol->nb; b ++) body_step_begin(sol, b);
@@ -922,6 +824,7 @@ int SOLFEC_Step_Begin(SOLFEC *sol)
 
 int SOLFEC_Step_End(SOLFEC *sol)
 {
+  if (sol->devloop) return dev_step_end(sol);
   double t0 = now_s();
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < sol->nb; b ++) body_step_end(sol, b);
@@ -957,6 +860,8 @@ double SOLFEC_Timing(SOLFEC *sol, const char *label)
 int SOLFEC_Get_Bodies(SOLFEC *sol, int *kind, double *rot, double *x, double *x0, double *jinv,
                       double *minv, double *velo, double *velo0, double *radius)
 {
+  int rc = dev_pull(sol);
+  if (rc) return rc;
   int nb = sol->nb;
   if (kind) memcpy(kind, sol->kind, sizeof(int)*(size_t)nb);
   if (rot) memcpy(rot, sol->rot, sizeof(double)*9*(size_t)nb);
@@ -974,6 +879,8 @@ int SOLFEC_Get_Contacts(SOLFEC *sol, int *master, int *slave, double *mpnt, doub
                         double *gap, double *friction, double *restitution, double *cohesion,
                         double *R, double *U)
 {
+  int rc = dev_pull(sol);
+  if (rc) return rc;
   size_t n = (size_t)sol->nc;
   if (master) memcpy(master, sol->c_master, sizeof(int)*n);
   if (slave) memcpy(slave, sol->c_slave, sizeof(int)*n);
@@ -991,6 +898,7 @@ int SOLFEC_Get_Contacts(SOLFEC *sol, int *master, int *slave, double *mpnt, doub
 
 int SOLFEC_Set_RU(SOLFEC *sol, const double *R, const double *U)
 {
+  dev_edit(sol);
   size_t n = (size_t)sol->nc;
   if (R) memcpy(sol->c_R, R, sizeof(double)*3*n);
   if (U) memcpy(sol->c_U, U, sizeof(double)*3*n);
@@ -1000,6 +908,7 @@ int SOLFEC_Set_RU(SOLFEC *sol, const double *R, const double *U)
 /* upload, assemble, solve and update body velocities on the GPU */
 int SOLFEC_Step_Solve(SOLFEC *sol, GAUSS_SEIDEL *gs)
 {
+  if (sol->devloop) return dev_step_solve(sol, gs);
   sfgs_ctx *g = SOLFEC_Gpu(sol);
   if (!g) return SOL_ERR_GPU;
   double t0 = now_s();
@@ -1169,7 +1078,8 @@ int SOLFEC_Locdyn_Dump(SOLFEC *sol, const char *path)
 {
   int *rowptr = NULL, *col = NULL;
   double *W = NULL, *Wd = NULL, *B = NULL, *V = NULL;
-  int rc = locdyn_download(sol, &rowptr, &col, &W, &Wd, &B, &V);
+  int rc = dev_pull(sol);
+  if (!rc) rc = locdyn_download(sol, &rowptr, &col, &W, &Wd, &B, &V);
   if (!rc) rc = SOLFEC_Locdyn_Write(path, sol->nc, sol->c_point, sol->c_master, sol->c_slave, sol->c_base, Wd, B, V,
                                     sol->c_U, sol->c_R, rowptr, col, W);
   free(rowptr); free(col); free(W); free(Wd); free(B); free(V);
@@ -1180,7 +1090,8 @@ int SOLFEC_W_MatrixMarket(SOLFEC *sol, const char *path)
 {
   int *rowptr = NULL, *col = NULL;
   double *W = NULL, *Wd = NULL, *B = NULL, *V = NULL;
-  int rc = locdyn_download(sol, &rowptr, &col, &W, &Wd, &B, &V);
+  int rc = dev_pull(sol);
+  if (!rc) rc = locdyn_download(sol, &rowptr, &col, &W, &Wd, &B, &V);
   if (!rc) rc = SOLFEC_W_MatrixMarket_Write(path, sol->nc, Wd, rowptr, col, W);
   free(rowptr); free(col); free(W); free(Wd); free(B); free(V);
   return rc;

@@ -52,6 +52,11 @@ struct solfec {
   float *capture;
   int capture_cap, captured;
 
+  /* device loop (SOLFEC_Device_Loop): the step functions keep bodies and
+   * contacts on the device; dev_valid: the device holds the current state;
+   * host_stale: the host arrays above are behind it */
+  int devloop, dev_valid, host_stale;
+
   /* phase timers (sol.c:392-439 labels), seconds */
   double t_timint, t_conupd, t_condet, t_locdyn, t_consol;
 };

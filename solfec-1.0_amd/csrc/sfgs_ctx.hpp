@@ -38,6 +38,7 @@ struct RowData;  // forward
 
 struct sfgs_ctx;
 void sfgs_mlp_release(sfgs_ctx *ctx);
+void sfgs_dom_release(sfgs_ctx *ctx);
 int sfgs_mlp_apply(sfgs_ctx *ctx, const float *feat, int n, double *R);
 
 struct sfgs_ctx {
@@ -117,4 +118,22 @@ struct sfgs_ctx {
   sfgs::DBuf<int> babort;
   sfgs::DBuf<int> u_clist, u_sscene, u_sbound, u_scene, u_active;   // per-colour batch schedule
   sfgs::DBuf<double> u_spart;
+
+  // ---- device-resident time stepping (sfgs_dom.hip): body shapes and state
+  // beyond the assembly's, materials, contact points and detection scratch
+  int dom = 0;                            // shapes uploaded for the current bodies
+  sfgs::DBuf<int> shape, surf, planes;    // planes: plane body indices, ascending
+  sfgs::DBuf<double> rad, plane, sca, rot0, bx, J, mass, aux, force, point;
+  int nplanes = 0;
+  double rmax = 0.0;                      // largest bounding radius of a convex body
+  int nmat = 0;
+  sfgs::DBuf<int> mat_s;                  // surface pairs (smaller id first), 2 per material
+  sfgs::DBuf<double> mat_v;               // friction, restitution, cohesion, 3 per material
+  double defmat[3] = {0.0, 0.0, 0.0};
+  sfgs::DBuf<double> geo, geo2, dpart;
+  sfgs::DBuf<int> okf, okpos, ok2, okpos2, ok3, okpos3, cand_m, cand_s;
+  sfgs::DBuf<int> t_master, t_slave;
+  sfgs::DBuf<double> t_R, t_U;
+  sfgs::DBuf<unsigned long long> hkeys;   // existing body pairs (open addressing)
+  sfgs::DBuf<int> cell_cnt, cell_start, cell_body, cell_cur, ncand, candoff, cand_a, cand_b;
 };

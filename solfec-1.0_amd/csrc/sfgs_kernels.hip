@@ -997,6 +997,7 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   release(ctx->body_out);
   release(ctx->feat);
   sfgs_mlp_release(ctx);
+  sfgs_dom_release(ctx);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1020,6 +1021,7 @@ int SFGS_Bodies_Upload(sfgs_ctx *ctx, int nb, const int *kind, const double *rot
   ctx->have_bodies = 1;
   ctx->assembled = 0;
   ctx->nscenes = 1;     // scene ids refer to one body upload
+  ctx->dom = 0;         // so do the shapes of the device loop
   return 0;
 }
 

@@ -62,6 +62,20 @@ class SOLFEC:
     def ncon(self):
         return self.lib.SOLFEC_Ncon(self.sol)
 
+    def set_velocity(self, body, velo6):
+        """Velocity [W v] (6 values) of the body with index `body` (SOLFEC_Set_Velocity)."""
+        u = f64(velo6, 6)
+        if self.lib.SOLFEC_Set_Velocity(self.sol, int(body), ptr(u)):
+            raise ValueError("invalid body")
+
+    def device_loop(self, on=True):
+        """Keep bodies and contacts on the GPU across steps (SOLFEC_Device_Loop):
+        the half steps, contact update and detection run on the device; reading
+        bodies() or contacts() downloads the current state."""
+        rc = self.lib.SOLFEC_Device_Loop(self.sol, 1 if on else 0)
+        if rc:
+            raise RuntimeError(self.lib.SOLFEC_Last_Error(self.sol).decode())
+
     def timing(self, label):
         """Accumulated wall-clock seconds of a phase (TIMING, lng.c:10892)."""
         return self.lib.SOLFEC_Timing(self.sol, label.encode())

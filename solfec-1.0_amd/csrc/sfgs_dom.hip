@@ -217,14 +217,26 @@ __global__ void k_bounds_part(int nb, const int *shape, const double *x, double 
     for (int k = 0; k < 6; k ++) part[6*(size_t)blockIdx.x + k] = s[k][0];
 }
 
+// one block of TPB threads
 __global__ void k_bounds_final(int nparts, const double *part, double *out)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  __shared__ double s[6][TPB];
   double v[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
-  for (int p = 0; p < nparts; p ++)
+  for (int p = threadIdx.x; p < nparts; p += TPB)
     for (int k = 0; k < 6; k ++)
       v[k] = k < 3 ? fmin(v[k], part[6*(size_t)p + k]) : fmax(v[k], part[6*(size_t)p + k]);
-  for (int k = 0; k < 6; k ++) out[k] = v[k];
+  for (int k = 0; k < 6; k ++) s[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = TPB/2; w > 0; w >>= 1)
+  {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 6; k ++)
+        s[k][threadIdx.x] = k < 3 ? fmin(s[k][threadIdx.x], s[k][threadIdx.x + w])
+                                  : fmax(s[k][threadIdx.x], s[k][threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; k ++) out[k] = s[k][0];
 }
 
 struct Grid {
@@ -514,7 +526,7 @@ int SFGS_Dom_Update_Begin(sfgs_ctx *ctx, double step, const double *gravity, int
     const int nparts = std::min(nblk(nb), 1024);
     CHK(ensure(ctx, ctx->dpart, 6*(size_t)nparts + 6));
     hipLaunchKernelGGL(k_bounds_part, dim3(nparts), dim3(TPB), 0, st, nb, ctx->shape.p, ctx->bx.p, ctx->dpart.p);
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, st, nparts, ctx->dpart.p, ctx->dpart.p + 6*(size_t)nparts);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(TPB), 0, st, nparts, ctx->dpart.p, ctx->dpart.p + 6*(size_t)nparts);
     double bb[6];
     HIPCHK(hipMemcpyAsync(bb, ctx->dpart.p + 6*(size_t)nparts, sizeof(bb), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));

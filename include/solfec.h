@@ -130,6 +130,15 @@ int SOLFEC_W_MatrixMarket_Write(const char *path, int n, const double *Wd, const
 /* snapshot writer: after each assembly, the features of every contact (18
  * floats, natural order, see SFGS_Features) are copied to buffer when the step
  * has at most cap contacts; SOLFEC_Captured gives the count of the last step */
+/* State file for restart and warm-start persistence: body records as
+ * BODY_Write_State (bod.c:2202-2221) and constraint records as write_constraint
+ * (dio.c:19-59, IOVER 4), XDR-encoded, after a header (see sol_state.c).  Read
+ * into a domain with the same bodies (same scene construction) and step; the
+ * contacts, their R and U, the body configurations, velocities and the time are
+ * replaced. */
+int SOLFEC_Write_State(SOLFEC *sol, const char *path);
+int SOLFEC_Read_State(SOLFEC *sol, const char *path);
+
 int SOLFEC_Capture(SOLFEC *sol, float *buffer, int cap);
 int SOLFEC_Captured(SOLFEC *sol);
 

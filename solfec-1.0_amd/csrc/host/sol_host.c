@@ -57,6 +57,7 @@ static void dev_edit(SOLFEC *sol)
 {
   if (sol->devloop) (void)dev_pull(sol);
   sol->dev_valid = 0;
+  sol->locdyn_current = 0;
 }
 
 /* ---------------- domain construction ---------------- */
@@ -732,6 +733,10 @@ static int dev_pull(SOLFEC *sol)
   return 0;
 }
 
+int sol_dev_pull(SOLFEC *sol) { return dev_pull(sol); }
+void sol_dev_edit(SOLFEC *sol) { dev_edit(sol); }
+void sol_ensure_contacts(SOLFEC *sol, int n) { ensure_contacts(sol, n); }
+
 int SOLFEC_Device_Loop(SOLFEC *sol, int on)
 {
   if (!sol) return SOL_ERR_ARG;
@@ -776,6 +781,7 @@ static int dev_step_solve(SOLFEC *sol, GAUSS_SEIDEL *gs)
   int rc = SFGS_Locdyn_Update_Begin(g, sol->step, sol->dynamic);
   double t1 = now_s();
   sol->t_locdyn += t1 - t0;
+  sol->locdyn_current = rc == 0;
   if (!rc && sol->capture)
   {
     sol->captured = sol->nc <= sol->capture_cap ? sol->nc : 0;
@@ -813,6 +819,7 @@ static int dev_step_end(SOLFEC *sol)
 
 int SOLFEC_Step_Begin(SOLFEC *sol)
 {
+  sol->locdyn_current = 0;
   if (sol->devloop) return dev_step_begin(sol);
   double t0 = now_s();
   #pragma omp parallel for schedule(static)
@@ -918,6 +925,7 @@ int SOLFEC_Step_Solve(SOLFEC *sol, GAUSS_SEIDEL *gs)
   if (!rc) rc = SFGS_Locdyn_Update_Begin(g, sol->step, sol->dynamic);
   double t1 = now_s();
   sol->t_locdyn += t1 - t0;
+  sol->locdyn_current = rc == 0;
   if (!rc && sol->capture)
   {
     sol->captured = sol->nc <= sol->capture_cap ? sol->nc : 0;

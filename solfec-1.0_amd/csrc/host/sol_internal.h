@@ -56,6 +56,7 @@ struct solfec {
    * contacts on the device; dev_valid: the device holds the current state;
    * host_stale: the host arrays above are behind it */
   int devloop, dev_valid, host_stale;
+  int locdyn_current;   /* the device holds the local dynamics of the current contact list */
 
   /* phase timers (sol.c:392-439 labels), seconds */
   double t_timint, t_conupd, t_condet, t_locdyn, t_consol;
@@ -74,6 +75,11 @@ struct gauss_seidel {
   int itershistcount, itershistsize;
   SFGS_REPORT rep;
 };
+
+/* host-layer internals shared by sol_host.c and sol_state.c */
+int sol_dev_pull(SOLFEC *sol);             /* device state into the host arrays, when behind */
+void sol_dev_edit(SOLFEC *sol);            /* before a host-side change of the state */
+void sol_ensure_contacts(SOLFEC *sol, int n);
 
 /* run the device solve for the current system of g; maps failures to SOL_ERR_* */
 int GAUSS_SEIDEL_Solve_Device(GAUSS_SEIDEL *gs, sfgs_ctx *g, char *err, size_t errlen);

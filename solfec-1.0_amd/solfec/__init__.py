@@ -68,6 +68,17 @@ class SOLFEC:
         if self.lib.SOLFEC_Set_Velocity(self.sol, int(body), ptr(u)):
             raise ValueError("invalid body")
 
+    def write_state(self, path):
+        """State file for restart (SOLFEC_Write_State): the body and constraint records
+        of BODY_Write_State / write_constraint, XDR-encoded."""
+        if self.lib.SOLFEC_Write_State(self.sol, str(path).encode()):
+            raise RuntimeError(self.lib.SOLFEC_Last_Error(self.sol).decode())
+
+    def read_state(self, path):
+        """Restart from a state file written by write_state for the same scene."""
+        if self.lib.SOLFEC_Read_State(self.sol, str(path).encode()):
+            raise RuntimeError(self.lib.SOLFEC_Last_Error(self.sol).decode())
+
     def device_loop(self, on=True):
         """Keep bodies and contacts on the GPU across steps (SOLFEC_Device_Loop):
         the half steps, contact update and detection run on the device; reading

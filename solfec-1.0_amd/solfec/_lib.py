@@ -104,6 +104,8 @@ _SOL_PROTOS = {
     "SOLFEC_W_MatrixMarket_Write": (_i, [C.c_char_p, _i, _p, _p, _p, _p]),
     "SOLFEC_Run": (_i, [_p, _p, _d]),
     "SOLFEC_Device_Loop": (_i, [_p, _i]),
+    "SOLFEC_Write_State": (_i, [_p, C.c_char_p]),
+    "SOLFEC_Read_State": (_i, [_p, C.c_char_p]),
     "SOLFEC_Step_Begin": (_i, [_p]),
     "SOLFEC_Step_Solve": (_i, [_p, _p]),
     "SOLFEC_Step_End": (_i, [_p]),

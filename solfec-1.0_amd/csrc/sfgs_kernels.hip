@@ -551,12 +551,15 @@ template <typename T, int G, bool MERIT>
 __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<T> a, int s0, int s1, int pbase, int upd_lo, int upd_hi,
                                                 int backward, int xcd, int *fin, int np, double *hist)
 {
-  if (a.ctrl->done) return;   // uniform: set only by k_finish between launches
+  // uniform: set only between launches (by the finish of an earlier sweep); read
+  // with the slice table, not before it
+  const int done = a.ctrl->done;
   const int bb = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   const int s = s0 + bb*(TPB/SLICE) + (threadIdx.x >> 6);
   double eu = 0.0, el = 0.0, up = 0.0;
   sweep_slice<T, G, MERIT>(a, s, s < s1, upd_lo, upd_hi, backward, &a.ctrl->sweeps, a.failkey, a.failcount,
-                           &a.ctrl->aborted, eu, el, up);
+                           &a.ctrl->aborted, eu, el, up, done == 0);
+  if (done) return;
   block_sum3(eu, el, up);
   if (threadIdx.x == 0)
   {

@@ -87,23 +87,22 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c)
 // One slice (64/G rows of one colour, one wavefront) of a sweep.  eu, el and
 // up receive this lane's contributions to the error sums and to the merit
 // numerator.  A failed local solve is recorded in failkey / failcount, keyed by
-// the sweep number *sweepno, and raises *abortflag under FAILURE_EXIT.
+// the sweep number *sweepno, and raises *abortflag under FAILURE_EXIT.  run =
+// false (a solve that has stopped) leaves the slice untouched; the caller reads
+// that flag together with the slice table, so both cost one memory round trip.
 template <typename T, int G, bool MERIT>
 __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool valid, int upd_lo, int upd_hi,
                                             int backward, const int *sweepno, unsigned long long *failkey,
-                                            int *failcount, int *abortflag, double &eu, double &el, double &up)
+                                            int *failcount, int *abortflag, double &eu, double &el, double &up,
+                                            bool run = true)
 {
   const int l = threadIdx.x & 63, r = l / G, g = l % G;
   const int n = a.n;
-  bool act = false;
-  int p = 0, len = 0;
+  int nr = 0, r0 = 0, p = 0, len = 0;
   long long off = 0;
-  if (valid)
-  {
-    act = r < a.nrows[s];
-    off = a.off[s];
-    if (act) { p = a.row0[s] + r; len = a.rowlen[p]; }
-  }
+  if (valid) { nr = a.nrows[s]; r0 = a.row0[s]; off = a.off[s]; }   // one round of loads
+  const bool act = valid && run && r < nr;
+  if (act) { p = r0 + r; len = a.rowlen[p]; }
   // the row's own data first, so that its loads overlap the gather below
   T Bi[3] = {T(0), T(0), T(0)}, Wi[9], R[3] = {T(0), T(0), T(0)}, U[3] = {T(0), T(0), T(0)};
   double Ai[9];

@@ -284,6 +284,26 @@ def bench_batch(args):
                     converged=int(np.sum((reps["merit"] <= 1e-6) & (reps["error"] <= eps))),
                     merit_max=float(reps["merit"].max())), R, reps
 
+    # off-diagonal blocks per scene: one per shared rigid body (LOCDYN_Insert, ldy.c:492-568)
+    kind, cm, csl = np.asarray(B["kind"]), np.asarray(Cn["master"]), np.asarray(Cn["slave"])
+    rig_m, rig_s = kind[cm] == 0, (csl >= 0) & (kind[np.maximum(csl, 0)] == 0)
+    deg = np.bincount(np.r_[cm[rig_m], csl[rig_s]], minlength=len(kind))
+    row_nnz = np.where(rig_m, deg[cm] - 1, 0) + np.where(rig_s, deg[np.maximum(csl, 0)] - 1, 0)
+    nnz_s = np.bincount(cs, weights=row_nnz, minlength=ns)
+    n_s = np.bincount(cs, minlength=ns).astype(float)
+    nnz_ok = int(row_nnz.sum()) == dev.sizes()[1]
+
+    def sweep_bytes(reps, prec):
+        """Algorithmic bytes of all sweeps of a batched solve: per scene, s (9 nnz + 26 n) + 4 nnz + 4 n
+        per sweep, s = 4 for the FP32 sweeps of MIXED (the last 2 are FP64 refinement), 8 for FP64."""
+        b32 = 4 * (9 * nnz_s + 26 * n_s) + 4 * nnz_s + 4 * n_s
+        b64 = 8 * (9 * nnz_s + 26 * n_s) + 4 * nnz_s + 4 * n_s
+        it = reps["iters"].astype(float)
+        if prec == "mixed":
+            ref = np.minimum(it, 2.0)
+            return float(np.sum((it - ref) * b32 + ref * b64))
+        return float(np.sum(it * b64))
+
     head, Rh, rh = run(pm, have_model, args.steps, args.warmup)
     log(f"[batch] learned+mixed: {head['ms_per_step']:.1f} ms/step, iters max {head['max_iters']}")
     plain, Rp, rp = run(pf, False, args.steps, args.warmup)
@@ -310,6 +330,12 @@ def bench_batch(args):
                    "scenes_converged_both": int(ok.sum()),
                    "scene_err_vs_fp64_max": max(errs), "scene_err_vs_fp64_median": float(np.median(errs))},
     }
+    if nnz_ok:
+        gbs = sweep_bytes(rh, "mixed") / (head["solve_ms"] * 1e-3) / 1e9
+        out["roofline"] = dict(bound="hbm", achieved=round(gbs, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                               frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None,
+                               kernel="k_usweep (algorithmic bytes of all sweeps / device time of the whole solve)")
+        log(f"[batch] whole-solve sweep bytes rate {gbs:.1f} GB/s")
     print(json.dumps(out), flush=True)
     dev.close()
 

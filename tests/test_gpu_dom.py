@@ -135,3 +135,40 @@ def test_restart_from_state_file_is_bit_exact(tmp_path, loop):
     nc = same_state(a, b, f"restart ({loop} loop)")
     assert a.time == b.time
     print(f"[dom] restart, {loop} loop: {nc} contacts identical after 10 steps", flush=True)
+
+
+@pytest.mark.parametrize("loop", ["host", "device"])
+def test_sphere_resting_on_a_plane(loop):
+    """A sphere at rest on a plane stays at rest; the contact carries its weight:
+    R = (0, 0, m g) in the contact frame (R is a force, u += h M^-1 H' R)."""
+    s = solfec.SOLFEC("DYNAMIC", 1e-3, "/tmp/solfec_test")
+    solfec.SURFACE_MATERIAL(s, friction=0.5, restitution=0.0)
+    solfec.GRAVITY(s, (0.0, 0.0, -9.81))
+    mat = solfec.BULK_MATERIAL(s, density=2000.0)
+    r = 0.05
+    solfec.BODY(s, "OBSTACLE", solfec.PLANE((0, 0, 0), (0, 0, 1), 0), mat)
+    solfec.BODY(s, "RIGID", solfec.SPHERE((0.3, 0.2, r), r, 1, 1), mat)
+    s.device_loop(loop == "device")
+    gs = solfec.GAUSS_SEIDEL_SOLVER(1e-10, 100)
+    solfec.RUN(s, gs, 0.05)
+    b, c = s.bodies(), s.contacts()
+    m = 2000.0 * 4.0 / 3.0 * np.pi * r ** 3
+    assert len(c["master"]) == 1
+    np.testing.assert_allclose(b["x"][3:6], [0.3, 0.2, r], atol=1e-9)
+    np.testing.assert_allclose(b["velo"][6:12], 0.0, atol=1e-9)
+    np.testing.assert_allclose(c["R"], [0.0, 0.0, m * 9.81], rtol=1e-9, atol=1e-12)
+    assert s.time == pytest.approx(0.05)
+
+
+def test_device_loop_without_convex_bodies():
+    """Planes only, and then nothing at all: the device loop runs empty steps."""
+    s = solfec.SOLFEC("DYNAMIC", 1e-3, "/tmp/solfec_test")
+    mat = solfec.BULK_MATERIAL(s, density=1000.0)
+    solfec.BODY(s, "OBSTACLE", solfec.PLANE((0, 0, 0), (0, 0, 1), 0), mat)
+    s.device_loop(True)
+    solfec.RUN(s, solfec.GAUSS_SEIDEL_SOLVER(1e-6, 100), 0.003)
+    assert s.ncon == 0 and s.time == pytest.approx(0.003)
+    e = solfec.SOLFEC("DYNAMIC", 1e-3, "/tmp/solfec_test")
+    e.device_loop(True)
+    solfec.RUN(e, solfec.GAUSS_SEIDEL_SOLVER(1e-6, 100), 0.002)
+    assert e.nbod == 0 and e.ncon == 0

@@ -24,6 +24,15 @@
 // the test of sweep m - 2 is known, so the state a stopping test names is
 // never overwritten.  Waits are bounded; a wait that exceeds the bound raises
 // a flag that ends every wait, and the solve reports an error.
+//
+// Publication without cache-wide fences: everything one wavefront hands to
+// another (reactions, versions, partial sums, the control block) is written
+// and read with device-scope relaxed atomics, which the per-XCD L2s keep
+// coherent; a writer waits for its own stores to complete (s_waitcnt) before
+// it stores the version or counter that publishes them, and a reader issues
+// its data loads only after it has seen that version.  A device-scope
+// release / acquire fence would instead write back / invalidate the whole L2
+// at every slice.
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cfloat>
@@ -62,6 +71,20 @@ __device__ __forceinline__ int ld_relaxed(const int *p)
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+template <typename V>
+__device__ __forceinline__ V cld(const V *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename V>
+__device__ __forceinline__ void cst(V *p, V v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// this wavefront's earlier device-scope stores are complete before what follows
+// (the compiler keeps memory operations on their side of it)
+__device__ __forceinline__ void publish_fence()
+{
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // the termination test after sweep k (bgs.c:1016-1036), as k_finish, on the
 // sums of all wavefronts' partials; run by one wavefront
 template <typename T>
@@ -78,10 +101,12 @@ __device__ void flow_decide(const FlowArgs<T> &f, int k)
   }
   eu = wave_sum(eu); el = wave_sum(el); up = wave_sum(up);
   if (lane != 0) return;
+  // the fields the tests of other wavefronts change are accessed coherently;
+  // nomerit, maxiter, eps, meritval and energy are set before the launch
   Ctrl *c = f.ctrl;
-  if (c->done) return;
+  if (cld(&c->done)) return;
   const double error_k = sqrt(eu) / sqrt(el == 0.0 ? 1.0 : el);
-  c->sweeps = k + 1;
+  cst(&c->sweeps, k + 1);
   const int aborted = ld_relaxed(&c->aborted);
   if (c->nomerit || aborted)
   {
@@ -90,31 +115,31 @@ __device__ void flow_decide(const FlowArgs<T> &f, int k)
     {
       f.hist[k] = error_k;
       f.hist[c->maxiter + k] = 0.0;
-      c->error = error_k;
-      c->merit = 0.0;
+      cst(&c->error, error_k);
+      cst(&c->merit, 0.0);
     }
-    c->iters = k + 1;
-    c->final_buf = (k + 1) % 3;
-    c->final_U = k % 2;
-    if (aborted || !(k + 1 < c->maxiter && error_k > c->eps)) c->done = 1;
+    cst(&c->iters, k + 1);
+    cst(&c->final_buf, (k + 1) % 3);
+    cst(&c->final_U, k % 2);
+    if (aborted || !(k + 1 < c->maxiter && error_k > c->eps)) cst(&c->done, 1);
     return;
   }
-  if (k == 0) { c->prev_error = error_k; return; }
+  if (k == 0) { cst(&c->prev_error, error_k); return; }
   const double lo = c->energy < DBL_EPSILON ? 1.0 : c->energy;
-  const double merit = (0.5*up) / lo, error = c->prev_error;
+  const double merit = (0.5*up) / lo, error = cld(&c->prev_error);
   f.hist[k - 1] = error;
   f.hist[c->maxiter + k - 1] = merit;
-  c->iters = k;
-  c->error = error;
-  c->merit = merit;
+  cst(&c->iters, k);
+  cst(&c->error, error);
+  cst(&c->merit, merit);
   if (!(k < c->maxiter && (error > c->eps || merit > c->meritval)))
   {
     // R after sweep k - 1, the merit U formed in sweep k
-    c->final_buf = k % 3;
-    c->final_U = 2 + k % 2;
-    c->done = 1;
+    cst(&c->final_buf, k % 3);
+    cst(&c->final_U, 2 + k % 2);
+    cst(&c->done, 1);
   }
-  else c->prev_error = error_k;
+  else cst(&c->prev_error, error_k);
 }
 
 template <typename T, int G>
@@ -135,7 +160,7 @@ __global__ __launch_bounds__(64) void k_flow(FlowArgs<T> f)
         if ((spins & 63u) == 0 && (ld_relaxed(f.abort_all) || ld_relaxed(&f.ctrl->done))) { ab = 1; break; }
         __builtin_amdgcn_s_sleep(1);
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
       if (ab || ld_relaxed(&f.ctrl->done)) return;
     }
     SweepArgs<T> a = a0;
@@ -148,47 +173,55 @@ __global__ __launch_bounds__(64) void k_flow(FlowArgs<T> f)
     double eu = 0.0, el = 0.0, up = 0.0;
     for (int s = W; s < f.nslices; s += f.nwaves)
     {
-      // wait for this lane's neighbours
+      // wait for this lane's neighbours: all their versions loaded at once, again
+      // after a pause while any is behind
       const int row0 = a.row0[s];
       const bool act = r < a.nrows[s];
       int ok = 1;
       if (act)
       {
         const int p = row0 + r, len = a.rowlen[p];
+        const int mine = len > g ? (len - g + G - 1) / G : 0;
         const int *cb = a.wcol + a.off[s] + lane;
-        for (int kk = g, m = 0; kk < len; kk += G, m ++)
+        unsigned spins = 0;
+        for (int m0 = 0; m0 < mine && ok; )
         {
-          const int j = cb[(size_t)m*SLICE];
-          const int need = k + (j < p ? 1 : 0);
-          unsigned spins = 0;
-          while (ld_relaxed(f.ver + j) < need)
-          {
-            if (++spins > f.spin_limit) { atomicExch(f.abort_all, 1); ok = 0; break; }
-            // a stopped solve leaves rows of later sweeps undone: stop waiting for them
-            if ((spins & 63u) == 0 && (ld_relaxed(f.abort_all) || ld_relaxed(&f.ctrl->done))) { ok = 0; break; }
-            __builtin_amdgcn_s_sleep(1);
-          }
-          if (!ok) break;
+          constexpr int CH = 8;
+          int jj[CH], vv[CH];
+          bool ready = true;
+#pragma unroll
+          for (int q = 0; q < CH; q ++) jj[q] = cb[(size_t)min(m0 + q, mine - 1)*SLICE];
+#pragma unroll
+          for (int q = 0; q < CH; q ++) vv[q] = ld_relaxed(f.ver + jj[q]);   // all in flight together
+#pragma unroll
+          for (int q = 0; q < CH; q ++) ready = ready && (m0 + q >= mine || vv[q] >= k + (jj[q] < p ? 1 : 0));
+          if (ready) { m0 += CH; continue; }
+          if (++spins > f.spin_limit) { atomicExch(f.abort_all, 1); ok = 0; break; }
+          // a stopped solve leaves rows of later sweeps undone: stop waiting for them
+          if ((spins & 63u) == 0 && (ld_relaxed(f.abort_all) || ld_relaxed(&f.ctrl->done))) { ok = 0; break; }
+          __builtin_amdgcn_s_sleep(1);
         }
       }
       if (__any(!ok)) return;
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
       // lower-colour neighbours (j < row0 <= p) are read from Q, the rest from P
       if (merit)
-        sweep_slice<T, G, true>(a, s, true, 0, row0, 0, f.sweepno, a.failkey, a.failcount, &f.ctrl->aborted, eu, el, up);
+        sweep_slice<T, G, true, true>(a, s, true, 0, row0, 0, f.sweepno, a.failkey, a.failcount, &f.ctrl->aborted,
+                                      eu, el, up);
       else
-        sweep_slice<T, G, false>(a, s, true, 0, row0, 0, f.sweepno, a.failkey, a.failcount, &f.ctrl->aborted, eu, el, up);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      if (act && g == 0) __hip_atomic_store(f.ver + row0 + r, k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sweep_slice<T, G, false, true>(a, s, true, 0, row0, 0, f.sweepno, a.failkey, a.failcount, &f.ctrl->aborted,
+                                       eu, el, up);
+      publish_fence();
+      if (act && g == 0) cst(f.ver + row0 + r, k + 1);
     }
     // this wavefront's share of sweep k
     eu = wave_sum(eu); el = wave_sum(el); up = wave_sum(up);
     if (lane == 0)
     {
       double *pt = f.part + 3*((size_t)f.nwaves*(k % 3) + W);
-      pt[0] = eu; pt[1] = el; pt[2] = up;
+      cst(pt, eu); cst(pt + 1, el); cst(pt + 2, up);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    publish_fence();
     unsigned last = 0;
     if (lane == 0) last = atomicAdd(f.cnt + k % 3, 1u) == (unsigned)f.nwaves - 1 ? 1u : 0u;
     last = __shfl(last, 0, 64);
@@ -202,15 +235,15 @@ __global__ __launch_bounds__(64) void k_flow(FlowArgs<T> f)
         if ((spins & 63u) == 0 && ld_relaxed(f.abort_all)) return;
         __builtin_amdgcn_s_sleep(1);
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
       flow_decide(f, k);
       if (lane == 0)
       {
-        *f.sweepno = k + 1;
-        __hip_atomic_store(f.cnt + k % 3, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        cst(f.sweepno, k + 1);
+        cst(f.cnt + k % 3, 0u);
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      if (lane == 0) __hip_atomic_store(f.decided, k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      publish_fence();
+      if (lane == 0) cst(f.decided, k + 1);
     }
   }
 }

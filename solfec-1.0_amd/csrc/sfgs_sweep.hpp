@@ -84,13 +84,32 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c)
   }
 }
 
+// Reactions shared between wavefronts of one persistent launch (the dataflow
+// schedule) are read and written with device-scope relaxed atomics: those bypass
+// the per-XCD L2's stale lines without the cache-wide write-back / invalidate
+// that a device-scope fence costs.  Ordinary accesses otherwise.
+template <bool COH, typename T>
+__device__ __forceinline__ T r_ld(const T *p)
+{
+  if (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return *p;
+}
+
+template <bool COH, typename T>
+__device__ __forceinline__ void r_st(T *p, T v)
+{
+  if (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *p = v;
+}
+
 // One slice (64/G rows of one colour, one wavefront) of a sweep.  eu, el and
 // up receive this lane's contributions to the error sums and to the merit
 // numerator.  A failed local solve is recorded in failkey / failcount, keyed by
 // the sweep number *sweepno, and raises *abortflag under FAILURE_EXIT.  run =
 // false (a solve that has stopped) leaves the slice untouched; the caller reads
 // that flag together with the slice table, so both cost one memory round trip.
-template <typename T, int G, bool MERIT>
+// COH: P and Q are shared with other wavefronts of the same launch (r_ld / r_st).
+template <typename T, int G, bool MERIT, bool COH = false>
 __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool valid, int upd_lo, int upd_hi,
                                             int backward, const int *sweepno, unsigned long long *failkey,
                                             int *failcount, int *abortflag, double &eu, double &el, double &up,
@@ -113,7 +132,7 @@ __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool v
 #pragma unroll
     for (int e = 0; e < 9; e ++) Wi[e] = a.Wd[(size_t)e*n + p];
 #pragma unroll
-    for (int k = 0; k < 3; k ++) R[k] = a.P[4*(size_t)p + k];
+    for (int k = 0; k < 3; k ++) R[k] = r_ld<COH>(a.P + 4*(size_t)p + k);
     if (MERIT)
     {
 #pragma unroll
@@ -154,23 +173,25 @@ __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool v
                 w5 = w[5*SLICE], w6 = w[6*SLICE], w7 = w[7*SLICE], w8 = w[8*SLICE];
         const bool upd = j >= upd_lo && j < upd_hi;
         const T *xj = (upd ? a.Q : a.P) + 4*(size_t)j;
-        const T r0 = xj[0], r1 = xj[1], r2 = xj[2];
+        const T r0 = r_ld<COH>(xj), r1 = r_ld<COH>(xj + 1), r2 = r_ld<COH>(xj + 2);
         T p0 = r0, p1 = r1, p2 = r2;
         if (MERIT && upd)
         {
           const T *pj = a.P + 4*(size_t)j;
-          p0 = pj[0]; p1 = pj[1]; p2 = pj[2];
+          p0 = r_ld<COH>(pj); p1 = r_ld<COH>(pj + 1); p2 = r_ld<COH>(pj + 2);
         }
         if (m < mine)
         {
-          b0 += w0*r0 + w3*r1 + w6*r2;
-          b1 += w1*r0 + w4*r1 + w7*r2;
-          b2 += w2*r0 + w5*r1 + w8*r2;
+          // explicit fused multiply-adds: the same rounding in every schedule
+          // that instantiates this gather, whatever the surrounding loads
+          b0 += fma(w6, r2, fma(w3, r1, w0*r0));
+          b1 += fma(w7, r2, fma(w4, r1, w1*r0));
+          b2 += fma(w8, r2, fma(w5, r1, w2*r0));
           if (MERIT)
           {
-            u0 += w0*p0 + w3*p1 + w6*p2;
-            u1 += w1*p0 + w4*p1 + w7*p2;
-            u2 += w2*p0 + w5*p1 + w8*p2;
+            u0 += fma(w6, p2, fma(w3, p1, w0*p0));
+            u1 += fma(w7, p2, fma(w4, p1, w1*p0));
+            u2 += fma(w8, p2, fma(w5, p1, w2*p0));
           }
         }
       }
@@ -210,7 +231,7 @@ __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool v
       {
         unsigned code = d < 0 ? SFGS_GS_DIAGONAL_FAILED : SFGS_GS_DIAGONAL_DIVERGED;
         unsigned long long pos = backward ? (unsigned long long)(n - 1 - p) : (unsigned long long)p;
-        atomicMax(failkey, ((unsigned long long)(*sweepno + 1) << 40) | (pos << 2) | code);
+        atomicMax(failkey, ((unsigned long long)(r_ld<COH>(sweepno) + 1) << 40) | (pos << 2) | code);
         atomicAdd(failcount, 1);
         if (a.failure == SFGS_FAILURE_EXIT) atomicExch(abortflag, 1);
       }
@@ -231,8 +252,8 @@ __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool v
     if (g == 0)
     {
 #pragma unroll
-      for (int k = 0; k < 3; k ++) { a.Q[4*(size_t)p + k] = R[k]; a.U[(size_t)k*n + p] = U[k]; }
-      a.Q[4*(size_t)p + 3] = T(0);
+      for (int k = 0; k < 3; k ++) { r_st<COH>(a.Q + 4*(size_t)p + k, R[k]); a.U[(size_t)k*n + p] = U[k]; }
+      r_st<COH>(a.Q + 4*(size_t)p + 3, T(0));
       const double d0 = (double)R[0] - R0[0], d1 = (double)R[1] - R0[1], d2 = (double)R[2] - R0[2];
       eu += d0*d0 + d1*d1 + d2*d2;
       el += (double)R[0]*R[0] + (double)R[1]*R[1] + (double)R[2]*R[2];

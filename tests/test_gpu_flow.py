@@ -55,6 +55,7 @@ def test_flow_equals_colour_schedule(dev, n, seed, lanes, kw):
     print(f"[flow] iters colour {r0.iters} flow {r1.iters}; merit {r0.merit:.3e} {r1.merit:.3e}; "
           f"solve ms {r0.solve_ms:.2f} {r1.solve_ms:.2f}")
     assert abs(r0.iters - r1.iters) <= 1
+    print(f"[flow] max |dR| {np.abs(R0g - R1g).max():.3e}, max |dU| {np.abs(U0g - U1g).max():.3e}", flush=True)
     if r0.iters == r1.iters:
         assert np.array_equal(R0g, R1g)
         assert np.array_equal(U0g, U1g)

@@ -97,8 +97,10 @@ class SOLFEC:
         out = dict(kind=np.zeros(nb, np.int32), rot=np.zeros(9 * nb), x=np.zeros(3 * nb), x0=np.zeros(3 * nb),
                    jinv=np.zeros(9 * nb), minv=np.zeros(nb), velo=np.zeros(6 * nb), velo0=np.zeros(6 * nb),
                    radius=np.zeros(nb))
-        self.lib.SOLFEC_Get_Bodies(self.sol, *[ptr(out[k]) for k in
-                                               ("kind", "rot", "x", "x0", "jinv", "minv", "velo", "velo0", "radius")])
+        if self.lib.SOLFEC_Get_Bodies(self.sol, *[ptr(out[k]) for k in
+                                                  ("kind", "rot", "x", "x0", "jinv", "minv", "velo", "velo0",
+                                                   "radius")]):
+            raise RuntimeError(self.lib.SOLFEC_Last_Error(self.sol).decode())
         return out
 
     def contacts(self):
@@ -106,9 +108,10 @@ class SOLFEC:
         out = dict(master=np.zeros(n, np.int32), slave=np.zeros(n, np.int32), mpnt=np.zeros(3 * n),
                    spnt=np.zeros(3 * n), base=np.zeros(9 * n), gap=np.zeros(n), friction=np.zeros(n),
                    restitution=np.zeros(n), cohesion=np.zeros(n), R=np.zeros(3 * n), U=np.zeros(3 * n))
-        self.lib.SOLFEC_Get_Contacts(self.sol, *[ptr(out[k]) for k in
-                                                 ("master", "slave", "mpnt", "spnt", "base", "gap", "friction",
-                                                  "restitution", "cohesion", "R", "U")])
+        if self.lib.SOLFEC_Get_Contacts(self.sol, *[ptr(out[k]) for k in
+                                                    ("master", "slave", "mpnt", "spnt", "base", "gap", "friction",
+                                                     "restitution", "cohesion", "R", "U")]):
+            raise RuntimeError(self.lib.SOLFEC_Last_Error(self.sol).decode())
         return out
 
     def set_RU(self, R=None, U=None):

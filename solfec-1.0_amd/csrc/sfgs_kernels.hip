@@ -543,6 +543,7 @@ __device__ __forceinline__ int xcd_block(int b, int nb)
   return x*q + min(x, rm) + k;
 }
 
+template <int NT = TPB>
 __device__ void finish_sweep(Ctrl *ctrl, const double *partials, int np, double *hist);
 
 // fin (the last launch of a sweep only): the block that completes the launch
@@ -598,14 +599,15 @@ __global__ __launch_bounds__(TPB) void k_merit(MeritArgs a, int nslices)
 // Without merit, sweep k closes iteration k.  With merit, sweep k >= 1 closes
 // iteration k-1 (its merit was formed during sweep k); sweep 0 only records its
 // error.  ctrl->final_buf / final_U name the buffers holding the final state.
+template <int NT>
 __device__ void finish_sweep(Ctrl *ctrl, const double *partials, int np, double *hist)
 {
   double eu = 0.0, el = 0.0, up = 0.0;
-  for (int k = threadIdx.x; k < np; k += TPB)
+  for (int k = threadIdx.x; k < np; k += NT)
   {
     eu += partials[3*(size_t)k]; el += partials[3*(size_t)k + 1]; up += partials[3*(size_t)k + 2];
   }
-  block_sum3(eu, el, up);
+  block_sum3<NT>(eu, el, up);
   if (threadIdx.x != 0) return;
   const int k = ctrl->sweeps;
   const double error_k = sqrt(eu) / sqrt(el == 0.0 ? 1.0 : el);

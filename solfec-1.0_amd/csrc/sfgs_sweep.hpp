@@ -69,10 +69,12 @@ __device__ __forceinline__ T gsum(T v)
   return v;
 }
 
-// deterministic block sum of three values; thread 0 returns them
+// deterministic block sum of three values over a block of NT threads; thread 0
+// returns them (the shared slots may be reused after the next barrier)
+template <int NT = TPB>
 __device__ __forceinline__ void block_sum3(double &a, double &b, double &c)
 {
-  __shared__ double sa[TPB/64], sb[TPB/64], sc[TPB/64];
+  __shared__ double sa[NT/64], sb[NT/64], sc[NT/64];
   a = wave_sum(a); b = wave_sum(b); c = wave_sum(c);
   int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) { sa[w] = a; sb[w] = b; sc[w] = c; }
@@ -80,7 +82,7 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c)
   if (threadIdx.x == 0)
   {
     a = sa[0]; b = sb[0]; c = sc[0];
-    for (int k = 1; k < TPB/64; k ++) { a += sa[k]; b += sb[k]; c += sc[k]; }
+    for (int k = 1; k < NT/64; k ++) { a += sa[k]; b += sb[k]; c += sc[k]; }
   }
 }
 

@@ -74,6 +74,7 @@ def lib():
             "OR_Plane_Contact": (_i, [_p, _p, _d, _p]),
             "OR_Detect": (C.c_longlong, [_i, _p, _p, _p, _p, _i, _p, _p, C.c_longlong, _p, _p, _p, _p]),
             "OR_Expmap": (None, [_p, _p]),
+            "OR_Penalty_Contact": (_i, [_i, _d, _d, _d, _d, _d, _d, _p, _p, _p, _p, _p]),
             "OR_Rig_Step_Begin": (None, [_i, _d, _p, _d, _d, _p, _p, _p, _p, _p, _p, _p]),
             "OR_Rig_Step_End": (None, [_i, _d, _p, _p, _p, _p, _p, _p]),
         }
@@ -291,3 +292,12 @@ def rig_step_end(kind, h, I, force, velo, rot, x, aux):
     velo, rot, x, aux = _f(velo).copy(), _f(rot).copy(), _f(x).copy(), _f(aux).copy()
     lib().OR_Rig_Step_End(int(kind), h, _ptr(_f(I)), _ptr(_f(force)), _ptr(velo), _ptr(rot), _ptr(x), _ptr(aux))
     return velo, rot, x, aux
+
+
+def penalty_contact(implicit, step, gap, spring, dashpot, hpow, friction, W, B, V, R0=(0.0, 0.0, 0.0)):
+    """PENALTY_Spring_Dashpot_Contact (pes.c:33-106): returns rc, R, U."""
+    R = _f(np.asarray(R0, float)).copy()
+    U = np.zeros(3)
+    rc = lib().OR_Penalty_Contact(int(implicit), step, gap, spring, dashpot, hpow, friction, _ptr(_f(W)), _ptr(_f(B)),
+                                  _ptr(_f(V)), _ptr(U), _ptr(R))
+    return rc, R, U

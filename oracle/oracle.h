@@ -11,6 +11,7 @@
  *   or_scf.c  -- scf.c:28-174 (cone linearisation / projection)
  *   or_bgs.c  -- bgs.c:913-1069 (serial Gauss-Seidel), mrf.c:35-184 (merit)
  *   or_bod.c  -- bod.c:418-435, 1425-1447 (body velocity update by reactions)
+ *   or_pes.c  -- pes.c:33-106 (spring-dashpot contact law of the penalty solver)
  *   or_dom.c  -- goc.c:50-59, 308-348, 421-455, gjk.c:560-584 (sphere and plane
  *                contacts), dom.c:348-483 (contact update and creation, brute
  *                force), bod.c:1313-1373, 1425-1492 (rigid step), alg.h:1138-1204
@@ -155,6 +156,12 @@ void OR_Rig_Step_Begin(int kind, double h, const double *gravity, double mass, d
                        const double *I, double *velo, double *velo0, double *R, double *x, double *A);
 void OR_Rig_Step_End(int kind, double h, const double *I, const double *force, double *velo, double *R, double *x,
                      double *A);
+
+/* spring-dashpot contact law (or_pes.c); W, B, V of the contact; R in: the
+ * carried reaction (its tangential part enters B_N), out: the reaction; returns
+ * 0, or -1 for a singular tangential block */
+int OR_Penalty_Contact(int implicit, double step, double gap, double spring, double dashpot, double hpow,
+                       double friction, const double *W, const double *B, const double *V, double *U, double *R);
 
 #ifdef __cplusplus
 }

@@ -265,10 +265,13 @@ typedef struct {
   const double *aux;     /* exp[-(h/2)W] J W + h T carried from step begin to step end, 3 per body */
 } SFGS_SHAPES;
 
-/* surface pair material (SURFACE_MATERIAL, mat.h); the pair is unordered */
+/* surface pair material (SURFACE_MATERIAL, sps.h:40-66); the pair is
+ * unordered; spring, dashpot and hpow (Hertz power, 1 for a linear spring) are
+ * the constants of the penalty solver's spring-dashpot law */
 typedef struct {
   int s1, s2;
   double friction, restitution, cohesion;
+  double spring, dashpot, hpow;
 } SFGS_MATERIAL;
 
 /* body shapes, the material table (defmat for pairs not in it) and the
@@ -297,6 +300,20 @@ int SFGS_Dom_Get_Bodies(sfgs_ctx *ctx, double *rot, double *x, double *velo, dou
 int SFGS_Dom_Get_Contacts(sfgs_ctx *ctx, int *master, int *slave, double *point, double *mpnt, double *spnt,
                           double *base, double *gap, double *friction, double *restitution, double *cohesion,
                           double *R, double *U);
+
+/* PENALTY_Solve (pes.c:247-281) for the uploaded bodies and contacts (or the
+ * device loop's): per contact W_ii, B and V as SFGS_Locdyn_Update_Begin forms
+ * them, without off-diagonal blocks (the penalty update, ldy.c:744), then the
+ * spring-dashpot law PENALTY_Spring_Dashpot_Contact (pes.c:33-106), IMPLICIT
+ * (implicit = 1) or EXPLICIT, from the carried R.  spring, dashpot and hpow
+ * are given per contact (natural order), or all three NULL: then they come
+ * from the material table of SFGS_Dom_Upload by the bodies' surfaces.  R and
+ * U follow from SFGS_Get_RU, velocities from SFGS_Body_Update or
+ * SFGS_Dom_Update_End.  The context holds no assembled system afterwards:
+ * SFGS_Solve, SFGS_Merit and SFGS_Get_Locdyn return SFGS_E_STATE until the
+ * next SFGS_Locdyn_Update_Begin.  Cohesive contacts are not modelled. */
+int SFGS_Penalty_Solve(sfgs_ctx *ctx, double step, int implicit, const double *spring, const double *dashpot,
+                       const double *hpow);
 
 #ifdef __cplusplus
 }

@@ -68,6 +68,18 @@ const char* SOLFEC_Last_Error(SOLFEC *sol);
 void SOLFEC_Gravity(SOLFEC *sol, double gx, double gy, double gz);
 /* surface pair material (SURFACE_MATERIAL, SIGNORINI_COULOMB); s1 = s2 = -1 sets the default */
 int SOLFEC_Surface_Material(SOLFEC *sol, int s1, int s2, double friction, double restitution, double cohesion);
+/* surface material models (sps.h:52-54) */
+#define SOLFEC_SIGNORINI_COULOMB 0
+#define SOLFEC_SPRING_DASHPOT 1
+/* SURFACE_MATERIAL with every constant (lng.c:2128-2190): model, friction,
+ * restitution in [0, 1], cohesion, spring, dashpot (negative: critical
+ * damping) and the Hertz power hpow (1 for a linear spring).  The constants
+ * spring, dashpot and hpow are read by the PENALTY_SOLVER for every contact
+ * (pes.c:272); a GAUSS_SEIDEL_SOLVER step refuses a domain that has a
+ * SPRING_DASHPOT material (the reference applies the spring law inside its
+ * sweep, dbs.c:521-526; this library does not) */
+int SOLFEC_Surface_Material_Ex(SOLFEC *sol, int s1, int s2, int model, double friction, double restitution,
+                               double cohesion, double spring, double dashpot, double hpow);
 /* rigid sphere of given density; returns the body index */
 int SOLFEC_Sphere(SOLFEC *sol, const double *centre, double radius, double density, int surface);
 /* rigid ellipsoid (ELLIP, eli.c) with semi-axes 3 along the columns of axes
@@ -90,6 +102,17 @@ int SOLFEC_Ellipsoid_Lattice_Scene(SOLFEC *sol, int n, double L, double amin, do
 
 /* SOLFEC_Run (sol.c:665-825): step until time >= t0 + duration */
 int SOLFEC_Run(SOLFEC *sol, GAUSS_SEIDEL *gs, double duration);
+
+/* ---- PENALTY_SOLVER (pes.c) ---- */
+typedef struct penalty PENALTY;
+/* PENALTY_Create (pes.c:236-245): implicit 1 = variant 'IMPLICIT', 0 = 'EXPLICIT' */
+PENALTY* PENALTY_Create(int implicit);
+void PENALTY_Destroy(PENALTY *ps);   /* pes.c:300-303 */
+/* LOCDYN_Update_Begin in its penalty form + PENALTY_Solve (pes.c:247-281) on
+ * the GPU (SFGS_Penalty_Solve) + the velocity update, as SOLFEC_Step_Solve */
+int SOLFEC_Step_Solve_Penalty(SOLFEC *sol, PENALTY *ps);
+/* SOLFEC_Run with a PENALTY_SOLVER (sol.c:471) */
+int SOLFEC_Run_Penalty(SOLFEC *sol, PENALTY *ps, double duration);
 /* single step phases (sol.c:761-773) for drivers that interpose work */
 /* 1: the step functions run DOM_Update_Begin / DOM_Update_End on the device
  * (SFGS_Dom_Update_Begin / _End) and keep bodies and contacts there; the host

@@ -294,10 +294,11 @@ def rig_step_end(kind, h, I, force, velo, rot, x, aux):
     return velo, rot, x, aux
 
 
-def penalty_contact(implicit, step, gap, spring, dashpot, hpow, friction, W, B, V, R0=(0.0, 0.0, 0.0)):
-    """PENALTY_Spring_Dashpot_Contact (pes.c:33-106): returns rc, R, U."""
+def penalty_contact(implicit, step, gap, spring, dashpot, hpow, friction, W, B, V, R0=(0.0, 0.0, 0.0),
+                    U0=(0.0, 0.0, 0.0)):
+    """PENALTY_Spring_Dashpot_Contact (pes.c:33-106) from the carried R0, U0: returns rc, R, U."""
     R = _f(np.asarray(R0, float)).copy()
-    U = np.zeros(3)
+    U = _f(np.asarray(U0, float)).copy()
     rc = lib().OR_Penalty_Contact(int(implicit), step, gap, spring, dashpot, hpow, friction, _ptr(_f(W)), _ptr(_f(B)),
                                   _ptr(_f(V)), _ptr(U), _ptr(R))
     return rc, R, U

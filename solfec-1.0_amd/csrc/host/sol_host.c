@@ -70,9 +70,9 @@ SOLFEC* SOLFEC_Create(int dynamic, double step, const char *outpath)
   sol->dynamic = dynamic ? 1 : 0;
   sol->step = step;
   sol->time = 0.0;
-  sol->defmat.friction = 0.0;
-  sol->defmat.restitution = 0.0;
-  sol->defmat.cohesion = 0.0;
+  sol->defmat.s1 = sol->defmat.s2 = -1;
+  sol->defmat.model = SOLFEC_SIGNORINI_COULOMB;
+  sol->defmat.hpow = 1.0;   /* the other constants 0 (lng.c:2137-2146) */
   sol->device = 0;
   snprintf(sol->outpath, sizeof(sol->outpath), "%s", outpath ? outpath : "");
   return sol;
@@ -122,20 +122,29 @@ void SOLFEC_Gravity(SOLFEC *sol, double gx, double gy, double gz)
 
 int SOLFEC_Surface_Material(SOLFEC *sol, int s1, int s2, double friction, double restitution, double cohesion)
 {
-  if (!sol || friction < 0.0 || restitution < 0.0 || cohesion < 0.0) return SOL_ERR_ARG;
+  return SOLFEC_Surface_Material_Ex(sol, s1, s2, SOLFEC_SIGNORINI_COULOMB, friction, restitution, cohesion, 0.0, 0.0, 1.0);
+}
+
+int SOLFEC_Surface_Material_Ex(SOLFEC *sol, int s1, int s2, int model, double friction, double restitution,
+                               double cohesion, double spring, double dashpot, double hpow)
+{
+  if (!sol || friction < 0.0 || restitution < 0.0 || restitution > 1.0 || cohesion < 0.0 || spring < 0.0 ||
+      hpow < 0.0 || (model != SOLFEC_SIGNORINI_COULOMB && model != SOLFEC_SPRING_DASHPOT)) return SOL_ERR_ARG;
   dev_edit(sol);
+  MATPAIR m;
+  m.s1 = s1 < s2 ? s1 : s2;
+  m.s2 = s1 < s2 ? s2 : s1;
+  m.model = model;
+  m.friction = friction; m.restitution = restitution; m.cohesion = cohesion;
+  m.spring = spring; m.dashpot = dashpot; m.hpow = hpow;
   if (s1 < 0 && s2 < 0)
   {
-    sol->defmat.friction = friction;
-    sol->defmat.restitution = restitution;
-    sol->defmat.cohesion = cohesion;
+    m.s1 = m.s2 = -1;
+    sol->defmat = m;
     return 0;
   }
   sol->mat = xrealloc(sol->mat, sizeof(MATPAIR)*(size_t)(sol->nmat + 1));
-  MATPAIR *m = &sol->mat[sol->nmat ++];
-  m->s1 = s1 < s2 ? s1 : s2;
-  m->s2 = s1 < s2 ? s2 : s1;
-  m->friction = friction; m->restitution = restitution; m->cohesion = cohesion;
+  sol->mat[sol->nmat ++] = m;
   return 0;
 }
 
@@ -536,8 +545,13 @@ static int update_and_detect(SOLFEC *sol)
     long long ncell = 1;
     for (int k = 0; k < 3; k ++)
     {
-      dim[k] = (int)floor((hi[k] - lo[k])/cell) + 1;
-      if (dim[k] < 1) dim[k] = 1;
+      const double ext = hi[k] - lo[k];
+      if (!(ext >= 0.0) || !(ext < 1e12*cell))
+      {
+        snprintf(sol->err, sizeof(sol->err), "body positions are not finite (the time integration diverged)");
+        return SOL_ERR_ARG;
+      }
+      dim[k] = (int)floor(ext/cell) + 1;
       ncell *= dim[k];
     }
     while (ncell > 4LL*nb + 1024)   /* very sparse scenes: coarser grid */
@@ -693,9 +707,11 @@ static int dev_push(SOLFEC *sol, sfgs_ctx *g)
       mat[k].s1 = sol->mat[k].s1; mat[k].s2 = sol->mat[k].s2;
       mat[k].friction = sol->mat[k].friction; mat[k].restitution = sol->mat[k].restitution;
       mat[k].cohesion = sol->mat[k].cohesion;
+      mat[k].spring = sol->mat[k].spring; mat[k].dashpot = sol->mat[k].dashpot; mat[k].hpow = sol->mat[k].hpow;
     }
     def.s1 = def.s2 = -1;
     def.friction = sol->defmat.friction; def.restitution = sol->defmat.restitution; def.cohesion = sol->defmat.cohesion;
+    def.spring = sol->defmat.spring; def.dashpot = sol->defmat.dashpot; def.hpow = sol->defmat.hpow;
     rc = SFGS_Dom_Upload(g, &sh, sol->nmat, mat, &def, sol->c_point);
     free(mat);
   }
@@ -912,9 +928,24 @@ int SOLFEC_Set_RU(SOLFEC *sol, const double *R, const double *U)
   return 0;
 }
 
+/* a SPRING_DASHPOT material in the domain (the Gauss-Seidel sweep here models
+ * SIGNORINI_COULOMB contacts only) */
+static int has_spring_dashpot(SOLFEC *sol)
+{
+  if (sol->defmat.model == SOLFEC_SPRING_DASHPOT) return 1;
+  for (int k = 0; k < sol->nmat; k ++) if (sol->mat[k].model == SOLFEC_SPRING_DASHPOT) return 1;
+  return 0;
+}
+
 /* upload, assemble, solve and update body velocities on the GPU */
 int SOLFEC_Step_Solve(SOLFEC *sol, GAUSS_SEIDEL *gs)
 {
+  if (!sol || !gs) return SOL_ERR_ARG;
+  if (has_spring_dashpot(sol))
+  {
+    snprintf(sol->err, sizeof(sol->err), "SPRING_DASHPOT materials need the PENALTY_SOLVER here");
+    return SOL_ERR_ARG;
+  }
   if (sol->devloop) return dev_step_solve(sol, gs);
   sfgs_ctx *g = SOLFEC_Gpu(sol);
   if (!g) return SOL_ERR_GPU;
@@ -1103,6 +1134,86 @@ int SOLFEC_W_MatrixMarket(SOLFEC *sol, const char *path)
   if (!rc) rc = SOLFEC_W_MatrixMarket_Write(path, sol->nc, Wd, rowptr, col, W);
   free(rowptr); free(col); free(W); free(Wd); free(B); free(V);
   return rc;
+}
+
+/* ---------------- PENALTY_SOLVER ---------------- */
+
+PENALTY* PENALTY_Create(int implicit)
+{
+  PENALTY *ps = calloc(1, sizeof(PENALTY));
+  if (ps) ps->implicit = implicit ? 1 : 0;
+  return ps;
+}
+
+void PENALTY_Destroy(PENALTY *ps)
+{
+  if (!ps) return;
+  free(ps->kcp);
+  free(ps);
+}
+
+static int penalty_failed(SOLFEC *sol, sfgs_ctx *g, int rc, const char *what)
+{
+  snprintf(sol->err, sizeof(sol->err), "%s failed: %s (%s)", what, SFGS_Error_String(rc), SFGS_Last_Error(g));
+  return SOL_ERR_GPU;
+}
+
+/* PENALTY_Solve (pes.c:247-281): the spring-dashpot law per contact with the
+ * material of its surface pair (pes.c:272), then the velocity update */
+int SOLFEC_Step_Solve_Penalty(SOLFEC *sol, PENALTY *ps)
+{
+  if (!sol || !ps) return SOL_ERR_ARG;
+  sfgs_ctx *g = SOLFEC_Gpu(sol);
+  if (!g) return SOL_ERR_GPU;
+  double t0 = now_s();
+  int rc;
+  sol->locdyn_current = 0;   /* the penalty update assembles no off-diagonal blocks to dump */
+  if (sol->devloop)
+  {
+    rc = SFGS_Penalty_Solve(g, sol->step, ps->implicit, NULL, NULL, NULL);
+    if (rc) return penalty_failed(sol, g, rc, "PENALTY_SOLVER on GPU");
+    sol->t_consol += now_s() - t0;
+    return 0;
+  }
+  const int n = sol->nc;
+  if (n > ps->kcp_cap)
+  {
+    ps->kcp = xrealloc(ps->kcp, sizeof(double)*3*(size_t)n);
+    ps->kcp_cap = n;
+  }
+  for (int i = 0; i < n; i ++)
+  {
+    const MATPAIR *m = find_material(sol, sol->surf[sol->c_master[i]], sol->surf[sol->c_slave[i]]);
+    ps->kcp[i] = m->spring;
+    ps->kcp[(size_t)n + i] = m->dashpot;
+    ps->kcp[2*(size_t)n + i] = m->hpow;
+  }
+  rc = SFGS_Bodies_Upload(g, sol->nb, sol->kind, sol->rot, sol->x0, sol->jinv, sol->minv, sol->velo, sol->velo0);
+  if (!rc) rc = SFGS_Contacts_Upload(g, sol->nc, sol->c_master, sol->c_slave, sol->c_mpnt, sol->c_spnt, sol->c_base,
+                                     sol->c_gap, sol->c_fri, sol->c_rest, sol->c_coh, sol->c_R, sol->c_U);
+  if (!rc) rc = SFGS_Penalty_Solve(g, sol->step, ps->implicit, ps->kcp, ps->kcp + n, ps->kcp + 2*(size_t)n);
+  if (!rc) rc = SFGS_Get_RU(g, sol->c_R, sol->c_U);
+  if (!rc) rc = SFGS_Body_Update(g, sol->step, sol->tmp_velo, sol->force);
+  if (rc) return penalty_failed(sol, g, rc, "PENALTY_SOLVER on GPU");
+  memcpy(sol->velo, sol->tmp_velo, sizeof(double)*6*(size_t)sol->nb);
+  sol->t_consol += now_s() - t0;
+  return 0;
+}
+
+int SOLFEC_Run_Penalty(SOLFEC *sol, PENALTY *ps, double duration)
+{
+  if (!sol || !ps || !(duration > 0.0)) return SOL_ERR_ARG;
+  double t0 = sol->time;
+  while (sol->time < t0 + duration - 1e-12*sol->step)
+  {
+    int rc = SOLFEC_Step_Begin(sol);
+    if (rc) return rc;
+    rc = SOLFEC_Step_Solve_Penalty(sol, ps);
+    if (rc) return rc;
+    rc = SOLFEC_Step_End(sol);
+    if (rc) return rc;
+  }
+  return SOL_OK;
 }
 
 int SOLFEC_Run(SOLFEC *sol, GAUSS_SEIDEL *gs, double duration)

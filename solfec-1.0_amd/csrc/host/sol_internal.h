@@ -7,7 +7,9 @@
 
 typedef struct {
   int s1, s2;
+  int model;                         /* SOLFEC_SIGNORINI_COULOMB or SOLFEC_SPRING_DASHPOT */
   double friction, restitution, cohesion;
+  double spring, dashpot, hpow;      /* the penalty solver's spring-dashpot law */
 } MATPAIR;
 
 struct solfec {
@@ -74,6 +76,12 @@ struct gauss_seidel {
   int *itershist;
   int itershistcount, itershistsize;
   SFGS_REPORT rep;
+};
+
+struct penalty {
+  int implicit;
+  double *kcp;            /* spring, dashpot, hpow per contact (host loop) */
+  int kcp_cap;
 };
 
 /* host-layer internals shared by sol_host.c and sol_state.c */

@@ -695,6 +695,7 @@ static int batch_solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TU
                        SFGS_REPORT *total, float *ms_out)
 {
   if (!ctx->assembled) { ctx->err = "solve before LOCDYN assembly"; return SFGS_E_STATE; }
+  if (ctx->diag_only) { ctx->err = "the store holds a penalty solve, not an assembled system"; return SFGS_E_STATE; }
   if (tn.lanes_per_row != ctx->G_built)
   {
     ctx->err = "batch shape has " + std::to_string(tn.lanes_per_row) + " lanes per row, the system was assembled with " +
@@ -896,7 +897,7 @@ int SFGS_Batch_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TUNE
 int SFGS_Batch_Autotune(sfgs_ctx *ctx, const SFGS_PARAM *par, int trial_iters, SFGS_BATCH_TUNE *best, double *best_ms)
 {
   if (!ctx || !par || trial_iters <= 0) return SFGS_E_ARG;
-  if (!ctx->assembled || !ctx->from_bodies) { ctx->err = "autotuning needs a system assembled from bodies"; return SFGS_E_STATE; }
+  if (!ctx->assembled || !ctx->from_bodies || ctx->diag_only) { ctx->err = "autotuning needs a system assembled from bodies"; return SFGS_E_STATE; }
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const int n = ctx->nc;

@@ -93,6 +93,8 @@ struct sfgs_ctx {
   double energy = 0.0, step = 0.0;
   int dynamic = 1;
   int have_bodies = 0, have_system = 0, assembled = 0, from_bodies = 0;
+  int diag_only = 0;                     // the store holds a penalty solve (R, U per contact), no system
+  sfgs::DBuf<double> pen;                // penalty solve: spring, dashpot, hpow per contact, or the default six
 
   // ---- reductions / control
   sfgs::DBuf<double> partials, mpartials, hist;
@@ -128,8 +130,8 @@ struct sfgs_ctx {
   double rmax = 0.0;                      // largest bounding radius of a convex body
   int nmat = 0;
   sfgs::DBuf<int> mat_s;                  // surface pairs (smaller id first), 2 per material
-  sfgs::DBuf<double> mat_v;               // friction, restitution, cohesion, 3 per material
-  double defmat[3] = {0.0, 0.0, 0.0};
+  sfgs::DBuf<double> mat_v;               // friction, restitution, cohesion, spring, dashpot, hpow, 6 per material
+  double defmat[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 1.0};
   sfgs::DBuf<double> geo, geo2, dpart;
   sfgs::DBuf<int> okf, okpos, ok2, okpos2, ok3, okpos3, cand_m, cand_s;
   sfgs::DBuf<int> t_master, t_slave;

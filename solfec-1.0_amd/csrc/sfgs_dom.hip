@@ -50,7 +50,7 @@ struct DomView {
 struct MatView {
   int nmat;
   const int *s;        // 2 per material, smaller surface first
-  const double *v;     // friction, restitution, cohesion
+  const double *v;     // friction, restitution, cohesion, spring, dashpot, hpow
   double def[3];
 };
 
@@ -103,7 +103,7 @@ __device__ void put_contact(const DomView &d, const MatView &mv, const ConOut &o
   int s1 = d.surf[master], s2 = d.surf[slave], a = s1 < s2 ? s1 : s2, b = s1 < s2 ? s2 : s1;
   const double *v = mv.def;
   for (int k = 0; k < mv.nmat; k ++)
-    if (mv.s[2*k] == a && mv.s[2*k + 1] == b) { v = mv.v + 3*k; break; }
+    if (mv.s[2*k] == a && mv.s[2*k + 1] == b) { v = mv.v + 6*k; break; }
   o.fri[i] = v[0];
   o.rest[i] = v[1];
   o.coh[i] = v[2];   // area = 1 for these contacts (goc.c:450)
@@ -466,18 +466,20 @@ int SFGS_Dom_Upload(sfgs_ctx *ctx, const SFGS_SHAPES *sh, int nmat, const SFGS_M
   CHK(upload(ctx, ctx->planes, planes.data(), planes.size()));
   CHK(upload(ctx, ctx->point, point, 3*(size_t)nc));
   std::vector<int> ms(2*(size_t)nmat);
-  std::vector<double> mvv(3*(size_t)nmat);
+  std::vector<double> mvv(6*(size_t)nmat);
   for (int k = 0; k < nmat; k ++)
   {
     ms[2*k] = std::min(mat[k].s1, mat[k].s2);
     ms[2*k + 1] = std::max(mat[k].s1, mat[k].s2);
-    mvv[3*k] = mat[k].friction; mvv[3*k + 1] = mat[k].restitution; mvv[3*k + 2] = mat[k].cohesion;
+    mvv[6*k] = mat[k].friction; mvv[6*k + 1] = mat[k].restitution; mvv[6*k + 2] = mat[k].cohesion;
+    mvv[6*k + 3] = mat[k].spring; mvv[6*k + 4] = mat[k].dashpot; mvv[6*k + 5] = mat[k].hpow;
   }
   CHK(upload(ctx, ctx->mat_s, ms.data(), ms.size()));
   CHK(upload(ctx, ctx->mat_v, mvv.data(), mvv.size()));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->nmat = nmat;
   ctx->defmat[0] = defmat->friction; ctx->defmat[1] = defmat->restitution; ctx->defmat[2] = defmat->cohesion;
+  ctx->defmat[3] = defmat->spring; ctx->defmat[4] = defmat->dashpot; ctx->defmat[5] = defmat->hpow;
   ctx->nplanes = (int)planes.size();
   ctx->rmax = rmax;
   ctx->dom = 1;

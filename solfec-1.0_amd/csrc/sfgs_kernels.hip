@@ -751,6 +751,137 @@ __global__ void k_body_update(BodyView bv, ConView cv, int nb, const int *bptr, 
 }
 
 // =====================================================================
+// penalty solver: the spring-dashpot law per contact (pes.c:33-106, 247-281)
+// =====================================================================
+
+// PENALTY_Spring_Dashpot_Contact (pes.c:33-106) for a contact that is not
+// cohesive; the operations and their order are those of the reference, with no
+// contraction into fused multiply-adds.  A singular tangential block leaves
+// R_T as carried and U as it was (pes.c:79, the return value that
+// PENALTY_Solve ignores).
+__device__ void penalty_law(int implicit, double h, double gap, double k, double c, double p, double mu,
+                            const double *W, const double *B, const double *V, double *U, double *R)
+{
+#pragma clang fp contract(off)
+  const double BN = B[2] + W[2]*R[0] + W[5]*R[1];
+  double g, s;
+  if (implicit)
+  {
+    g = gap + 0.25*h*(BN - V[2]);
+    g = g < 0.0 ? g : 0.0;
+    s = k*p*pow(-g, p - 1.0);
+    if (c < 0.0) c = 2.0*sqrt(h*s/W[8]);
+    R[2] = (k*pow(-g, p) - 0.5*c*(BN + V[2])) / (1.0 + (0.25*h*s + 0.5*c)*W[8]);
+  }
+  else
+  {
+    g = gap < 0.0 ? gap : 0.0;
+    s = k*p*pow(-g, p - 1.0);
+    if (c < 0.0) c = 2.0*sqrt(h*s/W[8]);
+    R[2] = k*pow(-g, p) - c*V[2];
+  }
+  if (R[2] < 0.0)
+  {
+    R[0] = R[1] = R[2] = 0.0;
+    U[0] = U[1] = U[2] = 0.0;
+    return;
+  }
+  const double BT0 = B[0] + W[6]*R[2], BT1 = B[1] + W[7]*R[2];
+  const double det = W[0]*W[4] - W[1]*W[3];
+  if (det == 0.0) return;
+  const double I0 = W[4]/det, I1 = -W[1]/det, I2 = -W[3]/det, I3 = W[0]/det;
+  R[0] = -I0*BT0 - I2*BT1;
+  R[1] = -I1*BT0 - I3*BT1;
+  const double len = sqrt(R[0]*R[0] + R[1]*R[1]), lim = mu*fabs(R[2]);
+  if (len > lim)
+  {
+    R[0] *= lim/len;
+    R[1] *= lim/len;
+  }
+  for (int r = 0; r < 3; r ++) U[r] = B[r] + (W[r]*R[0] + W[r + 3]*R[1] + W[r + 6]*R[2]);
+}
+
+// spring, dashpot and Hertz power of each contact: given per contact, or from
+// the material table (6 values per material: friction, restitution, cohesion,
+// spring, dashpot, hpow) by the surfaces of the contact's bodies
+struct PenMat {
+  const double *given;    // 3 per contact, or null
+  const int *surf, *mat_s;
+  const double *mat_v, *def;
+  int nmat;
+};
+
+__device__ void penalty_material(const PenMat &pm, const ConView &cv, int i, double *kcp)
+{
+  if (pm.given)
+  {
+    for (int e = 0; e < 3; e ++) kcp[e] = pm.given[3*(size_t)i + e];
+    return;
+  }
+  const int sl = cv.slave[i];
+  int s1 = pm.surf[cv.master[i]], s2 = sl >= 0 ? pm.surf[sl] : s1, a = s1 < s2 ? s1 : s2, b = s1 < s2 ? s2 : s1;
+  const double *v = pm.def;
+  for (int m = 0; m < pm.nmat; m ++)
+    if (pm.mat_s[2*m] == a && pm.mat_s[2*m + 1] == b) { v = pm.mat_v + 6*m; break; }
+  for (int e = 0; e < 3; e ++) kcp[e] = v[3 + e];
+}
+
+// PENALTY_Solve (pes.c:247-281) per contact i: W_ii, B, V as the assembly
+// computes them (ldy.c:364-397, 654-716; the penalty update forms no
+// off-diagonal blocks of contacts, ldy.c:744), then the law from the carried R.
+// R goes to the 4-per-row store and U to the SoA planes in the identity order.
+__global__ __launch_bounds__(TPB) void k_penalty(BodyView bv, ConView cv, PenMat pm, int n, int implicit,
+                                                  double step, double *Rs, double *Us)
+{
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int m = cv.master[i], s = cv.slave[i];
+  double bs[9], Hr[9], Wi[9], P[9], y[3], Vv[3], Bv[3], R[3], U[3], kcp[3];
+#pragma unroll
+  for (int k = 0; k < 9; k ++) bs[k] = cv.base[9*(size_t)i + k];
+  operator_Hr(bv.rot + 9*(size_t)m, bv.x0 + 3*(size_t)m, cv.mpnt + 3*(size_t)i, bs, Hr);
+  h_u(Hr, bs, bv.velo0 + 6*(size_t)m, Vv);
+  h_u(Hr, bs, bv.velo + 6*(size_t)m, Bv);
+#pragma unroll
+  for (int k = 0; k < 9; k ++) Wi[k] = 0.0;
+  if (couples(bv.kind, m))
+  {
+    h_im_ht(Hr, bs, Hr, bs, bv.jinv + 9*(size_t)m, bv.minv[m], P);
+#pragma unroll
+    for (int k = 0; k < 9; k ++) Wi[k] += P[k];
+  }
+  if (s >= 0)
+  {
+    operator_Hr(bv.rot + 9*(size_t)s, bv.x0 + 3*(size_t)s, cv.spnt + 3*(size_t)i, bs, Hr);
+    h_u(Hr, bs, bv.velo0 + 6*(size_t)s, y);
+    Vv[0] -= y[0]; Vv[1] -= y[1]; Vv[2] -= y[2];
+    h_u(Hr, bs, bv.velo + 6*(size_t)s, y);
+    Bv[0] -= y[0]; Bv[1] -= y[1]; Bv[2] -= y[2];
+    if (couples(bv.kind, s))
+    {
+      h_im_ht(Hr, bs, Hr, bs, bv.jinv + 9*(size_t)s, bv.minv[s], P);
+#pragma unroll
+      for (int k = 0; k < 9; k ++) Wi[k] += P[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; k ++) Wi[k] *= step;
+#pragma unroll
+  for (int k = 0; k < 3; k ++) { R[k] = cv.Rn[3*(size_t)i + k]; U[k] = cv.Un[3*(size_t)i + k]; }
+  penalty_material(pm, cv, i, kcp);
+  penalty_law(implicit, step, cv.gap[i], kcp[0], kcp[1], kcp[2], cv.fri[i], Wi, Bv, Vv, U, R);
+#pragma unroll
+  for (int k = 0; k < 3; k ++) { Rs[4*(size_t)i + k] = R[k]; Us[(size_t)k*n + i] = U[k]; }
+  Rs[4*(size_t)i + 3] = 0.0;
+}
+
+__global__ void k_identity(int n, int *perm, int *iperm)
+{
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i < n) { perm[i] = i; iperm[i] = i; }
+}
+
+// =====================================================================
 // host side
 // =====================================================================
 
@@ -912,6 +1043,32 @@ static int ensure_rows(sfgs_ctx *ctx, int n)
   return 0;
 }
 
+// body -> contact incidence (bptr, blist), ascending contact index per body (stable sort)
+static int build_incidence(sfgs_ctx *ctx)
+{
+  const int nc = ctx->nc, nb = ctx->nb;
+  hipStream_t st = ctx->stream;
+  CHK(ensure(ctx, ctx->bkeys, 2*(size_t)nc));
+  CHK(ensure(ctx, ctx->bkeys2, 2*(size_t)nc));
+  CHK(ensure(ctx, ctx->bvals, 2*(size_t)nc));
+  CHK(ensure(ctx, ctx->blist, 2*(size_t)nc));
+  CHK(ensure(ctx, ctx->bptr, (size_t)nb + 1));
+  CHK(ensure(ctx, ctx->counter, (size_t)nb + 1));
+  if (nc > 0)
+  {
+    hipLaunchKernelGGL(k_inc_keys, dim3(nblk(nc)), dim3(TPB), 0, st, nc, ctx->master.p, ctx->slave.p, ctx->kind.p,
+                       ctx->bkeys.p, ctx->bvals.p);
+    size_t bytes = 0;
+    HIPCHK(sfgs::sort_pairs_i32(nullptr, bytes, ctx->bkeys.p, ctx->bkeys2.p, ctx->bvals.p, ctx->blist.p, 2*nc, 32, st));
+    CHK(ensure(ctx, ctx->scratch, bytes));
+    HIPCHK(sfgs::sort_pairs_i32(ctx->scratch.p, bytes, ctx->bkeys.p, ctx->bkeys2.p, ctx->bvals.p, ctx->blist.p, 2*nc, 32, st));
+  }
+  HIPCHK(hipMemsetAsync(ctx->counter.p, 0, sizeof(int)*((size_t)nb + 1), st));
+  if (nc > 0) hipLaunchKernelGGL(k_inc_count, dim3(nblk(2*nc)), dim3(TPB), 0, st, 2*nc, ctx->bkeys2.p, ctx->counter.p);
+  CHK(scan_int(ctx, ctx->counter.p, ctx->bptr.p, nb + 1));
+  return 0;
+}
+
 #define COMMA ,
 #define SFGS_DISPATCH_G(G, CALL) \
   switch (G) { \
@@ -983,7 +1140,7 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   release(ctx->skeys); release(ctx->skeys2); release(ctx->svals); release(ctx->svals2); release(ctx->scratch);
   release(ctx->slice_row0); release(ctx->slice_nrows); release(ctx->slice_len); release(ctx->slice_off);
   release(ctx->rowlen); release(ctx->W); release(ctx->Wd); release(ctx->A); release(ctx->B); release(ctx->V);
-  release(ctx->rho); release(ctx->pfri); release(ctx->pcoh); release(ctx->shift); release(ctx->R); release(ctx->U);
+  release(ctx->rho); release(ctx->pfri); release(ctx->pcoh); release(ctx->shift); release(ctx->R); release(ctx->U); release(ctx->pen);
   release(ctx->Rb); release(ctx->Um); release(ctx->sep); release(ctx->wcol);
   release(ctx->W32); release(ctx->Wd32); release(ctx->B32); release(ctx->R32); release(ctx->R32b); release(ctx->U32); release(ctx->Um32);
   release(ctx->rho32); release(ctx->pfri32); release(ctx->pcoh32); release(ctx->shift32);
@@ -1091,28 +1248,10 @@ int SFGS_Locdyn_Update_Begin(sfgs_ctx *ctx, double step, int dynamic)
   if (!ctx || !(step > 0.0)) return SFGS_E_ARG;
   if (!ctx->have_bodies) { ctx->err = "no bodies"; return SFGS_E_STATE; }
   HIPCHK(hipSetDevice(ctx->device));
-  const int nc = ctx->nc, nb = ctx->nb;
+  const int nc = ctx->nc;
   hipStream_t st = ctx->stream;
 
-  // body -> contact incidence, ascending contact index per body (stable sort)
-  CHK(ensure(ctx, ctx->bkeys, 2*(size_t)nc));
-  CHK(ensure(ctx, ctx->bkeys2, 2*(size_t)nc));
-  CHK(ensure(ctx, ctx->bvals, 2*(size_t)nc));
-  CHK(ensure(ctx, ctx->blist, 2*(size_t)nc));
-  CHK(ensure(ctx, ctx->bptr, (size_t)nb + 1));
-  CHK(ensure(ctx, ctx->counter, (size_t)nb + 1));
-  if (nc > 0)
-  {
-    hipLaunchKernelGGL(k_inc_keys, dim3(nblk(nc)), dim3(TPB), 0, st, nc, ctx->master.p, ctx->slave.p, ctx->kind.p,
-                       ctx->bkeys.p, ctx->bvals.p);
-    size_t bytes = 0;
-    HIPCHK(sfgs::sort_pairs_i32(nullptr, bytes, ctx->bkeys.p, ctx->bkeys2.p, ctx->bvals.p, ctx->blist.p, 2*nc, 32, st));
-    CHK(ensure(ctx, ctx->scratch, bytes));
-    HIPCHK(sfgs::sort_pairs_i32(ctx->scratch.p, bytes, ctx->bkeys.p, ctx->bkeys2.p, ctx->bvals.p, ctx->blist.p, 2*nc, 32, st));
-  }
-  HIPCHK(hipMemsetAsync(ctx->counter.p, 0, sizeof(int)*((size_t)nb + 1), st));
-  if (nc > 0) hipLaunchKernelGGL(k_inc_count, dim3(nblk(2*nc)), dim3(TPB), 0, st, 2*nc, ctx->bkeys2.p, ctx->counter.p);
-  CHK(scan_int(ctx, ctx->counter.p, ctx->bptr.p, nb + 1));
+  CHK(build_incidence(ctx));
 
   // contact adjacency CSR (natural order)
   CHK(ensure(ctx, ctx->rowlen_n, (size_t)nc + 1));
@@ -1165,6 +1304,7 @@ int SFGS_Locdyn_Update_Begin(sfgs_ctx *ctx, double step, int dynamic)
   ctx->dynamic = dynamic;
   ctx->assembled = 1;
   ctx->from_bodies = 1;
+  ctx->diag_only = 0;
   ctx->mirror_valid = 0;
   if (singular) { ctx->err = std::to_string(singular) + " singular diagonal blocks"; return SFGS_E_SINGULAR; }
   return 0;
@@ -1224,6 +1364,7 @@ int SFGS_System_Upload(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, 
   ctx->dynamic = dynamic;
   ctx->assembled = 1;
   ctx->from_bodies = 0;
+  ctx->diag_only = 0;
   ctx->mirror_valid = 0;
   return 0;
 }
@@ -1365,6 +1506,7 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   if (par->precision != SFGS_PREC_FP64 && par->precision != SFGS_PREC_MIXED) return SFGS_E_ARG;
   if (par->precision == SFGS_PREC_MIXED && par->refine < 0) return SFGS_E_ARG;
   if (!ctx->assembled) { ctx->err = "solve before LOCDYN assembly"; return SFGS_E_STATE; }
+  if (ctx->diag_only) { ctx->err = "the store holds a penalty solve, not an assembled system"; return SFGS_E_STATE; }
   if (ctx->nscenes > 1) { ctx->err = "the system holds independent scenes: use SFGS_Batch_Solve"; return SFGS_E_STATE; }
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
@@ -1542,6 +1684,7 @@ int SFGS_Merit(sfgs_ctx *ctx, int update_U, double *merit, double *conmerit)
 {
   if (!ctx) return SFGS_E_ARG;
   if (!ctx->assembled) { ctx->err = "merit before LOCDYN assembly"; return SFGS_E_STATE; }
+  if (ctx->diag_only) { ctx->err = "the store holds a penalty solve, not an assembled system"; return SFGS_E_STATE; }
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const int n = ctx->nc, mgrid = (ctx->nslices + TPB/SLICE - 1) / (TPB/SLICE);
@@ -1586,7 +1729,7 @@ static int compute_features(sfgs_ctx *ctx, int natural)
 int SFGS_Features(sfgs_ctx *ctx, float *features)
 {
   if (!ctx || !features) return SFGS_E_ARG;
-  if (!ctx->assembled || !ctx->from_bodies) { ctx->err = "features need a system assembled from bodies"; return SFGS_E_STATE; }
+  if (!ctx->assembled || !ctx->from_bodies || ctx->diag_only) { ctx->err = "features need a system assembled from bodies"; return SFGS_E_STATE; }
   HIPCHK(hipSetDevice(ctx->device));
   CHK(compute_features(ctx, 1));
   if (ctx->nc > 0)
@@ -1598,7 +1741,7 @@ int SFGS_Features(sfgs_ctx *ctx, float *features)
 int SFGS_Warm_Start(sfgs_ctx *ctx)
 {
   if (!ctx) return SFGS_E_ARG;
-  if (!ctx->assembled || !ctx->from_bodies) { ctx->err = "warm start needs a system assembled from bodies"; return SFGS_E_STATE; }
+  if (!ctx->assembled || !ctx->from_bodies || ctx->diag_only) { ctx->err = "warm start needs a system assembled from bodies"; return SFGS_E_STATE; }
   HIPCHK(hipSetDevice(ctx->device));
   CHK(compute_features(ctx, 0));
   return sfgs_mlp_apply(ctx, ctx->feat.p, ctx->nc, ctx->R.p);
@@ -1655,6 +1798,7 @@ int SFGS_Get_Locdyn(sfgs_ctx *ctx, int *rowptr, int *col, int *via, double *W, d
 {
   if (!ctx) return SFGS_E_ARG;
   if (!ctx->assembled) { ctx->err = "no system"; return SFGS_E_STATE; }
+  if (ctx->diag_only) { ctx->err = "the store holds a penalty solve, not an assembled system"; return SFGS_E_STATE; }
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const int n = ctx->nc;
@@ -1766,6 +1910,54 @@ int SFGS_Body_Update(sfgs_ctx *ctx, double step, double *velo_out, double *force
   if (force_out) HIPCHK(hipMemcpyAsync(force_out, out.p + 6*(size_t)nb, sizeof(double)*6*(size_t)nb, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+
+// PENALTY_Solve (pes.c:247-281)
+int SFGS_Penalty_Solve(sfgs_ctx *ctx, double step, int implicit, const double *spring, const double *dashpot,
+                       const double *hpow)
+{
+  if (!ctx || !(step > 0.0)) return SFGS_E_ARG;
+  if (!ctx->have_bodies) { ctx->err = "no bodies"; return SFGS_E_STATE; }
+  const bool given = spring || dashpot || hpow;
+  if (given && !(spring && dashpot && hpow)) { ctx->err = "give spring, dashpot and hpow together, or none"; return SFGS_E_ARG; }
+  if (!given && !ctx->dom) { ctx->err = "no material table (SFGS_Dom_Upload) for the springs and dashpots"; return SFGS_E_STATE; }
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int n = ctx->nc;
+  CHK(build_incidence(ctx));
+  CHK(ensure(ctx, ctx->perm, (size_t)std::max(n, 1)));
+  CHK(ensure(ctx, ctx->iperm, (size_t)std::max(n, 1)));
+  CHK(ensure(ctx, ctx->R, 4*(size_t)std::max(n, 1)));
+  CHK(ensure(ctx, ctx->U, 3*(size_t)std::max(n, 1)));
+  if (given && n > 0)
+  {
+    std::vector<double> kcp(3*(size_t)n);
+    for (int i = 0; i < n; i ++) { kcp[3*(size_t)i] = spring[i]; kcp[3*(size_t)i + 1] = dashpot[i]; kcp[3*(size_t)i + 2] = hpow[i]; }
+    CHK(upload(ctx, ctx->pen, kcp.data(), kcp.size()));
+  }
+  if (!given) CHK(upload(ctx, ctx->pen, ctx->defmat, 6));   // the default material's six values
+  if (n > 0)
+  {
+    BodyView bv{ctx->kind.p, ctx->rot.p, ctx->x0.p, ctx->jinv.p, ctx->minv.p, ctx->velo.p, ctx->velo0.p};
+    ConView cv{ctx->master.p, ctx->slave.p, ctx->mpnt.p, ctx->spnt.p, ctx->base.p, ctx->gap.p, ctx->fri.p,
+               ctx->rest.p, ctx->coh.p, ctx->Rn.p, ctx->Un.p};
+    PenMat pm{given ? ctx->pen.p : nullptr, ctx->surf.p, ctx->mat_s.p, ctx->mat_v.p, given ? nullptr : ctx->pen.p,
+              ctx->nmat};
+    hipLaunchKernelGGL(k_identity, dim3(nblk(n)), dim3(TPB), 0, st, n, ctx->perm.p, ctx->iperm.p);
+    hipLaunchKernelGGL(k_penalty, dim3(nblk(n)), dim3(TPB), 0, st, bv, cv, pm, n, implicit ? 1 : 0, step, ctx->R.p,
+                       ctx->U.p);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  ctx->step = step;
+  ctx->assembled = 1;
+  ctx->from_bodies = 1;
+  ctx->diag_only = 1;
+  ctx->nnz = 0;
+  ctx->ncolours = 0;
+  ctx->mirror_valid = 0;
   return 0;
 }
 

@@ -19,7 +19,7 @@ from ._lib import SFGS_REPORT, f64, ptr, solfec_lib
 from .device import DIAGSOLVERS, FAILURES, GS_ERRORS, Device, SfgsError, make_param, default_diag  # noqa: F401
 
 __all__ = ["SOLFEC", "SURFACE_MATERIAL", "BULK_MATERIAL", "SPHERE", "ELLIP", "PLANE", "BODY", "GRAVITY",
-           "INITIAL_VELOCITY", "GAUSS_SEIDEL_SOLVER", "RUN", "LATTICE", "ELLIP_LATTICE", "LOCDYN_DUMP",
+           "INITIAL_VELOCITY", "GAUSS_SEIDEL_SOLVER", "PENALTY_SOLVER", "RUN", "LATTICE", "ELLIP_LATTICE", "LOCDYN_DUMP",
            "W_MATRIX_MARKET", "Device", "SfgsError"]
 
 _CALLBACK_T = C.CFUNCTYPE(C.c_int, C.c_void_p)
@@ -123,6 +123,8 @@ class SOLFEC:
         return self.lib.SOLFEC_Step_Begin(self.sol)
 
     def step_solve(self, solver):
+        if isinstance(solver, PENALTY_SOLVER):
+            return self.lib.SOLFEC_Step_Solve_Penalty(self.sol, solver.ps)
         return self.lib.SOLFEC_Step_Solve(self.sol, solver.gs)
 
     def step_end(self):
@@ -131,21 +133,35 @@ class SOLFEC:
 
 class SURFACE_MATERIAL:
     """SURFACE_MATERIAL(solfec, surf1=None, surf2=None, model='SIGNORINI_COULOMB', label=None,
-    friction=0.0, cohesion=0.0, restitution=0.0) (lng.c:2128)."""
+    friction=0.0, cohesion=0.0, restitution=0.0, spring=0.0, dashpot=0.0, hpow=1.0) (lng.c:2128-2190).
+
+    model 'SPRING_DASHPOT' contacts are solved by the PENALTY_SOLVER; a
+    GAUSS_SEIDEL_SOLVER step refuses a domain that has one.  spring, dashpot
+    (negative: critical damping) and hpow (Hertz power) enter the
+    PENALTY_SOLVER's law for every contact of the surface pair (pes.c:272)."""
+
+    MODELS = {"SIGNORINI_COULOMB": 0, "SPRING_DASHPOT": 1}
 
     def __init__(self, solfec, surf1=None, surf2=None, model="SIGNORINI_COULOMB", label=None,
-                 friction=0.0, cohesion=0.0, restitution=0.0):
-        if model != "SIGNORINI_COULOMB":
-            raise ValueError("only the SIGNORINI_COULOMB model is supported by this build")
+                 friction=0.0, cohesion=0.0, restitution=0.0, spring=0.0, dashpot=0.0, hpow=1.0):
+        if model not in self.MODELS:
+            raise ValueError("Unknown SURFACE_MATERIAL model")
+        for v, k in ((friction, "friction"), (cohesion, "cohesion"), (spring, "spring"), (hpow, "hpow")):
+            if not v >= 0:
+                raise ValueError(f"'{k}' must be non-negative")
+        if not 0 <= restitution <= 1:
+            raise ValueError("'restitution' must be in [0, 1]")
         s1 = -1 if surf1 is None else int(surf1)
         s2 = -1 if surf2 is None else int(surf2)
         if (s1 < 0) != (s2 < 0):
             raise ValueError("both surf1 and surf2 must be given")
-        rc = solfec.lib.SOLFEC_Surface_Material(solfec.sol, s1, s2, float(friction), float(restitution),
-                                                float(cohesion))
+        rc = solfec.lib.SOLFEC_Surface_Material_Ex(solfec.sol, s1, s2, self.MODELS[model], float(friction),
+                                                   float(restitution), float(cohesion), float(spring),
+                                                   float(dashpot), float(hpow))
         if rc:
             raise ValueError("invalid surface material parameters")
         self.friction, self.cohesion, self.restitution = friction, cohesion, restitution
+        self.spring, self.dashpot, self.hpow = spring, dashpot, hpow
         self.model, self.label = model, label
 
 
@@ -241,6 +257,29 @@ def LATTICE(solfec, n, L, rmin, rmax, jitter, density, seed):
     if solfec.lib.SOLFEC_Lattice_Scene(solfec.sol, int(n), float(L), float(rmin), float(rmax), float(jitter),
                                        float(density), int(seed)):
         raise ValueError("invalid lattice parameters")
+
+
+class PENALTY_SOLVER:
+    """PENALTY_SOLVER(variant='IMPLICIT') (lng.c:4659-4698): every contact is a
+    spring and a dashpot (pes.c:33-106) with the constants of its surface
+    material; variant 'IMPLICIT' or 'EXPLICIT'."""
+
+    def __init__(self, variant="IMPLICIT"):
+        if variant not in ("IMPLICIT", "EXPLICIT"):
+            raise ValueError("Invalid variant")
+        self.lib = solfec_lib()
+        self.variant = variant
+        self.ps = self.lib.PENALTY_Create(1 if variant == "IMPLICIT" else 0)
+        if not self.ps:
+            raise MemoryError("PENALTY_Create failed")
+
+    def __del__(self):
+        try:
+            if self.ps:
+                self.lib.PENALTY_Destroy(self.ps)
+                self.ps = None
+        except Exception:
+            pass
 
 
 class GAUSS_SEIDEL_SOLVER:
@@ -405,15 +444,19 @@ def W_MATRIX_MARKET(solfec, path):
 
 
 def RUN(solfec, solver, duration):
-    """RUN(solfec, solver, duration) (lng.c:8973-9025)."""
+    """RUN(solfec, solver, duration) (lng.c:8973-9025) with a GAUSS_SEIDEL_SOLVER
+    or a PENALTY_SOLVER (sol.c:471)."""
     if not isinstance(solfec, SOLFEC):
         raise TypeError("'solfec' must be a SOLFEC object")
-    if not isinstance(solver, GAUSS_SEIDEL_SOLVER):
+    if not isinstance(solver, (GAUSS_SEIDEL_SOLVER, PENALTY_SOLVER)):
         raise TypeError("'solver' must be a constraint solver object")
     if not duration > 0:
         raise ValueError("'duration' must be positive")
-    rc = solfec.lib.SOLFEC_Run(solfec.sol, solver.gs, float(duration))
+    if isinstance(solver, PENALTY_SOLVER):
+        rc = solfec.lib.SOLFEC_Run_Penalty(solfec.sol, solver.ps, float(duration))
+    else:
+        rc = solfec.lib.SOLFEC_Run(solfec.sol, solver.gs, float(duration))
     if rc:
         msg = solfec.lib.SOLFEC_Error_String(rc).decode()
         detail = solfec.lib.SOLFEC_Last_Error(solfec.sol).decode()
-        raise RuntimeError(msg + (f": {detail}" if detail and rc == 4 else ""))
+        raise RuntimeError(msg + (f": {detail}" if detail and rc in (4, 6) else ""))

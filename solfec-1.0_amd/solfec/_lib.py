@@ -57,6 +57,7 @@ _SFGS_PROTOS = {
     "SFGS_Get_Sizes": (_i, [_p, _p, _p, _p]),
     "SFGS_Colour": (_i, [_p, _i, _p, _p, _p]),
     "SFGS_Body_Update": (_i, [_p, _d, _p, _p]),
+    "SFGS_Penalty_Solve": (_i, [_p, _d, _i, _p, _p, _p]),
     "SFGS_Set_Layout": (_i, [_p, _i]),
     "SFGS_Set_Schedule": (_i, [_p, _i]),
     "SFGS_Set_Row_Order": (_i, [_p, _i]),
@@ -92,6 +93,11 @@ _SOL_PROTOS = {
     "SOLFEC_Last_Error": (_s, [_p]),
     "SOLFEC_Gravity": (None, [_p, _d, _d, _d]),
     "SOLFEC_Surface_Material": (_i, [_p, _i, _i, _d, _d, _d]),
+    "SOLFEC_Surface_Material_Ex": (_i, [_p, _i, _i, _i, _d, _d, _d, _d, _d, _d]),
+    "PENALTY_Create": (_p, [_i]),
+    "PENALTY_Destroy": (None, [_p]),
+    "SOLFEC_Step_Solve_Penalty": (_i, [_p, _p]),
+    "SOLFEC_Run_Penalty": (_i, [_p, _p, _d]),
     "SOLFEC_Sphere": (_i, [_p, _p, _d, _d, _i]),
     "SOLFEC_Plane": (_i, [_p, _p, _p, _i]),
     "SOLFEC_Set_Velocity": (_i, [_p, _i, _p]),

@@ -96,6 +96,17 @@ class Device:
     def assemble(self, step, dynamic=1):
         self._check(self.lib.SFGS_Locdyn_Update_Begin(self.ctx, step, dynamic), "SFGS_Locdyn_Update_Begin")
 
+    def penalty(self, step, implicit=1, spring=None, dashpot=None, hpow=None):
+        """SFGS_Penalty_Solve: the spring-dashpot law per uploaded contact; the
+        three constants per contact, or all None for the material table."""
+        n = self.sizes()[0]
+        if spring is None and dashpot is None and hpow is None:
+            rc = self.lib.SFGS_Penalty_Solve(self.ctx, float(step), int(implicit), None, None, None)
+        else:
+            a = [f64(spring, n), f64(dashpot, n), f64(hpow, n)]
+            rc = self.lib.SFGS_Penalty_Solve(self.ctx, float(step), int(implicit), *[ptr(x) for x in a])
+        self._check(rc, "SFGS_Penalty_Solve")
+
     def system(self, rowptr, col, W, Wd, A, B, V, rho, friction, restitution, cohesion, gap,
                free_energy, step, R, U, dynamic=1):
         n = len(rowptr) - 1

@@ -185,3 +185,36 @@ def test_ellipsoid_pair_and_plane_closed_forms():
     low = x - A @ np.array([0.0, 0.0, 1.0]) / hz
     X0 = s.bodies()["x0"].reshape(-1, 3)[2]
     np.testing.assert_allclose(c["mpnt"].reshape(n, 3)[i], X0 + (low - x), atol=1e-12)
+
+
+def test_spring_dashpot_material_and_penalty_solver_validation():
+    s = _domain()
+    with pytest.raises(ValueError):
+        solfec.SURFACE_MATERIAL(s, model="HERTZ")
+    with pytest.raises(ValueError):
+        solfec.SURFACE_MATERIAL(s, restitution=1.5)
+    with pytest.raises(ValueError):
+        solfec.SURFACE_MATERIAL(s, model="SPRING_DASHPOT", spring=-1.0)
+    with pytest.raises(ValueError):
+        solfec.SURFACE_MATERIAL(s, model="SPRING_DASHPOT", hpow=-0.5)
+    m = solfec.SURFACE_MATERIAL(s, model="SPRING_DASHPOT", spring=1e6, dashpot=-1.0, hpow=1.5)
+    assert (m.model, m.spring, m.dashpot, m.hpow) == ("SPRING_DASHPOT", 1e6, -1.0, 1.5)
+    assert solfec.PENALTY_SOLVER().variant == "IMPLICIT"
+    assert solfec.PENALTY_SOLVER("EXPLICIT").variant == "EXPLICIT"
+    with pytest.raises(ValueError):
+        solfec.PENALTY_SOLVER("SEMI")
+    with pytest.raises(TypeError):
+        solfec.RUN(s, object(), 0.001)
+
+
+def test_gauss_seidel_refuses_spring_dashpot_materials():
+    """The Gauss-Seidel sweep models SIGNORINI_COULOMB contacts only: a step is
+    refused (before any device work) when the domain has a SPRING_DASHPOT pair."""
+    s = _domain()
+    solfec.SURFACE_MATERIAL(s, 0, 1, model="SPRING_DASHPOT", spring=1e6)
+    solfec.LATTICE(s, 27, 0.3, 0.045, 0.055, 0.005, 2000.0, 1)
+    assert s.step_begin() == 0
+    assert s.step_solve(solfec.GAUSS_SEIDEL_SOLVER(1e-4, 100)) == 6
+    assert "PENALTY_SOLVER" in s.lib.SOLFEC_Last_Error(s.sol).decode()
+    with pytest.raises(RuntimeError, match="PENALTY_SOLVER"):
+        solfec.RUN(s, solfec.GAUSS_SEIDEL_SOLVER(1e-4, 100), 0.001)

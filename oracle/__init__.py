@@ -39,6 +39,16 @@ class OR_REPORT(C.Structure):
                 ("newton_iters", C.c_long), ("row_updates", C.c_long)]
 
 
+class OR_NTPARAM(C.Structure):
+    _fields_ = [("meritval", _d), ("maxiter", _i), ("linver", _i), ("linmaxiter", _i), ("maxmatvec", _i),
+                ("epsilon", _d), ("delta", _d), ("theta", _d), ("omega", _d), ("gsflag", _i), ("reldelta", _i)]
+
+
+class OR_NTREPORT(C.Structure):
+    _fields_ = [("iters", _i), ("matvecs", _i), ("converged", _i), ("gs_fallback", _i), ("gs_iters", _i),
+                ("singular", _i), ("merit", _d), ("W_norm", _d)]
+
+
 _lib = None
 
 
@@ -75,6 +85,9 @@ def lib():
             "OR_Detect": (C.c_longlong, [_i, _p, _p, _p, _p, _i, _p, _p, C.c_longlong, _p, _p, _p, _p]),
             "OR_Expmap": (None, [_p, _p]),
             "OR_Penalty_Contact": (_i, [_i, _d, _d, _d, _d, _d, _d, _p, _p, _p, _p, _p]),
+            "OR_Newton_Solve": (_i, [C.POINTER(OR_SYSTEM), C.POINTER(OR_NTPARAM), _p, _p, _p, _p, _p,
+                                     C.POINTER(OR_NTREPORT)]),
+            "OR_FlexGMRES_Dense": (_i, [_i, _p, _p, _p, _p, _i, _i, _i, _d, _p, _p]),
             "OR_Rig_Step_Begin": (None, [_i, _d, _p, _d, _d, _p, _p, _p, _p, _p, _p, _p]),
             "OR_Rig_Step_End": (None, [_i, _d, _p, _p, _p, _p, _p, _p]),
         }
@@ -302,3 +315,36 @@ def penalty_contact(implicit, step, gap, spring, dashpot, hpow, friction, W, B, 
     rc = lib().OR_Penalty_Contact(int(implicit), step, gap, spring, dashpot, hpow, friction, _ptr(_f(W)), _ptr(_f(B)),
                                   _ptr(_f(V)), _ptr(U), _ptr(R))
     return rc, R, U
+
+
+def nt_param(meritval=1e-8, maxiter=1000, linver="GMRES", linmaxiter=10, maxmatvec=10000, epsilon=0.25, delta=0.0,
+             theta=0.25, omega=1e-10, gsflag="ON", reldelta="OFF"):
+    """NEWTON_SOLVER parameters with the Python defaults of lng.c:4740-4857."""
+    return OR_NTPARAM(meritval, maxiter, {"GMRES": 0, "DIAG": 1}[linver], linmaxiter, maxmatvec, epsilon, delta,
+                      theta, omega, 1 if gsflag == "ON" else 0,
+                      {"OFF": 0, "avgWii": 1, "minWii": 2, "maxWii": 3}[reldelta])
+
+
+def newton_solve(sysobj, param, R0, U0=None, order=None):
+    """NEWTON_Solve (nts.c:1333-1379): returns rep, R, U, merhist, mvhist."""
+    n = sysobj.s.n
+    R = _f(R0).copy()
+    U = np.zeros(3 * n) if U0 is None else _f(U0).copy()
+    mer, mv = np.zeros(param.maxiter), np.zeros(param.maxiter, np.int32)
+    rep = OR_NTREPORT()
+    order = None if order is None else _n(order)
+    lib().OR_Newton_Solve(C.byref(sysobj.s), C.byref(param), _ptr(order), _ptr(R), _ptr(U), _ptr(mer), _ptr(mv),
+                          C.byref(rep))
+    return rep, R.reshape(n, 3), U.reshape(n, 3), mer[:rep.iters], mv[:rep.iters]
+
+
+def flexgmres_dense(A, b, x0=None, P=None, kdim=20, maxiter=10, miniter=1, atol=0.0):
+    """The Newton solve's flexible GMRES on a dense operator: returns iters, x, rnorm, matvecs."""
+    A = _f(np.asarray(A, float))
+    n = len(b)
+    x = np.zeros(n) if x0 is None else _f(np.asarray(x0, float)).copy()
+    rn, mv = C.c_double(0.0), C.c_int(0)
+    Pp = None if P is None else _f(np.asarray(P, float))
+    it = lib().OR_FlexGMRES_Dense(n, _ptr(A), _ptr(Pp), _ptr(_f(np.asarray(b, float))), _ptr(x), kdim, maxiter,
+                                  miniter, atol, C.byref(rn), C.byref(mv))
+    return it, x, rn.value, mv.value

@@ -12,6 +12,7 @@
  *   or_bgs.c  -- bgs.c:913-1069 (serial Gauss-Seidel), mrf.c:35-184 (merit)
  *   or_bod.c  -- bod.c:418-435, 1425-1447 (body velocity update by reactions)
  *   or_pes.c  -- pes.c:33-106 (spring-dashpot contact law of the penalty solver)
+ *   or_nts.c  -- nts.c:937-1379 and ext/krylov/flexgmres.c:82-700 (NEWTON_SOLVER, LOCDYN_ON)
  *   or_dom.c  -- goc.c:50-59, 308-348, 421-455, gjk.c:560-584 (sphere and plane
  *                contacts), dom.c:348-483 (contact update and creation, brute
  *                force), bod.c:1313-1373, 1425-1492 (rigid step), alg.h:1138-1204
@@ -162,6 +163,32 @@ void OR_Rig_Step_End(int kind, double h, const double *I, const double *force, d
  * 0, or -1 for a singular tangential block */
 int OR_Penalty_Contact(int implicit, double step, double gap, double spring, double dashpot, double hpow,
                        double friction, const double *W, const double *B, const double *V, double *U, double *R);
+
+/* NEWTON_SOLVER parameters (nts.h:31-66) and results (or_nts.c) */
+typedef struct {
+  double meritval;
+  int maxiter;
+  int linver;        /* 0 GMRES (PQN_GMRES), 1 DIAG (PQN_DIAG) */
+  int linmaxiter, maxmatvec;
+  double epsilon, delta, theta, omega;
+  int gsflag;        /* 1: Gauss-Seidel after an unconverged solve */
+  int reldelta;      /* 0 OFF, 1 avgWii, 2 minWii, 3 maxWii */
+} OR_NTPARAM;
+
+typedef struct {
+  int iters, matvecs, converged, gs_fallback, gs_iters, singular;
+  double merit, W_norm;
+} OR_NTREPORT;
+
+/* NEWTON_Solve with assembled local dynamics; order: the Gauss-Seidel row
+ * order of the fallback (NULL: natural); merhist, mvhist: maxiter entries or NULL */
+int OR_Newton_Solve(const OR_SYSTEM *sys, const OR_NTPARAM *par, const int *order, double *R, double *U,
+                    double *merhist, int *mvhist, OR_NTREPORT *rep);
+/* the flexible GMRES of the Newton solve on a dense row-major n x n operator A
+ * with a dense preconditioner P (NULL: identity); x: initial guess in, solution
+ * out; returns the iterations (-1: non-finite input) */
+int OR_FlexGMRES_Dense(int n, const double *A, const double *P, const double *b, double *x, int kdim, int maxiter,
+                       int miniter, double atol, double *rnorm, int *matvecs);
 
 #ifdef __cplusplus
 }

@@ -301,6 +301,49 @@ int SFGS_Dom_Get_Contacts(sfgs_ctx *ctx, int *master, int *slave, double *point,
                           double *base, double *gap, double *friction, double *restitution, double *cohesion,
                           double *R, double *U);
 
+/* NEWTON_SOLVER (nts.h:31-66): linear solver of the Newton step */
+#define SFGS_NT_GMRES 0   /* PQN_GMRES: flexible GMRES with a block-diagonal preconditioner */
+#define SFGS_NT_DIAG  1   /* PQN_DIAG: per-row steps with theta averaging */
+
+/* the fields of struct newton (nts.h:31-66) with the defaults of
+ * NEWTON_SOLVER, lng.c:4740-4857: meritval 1e-8, maxiter 1000, GMRES,
+ * linmaxiter 10, maxmatvec 10000, epsilon 0.25, delta 0, theta 0.25,
+ * omega 1e-10, gsflag 1, reldelta 0 */
+typedef struct {
+  double meritval;     /* merit value sufficient for termination */
+  int maxiter;         /* Newton steps bound */
+  int linver;          /* SFGS_NT_* */
+  int linmaxiter;      /* GMRES iterations bound per step */
+  int maxmatvec;       /* operator products bound (GMRES) */
+  double epsilon;      /* GMRES tolerance, relative to |b| */
+  double delta;        /* diagonal regularisation of the step operator */
+  double theta;        /* DIAG relaxation, (0, 1] */
+  double omega;        /* smoothing of the cone residual, > 0 */
+  int gsflag;          /* 1: Gauss-Seidel from the initial R after an unconverged solve */
+  int reldelta;        /* delta relative to W: 0 off, 1 mean, 2 min, 3 max of the W_ii diagonals */
+} SFGS_NEWTON_PARAM;
+
+typedef struct {
+  int iters;           /* Newton steps (ns->iters) */
+  int matvecs;         /* operator products */
+  int converged;       /* 1: merit <= meritval */
+  int gs_fallback;     /* 1: the Gauss-Seidel fallback ran */
+  int gs_iters;        /* its iterations */
+  int singular;        /* rows with a singular step block (the reference aborts there) */
+  double merit;        /* final merit */
+  double W_norm;       /* the W norm of reldelta (1 when off) */
+} SFGS_NEWTON_REPORT;
+
+/* NEWTON_Solve (nts.h:83, nts.c:1333-1379) with assembled local dynamics
+ * (locdyn 'ON') on the system of SFGS_Locdyn_Update_Begin or
+ * SFGS_System_Upload, from its current R: gmres_based_solve (nts.c:1184-1244)
+ * or diagonalized_solve (nts.c:1246-1303), then, when unconverged and gsflag
+ * is set, R reset and GAUSS_SEIDEL_SOLVER(1.0, maxiter, meritval, CONTINUE,
+ * 1e-9, 100, SEMISMOOTH_NEWTON) (nts.c:1358-1372).  merhist and mvhist take
+ * maxiter entries (may be NULL).  R and U follow from SFGS_Get_RU. */
+int SFGS_Newton_Solve(sfgs_ctx *ctx, const SFGS_NEWTON_PARAM *par, double *merhist, int *mvhist,
+                      SFGS_NEWTON_REPORT *rep);
+
 /* PENALTY_Solve (pes.c:247-281) for the uploaded bodies and contacts (or the
  * device loop's): per contact W_ii, B and V as SFGS_Locdyn_Update_Begin forms
  * them, without off-diagonal blocks (the penalty update, ldy.c:744), then the

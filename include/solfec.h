@@ -103,6 +103,28 @@ int SOLFEC_Ellipsoid_Lattice_Scene(SOLFEC *sol, int n, double L, double amin, do
 /* SOLFEC_Run (sol.c:665-825): step until time >= t0 + duration */
 int SOLFEC_Run(SOLFEC *sol, GAUSS_SEIDEL *gs, double duration);
 
+/* ---- NEWTON_SOLVER (nts.c, locdyn 'ON') ---- */
+typedef struct newton NEWTON;
+/* NEWTON_Create (nts.c:1305-1331) with the Python constructor's defaults
+ * (lng.c:4740-4857) for the other fields (see SFGS_NEWTON_PARAM) */
+NEWTON* NEWTON_Create(double meritval, int maxiter);
+void NEWTON_Destroy(NEWTON *ns);   /* nts.c:1388-1394 */
+/* members by name: meritval, maxiter, linver (SFGS_NT_*), linmaxiter,
+ * maxmatvec, epsilon, delta, theta, omega, gsflag, reldelta (0 OFF, 1 avgWii,
+ * 2 minWii, 3 maxWii), iters; "itershist" (set only) switches the per-solve
+ * iteration history on or off.  NEWTON_Set returns -1 for an invalid value */
+double NEWTON_Get(NEWTON *ns, const char *member);
+int NEWTON_Set(NEWTON *ns, const char *member, double value);
+/* merit and operator-product histories of the last solve; returns iters */
+int NEWTON_History(NEWTON *ns, double *merhist, int *mvhist);
+int NEWTON_Itershist(NEWTON *ns, int *out, int cap);
+int NEWTON_Report(NEWTON *ns, SFGS_NEWTON_REPORT *rep);
+/* LOCDYN_Update_Begin + NEWTON_Solve (sol.c:472) on the GPU + the velocity
+ * update, as SOLFEC_Step_Solve */
+int SOLFEC_Step_Solve_Newton(SOLFEC *sol, NEWTON *ns);
+/* SOLFEC_Run with a NEWTON_SOLVER */
+int SOLFEC_Run_Newton(SOLFEC *sol, NEWTON *ns, double duration);
+
 /* ---- PENALTY_SOLVER (pes.c) ---- */
 typedef struct penalty PENALTY;
 /* PENALTY_Create (pes.c:236-245): implicit 1 = variant 'IMPLICIT', 0 = 'EXPLICIT' */

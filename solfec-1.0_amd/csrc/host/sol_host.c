@@ -789,31 +789,6 @@ static int dev_step_begin(SOLFEC *sol)
   return 0;
 }
 
-static int dev_step_solve(SOLFEC *sol, GAUSS_SEIDEL *gs)
-{
-  sfgs_ctx *g = SOLFEC_Gpu(sol);
-  if (!g) return SOL_ERR_GPU;
-  double t0 = now_s();
-  int rc = SFGS_Locdyn_Update_Begin(g, sol->step, sol->dynamic);
-  double t1 = now_s();
-  sol->t_locdyn += t1 - t0;
-  sol->locdyn_current = rc == 0;
-  if (!rc && sol->capture)
-  {
-    sol->captured = sol->nc <= sol->capture_cap ? sol->nc : 0;
-    if (sol->captured) rc = SFGS_Features(g, sol->capture);
-  }
-  if (!rc && gs->warmstart) rc = SFGS_Warm_Start(g);
-  if (rc)
-  {
-    snprintf(sol->err, sizeof(sol->err), "LOCDYN on GPU failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));
-    return SOL_ERR_GPU;
-  }
-  int err = GAUSS_SEIDEL_Solve_Device(gs, g, sol->err, sizeof(sol->err));
-  sol->t_consol += now_s() - t1;
-  return err;
-}
-
 static int dev_step_end(SOLFEC *sol)
 {
   sfgs_ctx *g = SOLFEC_Gpu(sol);
@@ -938,6 +913,52 @@ static int has_spring_dashpot(SOLFEC *sol)
 }
 
 /* upload, assemble, solve and update body velocities on the GPU */
+/* the assembly of a step: in the host loop bodies and contacts go to the
+ * device first; gs (may be NULL) asks for the learned warm start */
+static int step_assemble(SOLFEC *sol, sfgs_ctx *g, GAUSS_SEIDEL *gs)
+{
+  double t0 = now_s();
+  int rc = 0;
+  if (!sol->devloop)
+  {
+    rc = SFGS_Bodies_Upload(g, sol->nb, sol->kind, sol->rot, sol->x0, sol->jinv, sol->minv, sol->velo, sol->velo0);
+    if (!rc) rc = SFGS_Contacts_Upload(g, sol->nc, sol->c_master, sol->c_slave, sol->c_mpnt, sol->c_spnt, sol->c_base,
+                                       sol->c_gap, sol->c_fri, sol->c_rest, sol->c_coh, sol->c_R, sol->c_U);
+  }
+  if (!rc) rc = SFGS_Locdyn_Update_Begin(g, sol->step, sol->dynamic);
+  sol->t_locdyn += now_s() - t0;
+  sol->locdyn_current = rc == 0;
+  if (!rc && sol->capture)
+  {
+    sol->captured = sol->nc <= sol->capture_cap ? sol->nc : 0;
+    if (sol->captured) rc = SFGS_Features(g, sol->capture);
+  }
+  if (!rc && gs && gs->warmstart) rc = SFGS_Warm_Start(g);
+  if (rc)
+  {
+    snprintf(sol->err, sizeof(sol->err), "LOCDYN on GPU failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));
+    return SOL_ERR_GPU;
+  }
+  return 0;
+}
+
+/* after the solve, in the host loop: R, U into the contact list and the
+ * velocity update (the device loop does both in SFGS_Dom_Update_End) */
+static int step_finish(SOLFEC *sol, sfgs_ctx *g)
+{
+  if (sol->devloop) return 0;
+  int rc = SFGS_Get_RU(g, sol->c_R, sol->c_U);
+  if (!rc) rc = SFGS_Body_Update(g, sol->step, sol->tmp_velo, sol->force);
+  if (rc)
+  {
+    snprintf(sol->err, sizeof(sol->err), "GPU body update failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));
+    return SOL_ERR_GPU;
+  }
+  memcpy(sol->velo, sol->tmp_velo, sizeof(double)*6*(size_t)sol->nb);
+  return 0;
+}
+
+/* upload, assemble, solve and update body velocities on the GPU */
 int SOLFEC_Step_Solve(SOLFEC *sol, GAUSS_SEIDEL *gs)
 {
   if (!sol || !gs) return SOL_ERR_ARG;
@@ -946,40 +967,48 @@ int SOLFEC_Step_Solve(SOLFEC *sol, GAUSS_SEIDEL *gs)
     snprintf(sol->err, sizeof(sol->err), "SPRING_DASHPOT materials need the PENALTY_SOLVER here");
     return SOL_ERR_ARG;
   }
-  if (sol->devloop) return dev_step_solve(sol, gs);
   sfgs_ctx *g = SOLFEC_Gpu(sol);
   if (!g) return SOL_ERR_GPU;
-  double t0 = now_s();
-  int rc = SFGS_Bodies_Upload(g, sol->nb, sol->kind, sol->rot, sol->x0, sol->jinv, sol->minv, sol->velo, sol->velo0);
-  if (!rc) rc = SFGS_Contacts_Upload(g, sol->nc, sol->c_master, sol->c_slave, sol->c_mpnt, sol->c_spnt, sol->c_base,
-                                     sol->c_gap, sol->c_fri, sol->c_rest, sol->c_coh, sol->c_R, sol->c_U);
-  if (!rc) rc = SFGS_Locdyn_Update_Begin(g, sol->step, sol->dynamic);
+  int rc = step_assemble(sol, g, gs);
+  if (rc) return rc;
   double t1 = now_s();
-  sol->t_locdyn += t1 - t0;
-  sol->locdyn_current = rc == 0;
-  if (!rc && sol->capture)
-  {
-    sol->captured = sol->nc <= sol->capture_cap ? sol->nc : 0;
-    if (sol->captured) rc = SFGS_Features(g, sol->capture);
-  }
-  if (!rc && gs->warmstart) rc = SFGS_Warm_Start(g);
-  if (rc)
-  {
-    snprintf(sol->err, sizeof(sol->err), "LOCDYN on GPU failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));
-    return SOL_ERR_GPU;
-  }
   int err = GAUSS_SEIDEL_Solve_Device(gs, g, sol->err, sizeof(sol->err));
   if (err == SOL_ERR_GPU) return err;
-  rc = SFGS_Get_RU(g, sol->c_R, sol->c_U);
-  if (!rc) rc = SFGS_Body_Update(g, sol->step, sol->tmp_velo, sol->force);
-  if (rc)
-  {
-    snprintf(sol->err, sizeof(sol->err), "GPU body update failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));
-    return SOL_ERR_GPU;
-  }
-  memcpy(sol->velo, sol->tmp_velo, sizeof(double)*6*(size_t)sol->nb);
+  rc = step_finish(sol, g);
   sol->t_consol += now_s() - t1;
-  return err;
+  return rc ? rc : err;
+}
+
+/* the same with a NEWTON_SOLVER (sol.c:472): every contact is solved as a
+ * SIGNORINI_COULOMB contact (nts.c:1072-1081) */
+int SOLFEC_Step_Solve_Newton(SOLFEC *sol, NEWTON *ns)
+{
+  if (!sol || !ns) return SOL_ERR_ARG;
+  sfgs_ctx *g = SOLFEC_Gpu(sol);
+  if (!g) return SOL_ERR_GPU;
+  int rc = step_assemble(sol, g, NULL);
+  if (rc) return rc;
+  double t1 = now_s();
+  rc = NEWTON_Solve_Device(ns, g, sol->err, sizeof(sol->err));
+  if (!rc) rc = step_finish(sol, g);
+  sol->t_consol += now_s() - t1;
+  return rc;
+}
+
+int SOLFEC_Run_Newton(SOLFEC *sol, NEWTON *ns, double duration)
+{
+  if (!sol || !ns || !(duration > 0.0)) return SOL_ERR_ARG;
+  double t0 = sol->time;
+  while (sol->time < t0 + duration - 1e-12*sol->step)
+  {
+    int rc = SOLFEC_Step_Begin(sol);
+    if (rc) return rc;
+    rc = SOLFEC_Step_Solve_Newton(sol, ns);
+    if (rc) return rc;
+    rc = SOLFEC_Step_End(sol);
+    if (rc) return rc;
+  }
+  return SOL_OK;
 }
 
 /* ---------------- LOCDYN dump formats ---------------- */

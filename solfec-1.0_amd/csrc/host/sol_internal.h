@@ -78,6 +78,17 @@ struct gauss_seidel {
   SFGS_REPORT rep;
 };
 
+struct newton {
+  SFGS_NEWTON_PARAM par;
+  int iters;
+  double *merhist;
+  int *mvhist;
+  int hist_cap;
+  int *itershist;
+  int itershistcount, itershistsize;
+  SFGS_NEWTON_REPORT rep;
+};
+
 struct penalty {
   int implicit;
   double *kcp;            /* spring, dashpot, hpow per contact (host loop) */
@@ -88,6 +99,9 @@ struct penalty {
 int sol_dev_pull(SOLFEC *sol);             /* device state into the host arrays, when behind */
 void sol_dev_edit(SOLFEC *sol);            /* before a host-side change of the state */
 void sol_ensure_contacts(SOLFEC *sol, int n);
+
+/* run the device Newton solve for the current system of g (nts_host.c) */
+int NEWTON_Solve_Device(NEWTON *ns, sfgs_ctx *g, char *err, size_t errlen);
 
 /* run the device solve for the current system of g; maps failures to SOL_ERR_* */
 int GAUSS_SEIDEL_Solve_Device(GAUSS_SEIDEL *gs, sfgs_ctx *g, char *err, size_t errlen);

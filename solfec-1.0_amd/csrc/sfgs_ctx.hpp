@@ -39,6 +39,7 @@ struct RowData;  // forward
 struct sfgs_ctx;
 void sfgs_mlp_release(sfgs_ctx *ctx);
 void sfgs_dom_release(sfgs_ctx *ctx);
+void sfgs_newton_release(sfgs_ctx *ctx);
 int sfgs_mlp_apply(sfgs_ctx *ctx, const float *feat, int n, double *R);
 
 struct sfgs_ctx {
@@ -95,6 +96,7 @@ struct sfgs_ctx {
   int have_bodies = 0, have_system = 0, assembled = 0, from_bodies = 0;
   int diag_only = 0;                     // the store holds a penalty solve (R, U per contact), no system
   sfgs::DBuf<double> pen;                // penalty solve: spring, dashpot, hpow per contact, or the default six
+  sfgs::DBuf<double> nt_mem;             // Newton solve: linearisation, Krylov vectors, partials
 
   // ---- reductions / control
   sfgs::DBuf<double> partials, mpartials, hist;

@@ -125,21 +125,6 @@ __device__ double eig3_max(const double *W)
   return fmax(a[0][0], fmax(a[1][1], a[2][2]));
 }
 
-// 3x3 inverse by LU with partial pivoting; 0 on success
-__device__ int inverse3(const double *W, double *A)
-{
-#pragma unroll
-  for (int c = 0; c < 3; c ++)
-  {
-    double M[9], x[3] = {c == 0 ? 1.0 : 0.0, c == 1 ? 1.0 : 0.0, c == 2 ? 1.0 : 0.0};
-#pragma unroll
-    for (int k = 0; k < 9; k ++) M[k] = W[k];
-    if (solve3(M, x)) return -1;
-    A[3*c] = x[0]; A[3*c + 1] = x[1]; A[3*c + 2] = x[2];
-  }
-  return 0;
-}
-
 // deterministic block sum of two values; thread 0 returns them
 __device__ __forceinline__ void block_sum2(double &a, double &b)
 {
@@ -1140,7 +1125,7 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   release(ctx->skeys); release(ctx->skeys2); release(ctx->svals); release(ctx->svals2); release(ctx->scratch);
   release(ctx->slice_row0); release(ctx->slice_nrows); release(ctx->slice_len); release(ctx->slice_off);
   release(ctx->rowlen); release(ctx->W); release(ctx->Wd); release(ctx->A); release(ctx->B); release(ctx->V);
-  release(ctx->rho); release(ctx->pfri); release(ctx->pcoh); release(ctx->shift); release(ctx->R); release(ctx->U); release(ctx->pen);
+  release(ctx->rho); release(ctx->pfri); release(ctx->pcoh); release(ctx->shift); release(ctx->R); release(ctx->U); release(ctx->pen); sfgs_newton_release(ctx);
   release(ctx->Rb); release(ctx->Um); release(ctx->sep); release(ctx->wcol);
   release(ctx->W32); release(ctx->Wd32); release(ctx->B32); release(ctx->R32); release(ctx->R32b); release(ctx->U32); release(ctx->Um32);
   release(ctx->rho32); release(ctx->pfri32); release(ctx->pcoh32); release(ctx->shift32);

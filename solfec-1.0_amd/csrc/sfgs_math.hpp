@@ -94,6 +94,23 @@ __device__ __forceinline__ int solve3(T *a, T *b)
   return 0;
 }
 
+// 3x3 inverse by LU with partial pivoting (dgetrf/dgetri as MX_Inverse calls
+// them); 0 on success
+template <typename T>
+__device__ __forceinline__ int inverse3(const T *W, T *A)
+{
+#pragma unroll
+  for (int c = 0; c < 3; c ++)
+  {
+    T M[9], x[3] = {c == 0 ? T(1) : T(0), c == 1 ? T(1) : T(0), c == 2 ? T(1) : T(0)};
+#pragma unroll
+    for (int k = 0; k < 9; k ++) M[k] = W[k];
+    if (solve3(M, x)) return -1;
+    A[3*c] = x[0]; A[3*c + 1] = x[1]; A[3*c + 2] = x[2];
+  }
+  return 0;
+}
+
 // spectral factorisation of S with respect to the cone (orientation of scf.c)
 template <typename T>
 __device__ __forceinline__ void spectral(T fri, const T *S, T &l1, T &l2, T *u1, T *u2, T &slen)
@@ -367,6 +384,59 @@ __device__ __forceinline__ int local_solve(int solver, const RowLaw<T> &L, T eps
   case 3: return projected_newton(L, eps, maxiter, W, B, U, R);
   default: return semismooth_newton(L, eps, maxiter, W, B, U, R);
   }
+}
+
+// SCF_Linearize (scf.c:28-133) with UT < 0 and smoothing omega = eps > 0, as
+// the Newton solver calls it (nts.c:1075): C(U, R), X = dC/dU, Y = dC/dR with
+// |U_T| -> sqrt(|U_T|^2 + eps^2) and |l| -> sqrt(l^2 + 4 eps^2 (mu^2)).
+__device__ __forceinline__ void scf_linearize_smooth(double fri, double coh, double shift, double eps, const double *U,
+                                                     const double *R, double *C, double *X, double *Y)
+{
+  const double eps2 = eps*eps, fri2 = fri*fri, onefri2 = 1.0 + fri2;
+  const double udash = U[2] + shift;
+  const double ulen = sqrt(U[0]*U[0] + U[1]*U[1] + eps2);
+  const double F[3] = {U[0], U[1], udash + fri*(ulen - eps)};
+  double S[3] = {R[0] - F[0], R[1] - F[1], R[2] - F[2] + coh};
+  const double sdot = S[0]*S[0] + S[1]*S[1], slen = sqrt(sdot);
+  const double l1 = -(S[2] + fri*slen)/onefri2, l2 = (slen - fri*S[2])/onefri2;
+  double u1[3], u2[3];
+  if (slen != 0.0) { u2[0] = S[0]/slen; u2[1] = S[1]/slen; }
+  else { u2[0] = 1.0; u2[1] = 0.0; }
+  u2[2] = -fri;
+  u1[0] = -fri*u2[0]; u1[1] = -fri*u2[1]; u1[2] = -1.0;
+  const double sq1 = sqrt(l1*l1 + 4.0*eps2), sq2 = sqrt(l2*l2 + 4.0*eps2*fri2);
+  const double g1 = 0.5*(sq1 + l1), g2 = 0.5*(sq2 + l2);
+  C[0] = F[0] + (g1*u1[0] + g2*u2[0]);
+  C[1] = F[1] + (g1*u1[1] + g2*u2[1]);
+  C[2] = F[2] + (g1*u1[2] + g2*u2[2]);
+  const double dF[9] = {1.0, 0.0, fri*U[0]/ulen, 0.0, 1.0, fri*U[1]/ulen, 0.0, 0.0, 1.0};
+  const double dg1 = 0.5*(1.0 + l1/sq1);
+  if (slen != 0.0)
+  {
+    const double dg2 = 0.5*(1.0 + l2/sq2);
+    const double a = 0.5*(1.0 + (l2 + fri*l1)/(sq2 + fri*sq1));
+    const double b = (fri2*dg1 + dg2)/onefri2, c = (fri*(dg1 - dg2))/onefri2, d = (dg1 + fri2*dg2)/onefri2;
+    Y[0] = a + (b - a)*S[0]*S[0]/sdot;
+    Y[1] = (b - a)*S[1]*S[0]/sdot;
+    Y[2] = c*S[0]/slen;
+    Y[3] = Y[1];
+    Y[4] = a + (b - a)*S[1]*S[1]/sdot;
+    Y[5] = c*S[1]/slen;
+    Y[6] = Y[2];
+    Y[7] = Y[5];
+    Y[8] = d;
+  }
+  else
+  {
+    Y[1] = Y[2] = Y[3] = Y[5] = Y[6] = Y[7] = 0.0;
+    Y[0] = Y[4] = Y[8] = dg1;
+  }
+  // X = dF - Y dF  ([I - dm/dS] dF/dU)
+#pragma unroll
+  for (int r = 0; r < 3; r ++)
+#pragma unroll
+    for (int c = 0; c < 3; c ++)
+      X[r + 3*c] = dF[r + 3*c] - (Y[r]*dF[3*c] + Y[r + 3]*dF[3*c + 1] + Y[r + 6]*dF[3*c + 2]);
 }
 
 } // namespace sfgs

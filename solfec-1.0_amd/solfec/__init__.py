@@ -15,11 +15,12 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import SFGS_REPORT, f64, ptr, solfec_lib
-from .device import DIAGSOLVERS, FAILURES, GS_ERRORS, Device, SfgsError, make_param, default_diag  # noqa: F401
+from ._lib import SFGS_NEWTON_REPORT, SFGS_REPORT, f64, ptr, solfec_lib
+from .device import (DIAGSOLVERS, FAILURES, GS_ERRORS, Device, SfgsError, make_param, default_diag,  # noqa: F401
+                     newton_param)
 
 __all__ = ["SOLFEC", "SURFACE_MATERIAL", "BULK_MATERIAL", "SPHERE", "ELLIP", "PLANE", "BODY", "GRAVITY",
-           "INITIAL_VELOCITY", "GAUSS_SEIDEL_SOLVER", "PENALTY_SOLVER", "RUN", "LATTICE", "ELLIP_LATTICE", "LOCDYN_DUMP",
+           "INITIAL_VELOCITY", "GAUSS_SEIDEL_SOLVER", "PENALTY_SOLVER", "NEWTON_SOLVER", "RUN", "LATTICE", "ELLIP_LATTICE", "LOCDYN_DUMP",
            "W_MATRIX_MARKET", "Device", "SfgsError"]
 
 _CALLBACK_T = C.CFUNCTYPE(C.c_int, C.c_void_p)
@@ -123,6 +124,8 @@ class SOLFEC:
         return self.lib.SOLFEC_Step_Begin(self.sol)
 
     def step_solve(self, solver):
+        if isinstance(solver, NEWTON_SOLVER):
+            return self.lib.SOLFEC_Step_Solve_Newton(self.sol, solver.ns)
         if isinstance(solver, PENALTY_SOLVER):
             return self.lib.SOLFEC_Step_Solve_Penalty(self.sol, solver.ps)
         return self.lib.SOLFEC_Step_Solve(self.sol, solver.gs)
@@ -257,6 +260,133 @@ def LATTICE(solfec, n, L, rmin, rmax, jitter, density, seed):
     if solfec.lib.SOLFEC_Lattice_Scene(solfec.sol, int(n), float(L), float(rmin), float(rmax), float(jitter),
                                        float(density), int(seed)):
         raise ValueError("invalid lattice parameters")
+
+
+class NEWTON_SOLVER:
+    """NEWTON_SOLVER(meritval=1E-8, maxiter=1000, locdyn='ON', linver='GMRES', linmaxiter=10,
+    maxmatvec=10000, epsilon=0.25, delta=0.0, theta=0.25, omega=1E-10, gsflag='ON', reldelta='OFF')
+    (lng.c:4740-4857): a projected semismooth Newton solve of all contacts at once
+    (nts.c) on the GPU; linver 'GMRES' (flexible GMRES step) or 'DIAG' (per-row
+    steps); gsflag 'ON' runs a Gauss-Seidel solve from the initial reactions when
+    the Newton solve ends above meritval.  Only locdyn 'ON' (assembled local
+    dynamics) is built.
+
+    Read-only members: merhist, mvhist, iters, itershist (set 'ON' to collect).
+    Read/write members: meritval, maxiter, locdyn, linver, linmaxiter, maxmatvec,
+    epsilon, delta, theta, omega, gsflag."""
+
+    _LINVER = {"GMRES": 0, "DIAG": 1}
+    _RELDELTA = {"OFF": 0, "avgWii": 1, "minWii": 2, "maxWii": 3}
+    _RO = ("merhist", "mvhist", "iters")
+
+    def __init__(self, meritval=1e-8, maxiter=1000, locdyn=None, linver=None, linmaxiter=10, maxmatvec=10000,
+                 epsilon=0.25, delta=0.0, theta=0.25, omega=1e-10, gsflag=None, reldelta=None):
+        for v, k in ((meritval, "meritval"), (maxiter, "maxiter"), (linmaxiter, "linmaxiter"),
+                     (maxmatvec, "maxmatvec"), (epsilon, "epsilon"), (omega, "omega")):
+            if not v > 0:
+                raise ValueError(f"'{k}' must be positive")
+        if not delta >= 0:
+            raise ValueError("'delta' must be non-negative")
+        if not 0 < theta <= 1:
+            raise ValueError("'theta' must be in (0, 1]")
+        lib = solfec_lib()
+        object.__setattr__(self, "lib", lib)
+        ns = lib.NEWTON_Create(float(meritval), int(maxiter))
+        if not ns:
+            raise ValueError("invalid NEWTON_SOLVER parameters")
+        object.__setattr__(self, "ns", ns)
+        for k, v in (("linmaxiter", linmaxiter), ("maxmatvec", maxmatvec), ("epsilon", epsilon), ("delta", delta),
+                     ("theta", theta), ("omega", omega)):
+            lib.NEWTON_Set(ns, k.encode(), float(v))
+        if locdyn is not None:
+            self.locdyn = locdyn
+        if linver is not None:
+            self.linver = linver
+        if gsflag is not None:
+            self.gsflag = gsflag
+        if reldelta is not None:
+            if reldelta not in self._RELDELTA:
+                raise ValueError("Invalid reldelta value: neither of {OFF, avgWii, minWii, maxWii} ")
+            lib.NEWTON_Set(ns, b"reldelta", float(self._RELDELTA[reldelta]))
+
+    def __del__(self):
+        try:
+            if self.ns:
+                self.lib.NEWTON_Destroy(self.ns)
+                object.__setattr__(self, "ns", None)
+        except Exception:
+            pass
+
+    def _get(self, m):
+        return self.lib.NEWTON_Get(self.ns, m.encode())
+
+    def __getattr__(self, name):
+        if name in ("ns", "lib"):
+            raise AttributeError(name)
+        lib, ns = self.lib, self.ns
+        if name in ("meritval", "epsilon", "delta", "theta", "omega"):
+            return self._get(name)
+        if name in ("maxiter", "linmaxiter", "maxmatvec", "iters"):
+            return int(self._get(name))
+        if name == "locdyn":
+            return "ON"
+        if name == "linver":
+            return "DIAG" if int(self._get("linver")) == 1 else "GMRES"
+        if name == "gsflag":
+            return "ON" if int(self._get("gsflag")) else "OFF"
+        if name in ("merhist", "mvhist"):
+            n = int(self._get("iters"))
+            a, b = np.zeros(max(n, 1)), np.zeros(max(n, 1), np.int32)
+            lib.NEWTON_History(ns, ptr(a), ptr(b))
+            return [float(x) for x in a[:n]] if name == "merhist" else [float(x) for x in b[:n]]
+        if name == "itershist":
+            n = lib.NEWTON_Itershist(ns, None, 0)
+            if n <= 0:
+                return None
+            out = np.zeros(n, np.int32)
+            lib.NEWTON_Itershist(ns, ptr(out), n)
+            return [int(x) for x in out]
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        lib, ns = self.lib, self.ns
+        if name in self._RO:
+            raise ValueError("Writing to a read-only member")
+        if name in ("meritval", "maxiter", "linmaxiter", "maxmatvec", "epsilon", "omega"):
+            if not value > 0:
+                raise ValueError(f"'{name}' must be positive")
+            lib.NEWTON_Set(ns, name.encode(), float(value))
+        elif name == "delta":
+            if not value >= 0:
+                raise ValueError("'delta' must be non-negative")
+            lib.NEWTON_Set(ns, b"delta", float(value))
+        elif name == "theta":
+            if not 0 < value <= 1:
+                raise ValueError("'theta' must be in (0, 1]")
+            lib.NEWTON_Set(ns, b"theta", float(value))
+        elif name == "locdyn":
+            if value not in ("ON", "OFF"):
+                raise ValueError("Invalid locdyn value: neither ON nor OFF")
+            if value == "OFF":
+                raise ValueError("locdyn 'OFF' (body-space Newton) is not built; use 'ON'")
+        elif name == "linver":
+            if value not in self._LINVER:
+                raise ValueError("Invalid linver value: neither GMRES nor DIAG")
+            lib.NEWTON_Set(ns, b"linver", float(self._LINVER[value]))
+        elif name == "gsflag":
+            if value not in ("ON", "OFF"):
+                raise ValueError("Invalid gsflag value: neither ON nor OFF")
+            lib.NEWTON_Set(ns, b"gsflag", 1.0 if value == "ON" else 0.0)
+        elif name == "itershist":
+            if value == "ON":
+                lib.NEWTON_Set(ns, b"itershist", 1.0)
+        else:
+            raise AttributeError(name)
+
+    def report(self):
+        rep = SFGS_NEWTON_REPORT()
+        self.lib.NEWTON_Report(self.ns, C.byref(rep))
+        return rep
 
 
 class PENALTY_SOLVER:
@@ -444,15 +574,17 @@ def W_MATRIX_MARKET(solfec, path):
 
 
 def RUN(solfec, solver, duration):
-    """RUN(solfec, solver, duration) (lng.c:8973-9025) with a GAUSS_SEIDEL_SOLVER
-    or a PENALTY_SOLVER (sol.c:471)."""
+    """RUN(solfec, solver, duration) (lng.c:8973-9025) with a GAUSS_SEIDEL_SOLVER,
+    a PENALTY_SOLVER or a NEWTON_SOLVER (sol.c:469-473)."""
     if not isinstance(solfec, SOLFEC):
         raise TypeError("'solfec' must be a SOLFEC object")
-    if not isinstance(solver, (GAUSS_SEIDEL_SOLVER, PENALTY_SOLVER)):
+    if not isinstance(solver, (GAUSS_SEIDEL_SOLVER, PENALTY_SOLVER, NEWTON_SOLVER)):
         raise TypeError("'solver' must be a constraint solver object")
     if not duration > 0:
         raise ValueError("'duration' must be positive")
-    if isinstance(solver, PENALTY_SOLVER):
+    if isinstance(solver, NEWTON_SOLVER):
+        rc = solfec.lib.SOLFEC_Run_Newton(solfec.sol, solver.ns, float(duration))
+    elif isinstance(solver, PENALTY_SOLVER):
         rc = solfec.lib.SOLFEC_Run_Penalty(solfec.sol, solver.ps, float(duration))
     else:
         rc = solfec.lib.SOLFEC_Run(solfec.sol, solver.gs, float(duration))

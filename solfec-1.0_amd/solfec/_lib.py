@@ -32,6 +32,16 @@ class SFGS_REPORT(C.Structure):
                 ("sweep_launches", _ll), ("row_updates", _ll), ("nnz", _ll)]
 
 
+class SFGS_NEWTON_PARAM(C.Structure):
+    _fields_ = [("meritval", _d), ("maxiter", _i), ("linver", _i), ("linmaxiter", _i), ("maxmatvec", _i),
+                ("epsilon", _d), ("delta", _d), ("theta", _d), ("omega", _d), ("gsflag", _i), ("reldelta", _i)]
+
+
+class SFGS_NEWTON_REPORT(C.Structure):
+    _fields_ = [("iters", _i), ("matvecs", _i), ("converged", _i), ("gs_fallback", _i), ("gs_iters", _i),
+                ("singular", _i), ("merit", _d), ("W_norm", _d)]
+
+
 class SFGS_BATCH_TUNE(C.Structure):
     _fields_ = [("lanes_per_row", _i), ("waves_per_group", _i), ("rows_per_group", _i), ("schedule", _i)]
 
@@ -58,6 +68,7 @@ _SFGS_PROTOS = {
     "SFGS_Colour": (_i, [_p, _i, _p, _p, _p]),
     "SFGS_Body_Update": (_i, [_p, _d, _p, _p]),
     "SFGS_Penalty_Solve": (_i, [_p, _d, _i, _p, _p, _p]),
+    "SFGS_Newton_Solve": (_i, [_p, C.POINTER(SFGS_NEWTON_PARAM), _p, _p, C.POINTER(SFGS_NEWTON_REPORT)]),
     "SFGS_Set_Layout": (_i, [_p, _i]),
     "SFGS_Set_Schedule": (_i, [_p, _i]),
     "SFGS_Set_Row_Order": (_i, [_p, _i]),
@@ -94,6 +105,15 @@ _SOL_PROTOS = {
     "SOLFEC_Gravity": (None, [_p, _d, _d, _d]),
     "SOLFEC_Surface_Material": (_i, [_p, _i, _i, _d, _d, _d]),
     "SOLFEC_Surface_Material_Ex": (_i, [_p, _i, _i, _i, _d, _d, _d, _d, _d, _d]),
+    "NEWTON_Create": (_p, [_d, _i]),
+    "NEWTON_Destroy": (None, [_p]),
+    "NEWTON_Get": (_d, [_p, _s]),
+    "NEWTON_Set": (_i, [_p, _s, _d]),
+    "NEWTON_History": (_i, [_p, _p, _p]),
+    "NEWTON_Itershist": (_i, [_p, _p, _i]),
+    "NEWTON_Report": (_i, [_p, C.POINTER(SFGS_NEWTON_REPORT)]),
+    "SOLFEC_Step_Solve_Newton": (_i, [_p, _p]),
+    "SOLFEC_Run_Newton": (_i, [_p, _p, _d]),
     "PENALTY_Create": (_p, [_i]),
     "PENALTY_Destroy": (None, [_p]),
     "SOLFEC_Step_Solve_Penalty": (_i, [_p, _p]),

@@ -6,7 +6,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import SFGS_BATCH_TUNE, SFGS_PARAM, SFGS_REPORT, SFGS_SCENE_REPORT, f64, i32, ptr, sfgs
+from ._lib import (SFGS_BATCH_TUNE, SFGS_NEWTON_PARAM, SFGS_NEWTON_REPORT, SFGS_PARAM, SFGS_REPORT,
+                   SFGS_SCENE_REPORT, f64, i32, ptr, sfgs)
 
 FP64, MIXED = 0, 1
 DIAGSOLVERS = {"PROJECTED_GRADIENT": 0, "DE_SAXCE_FENG": 1, "SEMISMOOTH_NEWTON": 2, "PROJECTED_NEWTON": 3}
@@ -16,6 +17,14 @@ GS_ERRORS = {0: "OK", 1: "DIVERGED", 2: "DIAGONAL_DIVERGED", 3: "DIAGONAL_FAILED
 
 class SfgsError(RuntimeError):
     pass
+
+
+def newton_param(meritval=1e-8, maxiter=1000, linver="GMRES", linmaxiter=10, maxmatvec=10000, epsilon=0.25,
+                 delta=0.0, theta=0.25, omega=1e-10, gsflag="ON", reldelta="OFF"):
+    """SFGS_NEWTON_PARAM with the defaults of NEWTON_SOLVER (lng.c:4740-4857)."""
+    return SFGS_NEWTON_PARAM(meritval, maxiter, {"GMRES": 0, "DIAG": 1}[linver], linmaxiter, maxmatvec, epsilon,
+                             delta, theta, omega, 1 if gsflag == "ON" else 0,
+                             {"OFF": 0, "avgWii": 1, "minWii": 2, "maxWii": 3}[reldelta])
 
 
 def default_diag(epsilon, maxiter, meritval=1.0):
@@ -95,6 +104,14 @@ class Device:
 
     def assemble(self, step, dynamic=1):
         self._check(self.lib.SFGS_Locdyn_Update_Begin(self.ctx, step, dynamic), "SFGS_Locdyn_Update_Begin")
+
+    def newton(self, param):
+        """SFGS_Newton_Solve: returns the report, merhist and mvhist."""
+        mer, mv = np.zeros(param.maxiter), np.zeros(param.maxiter, np.int32)
+        rep = SFGS_NEWTON_REPORT()
+        self._check(self.lib.SFGS_Newton_Solve(self.ctx, C.byref(param), ptr(mer), ptr(mv), C.byref(rep)),
+                    "SFGS_Newton_Solve")
+        return rep, mer[:rep.iters], mv[:rep.iters]
 
     def penalty(self, step, implicit=1, spring=None, dashpot=None, hpow=None):
         """SFGS_Penalty_Solve: the spring-dashpot law per uploaded contact; the

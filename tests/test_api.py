@@ -218,3 +218,25 @@ def test_gauss_seidel_refuses_spring_dashpot_materials():
     assert "PENALTY_SOLVER" in s.lib.SOLFEC_Last_Error(s.sol).decode()
     with pytest.raises(RuntimeError, match="PENALTY_SOLVER"):
         solfec.RUN(s, solfec.GAUSS_SEIDEL_SOLVER(1e-4, 100), 0.001)
+
+
+def test_newton_solver_members_and_validation():
+    ns = solfec.NEWTON_SOLVER()
+    assert (ns.meritval, ns.maxiter, ns.locdyn, ns.linver, ns.linmaxiter, ns.maxmatvec) == \
+        (1e-8, 1000, "ON", "GMRES", 10, 10000)
+    assert (ns.epsilon, ns.delta, ns.theta, ns.omega, ns.gsflag, ns.iters) == (0.25, 0.0, 0.25, 1e-10, "ON", 0)
+    assert ns.merhist == [] and ns.mvhist == [] and ns.itershist is None
+    ns.linver = "DIAG"
+    ns.theta = 1.0
+    ns.gsflag = "OFF"
+    assert (ns.linver, ns.theta, ns.gsflag) == ("DIAG", 1.0, "OFF")
+    for k, v in (("linver", "CG"), ("theta", 0.0), ("theta", 1.5), ("delta", -1.0), ("maxiter", 0),
+                 ("locdyn", "MAYBE"), ("locdyn", "OFF"), ("gsflag", "YES")):
+        with pytest.raises(ValueError):
+            setattr(ns, k, v)
+    with pytest.raises(ValueError):
+        ns.iters = 3
+    for kw in (dict(meritval=0.0), dict(theta=2.0), dict(linver="LU"), dict(reldelta="meanWii"), dict(omega=0.0)):
+        with pytest.raises(ValueError):
+            solfec.NEWTON_SOLVER(**kw)
+    assert solfec.NEWTON_SOLVER(reldelta="maxWii", linver="DIAG").linver == "DIAG"

@@ -1,0 +1,145 @@
+"""NEWTON_SOLVER on the GPU (SFGS_Newton_Solve) against the oracle's restatement
+of nts.c and ext/krylov/flexgmres.c on the same assembled systems: a fixed
+number of Newton steps must give the same reactions, merit and product
+histories (the step is FP64 throughout; the device sums dot products per block
+and W x in slice order, the oracle sequentially, so iterates agree to rounding:
+1e-8 relative to max|R| for GMRES steps, 1e-10 for DIAG steps); a converged
+solve satisfies the contact law; the Gauss-Seidel fallback matches the oracle's;
+RUN with a NEWTON_SOLVER gives the same bits in the host and the device loop
+and carries a resting sphere's weight."""
+import numpy as np
+import pytest
+
+import oracle
+import solfec
+from scenes import lattice
+from test_oracle_scf import residual
+
+pytestmark = pytest.mark.gpu
+
+H = 1e-3
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = solfec.Device(0)
+    yield d
+    d.close()
+
+
+def upload_system(dev, O, R0):
+    n = O["n"]
+    dev.system(O["rowptr"], O["col"], O["W"], O["Wd"], O["A"], O["B"], O["V"], O["rho"], O["mu"], O["rest"],
+               O["coh"], O["gap"], O["free_energy"], O["step"], R0, np.zeros(3 * n))
+
+
+def assembled(n, seed):
+    s = lattice(n, seed=seed)
+    b, c = s.bodies(), s.contacts()
+    O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), H)
+    return O, oracle.System(O), b, c
+
+
+@pytest.mark.parametrize("linver,steps,rtol", [("GMRES", 4, 1e-8), ("DIAG", 12, 1e-10)])
+@pytest.mark.parametrize("n,seed", [(216, 3), (1000, 1)])
+def test_newton_steps_match_oracle(dev, n, seed, linver, steps, rtol):
+    print(f"[newton] n={n} seed={seed} {linver} {steps} steps", flush=True)
+    O, S, b, c = assembled(n, seed)
+    R0 = np.random.default_rng(seed).normal(scale=1e-2, size=3 * O["n"])
+    kw = dict(linver=linver, maxiter=steps, meritval=1e-30, gsflag="OFF")
+    upload_system(dev, O, R0)
+    rep, mer, mv = dev.newton(solfec.newton_param(**kw))
+    Rg, Ug = dev.get_RU()
+    orep, Ro, Uo, omer, omv = oracle.newton_solve(S, oracle.nt_param(**kw), R0)
+    dR = np.abs(Rg - Ro).max() / np.abs(Ro).max()
+    print(f"[newton] merit {rep.merit:.3e} / {orep.merit:.3e}, products {rep.matvecs} / {orep.matvecs},"
+          f" max |dR|/max|R| {dR:.2e}", flush=True)
+    assert rep.iters == orep.iters == steps and not rep.gs_fallback and rep.singular == 0
+    np.testing.assert_array_equal(mv, omv)
+    np.testing.assert_allclose(mer, omer, rtol=1e3 * rtol)
+    assert dR <= rtol
+    assert np.abs(Ug - Uo).max() <= rtol * 1e3 * np.abs(Uo).max()
+
+
+@pytest.mark.parametrize("linver", ["GMRES", "DIAG"])
+def test_newton_converges_to_the_contact_law(dev, linver):
+    O, S, b, c = assembled(1000, 1)
+    n = O["n"]
+    meritval = 1e-10 if linver == "GMRES" else 1e-8
+    upload_system(dev, O, np.zeros(3 * n))
+    rep, mer, mv = dev.newton(solfec.newton_param(linver=linver, maxiter=2000, meritval=meritval, gsflag="OFF"))
+    R, U = dev.get_RU()
+    print(f"[newton] {linver}: {rep.iters} steps, {rep.matvecs} products, merit {rep.merit:.3e}", flush=True)
+    assert rep.converged and rep.merit <= meritval and mer[-1] == rep.merit
+    Cv = np.array([residual(U[i], R[i], O["mu"][i]) for i in range(n)])
+    assert np.abs(Cv).max() <= 1e-4 * max(np.abs(R).max(), np.abs(U).max())
+    m, Um, _ = oracle.merit(S, R.ravel(), U.ravel(), 1)
+    np.testing.assert_allclose(Um, U, rtol=0, atol=1e-10 * np.abs(U).max())
+
+
+def test_newton_fallback_matches_oracle(dev):
+    """Three Newton steps above meritval, then R reset and the Gauss-Seidel
+    fallback (three sweeps here: maxiter of the Newton solver).  Three sweeps
+    from a random R0 amplify rounding: on the CPU a 1e-12 relative change of R0
+    moves the oracle's own R by 1.5e-6 relative, so R is compared at 1e-3 and
+    the merit, a smooth function of R, at 1e-6."""
+    O, S, b, c = assembled(512, 6)
+    n = O["n"]
+    R0 = np.random.default_rng(2).normal(scale=1e-2, size=3 * n)
+    kw = dict(maxiter=3, meritval=1e-12)
+    upload_system(dev, O, R0)
+    rep, mer, mv = dev.newton(solfec.newton_param(**kw))
+    Rg, Ug = dev.get_RU()
+    colours = dev.locdyn()["colour"]
+    order, _ = oracle.colour_order(colours, int(colours.max()) + 1)
+    orep, Ro, Uo, omer, omv = oracle.newton_solve(S, oracle.nt_param(**kw), R0, order=order)
+    print(f"[newton] fallback: {rep.gs_iters} / {orep.gs_iters} sweeps, merit {rep.merit:.3e} / {orep.merit:.3e}",
+          flush=True)
+    assert rep.gs_fallback == orep.gs_fallback == 1 and rep.gs_iters == orep.gs_iters == 3
+    assert np.abs(Rg - Ro).max() <= 1e-3 * np.abs(Ro).max()
+    assert rep.merit == pytest.approx(orep.merit, rel=1e-6)
+
+
+def newton_scene(seed):
+    s = solfec.SOLFEC("DYNAMIC", H, "/tmp/solfec_test")
+    solfec.SURFACE_MATERIAL(s, friction=0.3)
+    solfec.GRAVITY(s, (0.0, 0.0, -9.81))
+    solfec.LATTICE(s, 343, 0.7, 0.045, 0.055, 0.005, 2000.0, seed)
+    return s
+
+
+def test_newton_run_device_loop_equals_host_loop():
+    ns = solfec.NEWTON_SOLVER(meritval=1e-8, maxiter=200)
+    ns.itershist = "ON"
+    host, dvc = newton_scene(5), newton_scene(5)
+    dvc.device_loop(True)
+    solfec.RUN(host, ns, 0.01)
+    solfec.RUN(dvc, ns, 0.01)
+    bh, bd = host.bodies(), dvc.bodies()
+    for k in ("x", "rot", "velo"):
+        assert np.array_equal(bh[k], bd[k]), k
+    ch, cd = host.contacts(), dvc.contacts()
+    assert len(ch["master"]) == len(cd["master"]) > 100
+    for k in ("R", "U"):
+        assert np.array_equal(ch[k], cd[k]), k
+    hist = ns.itershist
+    print(f"[newton] RUN: iterations per step {hist}", flush=True)
+    assert len(hist) == 20 and hist[:10] == hist[10:]
+
+
+@pytest.mark.parametrize("linver", ["GMRES", "DIAG"])
+def test_sphere_resting_on_a_plane(linver):
+    s = solfec.SOLFEC("DYNAMIC", H, "/tmp/solfec_test")
+    solfec.SURFACE_MATERIAL(s, friction=0.5)
+    solfec.GRAVITY(s, (0.0, 0.0, -9.81))
+    mat = solfec.BULK_MATERIAL(s, density=2000.0)
+    r = 0.05
+    solfec.BODY(s, "OBSTACLE", solfec.PLANE((0, 0, 0), (0, 0, 1), 0), mat)
+    solfec.BODY(s, "RIGID", solfec.SPHERE((0.3, 0.2, r), r, 1, 1), mat)
+    ns = solfec.NEWTON_SOLVER(meritval=1e-14, maxiter=100, linver=linver, theta=1.0)
+    solfec.RUN(s, ns, 0.02)
+    b, c = s.bodies(), s.contacts()
+    m = 2000.0 * 4.0 / 3.0 * np.pi * r ** 3
+    np.testing.assert_allclose(b["velo"][6:12], 0.0, atol=1e-9)
+    np.testing.assert_allclose(c["R"], [0.0, 0.0, m * 9.81], rtol=1e-6, atol=1e-9)
+    assert ns.report().gs_fallback == 0

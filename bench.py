@@ -118,6 +118,82 @@ def bench_run(args):
     print(json.dumps(line), flush=True)
 
 
+def settled_state(args):
+    """Bodies and contacts of the settled packing: settled here, or read from
+    --snapshot; written to --save-snapshot when asked."""
+    if args.snapshot:
+        z = np.load(args.snapshot)
+        b = {k[2:]: z[k] for k in z.files if k.startswith("b_")}
+        c = {k[2:]: z[k] for k in z.files if k.startswith("c_")}
+        return b, c
+    s = settle(args.spheres, args.box, args.settle, args.seed, args.friction, args.shape,
+               device_loop=not args.host_loop)
+    b, c = s.bodies(), s.contacts()
+    if args.save_snapshot:
+        np.savez(args.save_snapshot, **{"b_" + k: v for k, v in b.items()}, **{"c_" + k: v for k, v in c.items()})
+    return b, c
+
+
+def bench_consumers(args):
+    """The other consumers of the local dynamics on the settled packing of the
+    Gauss-Seidel workload: NEWTON_SOLVER (GMRES and DIAG steps) to merit 1e-6
+    against the FP64 Gauss-Seidel solve of the same system, and the
+    PENALTY_SOLVER's per-contact law (one fused kernel per call)."""
+    import solfec
+    h = 1e-3
+    b, c = settled_state(args)
+    dev = solfec.Device(0)
+    dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
+
+    def fresh(assemble=True):
+        dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
+                     c["restitution"], c["cohesion"], c["R"], c["U"])
+        if assemble:
+            dev.assemble(h)
+
+    out = {}
+    fresh()
+    n, nnz, ncol = dev.sizes()
+    t0 = time.perf_counter()
+    rep, _, _ = dev.solve(solfec.make_param(1e-6, args.maxiter, meritval=1e-6), history=False)
+    dt = time.perf_counter() - t0
+    out["gauss_seidel_fp64"] = dict(ms=1e3 * dt, iters=rep.iters, merit=rep.merit, converged=bool(rep.merit <= 1e-6))
+    log(f"[consumers] GS FP64: {1e3 * dt:.1f} ms, {rep.iters} iterations, merit {rep.merit:.3e}")
+    for lv, maxiter in (("GMRES", args.nt_maxiter), ("DIAG", 4 * args.nt_maxiter)):
+        fresh()
+        t0 = time.perf_counter()
+        rep, mer, mv = dev.newton(solfec.newton_param(meritval=1e-6, maxiter=maxiter, linver=lv, gsflag="OFF"))
+        dt = time.perf_counter() - t0
+        out["newton_" + lv.lower()] = dict(ms=1e3 * dt, steps=rep.iters, products=rep.matvecs, merit=rep.merit,
+                                           converged=bool(rep.converged),
+                                           merit_every_10=[float(x) for x in mer[9::10][:30]])
+        log(f"[consumers] NEWTON {lv}: {1e3 * dt:.1f} ms, {rep.iters} steps, {rep.matvecs} products, "
+            f"merit {rep.merit:.3e}")
+    # penalty: a linear spring with critical damping on every contact
+    fresh(assemble=False)
+    k = np.full(n, 1e6)
+    dev.penalty(h, 1, k, np.full(n, -1.0), np.ones(n))
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dev.penalty(h, 1, k, np.full(n, -1.0), np.ones(n))
+    dt = (time.perf_counter() - t0) / reps
+    R, _ = dev.get_RU()
+    out["penalty_implicit"] = dict(ms_per_call=1e3 * dt, contacts_per_s=n / dt, active=int((R[:, 2] > 0).sum()),
+                                   bytes_per_contact=248, bytes_per_body=276)
+    log(f"[consumers] PENALTY: {1e3 * dt:.3f} ms per call ({n} contacts, {(R[:, 2] > 0).sum()} compressed)")
+    head = min(out["newton_gmres"]["ms"], out["newton_diag"]["ms"])
+    line = {"metric": "NEWTON_SOLVER ms to merit<=1e-6 and PENALTY_SOLVER calls on the settled packing",
+            "value": head, "unit": "ms", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": head,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"{args.spheres}-{args.shape} packing settled {args.settle} steps: NEWTON_SOLVER "
+                                   "(GMRES, DIAG; gsflag OFF) against GAUSS_SEIDEL FP64 to merit 1e-6, and "
+                                   "PENALTY_SOLVER IMPLICIT (k 1e6, critical damping)",
+                       "contacts": n, "offdiag_blocks": nnz, "colours": ncol, **out}}
+    print(json.dumps(line), flush=True)
+    dev.close()
+
+
 def cpu_baseline(b, c, colours, n, seconds, maxiter):
     """The oracle (C restatement, 1 core) on the same system: GS iterations of
     sweep + merit, as the GPU solve does, for about `seconds` of CPU time.  The
@@ -369,21 +445,22 @@ def main():
                     help="only run 3 device copies of MIB MiB (scale of the traffic counters) and exit")
     ap.add_argument("--shape", default="sphere", choices=["sphere", "ellipsoid"],
                     help="packing bodies: spheres (config 2) or rigid ellipsoids (the config-3 variant)")
-    ap.add_argument("--workload", default="packing", choices=["packing", "batch", "run"],
+    ap.add_argument("--workload", default="packing", choices=["packing", "batch", "run", "consumers"],
                     help="packing: the 200k-sphere settled packing (headline); batch: independent scenes; "
                          "run: whole time steps, device loop against host loop")
     ap.add_argument("--run-steps", type=int, default=100, help="config-2 settle steps timed by --workload run")
     ap.add_argument("--run-configs", default="config1,config2_settle", help="cases of --workload run")
     ap.add_argument("--host-loop", action="store_true", help="settle with the host loop instead of the device loop")
+    ap.add_argument("--nt-maxiter", type=int, default=500, help="Newton steps bound of --workload consumers (DIAG: 4x)")
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--ws-new-only", type=int, default=1, help="learned warm start only on new contacts")
     ap.add_argument("--ws-scale", type=float, default=0.5, help="trust factor of the learned warm start")
     args = ap.parse_args()
-    if args.workload in ("batch", "run"):
+    if args.workload in ("batch", "run", "consumers"):
         if args.gpus != 1:
             print("bench.py measures one GPU", file=sys.stderr)
             sys.exit(2)
-        (bench_batch if args.workload == "batch" else bench_run)(args)
+        {"batch": bench_batch, "run": bench_run, "consumers": bench_consumers}[args.workload](args)
         return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -395,16 +472,7 @@ def main():
 
     import solfec
     h = 1e-3
-    if args.snapshot:
-        z = np.load(args.snapshot)
-        b = {k[2:]: z[k] for k in z.files if k.startswith("b_")}
-        c = {k[2:]: z[k] for k in z.files if k.startswith("c_")}
-    else:
-        s = settle(args.spheres, args.box, args.settle, args.seed, args.friction, args.shape,
-                   device_loop=not args.host_loop)
-        b, c = s.bodies(), s.contacts()
-        if args.save_snapshot:
-            np.savez(args.save_snapshot, **{"b_" + k: v for k, v in b.items()}, **{"c_" + k: v for k, v in c.items()})
+    b, c = settled_state(args)
     nb = len(b["kind"])
     dev = solfec.Device(0)
     dev.set_layout(args.lanes)

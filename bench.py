@@ -438,9 +438,9 @@ def main():
     ap.add_argument("--maxiter", type=int, default=5000, help="iteration bound (5000 is the workload's)")
     ap.add_argument("--row-order", type=int, default=2, choices=[0, 1, 2],
                     help="rows inside a colour: 0 by decreasing length, 1 by contact index, 2 by body block")
-    ap.add_argument("--schedule", type=int, default=0, choices=[0, 1, 2],
+    ap.add_argument("--schedule", type=int, default=0, choices=[0, 1, 2, 3],
                     help="sweep schedule: 0 one launch per colour (XCD-aware), 1 persistent dataflow, "
-                         "2 one launch per colour (hardware order)")
+                         "2 one launch per colour (hardware order), 3 persistent with a barrier per colour")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (scale of the traffic counters) and exit")
     ap.add_argument("--shape", default="sphere", choices=["sphere", "ellipsoid"],
@@ -550,7 +550,7 @@ def main():
     bytes_per_sweep = sz * (9 * nnz + 26 * n) + 4 * nnz + 4 * n
     sweeps_run = prep.row_updates / max(n, 1)
     achieved = bytes_per_sweep * sweeps_run / (prep.sweep_ms * 1e-3) / 1e9
-    kname = ("k_flow<%s>" if args.schedule == 1 else "k_sweep<%s>") % ("float" if is32 else "double")
+    kname = ({1: "k_flow<%s>", 3: "k_bar<%s>"}.get(args.schedule, "k_sweep<%s>")) % ("float" if is32 else "double")
     roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, args),
                 kernel=kname,
@@ -591,7 +591,8 @@ def main():
             "precision": "mixed" if is32 else "fp64",
             "warm_start": "learned (new contacts, trust %.2g)" % args.ws_scale if head == "learned" else "previous-step R",
             "lanes_per_row": args.lanes,
-            "schedule": ["per-colour launches, XCD-aware", "dataflow", "per-colour launches"][args.schedule],
+            "schedule": ["per-colour launches, XCD-aware", "dataflow", "per-colour launches",
+                         "persistent, barrier per colour"][args.schedule],
             "row_order": ["decreasing row length", "contact index", "body block, row length"][args.row_order],
             "modes": modes,
             "speedup_vs_plain_fp64": modes["fp64"]["ms_per_step"] / ms_per_step if "fp64" in modes else None,

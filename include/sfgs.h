@@ -148,7 +148,9 @@ int SFGS_Set_Layout(sfgs_ctx *ctx, int lanes_per_row);
  * persistent dataflow launch in which each row waits only for its own
  * neighbours, with the same iterates (forward sweeps; a solve with reverse = 1,
  * or with 4 or 8 lanes per row, uses schedule 0); 2 = one launch per colour in
- * the hardware's workgroup order. */
+ * the hardware's workgroup order; 3 = one persistent launch whose workgroups
+ * meet at a device-wide barrier after every colour, with the same iterates
+ * (1 or 2 lanes per row; otherwise schedule 0). */
 int SFGS_Set_Schedule(sfgs_ctx *ctx, int schedule);
 
 /* Order of the rows inside a colour (no counterpart in the reference): 0 =

@@ -40,6 +40,7 @@ struct sfgs_ctx;
 void sfgs_mlp_release(sfgs_ctx *ctx);
 void sfgs_dom_release(sfgs_ctx *ctx);
 void sfgs_newton_release(sfgs_ctx *ctx);
+void sfgs_bar_release(sfgs_ctx *ctx);
 int sfgs_mlp_apply(sfgs_ctx *ctx, const float *feat, int n, double *R);
 
 struct sfgs_ctx {
@@ -82,7 +83,8 @@ struct sfgs_ctx {
   sfgs::DBuf<int> rowlen;                // per permuted row
   sfgs::DBuf<double> W, Wd, A, B, V, rho, pfri, pcoh, shift, R, U, Rb, Um;
   int G = 2;                             // lanes per row of the sweep layout (1, 2, 4, 8)
-  int schedule = 0;                      // sweeps: 0 one launch per colour, 1 persistent dataflow
+  int schedule = 0;                      // sweeps: 0 one launch per colour (XCD-aware), 1 persistent dataflow,
+                                         // 2 one launch per colour (hardware order), 3 persistent with barriers
   int row_order = 2;                     // inside a colour: 0 by decreasing row length, 1 by contact index,
                                          // 2 by master body block then decreasing row length
   sfgs::DBuf<unsigned char> flow_mem;    // dataflow schedule: third R buffer, U pairs, versions, partials
@@ -97,6 +99,7 @@ struct sfgs_ctx {
   int diag_only = 0;                     // the store holds a penalty solve (R, U per contact), no system
   sfgs::DBuf<double> pen;                // penalty solve: spring, dashpot, hpow per contact, or the default six
   sfgs::DBuf<double> nt_mem;             // Newton solve: linearisation, Krylov vectors, partials
+  sfgs::DBuf<unsigned char> bar_mem;     // barrier schedule: colour tables, partials, counter
 
   // ---- reductions / control
   sfgs::DBuf<double> partials, mpartials, hist;

@@ -85,6 +85,14 @@ int run_flow(sfgs_ctx *ctx, const SweepArgs<double> &sa, double *X0, double *X1,
 int run_flow(sfgs_ctx *ctx, const SweepArgs<float> &sa, float *X0, float *X1, int maxiter, int nomerit,
              double eps, double meritval, double *hist, Ctrl &hc, float *ms);
 
+// the sweeps of SFGS_Solve as one persistent launch with a device-wide barrier
+// per colour (sfgs_bar.hip); the final state is named by hc.final_buf / final_U
+// as after the per-colour launches
+int run_bar(sfgs_ctx *ctx, const SweepArgs<double> &sa, double *X0, double *X1, int maxiter, int nomerit,
+            double eps, double meritval, int reverse, double *hist, Ctrl &hc, float *ms);
+int run_bar(sfgs_ctx *ctx, const SweepArgs<float> &sa, float *X0, float *X1, int maxiter, int nomerit,
+            double eps, double meritval, int reverse, double *hist, Ctrl &hc, float *ms);
+
 // FP32 copies of the sweep data (sfgs_kernels.hip)
 int build_f32_mirror(sfgs_ctx *ctx);
 // re-run the assembly of the current bodies and contacts (after a layout change)

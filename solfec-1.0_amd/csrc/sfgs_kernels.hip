@@ -1125,7 +1125,7 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   release(ctx->skeys); release(ctx->skeys2); release(ctx->svals); release(ctx->svals2); release(ctx->scratch);
   release(ctx->slice_row0); release(ctx->slice_nrows); release(ctx->slice_len); release(ctx->slice_off);
   release(ctx->rowlen); release(ctx->W); release(ctx->Wd); release(ctx->A); release(ctx->B); release(ctx->V);
-  release(ctx->rho); release(ctx->pfri); release(ctx->pcoh); release(ctx->shift); release(ctx->R); release(ctx->U); release(ctx->pen); sfgs_newton_release(ctx);
+  release(ctx->rho); release(ctx->pfri); release(ctx->pcoh); release(ctx->shift); release(ctx->R); release(ctx->U); release(ctx->pen); sfgs_newton_release(ctx); sfgs_bar_release(ctx);
   release(ctx->Rb); release(ctx->Um); release(ctx->sep); release(ctx->wcol);
   release(ctx->W32); release(ctx->Wd32); release(ctx->B32); release(ctx->R32); release(ctx->R32b); release(ctx->U32); release(ctx->Um32);
   release(ctx->rho32); release(ctx->pfri32); release(ctx->pcoh32); release(ctx->shift32);
@@ -1501,6 +1501,8 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   const int refine = mixed ? par->refine : 0;
   // the dataflow schedule runs forward sweeps only
   const bool flow = ctx->schedule == 1 && !par->reverse && n > 0 && ctx->G_built <= 2;
+  // the barrier schedule: one persistent launch, a device-wide barrier per colour
+  const bool bar = ctx->schedule == 3 && n > 0 && ctx->G_built <= 2;
 
   GsPlan pl;
   pl.grid.assign((size_t)ncol, 0);
@@ -1549,6 +1551,14 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
       pl.sweep_ms += fms;
       pl.launches += 1;
     }
+    else if (bar)
+    {
+      float bms = 0.0f;
+      CHK(run_bar(ctx, sa, ctx->R.p, ctx->Rb.p, maxiter, nomerit, par->epsilon, par->meritval, par->reverse,
+                  ctx->hist.p, hc, &bms));
+      pl.sweep_ms += bms;
+      pl.launches += 1;
+    }
     else
       CHK(run_gs<double>(ctx, sa, ctx->R.p, ctx->Rb.p, maxiter, nomerit, par->epsilon, par->meritval, par->reverse,
                          ctx->hist.p, pl, hc));
@@ -1576,6 +1586,14 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
       CHK(run_flow(ctx, sa, ctx->R32.p, ctx->R32b.p, maxiter, nomerit, par->epsilon, par->meritval, ctx->hist.p, hc,
                    &fms));
       pl.sweep_ms += fms;
+      pl.launches += 1;
+    }
+    else if (bar)
+    {
+      float bms = 0.0f;
+      CHK(run_bar(ctx, sa, ctx->R32.p, ctx->R32b.p, maxiter, nomerit, par->epsilon, par->meritval, par->reverse,
+                  ctx->hist.p, hc, &bms));
+      pl.sweep_ms += bms;
       pl.launches += 1;
     }
     else
@@ -1734,7 +1752,7 @@ int SFGS_Warm_Start(sfgs_ctx *ctx)
 
 int SFGS_Set_Schedule(sfgs_ctx *ctx, int schedule)
 {
-  if (!ctx || schedule < 0 || schedule > 2) return SFGS_E_ARG;
+  if (!ctx || schedule < 0 || schedule > 3) return SFGS_E_ARG;
   ctx->schedule = schedule;
   return 0;
 }

@@ -1,5 +1,6 @@
-"""The dataflow schedule of the sweeps (one persistent launch, rows wait for
-their own neighbours) against the per-colour schedule on the same system:
+"""The persistent schedules of the sweeps -- dataflow (rows wait for their own
+neighbours) and barrier (a device-wide barrier per colour) -- against the
+per-colour schedule on the same system:
 every row reads the values the serial colour-major sweep reads, so the
 reactions and velocities are bit-identical whenever both stop after the same
 number of iterations (the error and merit sums are formed in a different order,
@@ -34,17 +35,19 @@ def _system(dev, O, R0):
     (1000, 3, 2, dict(epsilon=1e-6, maxiter=3000, meritval=1e-6)),
     (512, 4, 1, dict(epsilon=1e-5, maxiter=50, meritval=1e-12)),       # stops at maxiter
     (729, 5, 1, dict(epsilon=1e-6, maxiter=3000, meritval=1e-8, precision=1, refine=2)),
+    (1000, 6, 2, dict(epsilon=1e-6, maxiter=3000, meritval=1e-6, precision=1, refine=2)),
 ])
-def test_flow_equals_colour_schedule(dev, n, seed, lanes, kw):
-    print(f"[flow] n={n} seed={seed} lanes={lanes} {kw}", flush=True)
+@pytest.mark.parametrize("sched", [1, 3])
+def test_flow_equals_colour_schedule(dev, n, seed, lanes, kw, sched):
+    print(f"[flow] schedule {sched} n={n} seed={seed} lanes={lanes} {kw}", flush=True)
     s = lattice(n, seed=seed)
     b, c = s.bodies(), s.contacts()
     O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), 1e-3)
     R0 = np.random.default_rng(seed).normal(scale=1e-3, size=3 * O["n"])
     dev.set_layout(lanes)
     out = []
-    for sched in (0, 1):
-        dev.set_schedule(sched)
+    for sc in (0, sched):
+        dev.set_schedule(sc)
         _system(dev, O, R0)
         rep, rer, mer = dev.solve(solfec.make_param(**kw))
         R, U = dev.get_RU()
@@ -63,6 +66,30 @@ def test_flow_equals_colour_schedule(dev, n, seed, lanes, kw):
         np.testing.assert_allclose(mer1, mer0, rtol=1e-10, atol=0)
     scale = np.abs(R0g).max()
     assert np.abs(R1g - R0g).max() <= 1e-6 * scale
+
+
+@pytest.mark.parametrize("kw", [dict(epsilon=1e-6, maxiter=3000, meritval=1e-6, reverse=1),
+                                dict(epsilon=1e-6, maxiter=3000, failure=1)])
+def test_barrier_schedule_reverse_and_exit(dev, kw):
+    """The barrier schedule also runs alternating sweeps and FAILURE_EXIT."""
+    s = lattice(1000, seed=7)
+    b, c = s.bodies(), s.contacts()
+    O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), 1e-3)
+    R0 = np.random.default_rng(7).normal(scale=1e-3, size=3 * O["n"])
+    dev.set_layout(2)
+    out = []
+    for sc in (0, 3):
+        dev.set_schedule(sc)
+        _system(dev, O, R0)
+        rep, rer, mer = dev.solve(solfec.make_param(**kw))
+        out.append((rep, dev.get_RU()))
+    dev.set_schedule(0)
+    dev.set_layout(1)
+    (r0, (R0g, U0g)), (r3, (R3g, U3g)) = out
+    print(f"[bar] {kw}: iters {r0.iters} {r3.iters}, error {r0.error} {r3.error}", flush=True)
+    assert abs(r0.iters - r3.iters) <= 1 and r0.error == r3.error
+    if r0.iters == r3.iters:
+        assert np.array_equal(R0g, R3g) and np.array_equal(U0g, U3g)
 
 
 def test_flow_matches_oracle(dev):

@@ -1925,10 +1925,14 @@ int SFGS_Penalty_Solve(sfgs_ctx *ctx, double step, int implicit, const double *s
   if (!ctx->have_bodies) { ctx->err = "no bodies"; return SFGS_E_STATE; }
   const bool given = spring || dashpot || hpow;
   if (given && !(spring && dashpot && hpow)) { ctx->err = "give spring, dashpot and hpow together, or none"; return SFGS_E_ARG; }
-  if (!given && !ctx->dom) { ctx->err = "no material table (SFGS_Dom_Upload) for the springs and dashpots"; return SFGS_E_STATE; }
+  const int n = ctx->nc;
+  if (!given && !ctx->dom && n > 0)
+  {
+    ctx->err = "no material table (SFGS_Dom_Upload) for the springs and dashpots";
+    return SFGS_E_STATE;
+  }
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
-  const int n = ctx->nc;
   CHK(build_incidence(ctx));
   CHK(ensure(ctx, ctx->perm, (size_t)std::max(n, 1)));
   CHK(ensure(ctx, ctx->iperm, (size_t)std::max(n, 1)));
@@ -1940,7 +1944,7 @@ int SFGS_Penalty_Solve(sfgs_ctx *ctx, double step, int implicit, const double *s
     for (int i = 0; i < n; i ++) { kcp[3*(size_t)i] = spring[i]; kcp[3*(size_t)i + 1] = dashpot[i]; kcp[3*(size_t)i + 2] = hpow[i]; }
     CHK(upload(ctx, ctx->pen, kcp.data(), kcp.size()));
   }
-  if (!given) CHK(upload(ctx, ctx->pen, ctx->defmat, 6));   // the default material's six values
+  if (!given && n > 0) CHK(upload(ctx, ctx->pen, ctx->defmat, 6));   // the default material's six values
   if (n > 0)
   {
     BodyView bv{ctx->kind.p, ctx->rot.p, ctx->x0.p, ctx->jinv.p, ctx->minv.p, ctx->velo.p, ctx->velo0.p};

@@ -208,15 +208,77 @@ __global__ void k_nt_update(int n, double *R, const double *dr, const double *fr
   for (int k = 0; k < 3; k ++) R[4*(size_t)p + k] = Rp[k];
 }
 
+// the Arnoldi scalars of one GMRES cycle live on the device: the Hessenberg
+// matrix hh (row-major, (k_dim + 1) x k_dim), the right-hand side rs and the
+// Givens rotations c, s, and the last residual estimate
+struct NtScal {
+  double hh[(NT_KDIM + 1)*NT_KDIM];
+  double rs[NT_KDIM + 1], c[NT_KDIM], s[NT_KDIM];
+  double rnorm;
+};
+
+// the sum of a dot product's block partials, in a fixed order; sqrt of it when root
+__global__ __launch_bounds__(TPB) void k_nt_dotfin(const double *part, int np, double *out, int root)
+{
+  double v = threadIdx.x < np ? part[threadIdx.x] : 0.0, z1 = 0.0, z2 = 0.0;
+  block_sum3(v, z1, z2);
+  if (threadIdx.x == 0) *out = root ? sqrt(v) : v;
+}
+
+// y += sign * (*alpha) x
+__global__ void k_nt_axpy_dev(long long N, const double *alpha, double sign, const double *x, double *y)
+{
+  const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+  if (i < N) y[i] += (sign*(*alpha))*x[i];
+}
+
+// x *= 1 / (*t) unless *t is 0
+__global__ void k_nt_scale_inv(long long N, const double *t, double *x)
+{
+  const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+  const double tv = *t;
+  if (i < N && tv != 0.0) x[i] *= 1.0/tv;
+}
+
+// column i - 1 of the Arnoldi step: the previous rotations, a new one, the
+// updated right-hand side and the residual estimate |rs[i]| (flexgmres.c:531-557)
+__global__ void k_nt_givens(NtScal *sc, int i)
+{
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double *hh = sc->hh, *c = sc->c, *s = sc->s, *rs = sc->rs, t;
+#define HH(a, b) hh[(a)*NT_KDIM + (b)]
+  for (int j = 1; j < i; j ++)
+  {
+    t = HH(j - 1, i - 1);
+    HH(j - 1, i - 1) = s[j - 1]*HH(j, i - 1) + c[j - 1]*t;
+    HH(j, i - 1) = -s[j - 1]*t + c[j - 1]*HH(j, i - 1);
+  }
+  t = HH(i, i - 1)*HH(i, i - 1);
+  t += HH(i - 1, i - 1)*HH(i - 1, i - 1);
+  double gamma = sqrt(t);
+  if (gamma == 0.0) gamma = 1.e-16;
+  c[i - 1] = HH(i - 1, i - 1)/gamma;
+  s[i - 1] = HH(i, i - 1)/gamma;
+  rs[i] = -HH(i, i - 1)*rs[i - 1];
+  rs[i] /= gamma;
+  rs[i - 1] = c[i - 1]*rs[i - 1];
+  HH(i - 1, i - 1) = s[i - 1]*HH(i, i - 1) + c[i - 1]*HH(i - 1, i - 1);
+  sc->rnorm = fabs(rs[i]);
+#undef HH
+}
+
 // the device vectors of one Newton solve
 struct NtWork {
   sfgs_ctx *ctx;
   int n;
   long long N;
   double *X, *Y, *P, *b, *dr, *R0, *kry, *pre, *w, *r, *part;
+  NtScal *sc;
   NtMat mat;
   int matvecs = 0;
   std::vector<double> hpart;
+  double rs0 = 0.0, rnorm_h = 0.0;   // host ends of the small copies
 
   double *kv(int k) const { return kry + (size_t)k*N; }
   double *pv(int k) const { return pre + (size_t)k*N; }
@@ -244,6 +306,24 @@ struct NtWork {
     return 0;
   }
 
+  // x'y (or its square root) into device memory: no read-back
+  void dot_dev(const double *a, const double *b2, double *out, int root)
+  {
+    const int grid = std::max(1, std::min(NT_DOT_BLOCKS, nblk(N)));
+    hipLaunchKernelGGL(k_nt_dot, dim3(grid), dim3(TPB), 0, ctx->stream, N, a, b2, part);
+    hipLaunchKernelGGL(k_nt_dotfin, dim3(1), dim3(TPB), 0, ctx->stream, part, grid, out, root);
+  }
+
+  void axpy_dev(const double *alpha, double sign, const double *x, double *y)
+  {
+    if (N > 0) hipLaunchKernelGGL(k_nt_axpy_dev, dim3(nblk(N)), dim3(TPB), 0, ctx->stream, N, alpha, sign, x, y);
+  }
+
+  void scale_inv_dev(const double *t, double *x)
+  {
+    if (N > 0) hipLaunchKernelGGL(k_nt_scale_inv, dim3(nblk(N)), dim3(TPB), 0, ctx->stream, N, t, x);
+  }
+
   void axpy(double alpha, const double *x, double *y)
   {
     if (N > 0) hipLaunchKernelGGL(k_nt_axpy, dim3(nblk(N)), dim3(TPB), 0, ctx->stream, N, alpha, x, y);
@@ -266,15 +346,19 @@ struct NtWork {
   }
 
   // hypre_FlexGMRESSolve (flexgmres.c:260-700) with rel_change off, cf_tol 0,
-  // r_tol 0, k_dim 20: x updated in place
+  // r_tol 0, k_dim 20: x updated in place.  The Arnoldi step runs on the
+  // device (dot products reduced there, the Givens update in one thread); the
+  // host reads the residual estimate once per iteration and the small
+  // Hessenberg system once per cycle.
   int gmres(const double *bb, double *x, int max_iter, int min_iter, double a_tol)
   {
     const int k_dim = NT_KDIM;
-    double rs[NT_KDIM + 1] = {0}, c[NT_KDIM] = {0}, s[NT_KDIM] = {0}, hh[NT_KDIM + 1][NT_KDIM];
-    std::memset(hh, 0, sizeof(hh));
-    const double epsmac = 1.e-16;
-    double r_norm, b_norm, t, gamma;
+    NtScal h;
+    double r_norm, b_norm, t;
     int iter = 0, i = 0;
+    double *hh = sc->hh;
+#define HHD(a, b) (hh + (a)*NT_KDIM + (b))
+#define HHH(a, b) h.hh[(a)*NT_KDIM + (b)]
     CHK(copy(bb, kv(0)));
     CHK(matvec(-1.0, x, 1.0, kv(0)));
     CHK(dot(bb, bb, t)); b_norm = std::sqrt(t);
@@ -284,7 +368,6 @@ struct NtWork {
     const double epsilon = a_tol;
     while (iter < max_iter)
     {
-      rs[0] = r_norm;
       if (r_norm == 0.0) break;
       if (r_norm <= epsilon && iter >= min_iter)
       {
@@ -293,6 +376,8 @@ struct NtWork {
         CHK(dot(r, r, t)); r_norm = std::sqrt(t);
         if (r_norm <= epsilon) break;
       }
+      rs0 = r_norm;   // rs[0] of the cycle
+      HIPCHK(hipMemcpyAsync(sc->rs, &rs0, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
       scale(1.0/r_norm, kv(0));
       i = 0;
       while (i < k_dim && iter < max_iter)
@@ -304,42 +389,31 @@ struct NtWork {
         CHK(matvec(1.0, pv(i - 1), 0.0, kv(i)));
         for (int j = 0; j < i; j ++)   // modified Gram-Schmidt
         {
-          CHK(dot(kv(j), kv(i), hh[j][i - 1]));
-          axpy(-hh[j][i - 1], kv(j), kv(i));
+          dot_dev(kv(j), kv(i), HHD(j, i - 1), 0);
+          axpy_dev(HHD(j, i - 1), -1.0, kv(j), kv(i));
         }
-        CHK(dot(kv(i), kv(i), t)); t = std::sqrt(t);
-        hh[i][i - 1] = t;
-        if (t != 0.0) scale(1.0/t, kv(i));
-        for (int j = 1; j < i; j ++)
-        {
-          t = hh[j - 1][i - 1];
-          hh[j - 1][i - 1] = s[j - 1]*hh[j][i - 1] + c[j - 1]*t;
-          hh[j][i - 1] = -s[j - 1]*t + c[j - 1]*hh[j][i - 1];
-        }
-        t = hh[i][i - 1]*hh[i][i - 1];
-        t += hh[i - 1][i - 1]*hh[i - 1][i - 1];
-        gamma = std::sqrt(t);
-        if (gamma == 0.0) gamma = epsmac;
-        c[i - 1] = hh[i - 1][i - 1]/gamma;
-        s[i - 1] = hh[i][i - 1]/gamma;
-        rs[i] = -hh[i][i - 1]*rs[i - 1];
-        rs[i] /= gamma;
-        rs[i - 1] = c[i - 1]*rs[i - 1];
-        hh[i - 1][i - 1] = s[i - 1]*hh[i][i - 1] + c[i - 1]*hh[i - 1][i - 1];
-        r_norm = std::fabs(rs[i]);
+        dot_dev(kv(i), kv(i), HHD(i, i - 1), 1);
+        scale_inv_dev(HHD(i, i - 1), kv(i));
+        hipLaunchKernelGGL(k_nt_givens, dim3(1), dim3(64), 0, ctx->stream, sc, i);
+        HIPCHK(hipMemcpyAsync(&rnorm_h, &sc->rnorm, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        r_norm = rnorm_h;
         if (r_norm <= epsilon && iter >= min_iter) break;
       }
-      rs[i - 1] = rs[i - 1]/hh[i - 1][i - 1];
+      HIPCHK(hipMemcpyAsync(&h, sc, sizeof(NtScal), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      HIPCHK(hipGetLastError());
+      h.rs[i - 1] = h.rs[i - 1]/HHH(i - 1, i - 1);
       for (int k = i - 2; k >= 0; k --)
       {
         t = 0.0;
-        for (int j = k + 1; j < i; j ++) t -= hh[k][j]*rs[j];
-        t += rs[k];
-        rs[k] = t/hh[k][k];
+        for (int j = k + 1; j < i; j ++) t -= HHH(k, j)*h.rs[j];
+        t += h.rs[k];
+        h.rs[k] = t/HHH(k, k);
       }
       CHK(copy(pv(i - 1), w));
-      scale(rs[i - 1], w);
-      for (int j = i - 2; j >= 0; j --) axpy(rs[j], pv(j), w);
+      scale(h.rs[i - 1], w);
+      for (int j = i - 2; j >= 0; j --) axpy(h.rs[j], pv(j), w);
       axpy(1.0, w, x);
       if (r_norm <= epsilon && iter >= min_iter)
       {
@@ -352,17 +426,19 @@ struct NtWork {
       }
       for (int j = i; j > 0; j --)
       {
-        rs[j - 1] = -s[j - 1]*rs[j];
-        rs[j] = c[j - 1]*rs[j];
+        h.rs[j - 1] = -h.s[j - 1]*h.rs[j];
+        h.rs[j] = h.c[j - 1]*h.rs[j];
       }
-      if (i) axpy(rs[i] - 1.0, kv(i), kv(i));
-      for (int j = i - 1; j > 0; j --) axpy(rs[j], kv(j), kv(i));
+      if (i) axpy(h.rs[i] - 1.0, kv(i), kv(i));
+      for (int j = i - 1; j > 0; j --) axpy(h.rs[j], kv(j), kv(i));
       if (i)
       {
-        axpy(rs[0] - 1.0, kv(0), kv(0));
+        axpy(h.rs[0] - 1.0, kv(0), kv(0));
         axpy(1.0, kv(i), kv(0));
       }
     }
+#undef HHD
+#undef HHH
     HIPCHK(hipGetLastError());
     return 0;
   }
@@ -397,7 +473,8 @@ int SFGS_Newton_Solve(sfgs_ctx *ctx, const SFGS_NEWTON_PARAM *par, double *merhi
 
   // one allocation: X, Y, P (9 per row), b, dr, w, r (3 per row), R0 (4 per
   // row), the Krylov and the preconditioned vectors, dot partials
-  const size_t words = 27*n1 + 4*N1 + 4*n1 + (size_t)(2*NT_KDIM + 1)*N1 + NT_DOT_BLOCKS;
+  const size_t scal_words = (sizeof(NtScal) + 7)/8;
+  const size_t words = 27*n1 + 4*N1 + 4*n1 + (size_t)(2*NT_KDIM + 1)*N1 + NT_DOT_BLOCKS + scal_words;
   CHK(ensure(ctx, ctx->nt_mem, words));
   NtWork wk;
   double *m0 = ctx->nt_mem.p;
@@ -405,6 +482,7 @@ int SFGS_Newton_Solve(sfgs_ctx *ctx, const SFGS_NEWTON_PARAM *par, double *merhi
   wk.X = m0; wk.Y = wk.X + 9*n1; wk.P = wk.Y + 9*n1; wk.b = wk.P + 9*n1; wk.dr = wk.b + N1; wk.w = wk.dr + N1;
   wk.r = wk.w + N1; wk.R0 = wk.r + N1; wk.kry = wk.R0 + 4*n1; wk.pre = wk.kry + (size_t)(NT_KDIM + 1)*N1;
   wk.part = wk.pre + (size_t)NT_KDIM*N1;
+  wk.sc = (NtScal*)(wk.part + NT_DOT_BLOCKS);
   wk.hpart.resize(NT_DOT_BLOCKS);
   HIPCHK(hipMemsetAsync(m0, 0, sizeof(double)*words, st));
   if (n > 0) HIPCHK(hipMemcpyAsync(wk.R0, ctx->R.p, sizeof(double)*4*(size_t)n, hipMemcpyDeviceToDevice, st));

@@ -143,3 +143,23 @@ def test_sphere_resting_on_a_plane(linver):
     np.testing.assert_allclose(b["velo"][6:12], 0.0, atol=1e-9)
     np.testing.assert_allclose(c["R"], [0.0, 0.0, m * 9.81], rtol=1e-6, atol=1e-9)
     assert ns.report().gs_fallback == 0
+
+
+@pytest.mark.parametrize("solver", ["newton", "penalty"])
+@pytest.mark.parametrize("loop", ["host", "device"])
+def test_other_solvers_run_empty_domains(solver, loop):
+    """Planes only, and then nothing at all: the NEWTON and PENALTY solvers run
+    steps without contacts."""
+    for planes in (1, 0):
+        s = solfec.SOLFEC("DYNAMIC", H, "/tmp/solfec_test")
+        solfec.SURFACE_MATERIAL(s, model="SPRING_DASHPOT" if solver == "penalty" else "SIGNORINI_COULOMB",
+                                spring=1e5)
+        mat = solfec.BULK_MATERIAL(s, density=1000.0)
+        if planes:
+            solfec.BODY(s, "OBSTACLE", solfec.PLANE((0, 0, 0), (0, 0, 1), 0), mat)
+        s.device_loop(loop == "device")
+        sv = solfec.NEWTON_SOLVER() if solver == "newton" else solfec.PENALTY_SOLVER()
+        solfec.RUN(s, sv, 0.003)
+        assert s.ncon == 0 and s.time == pytest.approx(0.003)
+        if solver == "newton":
+            assert sv.iters == 0 and sv.report().converged == 1

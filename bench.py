@@ -441,6 +441,8 @@ def main():
     ap.add_argument("--schedule", type=int, default=0, choices=[0, 1, 2, 3],
                     help="sweep schedule: 0 one launch per colour (XCD-aware), 1 persistent dataflow, "
                          "2 one launch per colour (hardware order), 3 persistent with a barrier per colour")
+    ap.add_argument("--wpb", type=int, default=4, choices=[1, 2, 4],
+                    help="wavefronts per workgroup of the per-colour sweep launches")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (scale of the traffic counters) and exit")
     ap.add_argument("--shape", default="sphere", choices=["sphere", "ellipsoid"],
@@ -477,6 +479,7 @@ def main():
     dev = solfec.Device(0)
     dev.set_layout(args.lanes)
     dev.set_schedule(args.schedule)
+    dev.set_sweep_block(args.wpb)
     dev.set_row_order(args.row_order)
     dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
     dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
@@ -591,6 +594,7 @@ def main():
             "precision": "mixed" if is32 else "fp64",
             "warm_start": "learned (new contacts, trust %.2g)" % args.ws_scale if head == "learned" else "previous-step R",
             "lanes_per_row": args.lanes,
+            "waves_per_workgroup": args.wpb,
             "schedule": ["per-colour launches, XCD-aware", "dataflow", "per-colour launches",
                          "persistent, barrier per colour"][args.schedule],
             "row_order": ["decreasing row length", "contact index", "body block, row length"][args.row_order],

@@ -83,6 +83,7 @@ struct sfgs_ctx {
   sfgs::DBuf<int> rowlen;                // per permuted row
   sfgs::DBuf<double> W, Wd, A, B, V, rho, pfri, pcoh, shift, R, U, Rb, Um;
   int G = 2;                             // lanes per row of the sweep layout (1, 2, 4, 8)
+  int wpb = 4;                           // wavefronts per workgroup of the per-colour sweep launches (1, 2, 4)
   int schedule = 0;                      // sweeps: 0 one launch per colour (XCD-aware), 1 persistent dataflow,
                                          // 2 one launch per colour (hardware order), 3 persistent with barriers
   int row_order = 2;                     // inside a colour: 0 by decreasing row length, 1 by contact index,

@@ -533,20 +533,21 @@ __device__ void finish_sweep(Ctrl *ctrl, const double *partials, int np, double 
 
 // fin (the last launch of a sweep only): the block that completes the launch
 // last takes the sweep's termination test, as k_finish would after it
-template <typename T, int G, bool MERIT>
-__global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<T> a, int s0, int s1, int pbase, int upd_lo, int upd_hi,
-                                                int backward, int xcd, int *fin, int np, double *hist)
+// WPB wavefronts per workgroup (one slice each)
+template <typename T, int G, bool MERIT, int WPB = TPB/SLICE>
+__global__ __launch_bounds__(WPB*64) void k_sweep(SweepArgs<T> a, int s0, int s1, int pbase, int upd_lo, int upd_hi,
+                                                   int backward, int xcd, int *fin, int np, double *hist)
 {
   // uniform: set only between launches (by the finish of an earlier sweep); read
   // with the slice table, not before it
   const int done = a.ctrl->done;
   const int bb = xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-  const int s = s0 + bb*(TPB/SLICE) + (threadIdx.x >> 6);
+  const int s = s0 + bb*WPB + (threadIdx.x >> 6);
   double eu = 0.0, el = 0.0, up = 0.0;
   sweep_slice<T, G, MERIT>(a, s, s < s1, upd_lo, upd_hi, backward, &a.ctrl->sweeps, a.failkey, a.failcount,
                            &a.ctrl->aborted, eu, el, up, done == 0);
   if (done) return;
-  block_sum3(eu, el, up);
+  block_sum3<WPB*64>(eu, el, up);
   if (threadIdx.x == 0)
   {
     double *pt = a.partials + 3*(size_t)(pbase + blockIdx.x);
@@ -564,7 +565,7 @@ __global__ __launch_bounds__(TPB) void k_sweep(SweepArgs<T> a, int s0, int s1, i
     if (last)
     {
       __threadfence();   // every other block's partial is visible
-      finish_sweep(a.ctrl, a.partials, np, hist);
+      finish_sweep<WPB*64>(a.ctrl, a.partials, np, hist);
       if (threadIdx.x == 0) *fin = 0;
     }
   }
@@ -1360,6 +1361,22 @@ int SFGS_System_Upload(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, 
 // sweep k reads buffer X[k & 1] and writes X[(k + 1) & 1].  The last launch also
 // takes the termination test (fin); returns false when nothing was launched
 // (then k_finish takes it)
+// one colour launch of k_sweep with the context's lanes per row and wavefronts per workgroup
+#define SFGS_SWEEP_WPB(MER, GG) \
+  switch (ctx->wpb) { \
+  case 1: hipLaunchKernelGGL((k_sweep<T, GG, MER, 1>), dim3(grid[c]), dim3(64), 0, ctx->stream, a, ctx->h_cslice[c], \
+                             ctx->h_cslice[c + 1], pbase[c], lo, hi, backward, xcd, f, np, hist); break; \
+  case 2: hipLaunchKernelGGL((k_sweep<T, GG, MER, 2>), dim3(grid[c]), dim3(128), 0, ctx->stream, a, ctx->h_cslice[c], \
+                             ctx->h_cslice[c + 1], pbase[c], lo, hi, backward, xcd, f, np, hist); break; \
+  default: hipLaunchKernelGGL((k_sweep<T, GG, MER, 4>), dim3(grid[c]), dim3(256), 0, ctx->stream, a, ctx->h_cslice[c], \
+                              ctx->h_cslice[c + 1], pbase[c], lo, hi, backward, xcd, f, np, hist); break; }
+#define SFGS_DISPATCH_SWEEP(MER) \
+  switch (ctx->G_built) { \
+  case 1: SFGS_SWEEP_WPB(MER, 1) break; \
+  case 2: SFGS_SWEEP_WPB(MER, 2) break; \
+  case 8: SFGS_SWEEP_WPB(MER, 8) break; \
+  default: SFGS_SWEEP_WPB(MER, 4) break; }
+
 template <typename T>
 static bool launch_sweep(sfgs_ctx *ctx, const SweepArgs<T> &base, T *X0, T *X1, int k, int backward, bool merit,
                          const std::vector<int> &grid, const std::vector<int> &pbase, bool prof, int *fin, int np,
@@ -1385,15 +1402,11 @@ static bool launch_sweep(sfgs_ctx *ctx, const SweepArgs<T> &base, T *X0, T *X1, 
     if (prof) (void)hipEventRecord(ctx->prof_ev[2*ci], ctx->stream);
     if (merit)
     {
-      SFGS_DISPATCH_G(ctx->G_built, auto kern = k_sweep<T COMMA GG COMMA true>; hipLaunchKernelGGL(kern, dim3(grid[c]),
-                      dim3(TPB), 0, ctx->stream, a, ctx->h_cslice[c], ctx->h_cslice[c + 1], pbase[c], lo, hi, backward,
-                      xcd, f, np, hist));
+      SFGS_DISPATCH_SWEEP(true);
     }
     else
     {
-      SFGS_DISPATCH_G(ctx->G_built, auto kern = k_sweep<T COMMA GG COMMA false>; hipLaunchKernelGGL(kern, dim3(grid[c]),
-                      dim3(TPB), 0, ctx->stream, a, ctx->h_cslice[c], ctx->h_cslice[c + 1], pbase[c], lo, hi, backward,
-                      xcd, f, np, hist));
+      SFGS_DISPATCH_SWEEP(false);
     }
     if (prof) (void)hipEventRecord(ctx->prof_ev[2*ci + 1], ctx->stream);
   }
@@ -1510,7 +1523,7 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   for (int c = 0; c < ncol; c ++)
   {
     int ns = ctx->h_cslice[c + 1] - ctx->h_cslice[c];
-    pl.grid[c] = (ns + TPB/SLICE - 1) / (TPB/SLICE);
+    pl.grid[c] = (ns + ctx->wpb - 1) / ctx->wpb;
     pl.pbase[c + 1] = pl.pbase[c] + pl.grid[c];
   }
   pl.np = pl.pbase[ncol];
@@ -1754,6 +1767,13 @@ int SFGS_Set_Schedule(sfgs_ctx *ctx, int schedule)
 {
   if (!ctx || schedule < 0 || schedule > 3) return SFGS_E_ARG;
   ctx->schedule = schedule;
+  return 0;
+}
+
+int SFGS_Set_Sweep_Block(sfgs_ctx *ctx, int waves_per_block)
+{
+  if (!ctx || (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)) return SFGS_E_ARG;
+  ctx->wpb = waves_per_block;
   return 0;
 }
 

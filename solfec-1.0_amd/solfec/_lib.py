@@ -71,6 +71,7 @@ _SFGS_PROTOS = {
     "SFGS_Newton_Solve": (_i, [_p, C.POINTER(SFGS_NEWTON_PARAM), _p, _p, C.POINTER(SFGS_NEWTON_REPORT)]),
     "SFGS_Set_Layout": (_i, [_p, _i]),
     "SFGS_Set_Schedule": (_i, [_p, _i]),
+    "SFGS_Set_Sweep_Block": (_i, [_p, _i]),
     "SFGS_Set_Row_Order": (_i, [_p, _i]),
     "SFGS_Mlp_Set": (_i, [_p, _p, _ll, _p]),
     "SFGS_Mlp_Get": (_i, [_p, _p, _p]),

@@ -85,6 +85,10 @@ class Device:
         """0: rows of a colour by decreasing length; 1: by contact index (spatial order)."""
         self._check(self.lib.SFGS_Set_Row_Order(self.ctx, int(order)), "SFGS_Set_Row_Order")
 
+    def set_sweep_block(self, waves_per_block):
+        """SFGS_Set_Sweep_Block: wavefronts per workgroup of the per-colour sweep launches (1, 2, 4)."""
+        self._check(self.lib.SFGS_Set_Sweep_Block(self.ctx, int(waves_per_block)), "SFGS_Set_Sweep_Block")
+
     def set_schedule(self, schedule):
         """0: one launch per colour; 1: persistent dataflow sweeps (same iterates)."""
         self._check(self.lib.SFGS_Set_Schedule(self.ctx, int(schedule)), "SFGS_Set_Schedule")

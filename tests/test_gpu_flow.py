@@ -115,16 +115,18 @@ def test_flow_matches_oracle(dev):
 @pytest.mark.parametrize("lanes", [1, 2])
 def test_row_order_and_block_mapping_same_iterates(dev, lanes):
     """Rows of a colour are independent: ordering them by contact index or by body
-    block instead of by length, or mapping workgroups to XCDs differently, changes
-    only the layout, not the reactions."""
+    block instead of by length, mapping workgroups to XCDs differently, or
+    grouping 1, 2 or 4 slices per workgroup changes only the layout, not the
+    reactions."""
     s = lattice(1000, seed=2)
     b, c = s.bodies(), s.contacts()
     kw = dict(epsilon=1e-6, maxiter=3000, meritval=1e-6)
     out = []
     dev.set_layout(lanes)
-    for order, sched in ((0, 0), (1, 0), (2, 0), (2, 2)):
+    for order, sched, wpb in ((0, 0, 4), (1, 0, 4), (2, 0, 4), (2, 2, 4), (2, 0, 1), (2, 0, 2)):
         dev.set_row_order(order)
         dev.set_schedule(sched)
+        dev.set_sweep_block(wpb)
         dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
         dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
                      c["restitution"], c["cohesion"], c["R"], c["U"])
@@ -134,6 +136,7 @@ def test_row_order_and_block_mapping_same_iterates(dev, lanes):
         out.append((rep, R, U, dev.locdyn()["colour"]))
     dev.set_row_order(0)
     dev.set_schedule(0)
+    dev.set_sweep_block(4)
     dev.set_layout(1)
     r0, R0, U0, c0 = out[0]
     for r1, R1, U1, c1 in out[1:]:

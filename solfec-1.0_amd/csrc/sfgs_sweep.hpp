@@ -161,40 +161,55 @@ __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool v
     const int nsl = a.slen[s], mine = len > g ? (len - g + G - 1) / G : 0;
     for (int m0 = 0; m0 < nsl; m0 += CH)
     {
+      // branch-free: every load of the chunk is issued before any is waited for
+      // (slots past the slice's own re-read its last slot and are not added)
       int jj[CH];
 #pragma unroll
       for (int q = 0; q < CH; q ++) jj[q] = m0 + q < nsl ? cb[(size_t)(m0 + q)*SLICE] : 0;
+      // the blocks do not depend on the indices: issued with them, before any wait
+      T wv[CH][9], rv[CH][3], pv[CH][3];
 #pragma unroll
       for (int q = 0; q < CH; q ++)
       {
-        const int m = m0 + q;
-        if (m >= nsl) break;
-        const int j = jj[q];
+        const int m = min(m0 + q, nsl - 1);
         const T *w = Wb + 9*(size_t)m*SLICE;
-        const T w0 = w[0], w1 = w[SLICE], w2 = w[2*SLICE], w3 = w[3*SLICE], w4 = w[4*SLICE],
-                w5 = w[5*SLICE], w6 = w[6*SLICE], w7 = w[7*SLICE], w8 = w[8*SLICE];
+#pragma unroll
+        for (int e = 0; e < 9; e ++) wv[q][e] = w[e*SLICE];
+      }
+      // keep the block loads ahead of the first wait for the indices
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < CH; q ++)
+      {
+        const int j = jj[q];
         const bool upd = j >= upd_lo && j < upd_hi;
         const T *xj = (upd ? a.Q : a.P) + 4*(size_t)j;
-        const T r0 = r_ld<COH>(xj), r1 = r_ld<COH>(xj + 1), r2 = r_ld<COH>(xj + 2);
-        T p0 = r0, p1 = r1, p2 = r2;
-        if (MERIT && upd)
+        rv[q][0] = r_ld<COH>(xj); rv[q][1] = r_ld<COH>(xj + 1); rv[q][2] = r_ld<COH>(xj + 2);
+        if (MERIT)
         {
+          // the previous iterate of every neighbour (equal to rv where not yet updated)
           const T *pj = a.P + 4*(size_t)j;
-          p0 = r_ld<COH>(pj); p1 = r_ld<COH>(pj + 1); p2 = r_ld<COH>(pj + 2);
+          pv[q][0] = r_ld<COH>(pj); pv[q][1] = r_ld<COH>(pj + 1); pv[q][2] = r_ld<COH>(pj + 2);
         }
-        if (m < mine)
+      }
+#pragma unroll
+      for (int q = 0; q < CH; q ++)
+      {
+        const bool on = m0 + q < mine;
+        const T w0 = wv[q][0], w1 = wv[q][1], w2 = wv[q][2], w3 = wv[q][3], w4 = wv[q][4], w5 = wv[q][5],
+                w6 = wv[q][6], w7 = wv[q][7], w8 = wv[q][8];
+        const T r0 = rv[q][0], r1 = rv[q][1], r2 = rv[q][2];
+        // explicit fused multiply-adds: the same rounding in every schedule
+        // that instantiates this gather, whatever the surrounding loads
+        b0 = on ? b0 + fma(w6, r2, fma(w3, r1, w0*r0)) : b0;
+        b1 = on ? b1 + fma(w7, r2, fma(w4, r1, w1*r0)) : b1;
+        b2 = on ? b2 + fma(w8, r2, fma(w5, r1, w2*r0)) : b2;
+        if (MERIT)
         {
-          // explicit fused multiply-adds: the same rounding in every schedule
-          // that instantiates this gather, whatever the surrounding loads
-          b0 += fma(w6, r2, fma(w3, r1, w0*r0));
-          b1 += fma(w7, r2, fma(w4, r1, w1*r0));
-          b2 += fma(w8, r2, fma(w5, r1, w2*r0));
-          if (MERIT)
-          {
-            u0 += fma(w6, p2, fma(w3, p1, w0*p0));
-            u1 += fma(w7, p2, fma(w4, p1, w1*p0));
-            u2 += fma(w8, p2, fma(w5, p1, w2*p0));
-          }
+          const T p0 = pv[q][0], p1 = pv[q][1], p2 = pv[q][2];
+          u0 = on ? u0 + fma(w6, p2, fma(w3, p1, w0*p0)) : u0;
+          u1 = on ? u1 + fma(w7, p2, fma(w4, p1, w1*p0)) : u1;
+          u2 = on ? u2 + fma(w8, p2, fma(w5, p1, w2*p0)) : u2;
         }
       }
     }

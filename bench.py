@@ -382,6 +382,9 @@ def bench_batch(args):
 
     head, Rh, rh = run(pm, have_model, args.steps, args.warmup)
     log(f"[batch] learned+mixed: {head['ms_per_step']:.1f} ms/step, iters max {head['max_iters']}")
+    # the FP64 baseline on its own best shape (the autotuner re-assembles the store for it)
+    tune64, tms64 = dev.batch_autotune(pf, trial_iters=20)
+    log(f"[batch] autotuned FP64 shape {tune64} ({tms64:.2f} ms for 20 iterations)")
     plain, Rp, rp = run(pf, False, args.steps, args.warmup)
     log(f"[batch] plain FP64: {plain['ms_per_step']:.1f} ms/step, iters max {plain['max_iters']}")
     # per-scene agreement with the FP64 run, relative to each scene's max|R|, where both converged
@@ -401,6 +404,7 @@ def bench_batch(args):
                                + ("one launch per colour over all running scenes" if tune[3] == 1
                                   else "one persistent launch"), "scenes": args.scenes, "contacts": ncon,
                    "epsilon": eps, "meritval": 1e-6, "maxiter": maxiter, "shape": list(tune),
+                   "shape_fp64": list(tune64),
                    "learned_model": have_model, "learned_mixed": head, "plain_fp64": plain,
                    "speedup_vs_plain_fp64": plain["ms_per_step"] / head["ms_per_step"],
                    "scenes_converged_both": int(ok.sum()),

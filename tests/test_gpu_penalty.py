@@ -148,3 +148,28 @@ def test_penalty_store_refuses_the_gauss_seidel_calls(dev):
         dev.penalty(H, 1)
     dev.assemble(H)                             # a full assembly makes the store a system again
     dev.merit()
+
+
+@pytest.mark.parametrize("solver", ["penalty", "newton"])
+def test_restart_with_other_solvers_is_bit_exact(tmp_path, solver):
+    """Five steps, a state file, five more; a fresh domain of the same scene that
+    reads the file and runs the same five steps ends in the same bits (device loop)."""
+    def make():
+        s = spring_scene(4, True)
+        s.device_loop(True)
+        return s
+    sv = solfec.PENALTY_SOLVER() if solver == "penalty" else solfec.NEWTON_SOLVER(meritval=1e-8, maxiter=100)
+    a = make()
+    solfec.RUN(a, sv, 0.005)
+    a.write_state(tmp_path / "s.xdr")
+    solfec.RUN(a, sv, 0.005)
+    b = make()
+    b.read_state(tmp_path / "s.xdr")
+    solfec.RUN(b, sv, 0.005)
+    ba, bb = a.bodies(), b.bodies()
+    for k in ("x", "rot", "velo"):
+        assert np.array_equal(ba[k], bb[k]), k
+    ca, cb = a.contacts(), b.contacts()
+    assert len(ca["master"]) == len(cb["master"]) > 100
+    for k in ("R", "U", "gap"):
+        assert np.array_equal(ca[k], cb[k]), k

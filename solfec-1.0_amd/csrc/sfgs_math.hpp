@@ -22,6 +22,15 @@ template <typename T> struct V3 { T x, y, z; };
 template <typename T> __device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
 template <typename T> __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
 
+// Division and square root of the local solvers: IEEE in FP64; in the FP32
+// sweeps of the mixed mode the hardware reciprocal and square root (1 ulp),
+// whose error stays below the FP32 local tolerance (the FP64 refinement sweeps
+// settle the final state)
+template <typename T> __device__ __forceinline__ T fdiv(T a, T b) { return a / b; }
+template <> __device__ __forceinline__ float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <typename T> __device__ __forceinline__ T fsqrt(T x) { return sqrt(x); }
+template <> __device__ __forceinline__ float fsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // y = c + M v
 template <typename T>
 __device__ __forceinline__ void mv_add(const T *c, const T *M, const T *v, T *y)
@@ -37,7 +46,7 @@ __device__ __forceinline__ T rel_change(const T *R, const T *R0)
 {
   T d0 = R[0] - R0[0], d1 = R[1] - R0[1], d2 = R[2] - R0[2];
   T rr = R[0]*R[0] + R[1]*R[1] + R[2]*R[2];
-  return sqrt((d0*d0 + d1*d1 + d2*d2) / tmax(rr, T(1)));
+  return fsqrt(fdiv(d0*d0 + d1*d1 + d2*d2, tmax(rr, T(1))));
 }
 
 // LU with partial pivoting on a 3x3 (column-major), b overwritten with x.
@@ -73,7 +82,7 @@ __device__ __forceinline__ int solve3(T *a, T *b)
 #pragma unroll
       for (int i = k + 1; i < 3; i ++) if (i == p) b[i] = bk;
     }
-    T inv = T(1) / a[k + 3*k];
+    T inv = fdiv(T(1), a[k + 3*k]);
     a[k + 3*k] = inv;   // kept for the back substitution
 #pragma unroll
     for (int i = k + 1; i < 3; i ++)
@@ -116,9 +125,9 @@ template <typename T>
 __device__ __forceinline__ void spectral(T fri, const T *S, T &l1, T &l2, T *u1, T *u2, T &slen)
 {
   T onefri2 = T(1) + fri*fri;
-  slen = sqrt(S[0]*S[0] + S[1]*S[1]);
-  l1 = -(S[2] + fri*slen) / onefri2;
-  l2 = (slen - fri*S[2]) / onefri2;
+  slen = fsqrt(S[0]*S[0] + S[1]*S[1]);
+  l1 = fdiv(-(S[2] + fri*slen), onefri2);
+  l2 = fdiv(slen - fri*S[2], onefri2);
   if (slen != T(0)) { u2[0] = S[0]/slen; u2[1] = S[1]/slen; }
   else { u2[0] = T(1); u2[1] = T(0); }
   u2[2] = -fri;
@@ -265,23 +274,23 @@ __device__ __forceinline__ int semismooth_newton(const RowLaw<T> &L, T eps, int 
 
     if (d[2] >= T(0))
     {
-      T norm = sqrt(d[0]*d[0] + d[1]*d[1]);
+      T norm = fsqrt(d[0]*d[0] + d[1]*d[1]);
       T lim = fri * tmax(T(0), d[2]);
       if (norm >= lim)
       {
         if (lim > T(0))
         {
-          T len = sqrt(R[0]*R[0] + R[1]*R[1]);
-          const T inorm = T(1) / norm;
+          T len = fsqrt(R[0]*R[0] + R[1]*R[1]);
+          const T inorm = fdiv(T(1), norm);
           T den = tmax(lim, len) * norm;
-          const T iden = T(1) / den;
+          const T iden = fdiv(T(1), den);
           T e = lim * inorm, beta;
           if (len == T(0)) beta = T(1);
           else
           {
-            T alfa = (R[0]*d[0] + R[1]*d[1]) / (len*norm);
-            T delta = tmin(len/lim, T(1));
-            beta = alfa < T(0) ? T(1)/(T(1) - alfa*delta) : T(1);
+            T alfa = fdiv(R[0]*d[0] + R[1]*d[1], len*norm);
+            T delta = tmin(fdiv(len, lim), T(1));
+            beta = alfa < T(0) ? fdiv(T(1), T(1) - alfa*delta) : T(1);
           }
           T F0 = (R[0]*d[0])*iden, F1 = (R[1]*d[0])*iden, F2 = (R[0]*d[1])*iden, F3 = (R[1]*d[1])*iden;
           T M0 = e*(T(1) - F0), M1 = -e*F1, M2 = -e*F2, M3 = e*(T(1) - F3);
@@ -314,7 +323,7 @@ __device__ __forceinline__ int semismooth_newton(const RowLaw<T> &L, T eps, int 
       {
 #pragma unroll
         for (int k = 0; k < 9; k ++) a[k] = W[k];
-        const T id2 = T(1) / d[2];
+        const T id2 = fdiv(T(1), d[2]);
         a[6] += U[0]*id2;
         a[7] += U[1]*id2;
         T s = T(1) + rho*U[2]*id2;

@@ -130,6 +130,19 @@ int SFGS_Merit(sfgs_ctx *ctx, int update_U, double *merit, double *conmerit);
 /* Download the current R, U (natural order, 3 per contact); NULL skips. */
 int SFGS_Get_RU(sfgs_ctx *ctx, double *R, double *U);
 
+/* update_cohesion of LOCDYN_Update_End (ldy.c:69-95, 879-889), after a
+ * Gauss-Seidel or Newton solve (not after SFGS_Penalty_Solve, whose law breaks
+ * cohesion itself): a contact with cohesion state c > 0 (the cohesive force,
+ * cohesion times area) loses it, c = 0, when R_N + c < e (mode I) or
+ * |R_T| + e >= friction (R_N + c) (mode II), e = 1e-8 c (COHESION_EPSILON,
+ * sps.h:34), with the solved R.  The states are those of SFGS_Contacts_Upload
+ * or of the device loop; SFGS_Get_Cohesion reads them back. */
+int SFGS_Update_Cohesion(sfgs_ctx *ctx);
+
+/* Download the contacts' cohesion states (natural order, 1 per contact): what
+ * SURFACE_MATERIAL_Cohesion_Get (sps.h:127, sps.c:356) returns per CON. */
+int SFGS_Get_Cohesion(sfgs_ctx *ctx, double *cohesion);
+
 /* Download the assembled LOCDYN in natural order for inspection: rowptr
  * (n+1), col/via (nnz), W (9 nnz), Wd, A (9 n), B, V (3 n), rho (n), the free
  * energy and the colour of every contact; NULL skips an array. */
@@ -294,6 +307,13 @@ int SFGS_Dom_Upload(sfgs_ctx *ctx, const SFGS_SHAPES *shapes, int nmat, const SF
  * convex pairs before plane pairs, with R = U = 0 (mem.c:77,86).  *ncon
  * receives the new contact count.  SFGS_Locdyn_Update_Begin follows. */
 int SFGS_Dom_Update_Begin(sfgs_ctx *ctx, double step, const double *gravity, int *ncon);
+/* The domain time of the next SFGS_Dom_Update_Begin (dom->time): contacts it
+ * creates at time 0 take their material's cohesion as cohesion state, later
+ * ones none (SURFACE_MATERIAL_Transfer, sps.c:264-282).  A cohesive contact
+ * (state > 0) is kept when its bodies separate, with its referential points,
+ * its spatial point following the master body and its previous frame and gap
+ * (update_contact, dom.c:452-458). */
+int SFGS_Dom_Set_Time(sfgs_ctx *ctx, double time);
 /* DOM_Update_End (dom.c:3818) after SFGS_Solve: R and U back into the
  * contact list, the velocity update by the reactions (as SFGS_Body_Update)
  * and the rigid end of step (BODY_Dynamic_Step_End, bod.c:1425-1492) */
@@ -360,7 +380,9 @@ int SFGS_Newton_Solve(sfgs_ctx *ctx, const SFGS_NEWTON_PARAM *par, double *merhi
  * U follow from SFGS_Get_RU, velocities from SFGS_Body_Update or
  * SFGS_Dom_Update_End.  The context holds no assembled system afterwards:
  * SFGS_Solve, SFGS_Merit and SFGS_Get_Locdyn return SFGS_E_STATE until the
- * next SFGS_Locdyn_Update_Begin.  Cohesive contacts are not modelled. */
+ * next SFGS_Locdyn_Update_Begin.  A cohesive contact (cohesion state c > 0)
+ * may take tension down to R_N = -c; it loses its cohesion (c = 0) below that
+ * or when its tangential reaction reaches the cone (pes.c:59-99). */
 int SFGS_Penalty_Solve(sfgs_ctx *ctx, double step, int implicit, const double *spring, const double *dashpot,
                        const double *hpow);
 

@@ -84,7 +84,8 @@ def lib():
             "OR_Plane_Contact": (_i, [_p, _p, _d, _p]),
             "OR_Detect": (C.c_longlong, [_i, _p, _p, _p, _p, _i, _p, _p, C.c_longlong, _p, _p, _p, _p]),
             "OR_Expmap": (None, [_p, _p]),
-            "OR_Penalty_Contact": (_i, [_i, _d, _d, _d, _d, _d, _d, _p, _p, _p, _p, _p]),
+            "OR_Penalty_Contact": (_i, [_i, _d, _d, _d, _d, _d, _d, _p, _p, _p, _p, _p, _p]),
+            "OR_Update_Cohesion": (None, [_i, _p, _p, _p]),
             "OR_Newton_Solve": (_i, [C.POINTER(OR_SYSTEM), C.POINTER(OR_NTPARAM), _p, _p, _p, _p, _p,
                                      C.POINTER(OR_NTREPORT)]),
             "OR_FlexGMRES_Dense": (_i, [_i, _p, _p, _p, _p, _i, _i, _i, _d, _p, _p]),
@@ -308,13 +309,22 @@ def rig_step_end(kind, h, I, force, velo, rot, x, aux):
 
 
 def penalty_contact(implicit, step, gap, spring, dashpot, hpow, friction, W, B, V, R0=(0.0, 0.0, 0.0),
-                    U0=(0.0, 0.0, 0.0)):
-    """PENALTY_Spring_Dashpot_Contact (pes.c:33-106) from the carried R0, U0: returns rc, R, U."""
+                    U0=(0.0, 0.0, 0.0), cohesion=None):
+    """PENALTY_Spring_Dashpot_Contact (pes.c:33-106) from the carried R0, U0: returns rc, R, U, or
+    rc, R, U, cohesion after the law when a cohesion state is given."""
     R = _f(np.asarray(R0, float)).copy()
     U = _f(np.asarray(U0, float)).copy()
+    c = None if cohesion is None else np.array([float(cohesion)])
     rc = lib().OR_Penalty_Contact(int(implicit), step, gap, spring, dashpot, hpow, friction, _ptr(_f(W)), _ptr(_f(B)),
-                                  _ptr(_f(V)), _ptr(U), _ptr(R))
-    return rc, R, U
+                                  _ptr(_f(V)), _ptr(U), _ptr(R), None if c is None else _ptr(c))
+    return (rc, R, U) if c is None else (rc, R, U, float(c[0]))
+
+
+def update_cohesion(R, friction, cohesion):
+    """update_cohesion (ldy.c:69-95): the cohesion states after a solve with reactions R (3 per contact)."""
+    c = _f(np.asarray(cohesion, float)).copy()
+    lib().OR_Update_Cohesion(len(c), _ptr(_f(R)), _ptr(_f(friction)), _ptr(c))
+    return c
 
 
 def nt_param(meritval=1e-8, maxiter=1000, linver="GMRES", linmaxiter=10, maxmatvec=10000, epsilon=0.25, delta=0.0,

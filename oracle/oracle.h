@@ -11,7 +11,8 @@
  *   or_scf.c  -- scf.c:28-174 (cone linearisation / projection)
  *   or_bgs.c  -- bgs.c:913-1069 (serial Gauss-Seidel), mrf.c:35-184 (merit)
  *   or_bod.c  -- bod.c:418-435, 1425-1447 (body velocity update by reactions)
- *   or_pes.c  -- pes.c:33-106 (spring-dashpot contact law of the penalty solver)
+ *   or_pes.c  -- pes.c:33-106 (spring-dashpot contact law of the penalty solver),
+ *                ldy.c:69-95 (update_cohesion)
  *   or_nts.c  -- nts.c:937-1379 and ext/krylov/flexgmres.c:82-700 (NEWTON_SOLVER, LOCDYN_ON)
  *   or_dom.c  -- goc.c:50-59, 308-348, 421-455, gjk.c:560-584 (sphere and plane
  *                contacts), dom.c:348-483 (contact update and creation, brute
@@ -159,10 +160,14 @@ void OR_Rig_Step_End(int kind, double h, const double *I, const double *force, d
                      double *A);
 
 /* spring-dashpot contact law (or_pes.c); W, B, V of the contact; R in: the
- * carried reaction (its tangential part enters B_N), out: the reaction; returns
- * 0, or -1 for a singular tangential block */
+ * carried reaction (its tangential part enters B_N), out: the reaction; coh
+ * (NULL: not cohesive) in: the cohesion state, out: 0 when broken; returns 0,
+ * or -1 for a singular tangential block */
 int OR_Penalty_Contact(int implicit, double step, double gap, double spring, double dashpot, double hpow,
-                       double friction, const double *W, const double *B, const double *V, double *U, double *R);
+                       double friction, const double *W, const double *B, const double *V, double *U, double *R,
+                       double *coh);
+/* update_cohesion of LOCDYN_Update_End (or_pes.c): R 3 per contact, coh in/out */
+void OR_Update_Cohesion(int n, const double *R, const double *friction, double *coh);
 
 /* NEWTON_SOLVER parameters (nts.h:31-66) and results (or_nts.c) */
 typedef struct {

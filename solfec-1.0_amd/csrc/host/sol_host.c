@@ -459,6 +459,7 @@ static int map_find(SOLFEC *sol, int a, int b)
   return -1;
 }
 
+/* contact data from its geometry; the cohesion state is set apart (new_contact) */
 static void set_contact(SOLFEC *sol, int i, int master, int slave, const double *mp, const double *sp,
                         const double *normal, double gap)
 {
@@ -472,7 +473,35 @@ static void set_contact(SOLFEC *sol, int i, int master, int slave, const double 
   const MATPAIR *m = find_material(sol, sol->surf[master], sol->surf[slave]);
   sol->c_fri[i] = m->friction;
   sol->c_rest[i] = m->restitution;
-  sol->c_coh[i] = m->cohesion;   /* area = 1 for sphere contacts (goc.c:450) */
+}
+
+/* a cohesive contact keeps its material points (dom.c:452-458): the spatial
+ * point is the master's original point in the current configuration, and the
+ * frame and gap are those of the geometry update (the previous ones when the
+ * surfaces have separated) */
+static void set_cohesive(SOLFEC *sol, int i, const double *normal, double gap)
+{
+  const int m = sol->c_master[i];
+  geom_cur_point(sol->rot + 9*m, sol->x + 3*m, sol->x0 + 3*m, sol->c_mpnt + 3*i, sol->c_point + 3*i);
+  geom_localbase(normal, sol->c_base + 9*i);
+  sol->c_gap[i] = gap;
+  const MATPAIR *mp = find_material(sol, sol->surf[m], sol->surf[sol->c_slave[i]]);
+  sol->c_fri[i] = mp->friction;
+  sol->c_rest[i] = mp->restitution;
+}
+
+/* a contact found by the detection: zero reactions and the cohesion state of
+ * SURFACE_MATERIAL_Transfer (sps.c:264-282), the material's cohesion for a
+ * contact created at time 0 and none later; area = 1 for these contacts
+ * (goc.c:450), so the state is the cohesive force itself */
+static void new_contact(SOLFEC *sol, int i, int master, int slave, const double *mp, const double *sp,
+                        const double *normal, double gap)
+{
+  set_contact(sol, i, master, slave, mp, sp, normal, gap);
+  const MATPAIR *m = find_material(sol, sol->surf[master], sol->surf[slave]);
+  sol->c_coh[i] = sol->time == 0.0 && m->cohesion > 0.0 ? m->cohesion : 0.0;
+  memset(sol->c_R + 3*i, 0, 3*sizeof(double));
+  memset(sol->c_U + 3*i, 0, 3*sizeof(double));
 }
 
 /* update existing contacts (keep slot, R, U; remove separated), then detect
@@ -501,16 +530,28 @@ static int update_and_detect(SOLFEC *sol)
     if (sol->shape[s] == 1) sol->upd_ok[i] = plane_convex(sol, s, m, G, G + 3, G + 6, G + 9);
     else sol->upd_ok[i] = convex_pair(sol, s, m, G, G + 3, G + 6, G + 9);
   }
-  /* compaction keeps the insertion order; upd_ok receives the old index of each kept slot */
+  /* compaction keeps the insertion order; upd_ok receives the old index of each
+   * kept slot.  Separated contacts are removed unless cohesive (dom.c:452-480),
+   * and a separated cohesive contact keeps its previous normal and gap. */
   for (int i = 0; i < nc; i ++)
   {
-    if (!sol->upd_ok[i]) continue;
+    const int cohesive = sol->c_coh[i] > 0.0;
+    if (!sol->upd_ok[i] && !cohesive) continue;
+    if (!sol->upd_ok[i])
+    {
+      double *G = sol->upd_geo + 10*(size_t)i;
+      memcpy(G + 6, sol->c_base + 9*i + 6, 3*sizeof(double));
+      G[9] = sol->c_gap[i];
+    }
     if (kept != i)
     {
       sol->c_master[kept] = sol->c_master[i];
       sol->c_slave[kept] = sol->c_slave[i];
       memcpy(sol->c_R + 3*kept, sol->c_R + 3*i, 3*sizeof(double));
       memcpy(sol->c_U + 3*kept, sol->c_U + 3*i, 3*sizeof(double));
+      memcpy(sol->c_mpnt + 3*kept, sol->c_mpnt + 3*i, 3*sizeof(double));
+      memcpy(sol->c_spnt + 3*kept, sol->c_spnt + 3*i, 3*sizeof(double));
+      sol->c_coh[kept] = sol->c_coh[i];
     }
     sol->upd_ok[kept ++] = i;
   }
@@ -518,7 +559,8 @@ static int update_and_detect(SOLFEC *sol)
   for (int k = 0; k < kept; k ++)
   {
     const double *G = sol->upd_geo + 10*(size_t)sol->upd_ok[k];
-    set_contact(sol, k, sol->c_master[k], sol->c_slave[k], G + 3, G, G + 6, G[9]);
+    if (sol->c_coh[k] > 0.0) set_cohesive(sol, k, G + 6, G[9]);
+    else set_contact(sol, k, sol->c_master[k], sol->c_slave[k], G + 3, G, G + 6, G[9]);
   }
   sol->nc = kept;
   double t1 = now_s();
@@ -648,10 +690,7 @@ static int update_and_detect(SOLFEC *sol)
         if (geom_pntcmp(sol->x + 3*a, sol->x + 3*b) > 0) { slave = b; master = a; }
         if (!convex_pair(sol, slave, master, sp, mp, nrm, &g)) continue;
         ensure_contacts(sol, sol->nc + 1);
-        int i = sol->nc ++;
-        set_contact(sol, i, master, slave, mp, sp, nrm, g);
-        memset(sol->c_R + 3*i, 0, 3*sizeof(double));
-        memset(sol->c_U + 3*i, 0, 3*sizeof(double));
+        new_contact(sol, sol->nc ++, master, slave, mp, sp, nrm, g);
       }
   }
   /* planes against all spheres */
@@ -667,10 +706,7 @@ static int update_and_detect(SOLFEC *sol)
       if (map_find(sol, p, s) >= 0) continue;
       if (!plane_convex(sol, p, s, sp, mp, nrm, &g)) continue;
       ensure_contacts(sol, sol->nc + 1);
-      int i = sol->nc ++;
-      set_contact(sol, i, s, p, mp, sp, nrm, g);
-      memset(sol->c_R + 3*i, 0, 3*sizeof(double));
-      memset(sol->c_U + 3*i, 0, 3*sizeof(double));
+      new_contact(sol, sol->nc ++, s, p, mp, sp, nrm, g);
     }
   }
   sol->t_condet += now_s() - t1;
@@ -777,7 +813,8 @@ static int dev_step_begin(SOLFEC *sol)
     int rc = dev_push(sol, g);
     if (rc) return rc;
   }
-  int nc = 0, rc = SFGS_Dom_Update_Begin(g, sol->step, sol->gravity, &nc);
+  int nc = 0, rc = SFGS_Dom_Set_Time(g, sol->time);
+  if (!rc) rc = SFGS_Dom_Update_Begin(g, sol->step, sol->gravity, &nc);
   if (rc)
   {
     snprintf(sol->err, sizeof(sol->err), "device step begin failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));
@@ -912,7 +949,15 @@ static int has_spring_dashpot(SOLFEC *sol)
   return 0;
 }
 
-/* upload, assemble, solve and update body velocities on the GPU */
+/* a cohesive material in the domain: the contacts created at time 0 may carry
+ * cohesion states, which the solves update */
+static int has_cohesion(SOLFEC *sol)
+{
+  if (sol->defmat.cohesion > 0.0) return 1;
+  for (int k = 0; k < sol->nmat; k ++) if (sol->mat[k].cohesion > 0.0) return 1;
+  return 0;
+}
+
 /* the assembly of a step: in the host loop bodies and contacts go to the
  * device first; gs (may be NULL) asks for the learned warm start */
 static int step_assemble(SOLFEC *sol, sfgs_ctx *g, GAUSS_SEIDEL *gs)
@@ -946,8 +991,20 @@ static int step_assemble(SOLFEC *sol, sfgs_ctx *g, GAUSS_SEIDEL *gs)
  * velocity update (the device loop does both in SFGS_Dom_Update_End) */
 static int step_finish(SOLFEC *sol, sfgs_ctx *g)
 {
+  int rc = 0;
+  if (has_cohesion(sol))
+  {
+    /* LOCDYN_Update_End (ldy.c:879-889): the cohesion states after the solve */
+    rc = SFGS_Update_Cohesion(g);
+    if (!rc && !sol->devloop) rc = SFGS_Get_Cohesion(g, sol->c_coh);
+    if (rc)
+    {
+      snprintf(sol->err, sizeof(sol->err), "GPU cohesion update failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));
+      return SOL_ERR_GPU;
+    }
+  }
   if (sol->devloop) return 0;
-  int rc = SFGS_Get_RU(g, sol->c_R, sol->c_U);
+  rc = SFGS_Get_RU(g, sol->c_R, sol->c_U);
   if (!rc) rc = SFGS_Body_Update(g, sol->step, sol->tmp_velo, sol->force);
   if (rc)
   {
@@ -1221,6 +1278,7 @@ int SOLFEC_Step_Solve_Penalty(SOLFEC *sol, PENALTY *ps)
   if (!rc) rc = SFGS_Contacts_Upload(g, sol->nc, sol->c_master, sol->c_slave, sol->c_mpnt, sol->c_spnt, sol->c_base,
                                      sol->c_gap, sol->c_fri, sol->c_rest, sol->c_coh, sol->c_R, sol->c_U);
   if (!rc) rc = SFGS_Penalty_Solve(g, sol->step, ps->implicit, ps->kcp, ps->kcp + n, ps->kcp + 2*(size_t)n);
+  if (!rc && has_cohesion(sol)) rc = SFGS_Get_Cohesion(g, sol->c_coh);   /* broken by the law itself */
   if (!rc) rc = SFGS_Get_RU(g, sol->c_R, sol->c_U);
   if (!rc) rc = SFGS_Body_Update(g, sol->step, sol->tmp_velo, sol->force);
   if (rc) return penalty_failed(sol, g, rc, "PENALTY_SOLVER on GPU");

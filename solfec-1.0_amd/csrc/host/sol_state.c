@@ -6,8 +6,9 @@
  *               (4 values for RIG, none for OBS; kept as zeros here);
  *   constraints write_constraint (dio.c:19-59, IOVER 4, kind CONTACT): id, kind, R,
  *               U, V, point, base, merit, master and slave ids, the surface material
- *               state (SURFACE_MATERIAL_Write_State, sps.c:286-294: material index and
- *               state size 0), area, gap, surface pair.
+ *               state (SURFACE_MATERIAL_Write_State, sps.c:286-294: material index,
+ *               state size -- 1 when the material is cohesive, sps.c:65-69 -- and
+ *               the contact's cohesion state), area, gap, surface pair.
  * Encoding: XDR (RFC 4506), which the reference's non-HDF5 PBF files use: big-endian,
  * 4-byte ints (a short takes 4 bytes), 8-byte IEEE doubles, strings as length +
  * bytes padded to 4.  The container is this library's own: a header (the string
@@ -134,8 +135,11 @@ int SOLFEC_Write_State(SOLFEC *sol, const char *path)
     put_f64(f, merit[i]);
     put_u32(f, (unsigned)(m + 1));
     put_u32(f, s >= 0 ? (unsigned)(s + 1) : 0u);
-    put_u32(f, (unsigned)material_index(sol, sol->surf[m], s >= 0 ? sol->surf[s] : sol->surf[m]));
-    put_u32(f, 0);                                 /* material state size */
+    const int k = material_index(sol, sol->surf[m], s >= 0 ? sol->surf[s] : sol->surf[m]);
+    const MATPAIR *mp = k < sol->nmat ? &sol->mat[k] : &sol->defmat;
+    put_u32(f, (unsigned)k);
+    put_u32(f, mp->cohesion > 0.0 ? 1u : 0u);     /* material state size */
+    if (mp->cohesion > 0.0) put_f64(f, sol->c_coh[i]);
     put_f64(f, 1.0);                               /* area (goc.c:450) */
     put_f64(f, sol->c_gap[i]);
     put_u32(f, (unsigned)sol->surf[m]);
@@ -216,7 +220,8 @@ int SOLFEC_Read_State(SOLFEC *sol, const char *path)
     unsigned mid = get_u32(&r), sid = get_u32(&r);
     (void)get_u32(&r);           /* material index: the table is the domain's */
     int ssize = get_i32(&r);
-    if (ssize > 0 && ssize < 64) { double st[64]; get_f64s(&r, st, ssize); }
+    c[19] = 0.0;                 /* cohesion state */
+    if (ssize > 0 && ssize < 64) { double st[64]; get_f64s(&r, st, ssize); c[19] = st[0]; }
     (void)get_f64(&r);           /* area */
     c[18] = get_f64(&r);         /* gap */
     (void)get_u32(&r); (void)get_u32(&r);   /* surface pair */
@@ -259,7 +264,8 @@ int SOLFEC_Read_State(SOLFEC *sol, const char *path)
     const MATPAIR *mp = k < sol->nmat ? &sol->mat[k] : &sol->defmat;
     sol->c_fri[i] = mp->friction;
     sol->c_rest[i] = mp->restitution;
-    sol->c_coh[i] = mp->cohesion;
+    /* cohesive when the material is and the state holds cohesion (sps.c:72-80) */
+    sol->c_coh[i] = mp->cohesion > 0.0 && c[19] > 0.0 ? c[19] : 0.0;
   }
   rc = SOL_OK;
 done:

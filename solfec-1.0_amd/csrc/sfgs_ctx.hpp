@@ -138,6 +138,7 @@ struct sfgs_ctx {
   sfgs::DBuf<int> mat_s;                  // surface pairs (smaller id first), 2 per material
   sfgs::DBuf<double> mat_v;               // friction, restitution, cohesion, spring, dashpot, hpow, 6 per material
   double defmat[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 1.0};
+  double dom_time = 0.0;                  // domain time: contacts created at 0 take their material's cohesion
   sfgs::DBuf<double> geo, geo2, dpart;
   sfgs::DBuf<int> okf, okpos, ok2, okpos2, ok3, okpos3, cand_m, cand_s;
   sfgs::DBuf<int> t_master, t_slave;

@@ -52,7 +52,13 @@ struct MatView {
   const int *s;        // 2 per material, smaller surface first
   const double *v;     // friction, restitution, cohesion, spring, dashpot, hpow
   double def[3];
+  int first;           // the domain time is 0: new contacts take the material's cohesion
 };
+
+// geometry record of an existing contact: spnt 3, mpnt 3, normal 3, gap, then
+// its cohesion state; a cohesive contact carries its previous referential
+// points in the first six entries instead (it keeps them, dom.c:452-458)
+constexpr int GEO = 11;
 
 // contact record outputs of set_contact
 struct ConOut {
@@ -89,9 +95,20 @@ __device__ int contact_geometry(const DomView &d, int s, int m, double *G)
   return convex_pair(d, s, m, G, G + 3, G + 6, G + 9);
 }
 
-// set_contact of the host layer: points, referential points, frame, gap, material
+__device__ const double* material_of(const DomView &d, const MatView &mv, int master, int slave)
+{
+  int s1 = d.surf[master], s2 = d.surf[slave], a = s1 < s2 ? s1 : s2, b = s1 < s2 ? s2 : s1;
+  const double *v = mv.def;
+  for (int k = 0; k < mv.nmat; k ++)
+    if (mv.s[2*k] == a && mv.s[2*k + 1] == b) { v = mv.v + 6*k; break; }
+  return v;
+}
+
+// set_contact of the host layer: points, referential points, frame, gap,
+// material; a new contact (fresh) takes the cohesion state of
+// SURFACE_MATERIAL_Transfer (sps.c:264-282), an existing one keeps coh
 __device__ void put_contact(const DomView &d, const MatView &mv, const ConOut &o, int i, int master, int slave,
-                            const double *mp, const double *sp, const double *nrm, double g)
+                            const double *mp, const double *sp, const double *nrm, double g, bool fresh, double coh)
 {
   o.master[i] = master;
   o.slave[i] = slave;
@@ -100,13 +117,27 @@ __device__ void put_contact(const DomView &d, const MatView &mv, const ConOut &o
   geom_ref_point(d.rot + 9*(size_t)slave, d.x + 3*(size_t)slave, d.x0 + 3*(size_t)slave, sp, o.spnt + 3*(size_t)i);
   geom_localbase(nrm, o.base + 9*(size_t)i);
   o.gap[i] = g;
-  int s1 = d.surf[master], s2 = d.surf[slave], a = s1 < s2 ? s1 : s2, b = s1 < s2 ? s2 : s1;
-  const double *v = mv.def;
-  for (int k = 0; k < mv.nmat; k ++)
-    if (mv.s[2*k] == a && mv.s[2*k + 1] == b) { v = mv.v + 6*k; break; }
+  const double *v = material_of(d, mv, master, slave);
   o.fri[i] = v[0];
   o.rest[i] = v[1];
-  o.coh[i] = v[2];   // area = 1 for these contacts (goc.c:450)
+  o.coh[i] = fresh ? (mv.first && v[2] > 0.0 ? v[2] : 0.0) : coh;   // area = 1 for these contacts (goc.c:450)
+}
+
+// set_cohesive of the host layer: the referential points Mp, Sp are kept and the
+// spatial point is the master's original point in the current configuration
+__device__ void put_cohesive(const DomView &d, const MatView &mv, const ConOut &o, int i, int master, int slave,
+                             const double *Mp, const double *Sp, const double *nrm, double g, double coh)
+{
+  o.master[i] = master;
+  o.slave[i] = slave;
+  geom_cur_point(d.rot + 9*(size_t)master, d.x + 3*(size_t)master, d.x0 + 3*(size_t)master, Mp, o.point + 3*(size_t)i);
+  for (int k = 0; k < 3; k ++) { o.mpnt[3*(size_t)i + k] = Mp[k]; o.spnt[3*(size_t)i + k] = Sp[k]; }
+  geom_localbase(nrm, o.base + 9*(size_t)i);
+  o.gap[i] = g;
+  const double *v = material_of(d, mv, master, slave);
+  o.fri[i] = v[0];
+  o.rest[i] = v[1];
+  o.coh[i] = coh;
 }
 
 __device__ bool pair_known(const unsigned long long *keys, unsigned long long mask, int a, int b)
@@ -151,12 +182,30 @@ __global__ void k_dom_end(int nb, const int *kind, double h, const double *I, co
                     x + 3*(size_t)b, aux + 3*(size_t)b);
 }
 
-__global__ void k_con_update(int nc, DomView d, const int *master, const int *slave, double *geo, int *ok)
+// geometry of the existing contacts; a cohesive one is kept when separated,
+// with its previous normal and gap (dom.c:452-458)
+__global__ void k_con_update(int nc, DomView d, const int *master, const int *slave, const double *coh,
+                             const double *mpnt, const double *spnt, const double *base, const double *gap,
+                             double *geo, int *ok)
 {
   int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i > nc) return;
   if (i == nc) { ok[nc] = 0; return; }   // scan sentinel
-  ok[i] = contact_geometry(d, slave[i], master[i], geo + 10*(size_t)i);
+  double *G = geo + GEO*(size_t)i;
+  const int in = contact_geometry(d, slave[i], master[i], G);
+  const double c = coh[i];
+  G[10] = c;
+  if (c > 0.0)
+  {
+    for (int k = 0; k < 3; k ++) { G[k] = spnt[3*(size_t)i + k]; G[3 + k] = mpnt[3*(size_t)i + k]; }
+    if (!in)
+    {
+      for (int k = 0; k < 3; k ++) G[6 + k] = base[9*(size_t)i + 6 + k];
+      G[9] = gap[i];
+    }
+    ok[i] = 1;
+  }
+  else ok[i] = in;
 }
 
 __global__ void k_con_compact(int nc, DomView d, MatView mv, ConOut o, const int *ok, const int *pos,
@@ -166,8 +215,9 @@ __global__ void k_con_compact(int nc, DomView d, MatView mv, ConOut o, const int
   if (i >= nc || !ok[i]) return;
   int k = pos[i];
   for (int c = 0; c < 3; c ++) { o.R[3*(size_t)k + c] = R[3*(size_t)i + c]; o.U[3*(size_t)k + c] = U[3*(size_t)i + c]; }
-  const double *G = geo + 10*(size_t)i;
-  put_contact(d, mv, o, k, master[i], slave[i], G + 3, G, G + 6, G[9]);
+  const double *G = geo + GEO*(size_t)i;
+  if (G[10] > 0.0) put_cohesive(d, mv, o, k, master[i], slave[i], G + 3, G, G + 6, G[9], G[10]);
+  else put_contact(d, mv, o, k, master[i], slave[i], G + 3, G, G + 6, G[9], false, 0.0);
 }
 
 __global__ void k_hash_clear(size_t cap, unsigned long long *keys)
@@ -363,7 +413,7 @@ __global__ void k_append_pairs(int m, int base, DomView d, MatView mv, ConOut o,
   int k = base + pos[q];
   for (int c = 0; c < 3; c ++) { o.R[3*(size_t)k + c] = 0.0; o.U[3*(size_t)k + c] = 0.0; }
   const double *G = geo + 10*(size_t)q;
-  put_contact(d, mv, o, k, cm[q], cs[q], G + 3, G, G + 6, G[9]);
+  put_contact(d, mv, o, k, cm[q], cs[q], G + 3, G, G + 6, G[9], true, 0.0);
 }
 
 __global__ void k_append_planes(int np, int nb, int base, DomView d, MatView mv, ConOut o, const int *planes,
@@ -375,7 +425,7 @@ __global__ void k_append_planes(int np, int nb, int base, DomView d, MatView mv,
   double G[10];
   plane_convex(d, p, s, G, G + 3, G + 6, G + 9);
   for (int c = 0; c < 3; c ++) { o.R[3*(size_t)k + c] = 0.0; o.U[3*(size_t)k + c] = 0.0; }
-  put_contact(d, mv, o, k, s, p, G + 3, G, G + 6, G[9]);
+  put_contact(d, mv, o, k, s, p, G + 3, G, G + 6, G[9], true, 0.0);
 }
 
 int scan(sfgs_ctx *ctx, const int *in, int *out, int n)
@@ -407,6 +457,7 @@ MatView mat_view(sfgs_ctx *ctx)
   MatView m;
   m.nmat = ctx->nmat; m.s = ctx->mat_s.p; m.v = ctx->mat_v.p;
   for (int k = 0; k < 3; k ++) m.def[k] = ctx->defmat[k];
+  m.first = ctx->dom_time == 0.0;
   return m;
 }
 
@@ -503,11 +554,11 @@ int SFGS_Dom_Update_Begin(sfgs_ctx *ctx, double step, const double *gravity, int
                        ctx->rot.p, ctx->bx.p, ctx->aux.p);
 
   // existing contacts: geometry, kept flags and their order-preserving positions
-  CHK(ensure(ctx, ctx->geo, 10*(size_t)nc));
+  CHK(ensure(ctx, ctx->geo, GEO*(size_t)nc));
   CHK(ensure(ctx, ctx->okf, (size_t)nc + 1));
   CHK(ensure(ctx, ctx->okpos, (size_t)nc + 1));
   hipLaunchKernelGGL(k_con_update, dim3(nblk(nc + 1)), dim3(TPB), 0, st, nc, d, ctx->master.p, ctx->slave.p,
-                     ctx->geo.p, ctx->okf.p);
+                     ctx->coh.p, ctx->mpnt.p, ctx->spnt.p, ctx->base.p, ctx->gap.p, ctx->geo.p, ctx->okf.p);
   int kept = 0;
   CHK(scan_total(ctx, ctx->okf.p, ctx->okpos.p, nc, &kept));
 
@@ -596,7 +647,8 @@ int SFGS_Dom_Update_Begin(sfgs_ctx *ctx, double step, const double *gravity, int
   const size_t nn = (size_t)ntot;
   CHK(ensure(ctx, ctx->t_master, nn)); CHK(ensure(ctx, ctx->t_slave, nn));
   CHK(ensure(ctx, ctx->t_R, 3*nn)); CHK(ensure(ctx, ctx->t_U, 3*nn));
-  // the point and frame arrays are rewritten whole; their old contents are not read
+  // the point and frame arrays are rewritten whole; what the kept contacts need
+  // of their old contents is in geo (k_con_update)
   CHK(ensure(ctx, ctx->point, 3*nn)); CHK(ensure(ctx, ctx->mpnt, 3*nn)); CHK(ensure(ctx, ctx->spnt, 3*nn));
   CHK(ensure(ctx, ctx->base, 9*nn)); CHK(ensure(ctx, ctx->gap, nn)); CHK(ensure(ctx, ctx->fri, nn));
   CHK(ensure(ctx, ctx->rest, nn)); CHK(ensure(ctx, ctx->coh, nn));
@@ -620,6 +672,13 @@ int SFGS_Dom_Update_Begin(sfgs_ctx *ctx, double step, const double *gravity, int
   ctx->nc = (int)ntot;
   ctx->assembled = 0;
   if (ncon) *ncon = (int)ntot;
+  return 0;
+}
+
+int SFGS_Dom_Set_Time(sfgs_ctx *ctx, double time)
+{
+  if (!ctx || !(time >= 0.0)) return SFGS_E_ARG;
+  ctx->dom_time = time;
   return 0;
 }
 

@@ -740,13 +740,16 @@ __global__ void k_body_update(BodyView bv, ConView cv, int nb, const int *bptr, 
 // penalty solver: the spring-dashpot law per contact (pes.c:33-106, 247-281)
 // =====================================================================
 
-// PENALTY_Spring_Dashpot_Contact (pes.c:33-106) for a contact that is not
-// cohesive; the operations and their order are those of the reference, with no
-// contraction into fused multiply-adds.  A singular tangential block leaves
-// R_T as carried and U as it was (pes.c:79, the return value that
-// PENALTY_Solve ignores).
+// PENALTY_Spring_Dashpot_Contact (pes.c:33-106); the operations and their
+// order are those of the reference, with no contraction into fused
+// multiply-adds.  A singular tangential block leaves R_T as carried and U as it
+// was (pes.c:72, the return value that PENALTY_Solve ignores).  coh is the
+// contact's cohesion state (cohesive when > 0; it equals the material's
+// cohesion times the area while the contact is cohesive): a cohesive contact
+// takes tension down to -coh and loses its cohesion below it (mode I,
+// pes.c:76-82) or on the cone (mode II, pes.c:88-99).
 __device__ void penalty_law(int implicit, double h, double gap, double k, double c, double p, double mu,
-                            const double *W, const double *B, const double *V, double *U, double *R)
+                            const double *W, const double *B, const double *V, double *U, double *R, double *coh)
 {
 #pragma clang fp contract(off)
   const double BN = B[2] + W[2]*R[0] + W[5]*R[1];
@@ -766,7 +769,8 @@ __device__ void penalty_law(int implicit, double h, double gap, double k, double
     if (c < 0.0) c = 2.0*sqrt(h*s/W[8]);
     R[2] = k*pow(-g, p) - c*V[2];
   }
-  if (R[2] < 0.0)
+  bool cohesive = *coh > 0.0;
+  if (!cohesive && R[2] < 0.0)
   {
     R[0] = R[1] = R[2] = 0.0;
     U[0] = U[1] = U[2] = 0.0;
@@ -778,11 +782,18 @@ __device__ void penalty_law(int implicit, double h, double gap, double k, double
   const double I0 = W[4]/det, I1 = -W[1]/det, I2 = -W[3]/det, I3 = W[0]/det;
   R[0] = -I0*BT0 - I2*BT1;
   R[1] = -I1*BT0 - I3*BT1;
+  if (cohesive && R[2] < -*coh)
+  {
+    cohesive = false;
+    R[2] = -*coh;
+    *coh = 0.0;
+  }
   const double len = sqrt(R[0]*R[0] + R[1]*R[1]), lim = mu*fabs(R[2]);
   if (len > lim)
   {
     R[0] *= lim/len;
     R[1] *= lim/len;
+    if (cohesive) *coh = 0.0;
   }
   for (int r = 0; r < 3; r ++) U[r] = B[r] + (W[r]*R[0] + W[r + 3]*R[1] + W[r + 6]*R[2]);
 }
@@ -817,7 +828,7 @@ __device__ void penalty_material(const PenMat &pm, const ConView &cv, int i, dou
 // off-diagonal blocks of contacts, ldy.c:744), then the law from the carried R.
 // R goes to the 4-per-row store and U to the SoA planes in the identity order.
 __global__ __launch_bounds__(TPB) void k_penalty(BodyView bv, ConView cv, PenMat pm, int n, int implicit,
-                                                  double step, double *Rs, double *Us)
+                                                  double step, double *Rs, double *Us, double *coh)
 {
   int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -855,10 +866,26 @@ __global__ __launch_bounds__(TPB) void k_penalty(BodyView bv, ConView cv, PenMat
 #pragma unroll
   for (int k = 0; k < 3; k ++) { R[k] = cv.Rn[3*(size_t)i + k]; U[k] = cv.Un[3*(size_t)i + k]; }
   penalty_material(pm, cv, i, kcp);
-  penalty_law(implicit, step, cv.gap[i], kcp[0], kcp[1], kcp[2], cv.fri[i], Wi, Bv, Vv, U, R);
+  double c = coh[i];
+  penalty_law(implicit, step, cv.gap[i], kcp[0], kcp[1], kcp[2], cv.fri[i], Wi, Bv, Vv, U, R, &c);
+  if (c != coh[i]) coh[i] = c;
 #pragma unroll
   for (int k = 0; k < 3; k ++) { Rs[4*(size_t)i + k] = R[k]; Us[(size_t)k*n + i] = U[k]; }
   Rs[4*(size_t)i + 3] = 0.0;
+}
+
+// update_cohesion (ldy.c:69-95) per row p of the solved system, on the
+// contact's cohesion state in natural order
+__global__ void k_cohesion(int n, const int *perm, const double *R, const double *fri, double *coh)
+{
+#pragma clang fp contract(off)
+  int p = blockIdx.x*blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i = perm[p];
+  const double c = coh[i];
+  if (!(c > 0.0)) return;
+  const double e = 1e-8*c, r0 = R[4*(size_t)p], r1 = R[4*(size_t)p + 1], r2 = R[4*(size_t)p + 2];
+  if (r2 + c < e || sqrt(r0*r0 + r1*r1) + e >= fri[i]*(r2 + c)) coh[i] = 0.0;
 }
 
 __global__ void k_identity(int n, int *perm, int *iperm)
@@ -1807,6 +1834,38 @@ int SFGS_Get_RU(sfgs_ctx *ctx, double *R, double *U)
   return 0;
 }
 
+int SFGS_Update_Cohesion(sfgs_ctx *ctx)
+{
+  if (!ctx) return SFGS_E_ARG;
+  if (!ctx->assembled) { ctx->err = "no system"; return SFGS_E_STATE; }
+  if (!ctx->from_bodies) { ctx->err = "cohesion states belong to uploaded contacts, not to an uploaded system"; return SFGS_E_STATE; }
+  HIPCHK(hipSetDevice(ctx->device));
+  const int n = ctx->nc;
+  if (n == 0) return 0;
+  if (ctx->coh.cap < (size_t)n || ctx->fri.cap < (size_t)n || ctx->perm.cap < (size_t)n || ctx->R.cap < 4*(size_t)n)
+  {
+    ctx->err = "no contact records for the solved system";
+    return SFGS_E_STATE;
+  }
+  hipLaunchKernelGGL(k_cohesion, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->perm.p, ctx->R.p, ctx->fri.p,
+                     ctx->coh.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int SFGS_Get_Cohesion(sfgs_ctx *ctx, double *cohesion)
+{
+  if (!ctx || (ctx->nc > 0 && !cohesion)) return SFGS_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  const size_t n = (size_t)ctx->nc;
+  if (n == 0) return 0;
+  if (ctx->coh.cap < n) { ctx->err = "no contact records"; return SFGS_E_STATE; }
+  HIPCHK(hipMemcpyAsync(cohesion, ctx->coh.p, sizeof(double)*n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
 int SFGS_Get_Sizes(sfgs_ctx *ctx, int *n, long long *nnz, int *ncolours)
 {
   if (!ctx) return SFGS_E_ARG;
@@ -1974,7 +2033,7 @@ int SFGS_Penalty_Solve(sfgs_ctx *ctx, double step, int implicit, const double *s
               ctx->nmat};
     hipLaunchKernelGGL(k_identity, dim3(nblk(n)), dim3(TPB), 0, st, n, ctx->perm.p, ctx->iperm.p);
     hipLaunchKernelGGL(k_penalty, dim3(nblk(n)), dim3(TPB), 0, st, bv, cv, pm, n, implicit ? 1 : 0, step, ctx->R.p,
-                       ctx->U.p);
+                       ctx->U.p, ctx->coh.p);
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));

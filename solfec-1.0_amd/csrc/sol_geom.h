@@ -124,6 +124,14 @@ SOL_GEOM_FN void geom_ref_point(const double *rot, const double *x, const double
   for (int k = 0; k < 3; k ++) X[k] = x0[k] + t[k];
 }
 
+/* spatial image of a referential point (BODY_Cur_Point, bod.c:1706-1715): p = x + R(X - X0) */
+SOL_GEOM_FN void geom_cur_point(const double *rot, const double *x, const double *x0, const double *X, double *p)
+{
+  double d[3] = {X[0] - x0[0], X[1] - x0[1], X[2] - x0[2]}, t[3];
+  geom_mv(rot, d, t);
+  for (int k = 0; k < 3; k ++) p[k] = x[k] + t[k];
+}
+
 /* sphere pair given as (slave a, master b): 1 when in contact (gap below
  * GEOMETRIC_EPSILON, goc.c:658-677 / gjk.c:560-584), with the points, the
  * normal from a towards b and the gap (sphere_sphere_gap, goc.c:50-59) */

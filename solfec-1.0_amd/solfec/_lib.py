@@ -63,6 +63,8 @@ _SFGS_PROTOS = {
     "SFGS_Solve": (_i, [_p, C.POINTER(SFGS_PARAM), _p, _p, C.POINTER(SFGS_REPORT)]),
     "SFGS_Merit": (_i, [_p, _i, _p, _p]),
     "SFGS_Get_RU": (_i, [_p, _p, _p]),
+    "SFGS_Update_Cohesion": (_i, [_p]),
+    "SFGS_Get_Cohesion": (_i, [_p, _p]),
     "SFGS_Get_Locdyn": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "SFGS_Get_Sizes": (_i, [_p, _p, _p, _p]),
     "SFGS_Colour": (_i, [_p, _i, _p, _p, _p]),

@@ -168,6 +168,16 @@ class Device:
         self._check(self.lib.SFGS_Get_RU(self.ctx, ptr(R), ptr(U)), "SFGS_Get_RU")
         return R.reshape(n, 3), U.reshape(n, 3)
 
+    def update_cohesion(self):
+        """SFGS_Update_Cohesion: update_cohesion of LOCDYN_Update_End (ldy.c:69-95) after a solve."""
+        self._check(self.lib.SFGS_Update_Cohesion(self.ctx), "SFGS_Update_Cohesion")
+
+    def cohesion(self):
+        """The contacts' cohesion states, natural order."""
+        c = np.zeros(self.sizes()[0])
+        self._check(self.lib.SFGS_Get_Cohesion(self.ctx, ptr(c)), "SFGS_Get_Cohesion")
+        return c
+
     def locdyn(self):
         n, nnz, _ = self.sizes()
         out = dict(rowptr=np.zeros(n + 1, np.int32), col=np.zeros(nnz, np.int32), via=np.zeros(nnz, np.int32),

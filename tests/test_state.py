@@ -107,3 +107,25 @@ def test_state_file_mismatch_is_refused(tmp_path):
     with pytest.raises(RuntimeError):
         u.read_state(tmp_path / "cut")
     np.testing.assert_array_equal(u.bodies()["x"], before)    # a bad file leaves the domain as it was
+
+
+def test_cohesion_state_round_trip(tmp_path):
+    """Contacts created at time 0 with a cohesive material carry its cohesion
+    (SURFACE_MATERIAL_Transfer, sps.c:264-282); the state file holds it as the
+    surface material state of size 1 (sps.c:65-69, 286-294), and a fresh domain
+    reads it back into the contacts."""
+    s = lattice(125, L=0.45, seed=3, cohesion=0.7)
+    c = s.contacts()
+    assert len(c["master"]) > 100
+    np.testing.assert_array_equal(c["cohesion"], 0.7)
+    path = tmp_path / "state.xdr"
+    s.write_state(path)
+    _, _, _, _, _, cons = decode(path)
+    assert all(r["state"] == (0.7,) for r in cons)
+    t = lattice(125, L=0.45, seed=3, cohesion=0.7, steps_begin=0)
+    t.read_state(path)
+    np.testing.assert_array_equal(t.contacts()["cohesion"], 0.7)
+    # the same file read into a domain whose material is not cohesive: no cohesion (sps.c:72-80)
+    u = lattice(125, L=0.45, seed=3, steps_begin=0)
+    u.read_state(path)
+    np.testing.assert_array_equal(u.contacts()["cohesion"], 0.0)

@@ -347,6 +347,8 @@ typedef struct {
   double omega;        /* smoothing of the cone residual, > 0 */
   int gsflag;          /* 1: Gauss-Seidel from the initial R after an unconverged solve */
   int reldelta;        /* delta relative to W: 0 off, 1 mean, 2 min, 3 max of the W_ii diagonals */
+  int bodyspace;       /* 1: locdyn 'OFF' -- W x and U = B + W R through the bodies, H (h M^-1 H' x)
+                          (nts.c:396-431, 683-852); 0: the assembled W blocks (locdyn 'ON') */
 } SFGS_NEWTON_PARAM;
 
 typedef struct {

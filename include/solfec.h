@@ -111,7 +111,7 @@ NEWTON* NEWTON_Create(double meritval, int maxiter);
 void NEWTON_Destroy(NEWTON *ns);   /* nts.c:1388-1394 */
 /* members by name: meritval, maxiter, linver (SFGS_NT_*), linmaxiter,
  * maxmatvec, epsilon, delta, theta, omega, gsflag, reldelta (0 OFF, 1 avgWii,
- * 2 minWii, 3 maxWii), iters; "itershist" (set only) switches the per-solve
+ * 2 minWii, 3 maxWii), locdyn (1 ON, 0 OFF: body-space products), iters; "itershist" (set only) switches the per-solve
  * iteration history on or off.  NEWTON_Set returns -1 for an invalid value */
 double NEWTON_Get(NEWTON *ns, const char *member);
 int NEWTON_Set(NEWTON *ns, const char *member, double value);

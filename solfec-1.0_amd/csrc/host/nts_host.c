@@ -55,6 +55,7 @@ double NEWTON_Get(NEWTON *ns, const char *m)
   if (!strcmp(m, "omega")) return ns->par.omega;
   if (!strcmp(m, "gsflag")) return ns->par.gsflag;
   if (!strcmp(m, "reldelta")) return ns->par.reldelta;
+  if (!strcmp(m, "locdyn")) return ns->par.bodyspace ? 0.0 : 1.0;
   if (!strcmp(m, "iters")) return ns->iters;
   return -1.0;
 }
@@ -73,6 +74,7 @@ int NEWTON_Set(NEWTON *ns, const char *m, double v)
   if (!strcmp(m, "omega")) { if (!(v > 0.0)) return -1; ns->par.omega = v; return 0; }
   if (!strcmp(m, "gsflag")) { ns->par.gsflag = v != 0.0; return 0; }
   if (!strcmp(m, "reldelta")) { if (v < 0 || v > 3) return -1; ns->par.reldelta = (int)v; return 0; }
+  if (!strcmp(m, "locdyn")) { ns->par.bodyspace = v == 0.0; return 0; }   /* 1 ON, 0 OFF (nts.h:40) */
   if (!strcmp(m, "itershist")) { ns->itershistcount = v != 0.0 ? 0 : -1; return 0; }
   return -1;
 }

@@ -111,6 +111,7 @@ struct sfgs_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   std::vector<hipEvent_t> prof_ev;        // per-launch sweep timing (profile mode)
   sfgs::DBuf<double> body_out;            // body update results (velo, force)
+  sfgs::DBuf<double> bsv;                 // body-space products: h M^-1 H' x per body (6 per body)
   sfgs::DBuf<float> feat;                 // learned warm start: per-row features (18 per row)
   void *mlp = nullptr;                    // learned warm start model (sfgs_mlp.hip)
 

@@ -97,5 +97,11 @@ int run_bar(sfgs_ctx *ctx, const SweepArgs<float> &sa, float *X0, float *X1, int
 int build_f32_mirror(sfgs_ctx *ctx);
 // re-run the assembly of the current bodies and contacts (after a layout change)
 int reassemble(sfgs_ctx *ctx);
+// body-space operator of NEWTON_SOLVER with locdyn OFF: z = W x formed as
+// H (h M^-1 H' x) through the bodies (x at x + stride p for row p); mode 0:
+// y = beta y + alpha (X z + Y x) + alpha delta x (x 3 per row), mode 1:
+// U = B + z (U, B in SoA planes)
+int bodyspace_product(sfgs_ctx *ctx, const double *x, int stride, int mode, const double *X, const double *Y,
+                      double alpha, double beta, double delta, double *y);
 
 } // namespace sfgs

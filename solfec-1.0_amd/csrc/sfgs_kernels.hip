@@ -736,6 +736,114 @@ __global__ void k_body_update(BodyView bv, ConView cv, int nb, const int *bptr, 
   for (int k = 0; k < 6; k ++) velo_out[6*(size_t)b + k] = u[k];
 }
 
+// body-space products of NEWTON_SOLVER with locdyn OFF (U_WR_B, nts.c:396-431):
+// H_trans_R and BODY_Invvec per body, v_b = h M_b^-1 sum_c (+/-) H_c' x_c with
+// x_c at x + stride p, p = iperm[c]; obstacles carry none (bod.c:431)
+__global__ void k_bs_bodies(BodyView bv, ConView cv, int nb, const int *bptr, const int *blist, const int *iperm,
+                            const double *x, int stride, double step, double *v)
+{
+  int b = blockIdx.x*blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double f[6] = {0, 0, 0, 0, 0, 0};
+  if (bv.kind[b] == 0)
+  {
+    const double *Rot = bv.rot + 9*(size_t)b, *X0 = bv.x0 + 3*(size_t)b;
+    for (int q = bptr[b]; q < bptr[b + 1]; q ++)
+    {
+      int i = blist[q];
+      double sg = cv.master[i] == b ? 1.0 : -1.0, bs[9], Hr[9];
+      const double *X = sg > 0.0 ? cv.mpnt + 3*(size_t)i : cv.spnt + 3*(size_t)i;
+#pragma unroll
+      for (int e = 0; e < 9; e ++) bs[e] = cv.base[9*(size_t)i + e];
+      operator_Hr(Rot, X0, X, bs, Hr);
+      const double *r = x + (size_t)stride*iperm[i];
+      double r0 = r[0], r1 = r[1], r2 = r[2];
+#pragma unroll
+      for (int c = 0; c < 3; c ++)
+      {
+        f[c] += sg * (Hr[3*c]*r0 + Hr[3*c + 1]*r1 + Hr[3*c + 2]*r2);
+        f[3 + c] += sg * (bs[c]*r0 + bs[c + 3]*r1 + bs[c + 6]*r2);
+      }
+    }
+    const double *J = bv.jinv + 9*(size_t)b;
+    const double minv = bv.minv[b];
+    double u[6];
+#pragma unroll
+    for (int r = 0; r < 3; r ++)
+    {
+      u[r] = step * (J[r]*f[0] + J[r + 3]*f[1] + J[r + 6]*f[2]);
+      u[3 + r] = step * minv * f[3 + r];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k ++) f[k] = u[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k ++) v[6*(size_t)b + k] = f[k];
+}
+
+// H_times_u per row p: z = H_m v_m - H_s v_s; mode 0 the step operator of
+// Matvec (nts.c:584-652), y = beta y + alpha (X z + Y x_p) + alpha delta x_p;
+// mode 1 U = B + z
+__global__ __launch_bounds__(TPB) void k_bs_rows(BodyView bv, ConView cv, int n, const int *perm, const double *v,
+                                                  int mode, const double *Xm, const double *Ym, const double *x,
+                                                  double alpha, double beta, double delta, double *y,
+                                                  const double *B, double *U)
+{
+  int p = blockIdx.x*blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i = perm[p], m = cv.master[i], s = cv.slave[i];
+  double bs[9], Hr[9], z[3], t[3];
+#pragma unroll
+  for (int k = 0; k < 9; k ++) bs[k] = cv.base[9*(size_t)i + k];
+  operator_Hr(bv.rot + 9*(size_t)m, bv.x0 + 3*(size_t)m, cv.mpnt + 3*(size_t)i, bs, Hr);
+  h_u(Hr, bs, v + 6*(size_t)m, z);
+  if (s >= 0)
+  {
+    operator_Hr(bv.rot + 9*(size_t)s, bv.x0 + 3*(size_t)s, cv.spnt + 3*(size_t)i, bs, Hr);
+    h_u(Hr, bs, v + 6*(size_t)s, t);
+    z[0] -= t[0]; z[1] -= t[1]; z[2] -= t[2];
+  }
+  if (mode == 1)
+  {
+#pragma unroll
+    for (int k = 0; k < 3; k ++) U[(size_t)k*n + p] = B[(size_t)k*n + p] + z[k];
+    return;
+  }
+  const double xp[3] = {x[3*(size_t)p], x[3*(size_t)p + 1], x[3*(size_t)p + 2]};
+  double X[9], Y[9];
+#pragma unroll
+  for (int e = 0; e < 9; e ++) { X[e] = Xm[(size_t)e*n + p]; Y[e] = Ym[(size_t)e*n + p]; }
+  const double ad = alpha*delta;
+#pragma unroll
+  for (int k = 0; k < 3; k ++)
+  {
+    const double u = (X[k]*z[0] + X[k + 3]*z[1] + X[k + 6]*z[2]) + (Y[k]*xp[0] + Y[k + 3]*xp[1] + Y[k + 6]*xp[2]);
+    double q = beta*y[3*(size_t)p + k];
+    q = q + alpha*u;
+    y[3*(size_t)p + k] = q + ad*xp[k];
+  }
+}
+
+int sfgs::bodyspace_product(sfgs_ctx *ctx, const double *x, int stride, int mode, const double *X, const double *Y,
+                            double alpha, double beta, double delta, double *y)
+{
+  const int n = ctx->nc, nb = ctx->nb;
+  if (n == 0) return 0;
+  if (!ctx->from_bodies) { ctx->err = "body-space products need a system assembled from bodies"; return SFGS_E_STATE; }
+  CHK(ensure(ctx, ctx->bsv, 6*(size_t)std::max(nb, 1)));
+  hipStream_t st = ctx->stream;
+  BodyView bv{ctx->kind.p, ctx->rot.p, ctx->x0.p, ctx->jinv.p, ctx->minv.p, ctx->velo.p, ctx->velo0.p};
+  ConView cv{ctx->master.p, ctx->slave.p, ctx->mpnt.p, ctx->spnt.p, ctx->base.p, ctx->gap.p, ctx->fri.p,
+             ctx->rest.p, ctx->coh.p, ctx->Rn.p, ctx->Un.p};
+  if (nb > 0)
+    hipLaunchKernelGGL(k_bs_bodies, dim3(nblk(nb)), dim3(TPB), 0, st, bv, cv, nb, ctx->bptr.p, ctx->blist.p,
+                       ctx->iperm.p, x, stride, ctx->step, ctx->bsv.p);
+  hipLaunchKernelGGL(k_bs_rows, dim3(nblk(n)), dim3(TPB), 0, st, bv, cv, n, ctx->perm.p, ctx->bsv.p, mode, X, Y, x,
+                     alpha, beta, delta, y, ctx->B.p, ctx->U.p);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 // =====================================================================
 // penalty solver: the spring-dashpot law per contact (pes.c:33-106, 247-281)
 // =====================================================================
@@ -1170,6 +1278,7 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   if (ctx->ev3) (void)hipEventDestroy(ctx->ev3);
   for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
   release(ctx->body_out);
+  release(ctx->bsv);
   release(ctx->feat);
   sfgs_mlp_release(ctx);
   sfgs_dom_release(ctx);

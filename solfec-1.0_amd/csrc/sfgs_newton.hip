@@ -276,6 +276,7 @@ struct NtWork {
   double *X, *Y, *P, *b, *dr, *R0, *kry, *pre, *w, *r, *part;
   NtScal *sc;
   NtMat mat;
+  int bodyspace = 0;
   int matvecs = 0;
   std::vector<double> hpart;
   double rs0 = 0.0, rnorm_h = 0.0;   // host ends of the small copies
@@ -285,6 +286,12 @@ struct NtWork {
 
   int matvec(double alpha, const double *x, double beta, double *y)
   {
+    if (bodyspace)
+    {
+      CHK(bodyspace_product(ctx, x, 3, 0, X, Y, alpha, beta, mat.delta, y));
+      matvecs ++;
+      return 0;
+    }
     const int ns = ctx->nslices, grid = (ns + TPB/SLICE - 1) / (TPB/SLICE);
     if (grid > 0)
       NT_DISPATCH_G(ctx->G_built, auto kern = k_nt_matvec<GG>; hipLaunchKernelGGL(kern, dim3(grid), dim3(TPB), 0,
@@ -461,7 +468,7 @@ int SFGS_Newton_Solve(sfgs_ctx *ctx, const SFGS_NEWTON_PARAM *par, double *merhi
   if (!(par->meritval > 0.0) || par->maxiter <= 0 || par->linmaxiter <= 0 || par->maxmatvec <= 0 ||
       !(par->epsilon > 0.0) || !(par->delta >= 0.0) || !(par->theta > 0.0 && par->theta <= 1.0) ||
       !(par->omega > 0.0) || (par->linver != SFGS_NT_GMRES && par->linver != SFGS_NT_DIAG) ||
-      par->reldelta < 0 || par->reldelta > 3) return SFGS_E_ARG;
+      par->reldelta < 0 || par->reldelta > 3 || (par->bodyspace != 0 && par->bodyspace != 1)) return SFGS_E_ARG;
   if (!ctx->assembled) { ctx->err = "solve before LOCDYN assembly"; return SFGS_E_STATE; }
   if (ctx->diag_only) { ctx->err = "the store holds a penalty solve, not an assembled system"; return SFGS_E_STATE; }
   if (ctx->nscenes > 1) { ctx->err = "the system holds independent scenes"; return SFGS_E_STATE; }
@@ -516,8 +523,21 @@ int SFGS_Newton_Solve(sfgs_ctx *ctx, const SFGS_NEWTON_PARAM *par, double *merhi
   NtRows rows{n, ctx->R.p, ctx->U.p, ctx->Wd.p, ctx->pfri.p, ctx->pcoh.p, ctx->shift.p, wk.X, wk.Y, wk.P, wk.b, wk.dr,
               ctx->failcount.p};
 
+  wk.bodyspace = par->bodyspace;
+  if (par->bodyspace && !ctx->from_bodies)
+  {
+    ctx->err = "locdyn OFF needs a system assembled from bodies";
+    return SFGS_E_STATE;
+  }
+  // U = W R + B (U_WR_B, nts.c:255-431), then the merit (nts.c:929, 1199); in
+  // body space W R is formed through the bodies
+  auto merit_of = [&](double &m) -> int {
+    if (!par->bodyspace) return SFGS_Merit(ctx, 1, &m, nullptr);
+    CHK(bodyspace_product(ctx, ctx->R.p, 4, 1, nullptr, nullptr, 0.0, 0.0, 0.0, nullptr));
+    return SFGS_Merit(ctx, 0, &m, nullptr);
+  };
   double merit = 0.0;
-  CHK(SFGS_Merit(ctx, 1, &merit, nullptr));   // U = W R + B, then the merit (nts.c:929, 1199)
+  CHK(merit_of(merit));
   int iters = 0, nan = 0;
   const bool diag = par->linver == SFGS_NT_DIAG;
   while (iters < par->maxiter && (diag || wk.matvecs < par->maxmatvec) && merit > par->meritval)
@@ -535,7 +555,7 @@ int SFGS_Newton_Solve(sfgs_ctx *ctx, const SFGS_NEWTON_PARAM *par, double *merhi
         hipLaunchKernelGGL(k_nt_update, dim3(nblk(n)), dim3(TPB), 0, st, n, ctx->R.p, wk.dr, ctx->pfri.p, ctx->pcoh.p);
       HIPCHK(hipGetLastError());
     }
-    CHK(SFGS_Merit(ctx, 1, &merit, nullptr));
+    CHK(merit_of(merit));
     if (std::isnan(merit)) { nan = 1; break; }
     if (merhist) merhist[iters] = merit;
     if (mvhist) mvhist[iters] = diag ? 0 : wk.matvecs;
@@ -552,6 +572,14 @@ int SFGS_Newton_Solve(sfgs_ctx *ctx, const SFGS_NEWTON_PARAM *par, double *merhi
   r.singular = singular;
   r.merit = merit;
   r.W_norm = W_norm;
+  if (!r.converged && par->gsflag && par->bodyspace)
+  {
+    // the reference aborts here (nts.c:1370-1374: no Gauss-Seidel without the
+    // assembled blocks); this library reports the failure instead
+    if (rep) *rep = r;
+    ctx->err = "NEWTON_SOLVER failed in body space mode";
+    return SFGS_E_STATE;
+  }
   if (!r.converged && par->gsflag)
   {
     // R back to its initial value; GAUSS_SEIDEL_Create (1.0, maxiter, meritval,

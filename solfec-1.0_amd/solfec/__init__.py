@@ -268,8 +268,10 @@ class NEWTON_SOLVER:
     (lng.c:4740-4857): a projected semismooth Newton solve of all contacts at once
     (nts.c) on the GPU; linver 'GMRES' (flexible GMRES step) or 'DIAG' (per-row
     steps); gsflag 'ON' runs a Gauss-Seidel solve from the initial reactions when
-    the Newton solve ends above meritval.  Only locdyn 'ON' (assembled local
-    dynamics) is built.
+    the Newton solve ends above meritval.  locdyn 'OFF' forms the W products
+    through the bodies, H (h M^-1 H' x) (nts.c:396-431); a solve that ends
+    above meritval with gsflag 'ON' then raises, where the reference aborts
+    (nts.c:1370-1374).
 
     Read-only members: merhist, mvhist, iters, itershist (set 'ON' to collect).
     Read/write members: meritval, maxiter, locdyn, linver, linmaxiter, maxmatvec,
@@ -329,7 +331,7 @@ class NEWTON_SOLVER:
         if name in ("maxiter", "linmaxiter", "maxmatvec", "iters"):
             return int(self._get(name))
         if name == "locdyn":
-            return "ON"
+            return "ON" if int(self._get("locdyn")) else "OFF"
         if name == "linver":
             return "DIAG" if int(self._get("linver")) == 1 else "GMRES"
         if name == "gsflag":
@@ -367,8 +369,7 @@ class NEWTON_SOLVER:
         elif name == "locdyn":
             if value not in ("ON", "OFF"):
                 raise ValueError("Invalid locdyn value: neither ON nor OFF")
-            if value == "OFF":
-                raise ValueError("locdyn 'OFF' (body-space Newton) is not built; use 'ON'")
+            lib.NEWTON_Set(ns, b"locdyn", 1.0 if value == "ON" else 0.0)
         elif name == "linver":
             if value not in self._LINVER:
                 raise ValueError("Invalid linver value: neither GMRES nor DIAG")

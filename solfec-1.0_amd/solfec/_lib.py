@@ -34,7 +34,8 @@ class SFGS_REPORT(C.Structure):
 
 class SFGS_NEWTON_PARAM(C.Structure):
     _fields_ = [("meritval", _d), ("maxiter", _i), ("linver", _i), ("linmaxiter", _i), ("maxmatvec", _i),
-                ("epsilon", _d), ("delta", _d), ("theta", _d), ("omega", _d), ("gsflag", _i), ("reldelta", _i)]
+                ("epsilon", _d), ("delta", _d), ("theta", _d), ("omega", _d), ("gsflag", _i), ("reldelta", _i),
+                ("bodyspace", _i)]
 
 
 class SFGS_NEWTON_REPORT(C.Structure):

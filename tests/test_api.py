@@ -230,8 +230,10 @@ def test_newton_solver_members_and_validation():
     ns.theta = 1.0
     ns.gsflag = "OFF"
     assert (ns.linver, ns.theta, ns.gsflag) == ("DIAG", 1.0, "OFF")
+    ns.locdyn = "OFF"
+    assert ns.locdyn == "OFF" and solfec.NEWTON_SOLVER(locdyn="OFF").locdyn == "OFF"
     for k, v in (("linver", "CG"), ("theta", 0.0), ("theta", 1.5), ("delta", -1.0), ("maxiter", 0),
-                 ("locdyn", "MAYBE"), ("locdyn", "OFF"), ("gsflag", "YES")):
+                 ("locdyn", "MAYBE"), ("gsflag", "YES")):
         with pytest.raises(ValueError):
             setattr(ns, k, v)
     with pytest.raises(ValueError):

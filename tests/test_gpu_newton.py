@@ -163,3 +163,57 @@ def test_other_solvers_run_empty_domains(solver, loop):
         assert s.ncon == 0 and s.time == pytest.approx(0.003)
         if solver == "newton":
             assert sv.iters == 0 and sv.report().converged == 1
+
+
+@pytest.mark.parametrize("linver", ["GMRES", "DIAG"])
+def test_newton_body_space_equals_assembled(dev, linver):
+    """locdyn OFF forms W x and U = B + W R through the bodies, H (h M^-1 H' x)
+    (nts.c:396-431); the same Newton steps on the assembled blocks give the
+    same iterates up to the rounding of the products.  Three steps: inexact
+    GMRES steps let rounding differences grow over a long solve (at 60 steps
+    the two merits differ by orders of magnitude while both decrease), so the
+    operator is compared where the paths still coincide."""
+    print(f"[newton] body space against assembled, {linver}", flush=True)
+    s = lattice(343, seed=5)
+    b, c = s.bodies(), s.contacts()
+
+    def run(bodyspace):
+        dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
+        dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
+                     c["restitution"], c["cohesion"], c["R"], c["U"])
+        dev.assemble(H)
+        p = solfec.newton_param(meritval=1e-14, maxiter=3, linver=linver, gsflag="OFF")
+        p.bodyspace = bodyspace
+        rep, mer, mv = dev.newton(p)
+        R, U = dev.get_RU()
+        return rep, np.asarray(mer), R, U
+    ra, ma, Ra, Ua = run(0)
+    rb, mb, Rb, Ub = run(1)
+    k = min(ra.iters, rb.iters)
+    err = np.abs(Rb - Ra).max() / np.abs(Ra).max()
+    print(f"[newton] iters {ra.iters} / {rb.iters}, merit {ra.merit:.3e} / {rb.merit:.3e}, max |dR|/max|R| {err:.2e}",
+          flush=True)
+    assert ra.iters == rb.iters == 3
+    assert np.allclose(ma, mb, rtol=1e-8, atol=0.0)
+    assert err < 1e-8
+    assert np.abs(Ub - Ua).max() <= 1e-8 * np.abs(Ua).max()
+
+
+def test_newton_body_space_run_device_loop_equals_host_loop():
+    ns = solfec.NEWTON_SOLVER(meritval=1e-8, maxiter=200, locdyn="OFF", gsflag="OFF")
+    ref = solfec.NEWTON_SOLVER(meritval=1e-8, maxiter=200, gsflag="OFF")
+    host, dvc, asm = newton_scene(5), newton_scene(5), newton_scene(5)
+    dvc.device_loop(True)
+    solfec.RUN(host, ns, 0.01)
+    solfec.RUN(dvc, ns, 0.01)
+    solfec.RUN(asm, ref, 0.01)
+    bh, bd, ba = host.bodies(), dvc.bodies(), asm.bodies()
+    for k in ("x", "rot", "velo"):
+        assert np.array_equal(bh[k], bd[k]), k
+    ch, cd = host.contacts(), dvc.contacts()
+    for k in ("R", "U"):
+        assert np.array_equal(ch[k], cd[k]), k
+    d = np.abs(bh["velo"] - ba["velo"]).max()
+    print(f"[newton] body space against assembled after 10 steps: max |d velo| {d:.2e}"
+          f" of {np.abs(ba['velo']).max():.2e}", flush=True)
+    assert d <= 1e-3 * np.abs(ba["velo"]).max()

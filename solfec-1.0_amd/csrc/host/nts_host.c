@@ -29,6 +29,7 @@ NEWTON* NEWTON_Create(double meritval, int maxiter)
   ns->par.omega = 1e-10;
   ns->par.gsflag = 1;
   ns->par.reldelta = 0;
+  ns->par.bodyspace = 0;   /* locdyn ON (nts.c:1312) */
   ns->itershistcount = -1;
   return ns;
 }

@@ -35,6 +35,7 @@ typedef int (*GAUSS_SEIDEL_Callback)(void *data);  /* returns 0 to stop the prog
 #define SOL_ERR_GPU                            4
 #define SOL_ERR_CALLBACK_EXIT                  5
 #define SOL_ERR_ARG                            6
+#define SOL_ERR_MEM                            7   /* a host array could not be sized: count out of range or no memory */
 
 /* body kinds */
 #define SOL_RIG 0

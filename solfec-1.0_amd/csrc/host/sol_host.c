@@ -42,12 +42,32 @@ static double now_s(void)
   return (double)t.tv_sec + 1e-9*(double)t.tv_nsec;
 }
 
-static void* xrealloc(void *p, size_t bytes)
+/* no host array of a scene comes near this many bytes: a larger request comes
+ * from a corrupted or overflowed count */
+#define SOL_MAX_BYTES ((size_t)1 << 40)
+
+/* resizes the array *pp (any pointer type, passed by address) to count
+ * elements of elem bytes.  A negative count, one beyond SOL_MAX_BYTES, or a
+ * failed realloc leaves the array as it was, records the reason and returns
+ * SOL_ERR_MEM; nothing here ends the process. */
+static int grow(char *err, size_t errlen, void *pp, long long count, size_t elem)
 {
-  void *q = realloc(p, bytes ? bytes : 1);
-  if (!q) { fprintf(stderr, "solfec: out of memory (%zu bytes)\n", bytes); abort(); }
-  return q;
+  void **p = (void**)pp;
+  if (count < 0 || (unsigned long long)count > SOL_MAX_BYTES / elem)
+  {
+    if (err) snprintf(err, errlen, "refused to allocate %lld elements of %zu bytes", count, elem);
+    return SOL_ERR_MEM;
+  }
+  void *q = realloc(*p, count > 0 ? (size_t)count*elem : 1);
+  if (!q)
+  {
+    if (err) snprintf(err, errlen, "out of memory (%lld elements of %zu bytes)", count, elem);
+    return SOL_ERR_MEM;
+  }
+  *p = q;
+  return 0;
 }
+#define GROW(arr, count) grow(sol->err, sizeof(sol->err), &(arr), (long long)(count), sizeof(*(arr)))
 
 static int dev_pull(SOLFEC *sol);
 
@@ -109,6 +129,7 @@ const char* SOLFEC_Error_String(int code)
   case SOL_ERR_GPU: return "GPU contact solver failed";
   case SOL_ERR_CALLBACK_EXIT: return "GAUSS_SEIDEL_SOLVER failure callback requested exit";
   case SOL_ERR_ARG: return "invalid argument";
+  case SOL_ERR_MEM: return "host memory allocation refused or failed";
   }
   return "unknown error";
 }
@@ -143,7 +164,7 @@ int SOLFEC_Surface_Material_Ex(SOLFEC *sol, int s1, int s2, int model, double fr
     sol->defmat = m;
     return 0;
   }
-  sol->mat = xrealloc(sol->mat, sizeof(MATPAIR)*(size_t)(sol->nmat + 1));
+  if (GROW(sol->mat, sol->nmat + 1LL)) return SOL_ERR_MEM;
   sol->mat[sol->nmat ++] = m;
   return 0;
 }
@@ -160,27 +181,20 @@ static int add_body(SOLFEC *sol)
   dev_edit(sol);
   if (sol->nb == sol->bcap)
   {
-    int cap = sol->bcap ? 2*sol->bcap : 64;
-    sol->kind = xrealloc(sol->kind, sizeof(int)*(size_t)cap);
-    sol->shape = xrealloc(sol->shape, sizeof(int)*(size_t)cap);
-    sol->surf = xrealloc(sol->surf, sizeof(int)*(size_t)cap);
-    sol->rad = xrealloc(sol->rad, sizeof(double)*(size_t)cap);
-    sol->plane = xrealloc(sol->plane, sizeof(double)*6*(size_t)cap);
-    sol->sca = xrealloc(sol->sca, sizeof(double)*3*(size_t)cap);
-    sol->rot0 = xrealloc(sol->rot0, sizeof(double)*9*(size_t)cap);
-    sol->rot = xrealloc(sol->rot, sizeof(double)*9*(size_t)cap);
-    sol->x = xrealloc(sol->x, sizeof(double)*3*(size_t)cap);
-    sol->x0 = xrealloc(sol->x0, sizeof(double)*3*(size_t)cap);
-    sol->J = xrealloc(sol->J, sizeof(double)*9*(size_t)cap);
-    sol->jinv = xrealloc(sol->jinv, sizeof(double)*9*(size_t)cap);
-    sol->minv = xrealloc(sol->minv, sizeof(double)*(size_t)cap);
-    sol->mass = xrealloc(sol->mass, sizeof(double)*(size_t)cap);
-    sol->velo = xrealloc(sol->velo, sizeof(double)*6*(size_t)cap);
-    sol->velo0 = xrealloc(sol->velo0, sizeof(double)*6*(size_t)cap);
-    sol->aux = xrealloc(sol->aux, sizeof(double)*3*(size_t)cap);
-    sol->force = xrealloc(sol->force, sizeof(double)*6*(size_t)cap);
-    sol->tmp_velo = xrealloc(sol->tmp_velo, sizeof(double)*6*(size_t)cap);
-    sol->bcap = cap;
+    long long cap = sol->bcap ? 2LL*sol->bcap : 64;
+    if (cap > 0x3fffffff)
+    {
+      snprintf(sol->err, sizeof(sol->err), "too many bodies");
+      return -1;
+    }
+    /* arrays already grown keep their new size when a later one fails; bcap
+     * then stays at the size all of them have */
+    if (GROW(sol->kind, cap) || GROW(sol->shape, cap) || GROW(sol->surf, cap) || GROW(sol->rad, cap) ||
+        GROW(sol->plane, 6*cap) || GROW(sol->sca, 3*cap) || GROW(sol->rot0, 9*cap) || GROW(sol->rot, 9*cap) ||
+        GROW(sol->x, 3*cap) || GROW(sol->x0, 3*cap) || GROW(sol->J, 9*cap) || GROW(sol->jinv, 9*cap) ||
+        GROW(sol->minv, cap) || GROW(sol->mass, cap) || GROW(sol->velo, 6*cap) || GROW(sol->velo0, 6*cap) ||
+        GROW(sol->aux, 3*cap) || GROW(sol->force, 6*cap) || GROW(sol->tmp_velo, 6*cap)) return -1;
+    sol->bcap = (int)cap;
   }
   int b = sol->nb ++;
   static const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
@@ -206,6 +220,7 @@ int SOLFEC_Sphere(SOLFEC *sol, const double *centre, double radius, double densi
 {
   if (!sol || !(radius > 0.0) || !(density > 0.0)) return -1;
   int b = add_body(sol);
+  if (b < 0) return -1;
   /* sphere mass characteristics (sph.c:167-238): m = rho 4/3 pi r^3, J = 2/5 m r^2 I */
   double m = density * (4.0/3.0) * M_PI * radius*radius*radius, j = 0.4*m*radius*radius;
   sol->kind[b] = SOL_RIG;
@@ -227,6 +242,7 @@ int SOLFEC_Ellipsoid(SOLFEC *sol, const double *centre, const double *semi_axes,
   if (!sol || !(density > 0.0)) return -1;
   for (int k = 0; k < 3; k ++) if (!(semi_axes[k] > 0.0)) return -1;
   int b = add_body(sol);
+  if (b < 0) return -1;
   const double a = semi_axes[0], bb = semi_axes[1], c = semi_axes[2];
   /* ellipsoid mass characteristics (eli.c:170-227): m = rho 4/3 pi abc,
    * principal inertia m/5 (b^2+c^2, a^2+c^2, a^2+b^2), J = R0 diag R0' */
@@ -259,6 +275,7 @@ int SOLFEC_Plane(SOLFEC *sol, const double *point, const double *normal, int sur
   double len = sqrt(normal[0]*normal[0] + normal[1]*normal[1] + normal[2]*normal[2]);
   if (!sol || !(len > 0.0)) return -1;
   int b = add_body(sol);
+  if (b < 0) return -1;
   sol->kind[b] = SOL_OBS;
   sol->shape[b] = 1;
   sol->surf[b] = surface;
@@ -370,24 +387,24 @@ static void body_step_end(SOLFEC *sol, int b)
 
 /* ---------------- contacts ---------------- */
 
-static void ensure_contacts(SOLFEC *sol, int n)
+/* room for n contacts; a count outside [0, 2^30) (a corrupted size) is
+ * refused with SOL_ERR_MEM, as is a failed allocation */
+static int ensure_contacts(SOLFEC *sol, long long n)
 {
-  if (n <= sol->ccap) return;
-  int cap = sol->ccap ? sol->ccap : 1024;
+  if (n < 0 || n > 0x3fffffff)
+  {
+    snprintf(sol->err, sizeof(sol->err), "refused a contact count of %lld", n);
+    return SOL_ERR_MEM;
+  }
+  if (n <= sol->ccap) return 0;
+  long long cap = sol->ccap ? sol->ccap : 1024;
   while (cap < n) cap *= 2;
-  sol->c_master = xrealloc(sol->c_master, sizeof(int)*(size_t)cap);
-  sol->c_slave = xrealloc(sol->c_slave, sizeof(int)*(size_t)cap);
-  sol->c_mpnt = xrealloc(sol->c_mpnt, sizeof(double)*3*(size_t)cap);
-  sol->c_spnt = xrealloc(sol->c_spnt, sizeof(double)*3*(size_t)cap);
-  sol->c_point = xrealloc(sol->c_point, sizeof(double)*3*(size_t)cap);
-  sol->c_base = xrealloc(sol->c_base, sizeof(double)*9*(size_t)cap);
-  sol->c_gap = xrealloc(sol->c_gap, sizeof(double)*(size_t)cap);
-  sol->c_fri = xrealloc(sol->c_fri, sizeof(double)*(size_t)cap);
-  sol->c_rest = xrealloc(sol->c_rest, sizeof(double)*(size_t)cap);
-  sol->c_coh = xrealloc(sol->c_coh, sizeof(double)*(size_t)cap);
-  sol->c_R = xrealloc(sol->c_R, sizeof(double)*3*(size_t)cap);
-  sol->c_U = xrealloc(sol->c_U, sizeof(double)*3*(size_t)cap);
-  sol->ccap = cap;
+  if (GROW(sol->c_master, cap) || GROW(sol->c_slave, cap) || GROW(sol->c_mpnt, 3*cap) || GROW(sol->c_spnt, 3*cap) ||
+      GROW(sol->c_point, 3*cap) || GROW(sol->c_base, 9*cap) || GROW(sol->c_gap, cap) || GROW(sol->c_fri, cap) ||
+      GROW(sol->c_rest, cap) || GROW(sol->c_coh, cap) || GROW(sol->c_R, 3*cap) || GROW(sol->c_U, 3*cap))
+    return SOL_ERR_MEM;
+  sol->ccap = (int)cap;
+  return 0;
 }
 
 /* referential image of a spatial point (BODY_Ref_Point): X = X0 + R'(x - xc) */
@@ -426,14 +443,13 @@ static int plane_convex(SOLFEC *sol, int pl, int s, double *spnt, double *mpnt, 
 }
 
 /* open-addressing map from body pair to contact slot */
-static void map_build(SOLFEC *sol)
+static int map_build(SOLFEC *sol)
 {
   size_t cap = 16;
   while (cap < 2*(size_t)sol->nc + 16) cap *= 2;
   if (cap > sol->pair_cap)
   {
-    sol->pair_keys = xrealloc(sol->pair_keys, sizeof(unsigned long long)*cap);
-    sol->pair_vals = xrealloc(sol->pair_vals, sizeof(int)*cap);
+    if (GROW(sol->pair_keys, cap) || GROW(sol->pair_vals, cap)) return SOL_ERR_MEM;
     sol->pair_cap = cap;
   }
   sol->pair_mask = cap - 1;
@@ -446,6 +462,7 @@ static void map_build(SOLFEC *sol)
     sol->pair_keys[h] = key;
     sol->pair_vals[h] = i;
   }
+  return 0;
 }
 static int map_find(SOLFEC *sol, int a, int b)
 {
@@ -518,8 +535,7 @@ static int update_and_detect(SOLFEC *sol)
   /* geometry of every existing contact (independent per contact) */
   if ((size_t)nc > sol->upd_cap)
   {
-    sol->upd_geo = xrealloc(sol->upd_geo, sizeof(double)*10*(size_t)nc);
-    sol->upd_ok = xrealloc(sol->upd_ok, sizeof(int)*(size_t)nc);
+    if (GROW(sol->upd_geo, 10LL*nc) || GROW(sol->upd_ok, nc)) return SOL_ERR_MEM;
     sol->upd_cap = (size_t)nc;
   }
   #pragma omp parallel for schedule(static)
@@ -579,37 +595,27 @@ static int update_and_detect(SOLFEC *sol)
         if (sol->x[3*b + k] > hi[k]) hi[k] = sol->x[3*b + k];
       }
     }
-  map_build(sol);
+  if (map_build(sol)) return SOL_ERR_MEM;
   if (rmax > 0.0)
   {
+    /* at most 4 cells per body (very sparse scenes get a coarser grid); the
+     * count is formed without integer overflow however far a body has gone */
     double cell = 2.0*rmax + 2.0*GEOMETRIC_EPSILON;
     int dim[3];
     long long ncell = 1;
-    for (int k = 0; k < 3; k ++)
+    if (geom_grid_dims(lo, hi, 4LL*nb + 1024, &cell, dim, &ncell))
     {
-      const double ext = hi[k] - lo[k];
-      if (!(ext >= 0.0) || !(ext < 1e12*cell))
-      {
-        snprintf(sol->err, sizeof(sol->err), "body positions are not finite (the time integration diverged)");
-        return SOL_ERR_ARG;
-      }
-      dim[k] = (int)floor(ext/cell) + 1;
-      ncell *= dim[k];
-    }
-    while (ncell > 4LL*nb + 1024)   /* very sparse scenes: coarser grid */
-    {
-      cell *= 2.0;
-      ncell = 1;
-      for (int k = 0; k < 3; k ++) { dim[k] = (int)floor((hi[k] - lo[k])/cell) + 1; ncell *= dim[k]; }
+      snprintf(sol->err, sizeof(sol->err), "body positions are not finite (the time integration diverged)");
+      return SOL_ERR_ARG;
     }
     if ((size_t)ncell > sol->grid_cap)
     {
-      sol->grid_head = xrealloc(sol->grid_head, sizeof(int)*(size_t)ncell);
+      if (GROW(sol->grid_head, ncell)) return SOL_ERR_MEM;
       sol->grid_cap = (size_t)ncell;
     }
     if ((size_t)nb > sol->gnext_cap)
     {
-      sol->grid_next = xrealloc(sol->grid_next, sizeof(int)*(size_t)nb);
+      if (GROW(sol->grid_next, nb)) return SOL_ERR_MEM;
       sol->gnext_cap = (size_t)nb;
     }
     for (long long c = 0; c < ncell; c ++) sol->grid_head[c] = -1;
@@ -632,13 +638,12 @@ static int update_and_detect(SOLFEC *sol)
     }
     if (sol->cand_nthr < nthr)
     {
-      sol->cand = xrealloc(sol->cand, sizeof(int*)*(size_t)nthr);
-      sol->cand_n = xrealloc(sol->cand_n, sizeof(int)*(size_t)nthr);
-      sol->cand_cap = xrealloc(sol->cand_cap, sizeof(int)*(size_t)nthr);
+      if (GROW(sol->cand, nthr) || GROW(sol->cand_n, nthr) || GROW(sol->cand_cap, nthr)) return SOL_ERR_MEM;
       for (int t = sol->cand_nthr; t < nthr; t ++) { sol->cand[t] = NULL; sol->cand_cap[t] = 0; }
       sol->cand_nthr = nthr;
     }
-    #pragma omp parallel
+    int cand_failed = 0;   /* a candidate buffer that could not grow: the step fails after the region */
+    #pragma omp parallel reduction(|:cand_failed)
     {
       int t = omp_get_thread_num_();
       int chunk = (nb + nthr - 1) / nthr, a0 = t*chunk, a1 = a0 + chunk < nb ? a0 + chunk : nb;
@@ -663,8 +668,9 @@ static int update_and_detect(SOLFEC *sol)
                 if (map_find(sol, a, b) >= 0) continue;
                 if (sol->cand_n[t] + 2 > sol->cand_cap[t])
                 {
-                  sol->cand_cap[t] = sol->cand_cap[t] ? 2*sol->cand_cap[t] : 1024;
-                  sol->cand[t] = xrealloc(sol->cand[t], sizeof(int)*(size_t)sol->cand_cap[t]);
+                  long long cap = sol->cand_cap[t] ? 2LL*sol->cand_cap[t] : 1024;
+                  if (cap > 0x3fffffff || grow(NULL, 0, &sol->cand[t], cap, sizeof(int))) { cand_failed = 1; break; }
+                  sol->cand_cap[t] = (int)cap;
                 }
                 sol->cand[t][sol->cand_n[t] ++] = a;
                 sol->cand[t][sol->cand_n[t] ++] = b;
@@ -680,6 +686,11 @@ static int update_and_detect(SOLFEC *sol)
         }
       }
     }
+    if (cand_failed)
+    {
+      snprintf(sol->err, sizeof(sol->err), "out of memory for the candidate contact pairs");
+      return SOL_ERR_MEM;
+    }
     for (int t = 0; t < nthr; t ++)
       for (int q = 0; q < sol->cand_n[t]; q += 2)
       {
@@ -689,7 +700,7 @@ static int update_and_detect(SOLFEC *sol)
         int slave = a, master = b;
         if (geom_pntcmp(sol->x + 3*a, sol->x + 3*b) > 0) { slave = b; master = a; }
         if (!convex_pair(sol, slave, master, sp, mp, nrm, &g)) continue;
-        ensure_contacts(sol, sol->nc + 1);
+        if (ensure_contacts(sol, sol->nc + 1LL)) return SOL_ERR_MEM;
         new_contact(sol, sol->nc ++, master, slave, mp, sp, nrm, g);
       }
   }
@@ -705,7 +716,7 @@ static int update_and_detect(SOLFEC *sol)
       if (dist >= sol->rad[s] + GEOMETRIC_EPSILON) continue;
       if (map_find(sol, p, s) >= 0) continue;
       if (!plane_convex(sol, p, s, sp, mp, nrm, &g)) continue;
-      ensure_contacts(sol, sol->nc + 1);
+      if (ensure_contacts(sol, sol->nc + 1LL)) return SOL_ERR_MEM;
       new_contact(sol, sol->nc ++, s, p, mp, sp, nrm, g);
     }
   }
@@ -737,7 +748,8 @@ static int dev_push(SOLFEC *sol, sfgs_ctx *g)
   {
     SFGS_SHAPES sh = {sol->shape, sol->surf, sol->rad, sol->plane, sol->sca, sol->rot0, sol->x, sol->J, sol->mass,
                       sol->aux};
-    SFGS_MATERIAL *mat = xrealloc(NULL, sizeof(SFGS_MATERIAL)*(size_t)(sol->nmat + 1)), def;
+    SFGS_MATERIAL *mat = NULL, def;
+    if (GROW(mat, sol->nmat + 1LL)) return SOL_ERR_MEM;
     for (int k = 0; k < sol->nmat; k ++)
     {
       mat[k].s1 = sol->mat[k].s1; mat[k].s2 = sol->mat[k].s2;
@@ -771,7 +783,8 @@ static int dev_pull(SOLFEC *sol)
   if (!rc) rc = SFGS_Get_Sizes(g, &n, NULL, NULL);
   if (!rc)
   {
-    ensure_contacts(sol, n > 0 ? n : 1);
+    rc = ensure_contacts(sol, n > 0 ? n : 1);
+    if (rc) return rc;
     sol->nc = n;
     rc = SFGS_Dom_Get_Contacts(g, sol->c_master, sol->c_slave, sol->c_point, sol->c_mpnt, sol->c_spnt, sol->c_base,
                                sol->c_gap, sol->c_fri, sol->c_rest, sol->c_coh, sol->c_R, sol->c_U);
@@ -787,7 +800,7 @@ static int dev_pull(SOLFEC *sol)
 
 int sol_dev_pull(SOLFEC *sol) { return dev_pull(sol); }
 void sol_dev_edit(SOLFEC *sol) { dev_edit(sol); }
-void sol_ensure_contacts(SOLFEC *sol, int n) { ensure_contacts(sol, n); }
+int sol_ensure_contacts(SOLFEC *sol, long long n) { return ensure_contacts(sol, n); }
 
 int SOLFEC_Device_Loop(SOLFEC *sol, int on)
 {
@@ -853,8 +866,8 @@ int SOLFEC_Step_Begin(SOLFEC *sol)
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < sol->nb; b ++) body_step_begin(sol, b);
   sol->t_timint += now_s() - t0;
-  ensure_contacts(sol, 1);
-  return update_and_detect(sol);
+  int rc = ensure_contacts(sol, 1);
+  return rc ? rc : update_and_detect(sol);
 }
 
 int SOLFEC_Step_End(SOLFEC *sol)
@@ -1115,7 +1128,15 @@ int SOLFEC_Locdyn_Write(const char *path, int n, const double *point, const int 
             U[3*i], U[3*i + 1], U[3*i + 2], R[3*i], R[3*i + 1], R[3*i + 2]);
     /* distinct neighbours with summed blocks, insertion-sorted by dump_cmp */
     int m = 0, len = rowptr[i + 1] - rowptr[i];
-    if (len > cap) { cap = len; nb = xrealloc(nb, sizeof(int)*(size_t)cap); acc = xrealloc(acc, sizeof(double)*9*(size_t)cap); }
+    if (len > cap)
+    {
+      cap = len;
+      if (grow(NULL, 0, &nb, cap, sizeof(int)) || grow(NULL, 0, &acc, 9LL*cap, sizeof(double)))
+      {
+        free(nb); free(acc); fclose(f);
+        return SOL_ERR_MEM;
+      }
+    }
     for (int q = rowptr[i]; q < rowptr[i + 1]; q ++)
     {
       int j = col[q], at = 0;
@@ -1185,12 +1206,9 @@ static int locdyn_download(SOLFEC *sol, int **rowptr, int **col, double **W, dou
     snprintf(sol->err, sizeof(sol->err), "no assembled local dynamics to dump");
     return SOL_ERR_ARG;
   }
-  *rowptr = xrealloc(NULL, sizeof(int)*((size_t)n + 1));
-  *col = xrealloc(NULL, sizeof(int)*((size_t)nnz + 1));
-  *W = xrealloc(NULL, sizeof(double)*9*((size_t)nnz + 1));
-  *Wd = xrealloc(NULL, sizeof(double)*9*((size_t)n + 1));
-  *B = xrealloc(NULL, sizeof(double)*3*((size_t)n + 1));
-  *V = xrealloc(NULL, sizeof(double)*3*((size_t)n + 1));
+  /* the callers free what was allocated, also after a failure */
+  if (GROW(*rowptr, n + 1LL) || GROW(*col, nnz + 1) || GROW(*W, 9*(nnz + 1)) || GROW(*Wd, 9*(n + 1LL)) ||
+      GROW(*B, 3*(n + 1LL)) || GROW(*V, 3*(n + 1LL))) return SOL_ERR_MEM;
   if (SFGS_Get_Locdyn(g, *rowptr, *col, NULL, *W, *Wd, NULL, *B, *V, NULL, NULL, NULL))
   {
     snprintf(sol->err, sizeof(sol->err), "LOCDYN download failed: %s", SFGS_Last_Error(g));
@@ -1264,7 +1282,7 @@ int SOLFEC_Step_Solve_Penalty(SOLFEC *sol, PENALTY *ps)
   const int n = sol->nc;
   if (n > ps->kcp_cap)
   {
-    ps->kcp = xrealloc(ps->kcp, sizeof(double)*3*(size_t)n);
+    if (GROW(ps->kcp, 3LL*n)) return SOL_ERR_MEM;
     ps->kcp_cap = n;
   }
   for (int i = 0; i < n; i ++)

@@ -98,7 +98,7 @@ struct penalty {
 /* host-layer internals shared by sol_host.c and sol_state.c */
 int sol_dev_pull(SOLFEC *sol);             /* device state into the host arrays, when behind */
 void sol_dev_edit(SOLFEC *sol);            /* before a host-side change of the state */
-void sol_ensure_contacts(SOLFEC *sol, int n);
+int sol_ensure_contacts(SOLFEC *sol, long long n);
 
 /* run the device Newton solve for the current system of g (nts_host.c) */
 int NEWTON_Solve_Device(NEWTON *ns, sfgs_ctx *g, char *err, size_t errlen);

@@ -235,6 +235,7 @@ int SOLFEC_Read_State(SOLFEC *sol, const char *path)
     cs[i] = (int)sid - 1;
   }
   if (r.pos != r.n) { snprintf(sol->err, sizeof(sol->err), "%s has trailing data", path); goto done; }
+  if (sol_ensure_contacts(sol, nc > 0 ? nc : 1)) { rc = SOL_ERR_MEM; goto done; }
   /* install */
   sol->time = time;
   for (int b = 0; b < nb; b ++)
@@ -244,7 +245,6 @@ int SOLFEC_Read_State(SOLFEC *sol, const char *path)
     memcpy(sol->x + 3*(size_t)b, q + 9, 3*sizeof(double));
     memcpy(sol->velo + 6*(size_t)b, q + 12, 6*sizeof(double));
   }
-  sol_ensure_contacts(sol, nc > 0 ? nc : 1);
   sol->nc = nc;
   for (int i = 0; i < nc; i ++)
   {

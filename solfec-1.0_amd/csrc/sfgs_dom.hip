@@ -586,19 +586,14 @@ int SFGS_Dom_Update_Begin(sfgs_ctx *ctx, double step, const double *gravity, int
     Grid g;
     g.cell = 2.0*ctx->rmax + 2.0*SOL_GEOMETRIC_EPSILON;
     long long ncell = 1;
-    for (int k = 0; k < 3; k ++)
+    for (int k = 0; k < 3; k ++) g.lo[k] = bb[k];
+    // at most 4 cells per body (a coarser grid for very sparse scenes), sized
+    // without integer overflow however far a body has gone, so that every
+    // cell index of k_cell_* and k_cand_* fits an int and the cell arrays
+    if (geom_grid_dims(bb, bb + 3, 4LL*nb + 1024, &g.cell, g.dim, &ncell))
     {
-      g.lo[k] = bb[k];
-      double ext = bb[3 + k] - bb[k];
-      if (!(ext >= 0.0) || !(ext < 1e12*g.cell)) { ctx->err = "body positions are not finite"; return SFGS_E_ARG; }
-      g.dim[k] = std::max(1, (int)floor(ext/g.cell) + 1);
-      ncell *= g.dim[k];
-    }
-    while (ncell > 4LL*nb + 1024)   // very sparse scenes: coarser grid
-    {
-      g.cell *= 2.0;
-      ncell = 1;
-      for (int k = 0; k < 3; k ++) { g.dim[k] = (int)floor((bb[3 + k] - bb[k])/g.cell) + 1; ncell *= g.dim[k]; }
+      ctx->err = "body positions are not finite";
+      return SFGS_E_ARG;
     }
     CHK(ensure(ctx, ctx->cell_cnt, (size_t)ncell + 1));
     CHK(ensure(ctx, ctx->cell_start, (size_t)ncell + 1));

@@ -350,6 +350,37 @@ SOL_GEOM_FN int geom_cell_coord(double v, int dim)
   return (int)f;
 }
 
+/* cells of the broad-phase grid over the box [lo, hi]: the cell size starts at
+ * *cell and doubles until the cell count is at most cap (<= 2^31 - 1), so that
+ * every dimension, the count and a cell index fit an int however far apart the
+ * bodies are.  Counts are formed in double, where a product of large
+ * dimensions cannot wrap.  Returns -1 for a box that is not finite. */
+SOL_GEOM_FN int geom_grid_dims(const double *lo, const double *hi, long long cap, double *cell, int *dim,
+                               long long *ncell)
+{
+  for (int k = 0; k < 3; k ++)
+  {
+    const double ext = hi[k] - lo[k];
+    if (!(ext >= 0.0) || !(ext <= 1e300)) return -1;
+  }
+  for (;;)
+  {
+    double d[3], count = 1.0;
+    for (int k = 0; k < 3; k ++)
+    {
+      d[k] = floor((hi[k] - lo[k])/(*cell)) + 1.0;
+      count *= d[k];
+    }
+    if (count <= (double)cap)
+    {
+      for (int k = 0; k < 3; k ++) dim[k] = (int)d[k];
+      *ncell = (long long)count;
+      return 0;
+    }
+    *cell *= 2.0;
+  }
+}
+
 /* key of an unordered body pair and its hash (splitmix64 finaliser) */
 SOL_GEOM_FN unsigned long long geom_pair_key(int a, int b)
 {

@@ -592,4 +592,4 @@ def RUN(solfec, solver, duration):
     if rc:
         msg = solfec.lib.SOLFEC_Error_String(rc).decode()
         detail = solfec.lib.SOLFEC_Last_Error(solfec.sol).decode()
-        raise RuntimeError(msg + (f": {detail}" if detail and rc in (4, 6) else ""))
+        raise RuntimeError(msg + (f": {detail}" if detail and rc in (4, 6, 7) else ""))

@@ -15,10 +15,13 @@ call returns after synchronising its stream, so the host clock brackets
 completed device work.
 
 Printed line: value = contact-block updates/s (row updates of the GS sweeps per
-second of whole-step time), ms_per_step, the roofline of the sweep kernel
-(algorithmic bytes per row s(9k+26)+4k+4, k = off-diagonal blocks, over the
-HIP-event duration of every sweep launch) and the CPU oracle timed on a
-bounded sample of the same system.
+second of whole-step time), ms_per_step, per mode the merit reached, the first
+iteration with merit <= 1e-6 (none, on this packing: the reference GS stalls
+at merit ~1.5e-5) and the time to the merit the plain FP64 solve ends with;
+the roofline of the sweep kernel (algorithmic bytes per row s(9k+26)+4k+4,
+k = off-diagonal blocks, of every sweep of the timed steps over those solves'
+device time, HIP events on the library's stream); and the CPU oracle, built
+-march=native on this host, timed on a bounded sample of the same system.
 """
 import argparse
 import json
@@ -194,13 +197,39 @@ def bench_consumers(args):
     dev.close()
 
 
+def host_cpu():
+    """Model name and logical CPU count of this host (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    model, count = None, os.cpu_count()
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() == "Model name":
+                model = v.strip()
+            elif k.strip() == "CPU(s)" and v.strip().isdigit():
+                count = int(v.strip())
+    except Exception:
+        pass
+    if model is None and os.path.exists("/proc/cpuinfo"):
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    return model, count
+
+
 def cpu_baseline(b, c, colours, n, seconds, maxiter):
     """The oracle (C restatement, 1 core) on the same system: GS iterations of
     sweep + merit, as the GPU solve does, for about `seconds` of CPU time.  The
     local solver runs with the GPU solve's diagepsilon and diagmaxiter (those of
     epsilon = meritval = 1e-6); only the outer test is switched off, so that
-    exactly the planned number of iterations runs."""
+    exactly the planned number of iterations runs.  The oracle is compiled on
+    this host with -O3 -march=native first (the shipped build is kept if that
+    fails, and the line says which ran)."""
     import oracle
+    flags = oracle.use_native_build() or "-O3 -march=x86-64-v3 (prebuilt; native build failed)"
+    model, ncpu = host_cpu()
     O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), 1e-3)
     sysobj = oracle.System(O)
     order, _ = oracle.colour_order(colours, int(colours.max()) + 1)
@@ -226,7 +255,8 @@ def cpu_baseline(b, c, colours, n, seconds, maxiter):
     dtl = time.perf_counter() - t0
     return dict(value=rep.iters * n / dt, unit="contact-block updates/s", cores=1, kind="port",
                 sample=f"{rep.iters} GS iterations (colour-major sweep + merit) of the same {n}-contact "
-                       f"system, oracle/liboracle.so on 1 host core, {dt:.1f} s",
+                       f"system, the oracle on 1 host core, {dt:.1f} s",
+                host_cpu=model, host_logical_cpus=ncpu, compiler_flags=flags,
                 list_order=dict(value=repl.iters * n / dtl, iterations=int(repl.iters), seconds=round(dtl, 2)))
 
 
@@ -420,6 +450,18 @@ def bench_batch(args):
     dev.close()
 
 
+REF_ITERS = 2000
+
+
+def iso_merit(res, ref_merit):
+    """Time to a merit level: per timed step, the first iteration whose merit is at or below
+    ref_merit, and the step's device time scaled to that iteration."""
+    h = res["merhist_last"]
+    k = int(np.argmax(h <= ref_merit)) + 1 if np.any(h <= ref_merit) else None
+    res["iters_to_merit_ref"] = k
+    res["ms_to_merit_ref"] = float(np.mean(res["device_ms"])) * k / res["iters"][-1] if k else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -490,7 +532,7 @@ def main():
                  c["restitution"], c["cohesion"], c["R"], c["U"])
     n, nnz, ncol = 0, 0, 0
 
-    def run_mode(name, precision, steps, warmup, learned_ws=False):
+    def run_mode(name, precision, steps, warmup, learned_ws=False, ref_merit=None):
         par = solfec.make_param(1e-6, args.maxiter, meritval=1e-6, precision=precision, refine=2)
         # every mode starts from the step's own R0 (reading R back overwrites the uploaded one)
         dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
@@ -500,22 +542,39 @@ def main():
             dev.assemble(h)
             if learned_ws:
                 dev.warm_start()
-            rep, _, _ = dev.solve(par, history=False)
+            rep, rer, mer = dev.solve(par, history=True)
             dev.body_update(h, nb, download=False)
-            return rep
+            return rep, mer
 
         for _ in range(warmup):
-            rep = step()
-        reps = []
+            step()
+        reps, mers = [], []
         t0 = time.perf_counter()
         for _ in range(steps):
-            reps.append(step())
+            rep, mer = step()
+            reps.append(rep)
+            mers.append(mer)
         dt = time.perf_counter() - t0
         R, _ = dev.get_RU()
+        n_ = dev.sizes()[0]
         iters = [r.iters for r in reps]
+        # merit <= 1e-6 (the workload's meritval): first iteration that reaches it, per step
+        first = [int(np.argmax(m <= 1e-6)) + 1 if np.any(m <= 1e-6) else None for m in mers]
+        # sweeps run (the merit's closing sweep included) and the FP64 ones among them
+        sweeps = [r.row_updates // max(n_, 1) for r in reps]
+        fp64 = [r.fp64_sweeps for r in reps]
         res = dict(ms_per_step=1e3 * dt / steps, iters=iters, merit=reps[-1].merit, rel_error=reps[-1].rel_error,
-                   updates_per_s=sum(iters) * dev.sizes()[0] / dt)
-        log(f"[bench] {name}: {res['ms_per_step']:.2f} ms/step, iters={iters}, merit={res['merit']:.3e}")
+                   updates_per_s=sum(iters) * n_ / dt,
+                   merit_at_end=[float(r.merit) for r in reps], first_iter_merit_le_1e6=first,
+                   converged=[bool(r.iters < args.maxiter) for r in reps],
+                   merit_at=dict((str(k), float(mers[-1][k - 1])) for k in (100, 1000, 2000, 5000)
+                                 if k <= len(mers[-1])),
+                   device_ms=[float(r.solve_ms) for r in reps], sweeps=sweeps, fp64_sweeps=fp64)
+        res["merhist_last"] = mers[-1]
+        if ref_merit is not None:
+            iso_merit(res, ref_merit)
+        log(f"[bench] {name}: {res['ms_per_step']:.2f} ms/step, iters={iters}, merit={res['merit']:.3e}, "
+            f"first merit<=1e-6: {first}")
         return res, R
 
     modes = {}
@@ -525,8 +584,17 @@ def main():
     R64 = Rmx = None
     if "fp64" in want:
         modes["fp64"], R64 = run_mode("plain FP64 GS", solfec.device.FP64, args.steps, args.warmup)
+    # reference merit of the time-to-merit comparison: what plain FP64 reaches after REF_ITERS
+    # iterations, on the falling part of its merit curve (it plateaus at ~1.5e-5 further on)
+    mref = None
+    if "fp64" in modes:
+        f = modes["fp64"]
+        h64 = f["merhist_last"]
+        mref = float(h64[min(REF_ITERS, len(h64)) - 1])
+        iso_merit(f, mref)
     if "mixed" in want:
-        modes["mixed"], Rmx = run_mode("mixed FP32/FP64 GS", solfec.device.MIXED, args.steps, args.warmup)
+        modes["mixed"], Rmx = run_mode("mixed FP32/FP64 GS", solfec.device.MIXED, args.steps, args.warmup,
+                                       ref_merit=mref)
         if R64 is not None:
             modes["mixed"]["err_vs_fp64"] = float(np.abs(Rmx - R64).max() / max(np.abs(R64).max(), 1e-300))
     if "learned" in want:
@@ -534,7 +602,7 @@ def main():
         learned.install(dev)
         dev.warm_start_mode(args.ws_new_only, args.ws_scale)
         modes["learned"], Rl = run_mode("learned warm start + mixed GS", solfec.device.MIXED, args.steps, args.warmup,
-                                        learned_ws=True)
+                                        learned_ws=True, ref_merit=mref)
         if R64 is not None:
             modes["learned"]["err_vs_fp64"] = float(np.abs(Rl - R64).max() / max(np.abs(R64).max(), 1e-300))
     head = args.headline if args.headline in modes else ("mixed" if "mixed" in modes else "fp64")
@@ -543,27 +611,27 @@ def main():
     ms_per_step = modes[head]["ms_per_step"]
     iters = modes[head]["iters"]
 
-    # roofline of the sweep kernel of the headline mode: one profiled solve, HIP events around every launch
+    # roofline of the sweep kernel from the timed steps themselves: algorithmic
+    # bytes of every sweep of every timed solve (FP32 sweeps at s = 4, FP64 at
+    # s = 8) over the solves' device time (HIP events on the library's stream
+    # around each solve, so the termination kernels and launch gaps count
+    # against the sweeps); per launch, the same device time over the launches
     is32 = head in ("mixed", "learned")
-    dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
-                 c["restitution"], c["cohesion"], c["R"], c["U"])
-    dev.assemble(h)
-    if head == "learned":
-        dev.warm_start()
-    prec = solfec.device.MIXED if is32 else solfec.device.FP64
-    prof = solfec.make_param(1e-6, args.maxiter, meritval=1e-6, profile=1, precision=prec, refine=2)
-    prep, _, _ = dev.solve(prof, history=False)
-    sz = 4 if is32 else 8
-    bytes_per_sweep = sz * (9 * nnz + 26 * n) + 4 * nnz + 4 * n
-    sweeps_run = prep.row_updates / max(n, 1)
-    achieved = bytes_per_sweep * sweeps_run / (prep.sweep_ms * 1e-3) / 1e9
+    hm = modes[head]
+    b32 = 4 * (9 * nnz + 26 * n) + 4 * nnz + 4 * n
+    b64 = 8 * (9 * nnz + 26 * n) + 4 * nnz + 4 * n
+    nbytes = sum((sw - f) * (b32 if is32 else b64) + f * b64 for sw, f in zip(hm["sweeps"], hm["fp64_sweeps"]))
+    dev_s = 1e-3 * sum(hm["device_ms"])
+    launches = sum(hm["sweeps"]) * ncol
+    achieved = nbytes / dev_s / 1e9
     kname = ({1: "k_flow<%s>", 3: "k_bar<%s>"}.get(args.schedule, "k_sweep<%s>")) % ("float" if is32 else "double")
     roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, args),
-                kernel=kname,
-                avg_launch_us=round(1e3 * prep.sweep_ms / max(prep.sweep_launches, 1), 3),
-                launches=int(prep.sweep_launches), bytes_per_sweep=int(bytes_per_sweep))
-    log(f"[bench] sweep kernel: {prep.sweep_launches} launches, {prep.sweep_ms:.2f} ms, {achieved:.1f} GB/s")
+                kernel=kname, avg_launch_us=round(1e6 * dev_s / max(launches, 1), 3), launches=int(launches),
+                bytes_per_sweep=int(b32 if is32 else b64),
+                how="timed steps: algorithmic bytes of all sweeps / device time of the solves (HIP events)")
+    log(f"[bench] sweeps of the timed {head} steps: {launches} launches, {1e3 * dev_s:.2f} ms device, "
+        f"{achieved:.1f} GB/s")
     copy = stream_copy_rate()
     roof["copy_rate"] = dict(value=round(copy, 1), unit="GB/s", frac_of_copy=round(achieved / copy, 4),
                              what="hipMemcpyAsync device to device, 1 GiB x 10, read + write bytes")
@@ -575,6 +643,8 @@ def main():
         cpu = cpu_baseline(b, c, colours, n, args.cpu_seconds, args.maxiter)
         log(f"[bench] cpu baseline: {cpu['value']:.3e} updates/s")
 
+    for m in modes.values():
+        m.pop("merhist_last", None)     # arrays stay out of the JSON line
     out = {
         "metric": METRIC,
         "value": value,
@@ -604,6 +674,15 @@ def main():
             "row_order": ["decreasing row length", "contact index", "body block, row length"][args.row_order],
             "modes": modes,
             "speedup_vs_plain_fp64": modes["fp64"]["ms_per_step"] / ms_per_step if "fp64" in modes else None,
+            "merit_1e-6_reached": {k: any(f is not None for f in m["first_iter_merit_le_1e6"]) for k, m in modes.items()},
+            "merit_ref": {"value": mref, "defined_as": f"plain FP64 merit after {REF_ITERS} iterations"},
+            "time_to_merit_note": ("ms_per_step is the time to the reference termination test (bgs.c:1036: relative "
+                                   "error <= epsilon and merit <= meritval) or to maxiter; first_iter_merit_le_1e6 "
+                                   "lists, per timed step, the first iteration whose merit is <= 1e-6 (null: never "
+                                   "within maxiter; the reference GS plateaus near 1.5e-5 on this packing); "
+                                   "ms_to_merit_ref is each mode's device time to merit_ref"),
+            "speedup_to_merit_ref": (modes["fp64"]["ms_to_merit_ref"] / modes[head]["ms_to_merit_ref"]
+                                     if "fp64" in modes and modes[head].get("ms_to_merit_ref") else None),
         },
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -48,7 +48,7 @@ typedef struct sfgs_ctx sfgs_ctx;
 
 /* sweep precision */
 #define SFGS_PREC_FP64  0   /* FP64 storage and arithmetic (the reference) */
-#define SFGS_PREC_MIXED 1   /* FP32 W/B/R sweeps, FP64 merit, FP64 refinement sweeps */
+#define SFGS_PREC_MIXED 1   /* FP32 W/B/R sweeps, FP64 merit and termination, then FP64 sweeps */
 
 /* Gauss-Seidel parameters: the fields of struct gs, bgs.h:64-111, with the
  * defaults of GAUSS_SEIDEL_SOLVER, lng.c:4243-4349 */
@@ -62,7 +62,10 @@ typedef struct {
   int diagsolver;      /* SFGS_DS_* */
   int reverse;         /* 1: alternate forward / backward sweeps (bgs.c:936-950) */
   int precision;       /* SFGS_PREC_* */
-  int refine;          /* FP64 refinement sweeps after a mixed solve (default 2) */
+  int refine;          /* MIXED: FP32 sweeps stop refine iterations before maxiter (or at the FP32
+                          floor: relative error 1e-6, merit 1e-10), then FP64 sweeps, at least
+                          refine (default 2), run until the termination test holds in FP64 or the
+                          maxiter iterations are spent; maxiter <= refine solves in FP64 */
   int profile;         /* 1: time every sweep-kernel launch with HIP events (slower) */
 } SFGS_PARAM;
 
@@ -79,6 +82,7 @@ typedef struct {
   long long sweep_launches; /* sweep-kernel launches timed in profile mode */
   long long row_updates; /* rows visited = sweeps x rows */
   long long nnz;       /* off-diagonal blocks */
+  int fp64_sweeps;     /* MIXED: sweeps of the FP64 phase (the rest of row_updates / n are FP32) */
 } SFGS_REPORT;
 
 /* context: LOCDYN_Create, ldy.h:93 (one device-resident W/B/R store) */

@@ -11,6 +11,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _i, _d, _p = C.c_int, C.c_double, C.c_void_p
+_libpath = os.path.join(_HERE, "liboracle.so")
 
 
 class OR_BODIES(C.Structure):
@@ -52,10 +53,26 @@ class OR_NTREPORT(C.Structure):
 _lib = None
 
 
+def use_native_build():
+    """Build the oracle for this host's own CPU (-O3 -march=native, make target
+    `native`, into oracle/_native/) and load that build instead of the default
+    one; returns the compiler flags used, or None when the build failed (the
+    default build is then kept).  Called by the CPU-baseline leg of bench.py
+    before anything else loads the oracle."""
+    import subprocess
+    global _lib, _libpath
+    try:
+        subprocess.run(["make", "-s", "-C", _HERE, "native"], check=True, capture_output=True, timeout=300)
+    except Exception:
+        return None
+    _libpath, _lib = os.path.join(_HERE, "_native", "liboracle.so"), None
+    return "-O3 -march=native"
+
+
 def lib():
     global _lib
     if _lib is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        path = _libpath
         if not os.path.exists(path):
             raise RuntimeError(f"{path} missing: run `make -C oracle`")
         L = C.CDLL(path)

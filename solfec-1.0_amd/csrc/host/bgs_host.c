@@ -169,8 +169,16 @@ int GAUSS_SEIDEL_Solve_Device(GAUSS_SEIDEL *gs, sfgs_ctx *g, char *err, size_t e
   int maxiter = gs->par.maxiter;
   if (gs->hist_cap < maxiter)
   {
-    gs->rerhist = realloc(gs->rerhist, sizeof(double)*(size_t)maxiter);
-    gs->merhist = realloc(gs->merhist, sizeof(double)*(size_t)maxiter);
+    /* the histories hold at most maxiter values, MIXED included */
+    double *r = realloc(gs->rerhist, sizeof(double)*(size_t)maxiter);
+    if (r) gs->rerhist = r;
+    double *m = r ? realloc(gs->merhist, sizeof(double)*(size_t)maxiter) : NULL;
+    if (m) gs->merhist = m;
+    if (!r || !m)
+    {
+      snprintf(err, errlen, "out of memory for the histories of %d iterations", maxiter);
+      return SOL_ERR_MEM;
+    }
     gs->hist_cap = maxiter;
   }
   int rc = SFGS_Solve(g, &gs->par, gs->rerhist, gs->merhist, &gs->rep);
@@ -185,8 +193,14 @@ int GAUSS_SEIDEL_Solve_Device(GAUSS_SEIDEL *gs, sfgs_ctx *g, char *err, size_t e
   {
     if (gs->itershistcount >= gs->itershistsize)
     {
+      int *h = realloc(gs->itershist, sizeof(int)*((size_t)gs->itershistsize + 1024));
+      if (!h)
+      {
+        snprintf(err, errlen, "out of memory for the iteration history");
+        return SOL_ERR_MEM;
+      }
+      gs->itershist = h;
       gs->itershistsize += 1024;
-      gs->itershist = realloc(gs->itershist, sizeof(int)*(size_t)gs->itershistsize);
     }
     gs->itershist[gs->itershistcount ++] = gs->iters;
   }

@@ -858,6 +858,17 @@ static int dev_step_end(SOLFEC *sol)
 
 /* ---------------- time stepping ---------------- */
 
+/* stopfile (sol.c:344-367): a file named STOP in the output directory */
+static int stop_file(SOLFEC *sol)
+{
+  char path[sizeof(sol->outpath) + 8];
+  snprintf(path, sizeof(path), "%s/STOP", sol->outpath);
+  FILE *f = fopen(path, "r");
+  if (!f) return 0;
+  fclose(f);
+  return 1;
+}
+
 int SOLFEC_Step_Begin(SOLFEC *sol)
 {
   sol->locdyn_current = 0;
@@ -1043,7 +1054,7 @@ int SOLFEC_Step_Solve(SOLFEC *sol, GAUSS_SEIDEL *gs)
   if (rc) return rc;
   double t1 = now_s();
   int err = GAUSS_SEIDEL_Solve_Device(gs, g, sol->err, sizeof(sol->err));
-  if (err == SOL_ERR_GPU) return err;
+  if (err == SOL_ERR_GPU || err == SOL_ERR_MEM) return err;
   rc = step_finish(sol, g);
   sol->t_consol += now_s() - t1;
   return rc ? rc : err;
@@ -1077,6 +1088,7 @@ int SOLFEC_Run_Newton(SOLFEC *sol, NEWTON *ns, double duration)
     if (rc) return rc;
     rc = SOLFEC_Step_End(sol);
     if (rc) return rc;
+    if (stop_file(sol)) break;   /* a STOP file created by the user ends the run (sol.c:804-805) */
   }
   return SOL_OK;
 }
@@ -1317,6 +1329,7 @@ int SOLFEC_Run_Penalty(SOLFEC *sol, PENALTY *ps, double duration)
     if (rc) return rc;
     rc = SOLFEC_Step_End(sol);
     if (rc) return rc;
+    if (stop_file(sol)) break;   /* a STOP file created by the user ends the run (sol.c:804-805) */
   }
   return SOL_OK;
 }
@@ -1333,6 +1346,7 @@ int SOLFEC_Run(SOLFEC *sol, GAUSS_SEIDEL *gs, double duration)
     if (rc) return rc;
     rc = SOLFEC_Step_End(sol);
     if (rc) return rc;
+    if (stop_file(sol)) break;   /* a STOP file created by the user ends the run (sol.c:804-805) */
   }
   return SOL_OK;
 }

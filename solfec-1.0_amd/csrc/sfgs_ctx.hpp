@@ -25,6 +25,7 @@ struct Ctrl {
   int final_buf;    // R buffer (0 / 1) holding the final reactions
   int final_U;      // 0: U, 1: Um holds the final relative velocities
   int maxiter, nomerit;
+  int miniter;      // no convergence stop before this many iterations (the FP64 phase of MIXED)
   double error;     // last relative error
   double merit;     // last merit
   double prev_error;

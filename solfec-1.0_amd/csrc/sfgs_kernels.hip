@@ -20,6 +20,7 @@
 #include <cfloat>
 #include <climits>
 #include <algorithm>
+#include <string>
 #include "sfgs_ctx.hpp"
 #include "sfgs_math.hpp"
 #include "sfgs_prims.hpp"
@@ -607,7 +608,7 @@ __device__ void finish_sweep(Ctrl *ctrl, const double *partials, int np, double 
     ctrl->merit = 0.0;
     ctrl->final_buf = (k + 1) & 1;
     ctrl->final_U = 0;
-    if (ctrl->aborted || !(k + 1 < ctrl->maxiter && error_k > ctrl->eps)) ctrl->done = 1;
+    if (ctrl->aborted || !(k + 1 < ctrl->maxiter && (error_k > ctrl->eps || k + 1 < ctrl->miniter))) ctrl->done = 1;
     return;
   }
   if (ctrl->aborted)
@@ -626,7 +627,7 @@ __device__ void finish_sweep(Ctrl *ctrl, const double *partials, int np, double 
   ctrl->iters = k;
   ctrl->error = error;
   ctrl->merit = merit;
-  if (!(k < ctrl->maxiter && (error > ctrl->eps || merit > ctrl->meritval)))
+  if (!(k < ctrl->maxiter && (error > ctrl->eps || merit > ctrl->meritval || k < ctrl->miniter)))
   {
     ctrl->final_buf = k & 1;   // P of sweep k = R after sweep k-1
     ctrl->final_U = 1;         // U after the merit of iteration k-1
@@ -1560,12 +1561,13 @@ struct GsPlan {
 // Gauss-Seidel iterations until k_finish stops them; hc receives the final control block
 template <typename T>
 static int run_gs(sfgs_ctx *ctx, SweepArgs<T> sa, T *X0, T *X1, int maxiter, int nomerit, double eps,
-                  double meritval, int reverse, double *hist, GsPlan &pl, Ctrl &hc)
+                  double meritval, int reverse, double *hist, GsPlan &pl, Ctrl &hc, int miniter = 0)
 {
   hipStream_t st = ctx->stream;
   const int ncol = ctx->ncolours;
   std::memset(&hc, 0, sizeof(hc));
   hc.eps = eps; hc.meritval = meritval; hc.maxiter = maxiter; hc.nomerit = nomerit; hc.energy = ctx->energy;
+  hc.miniter = miniter;
   HIPCHK(hipMemcpyAsync(ctx->ctrl.p, &hc, sizeof(Ctrl), hipMemcpyHostToDevice, st));
   const int maxsweeps = nomerit ? maxiter : maxiter + 1;   // with merit, sweep k closes iteration k-1
   int batch = pl.prof ? 1 : 2, k = 0;
@@ -1646,8 +1648,13 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   hipStream_t st = ctx->stream;
   const int ncol = ctx->ncolours, maxiter = par->maxiter, n = ctx->nc;
   const int nomerit = par->meritval >= 1.0 ? 1 : 0;
-  const bool mixed = par->precision == SFGS_PREC_MIXED;
+  // MIXED: FP32 sweeps for at most maxiter - refine iterations, then FP64 sweeps
+  // (at least refine of them) until the reference's termination test holds in
+  // FP64 or the maxiter iterations are spent.  With maxiter <= refine the whole
+  // solve is FP64.
+  const bool mixed = par->precision == SFGS_PREC_MIXED && par->maxiter > par->refine;
   const int refine = mixed ? par->refine : 0;
+  const int maxiter32 = maxiter - refine;
   // the dataflow schedule runs forward sweeps only
   const bool flow = ctx->schedule == 1 && !par->reverse && n > 0 && ctx->G_built <= 2;
   // the barrier schedule: one persistent launch, a device-wide barrier per colour
@@ -1667,7 +1674,9 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   const int mgrid = (ctx->nslices + TPB/SLICE - 1) / (TPB/SLICE);
   CHK(ensure(ctx, ctx->partials, 3*(size_t)std::max(pl.np, 1)));
   CHK(ensure(ctx, ctx->mpartials, (size_t)std::max(mgrid, 1)));
-  CHK(ensure(ctx, ctx->hist, 2*(size_t)(maxiter + refine + 1)));
+  // histories: the first (or only) phase at hist (errors, then merits at
+  // + its iteration bound), the FP64 phase of MIXED at hist + 2 maxiter
+  CHK(ensure(ctx, ctx->hist, 4*(size_t)maxiter + 2));
   CHK(ensure(ctx, ctx->failkey, 1));
   CHK(ensure(ctx, ctx->failcount, 4));
   CHK(ensure(ctx, ctx->ctrl, 1));
@@ -1718,21 +1727,26 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   }
   else
   {
-    // FP32 sweeps (FP64 merit, errors and termination), then FP64 refinement sweeps
+    // FP32 sweeps (FP64 merit, errors and termination), then FP64 sweeps
     CHK(build_f32_mirror(ctx));
     if (n > 0)
     {
       hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(4LL*n)), dim3(TPB), 0, st, 4LL*n, ctx->R.p, ctx->R32.p);
       hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(3LL*n)), dim3(TPB), 0, st, 3LL*n, ctx->U.p, ctx->U32.p);
     }
-    // FP32 resolves a relative change of about 1e-7: keep the local tolerance above it
+    // FP32 resolves a relative change of about 1e-7, and the FP64 merit of an
+    // FP32 fixed point is of order 1e-12 (1e-11 on ellipsoid packings): the
+    // local tolerance, the relative error and the merit of the FP32 phase stop
+    // at 1e-6, 1e-6 and 1e-10; the FP64 phase takes the solve down to epsilon
+    // and meritval
     const double de32 = std::max(par->diagepsilon, 1e-6);
+    const double eps32 = std::max(par->epsilon, 1e-6), merit32 = std::max(par->meritval, 1e-10);
     SweepArgs<float> sa = sweep_args<float>(ctx, par, ctx->W32.p, ctx->Wd32.p, ctx->B32.p, ctx->rho32.p,
                                             ctx->pfri32.p, ctx->pcoh32.p, ctx->shift32.p, ctx->U32.p, ctx->Um32.p, de32);
     if (flow)
     {
       float fms = 0.0f;
-      CHK(run_flow(ctx, sa, ctx->R32.p, ctx->R32b.p, maxiter, nomerit, par->epsilon, par->meritval, ctx->hist.p, hc,
+      CHK(run_flow(ctx, sa, ctx->R32.p, ctx->R32b.p, maxiter32, nomerit, eps32, merit32, ctx->hist.p, hc,
                    &fms));
       pl.sweep_ms += fms;
       pl.launches += 1;
@@ -1740,14 +1754,14 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
     else if (bar)
     {
       float bms = 0.0f;
-      CHK(run_bar(ctx, sa, ctx->R32.p, ctx->R32b.p, maxiter, nomerit, par->epsilon, par->meritval, par->reverse,
+      CHK(run_bar(ctx, sa, ctx->R32.p, ctx->R32b.p, maxiter32, nomerit, eps32, merit32, par->reverse,
                   ctx->hist.p, hc, &bms));
       pl.sweep_ms += bms;
       pl.launches += 1;
     }
     else
-      CHK(run_gs<float>(ctx, sa, ctx->R32.p, ctx->R32b.p, maxiter, nomerit, par->epsilon, par->meritval, par->reverse,
-                        ctx->hist.p, pl, hc));
+      CHK(run_gs<float>(ctx, sa, ctx->R32.p, ctx->R32b.p, maxiter32, nomerit, eps32, merit32,
+                        par->reverse, ctx->hist.p, pl, hc));
     if (n > 0)
     {
       float *Rf = hc.final_buf == 1 ? ctx->R32b.p : ctx->R32.p, *Uf = hc.final_U == 1 ? ctx->Um32.p : ctx->U32.p;
@@ -1758,14 +1772,18 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
     {
       SweepArgs<double> sd = sweep_args<double>(ctx, par, ctx->W.p, ctx->Wd.p, ctx->B.p, ctx->rho.p, ctx->pfri.p,
                                                 ctx->pcoh.p, ctx->shift.p, ctx->U.p, ctx->Um.p, par->diagepsilon);
-      // exactly 'refine' FP64 sweeps (eps 0 never stops them early), histories after the FP32 ones
-      CHK(run_gs<double>(ctx, sd, ctx->R.p, ctx->Rb.p, refine, 1, 0.0, 1.0, 0, ctx->hist.p + 2*(size_t)maxiter, pl, hr));
+      // the reference's termination test on FP64 sweeps, not before `refine` of
+      // them, within the iterations the FP32 phase left (at least refine)
+      CHK(run_gs<double>(ctx, sd, ctx->R.p, ctx->Rb.p, maxiter - hc.iters, nomerit, par->epsilon, par->meritval,
+                         par->reverse, ctx->hist.p + 2*(size_t)maxiter, pl, hr, refine));
       if (n > 0 && hr.final_buf == 1)
         HIPCHK(hipMemcpyAsync(ctx->R.p, ctx->Rb.p, sizeof(double)*4*(size_t)n, hipMemcpyDeviceToDevice, st));
+      if (n > 0 && hr.final_U == 1)
+        HIPCHK(hipMemcpyAsync(ctx->U.p, ctx->Um.p, sizeof(double)*3*(size_t)n, hipMemcpyDeviceToDevice, st));
     }
-    // final FP64 merit with the merit's update of U (mrf.c:61-76)
-    if (!nomerit && mgrid > 0)
+    else if (!nomerit && mgrid > 0)
     {
+      // no FP64 sweeps: the FP64 merit of the FP32 state, with the merit's update of U (mrf.c:61-76)
       MeritArgs ma = merit_args(ctx, 1, nullptr);
       SFGS_DISPATCH_G(ctx->G_built, auto kern = k_merit<GG>; hipLaunchKernelGGL(kern, dim3(mgrid), dim3(TPB), 0, st, ma,
                       ctx->nslices));
@@ -1774,44 +1792,51 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   HIPCHK(hipEventRecord(ctx->ev1, st));
   unsigned long long fk = 0;
   int fc = 0;
-  std::vector<double> hist(2*(size_t)(maxiter + refine + 1), 0.0), mp((size_t)std::max(mgrid, 1), 0.0);
+  std::vector<double> hist(4*(size_t)maxiter + 2, 0.0), mp((size_t)std::max(mgrid, 1), 0.0);
+  const bool refined = mixed && refine > 0 && !hc.aborted;
   HIPCHK(hipMemcpyAsync(&fk, ctx->failkey.p, sizeof(fk), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&fc, ctx->failcount.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(hist.data(), ctx->hist.p, sizeof(double)*hist.size(), hipMemcpyDeviceToHost, st));
-  if (mixed && !nomerit && mgrid > 0)
+  if (mixed && !refined && !nomerit && mgrid > 0)
     HIPCHK(hipMemcpyAsync(mp.data(), ctx->mpartials.p, sizeof(double)*(size_t)mgrid, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());
   float ms = 0.0f;
   HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
 
+  // histories of the first phase: errors at hist[k], merits at hist[bound + k]
+  const int bound1 = mixed ? maxiter32 : maxiter;
   int iters = hc.iters;
   double rel_error = hc.error, merit = hc.merit;
-  if (mixed)
+  std::vector<double> rer((size_t)maxiter, 0.0), mer((size_t)maxiter, 0.0);
+  for (int k = 0; k < std::min(hc.iters, maxiter); k ++)
   {
-    const double *h2 = hist.data() + 2*(size_t)maxiter;   // refinement block: errors then merits (maxiter = refine)
-    int nref = hr.iters;
-    if (!nomerit)
-    {
-      double up = 0.0;
-      for (int k = 0; k < mgrid; k ++) up += mp[k];
-      double lo = ctx->energy < DBL_EPSILON ? 1.0 : ctx->energy;
-      merit = 0.5*up / lo;
-    }
-    if (nref > 0) rel_error = h2[nref - 1];
-    // append the refinement sweeps to the histories
-    for (int k = 0; k < nref && iters + k < maxiter + refine; k ++)
-    {
-      hist[(size_t)iters + k] = h2[k];
-    }
-    iters += nref;
+    rer[k] = hist[k];
+    mer[k] = hist[(size_t)bound1 + k];
   }
-  if (rerhist) std::memcpy(rerhist, hist.data(), sizeof(double)*(size_t)std::min(iters, maxiter));
-  if (merhist)
+  if (refined)
   {
-    std::memcpy(merhist, hist.data() + maxiter, sizeof(double)*(size_t)std::min(hc.iters, maxiter));
-    for (int k = hc.iters; k < std::min(iters, maxiter); k ++) merhist[k] = merit;
+    // the FP64 phase, appended: its bound is maxiter - hc.iters, so the total stays within maxiter
+    const double *h2 = hist.data() + 2*(size_t)maxiter;
+    const int bound2 = maxiter - hc.iters;
+    for (int k = 0; k < hr.iters && iters + k < maxiter; k ++)
+    {
+      rer[(size_t)iters + k] = h2[k];
+      mer[(size_t)iters + k] = h2[(size_t)bound2 + k];
+    }
+    iters += hr.iters;
+    rel_error = hr.error;
+    merit = hr.merit;
   }
+  else if (mixed && !nomerit)
+  {
+    double up = 0.0;
+    for (int k = 0; k < mgrid; k ++) up += mp[k];
+    double lo = ctx->energy < DBL_EPSILON ? 1.0 : ctx->energy;
+    merit = 0.5*up / lo;
+  }
+  if (rerhist) std::memcpy(rerhist, rer.data(), sizeof(double)*(size_t)std::min(iters, maxiter));
+  if (merhist) std::memcpy(merhist, mer.data(), sizeof(double)*(size_t)std::min(iters, maxiter));
   if (rep)
   {
     std::memset(rep, 0, sizeof(*rep));
@@ -1820,14 +1845,15 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
     rep->merit = merit;
     rep->ncolours = ncol;
     rep->fail_rows = fc;
-    rep->aborted = hc.aborted;
+    rep->aborted = hc.aborted || hr.aborted;
     rep->solve_ms = ms;
     rep->sweep_ms = pl.sweep_ms;
     rep->sweep_launches = pl.launches;
     rep->row_updates = (long long)(hc.sweeps + hr.sweeps) * n;   // every sweep run, the merit's extra one included
     rep->nnz = ctx->nnz;
+    rep->fp64_sweeps = mixed ? hr.sweeps : hc.sweeps;
     rep->error = fk ? (int)(fk & 3ull) : SFGS_GS_OK;
-    if (!hc.aborted && hc.iters >= maxiter) rep->error = SFGS_GS_DIVERGED;
+    if (!rep->aborted && iters >= maxiter) rep->error = SFGS_GS_DIVERGED;
   }
   return 0;
 }

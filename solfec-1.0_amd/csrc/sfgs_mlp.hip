@@ -40,7 +40,7 @@ __device__ __forceinline__ float dgelu(float x)
 }
 
 struct GemmArgs {
-  int M, N, K;           // C is M x N, contraction over K (multiple of 32)
+  int M, N, K;           // C is M x N, contraction over K
   const bf16 *A; int lda; int transA;   // transA = 0: A[m*lda + k]; 1: A[k*lda + m]
   const bf16 *B; int ldb; int transB;   // transB = 0: B[k*ldb + n]; 1: B[n*ldb + k]
   float *C; int ldc;                    // FP32 output (EPI_F32, EPI_BIAS_F32, EPI_BIAS_GELU keeps Z here)
@@ -48,49 +48,81 @@ struct GemmArgs {
   const float *bias;                    // length N (EPI_BIAS_*)
   const float *Z; int ldz;              // pre-activations for EPI_GELU_BWD
   int epi;
+  int kslice;            // split-K: workgroup z takes K range [z kslice, (z+1) kslice) and writes
+                         // its partial C at C + z M ldc (EPI_F32 only); 0 = whole K
 };
+
+// eight consecutive bf16 of one operand row (16-byte aligned when ld and the
+// base are multiples of 8 elements, which every operand here is)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u32x4 ld8(const bf16 *p) { return *reinterpret_cast<const u32x4*>(p); }
+__device__ __forceinline__ bf16 part(const u32x4 &v, int t)
+{
+  unsigned w = v[t >> 1];
+  unsigned short h = (t & 1) ? (unsigned short)(w >> 16) : (unsigned short)(w & 0xffffu);
+  return __builtin_bit_cast(bf16, h);
+}
+
+// Stage the 64 x 32 slice of an operand at (row0, k0) into S[row][k] (LDS, row
+// stride BK + LPAD).  trans = 0: element (r, k) at P[r ld + k], rows contiguous
+// in k: each thread loads 8 k values of one row with one 16-byte load.  trans =
+// 1: element at P[k ld + r], contiguous in r: each thread loads 8 rows of one k
+// and scatters them.  Rows at or past `rows` and k at or past K are zero.
+__device__ __forceinline__ void stage(bf16 (*S)[BK + LPAD], const bf16 *P, int ld, int trans, int row0, int rows,
+                                      int k0, int K, int tid)
+{
+  if (!trans)
+  {
+    const int r = tid >> 2, c = (tid & 3)*8;
+    const bf16 *q = P + (size_t)(row0 + r)*ld + k0 + c;
+    if (row0 + r < rows && k0 + c + 8 <= K)
+      *reinterpret_cast<u32x4*>(&S[r][c]) = ld8(q);
+    else
+    {
+#pragma unroll
+      for (int t = 0; t < 8; t ++) S[r][c + t] = row0 + r < rows && k0 + c + t < K ? q[t] : (bf16)0.0f;
+    }
+  }
+  else
+  {
+    const int k = tid >> 3, c = (tid & 7)*8;
+    const bf16 *q = P + (size_t)(k0 + k)*ld + row0 + c;
+    if (k0 + k < K && row0 + c + 8 <= rows)
+    {
+      const u32x4 v = ld8(q);
+#pragma unroll
+      for (int t = 0; t < 8; t ++) S[c + t][k] = part(v, t);
+    }
+    else
+    {
+#pragma unroll
+      for (int t = 0; t < 8; t ++) S[c + t][k] = k0 + k < K && row0 + c + t < rows ? q[t] : (bf16)0.0f;
+    }
+  }
+}
 
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
 {
-  __shared__ bf16 As[BM][BK + LPAD];
-  __shared__ bf16 Bt[BN][BK + LPAD];   // B tile stored transposed: Bt[n][k]
+  __shared__ __align__(16) bf16 As[BM][BK + LPAD];
+  __shared__ __align__(16) bf16 Bt[BN][BK + LPAD];   // B tile stored transposed: Bt[n][k]
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int m0 = blockIdx.y*BM, n0 = blockIdx.x*BN;
   const int wm = (wave >> 1)*32, wn = (wave & 1)*32;
+  const int kb = g.kslice ? blockIdx.z*g.kslice : 0, ke = g.kslice ? min(g.K, kb + g.kslice) : g.K;
   floatx4 acc[2][2];
   for (int i = 0; i < 2; i ++) for (int j = 0; j < 2; j ++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  for (int k0 = 0; k0 < g.K; k0 += BK)
+  for (int k0 = kb; k0 < ke; k0 += BK)
   {
-    // stage A (64 x 32) and B (32 x 64): 2048 elements each, 8 per thread
-    {
-      int e = tid*8;
-      int r = e / BK, c = e % BK;            // A tile row r, k c..c+7
-      int m = m0 + r;
-      for (int t = 0; t < 8; t ++)
-      {
-        int k = k0 + c + t;
-        bf16 v = (bf16)0.0f;
-        if (m < g.M && k < g.K) v = g.transA ? g.A[(size_t)k*g.lda + m] : g.A[(size_t)m*g.lda + k];
-        As[r][c + t] = v;
-      }
-      int nn = e / BK, kk = e % BK;          // B tile column nn, k kk..kk+7
-      int n = n0 + nn;
-      for (int t = 0; t < 8; t ++)
-      {
-        int k = k0 + kk + t;
-        bf16 v = (bf16)0.0f;
-        if (n < g.N && k < g.K) v = g.transB ? g.B[(size_t)n*g.ldb + k] : g.B[(size_t)k*g.ldb + n];
-        Bt[nn][kk + t] = v;
-      }
-    }
+    // A rows are m (A[m][k] or A[k][m]); B is staged by its n rows: B[n][k]
+    // when transB = 1, B[k][n] when transB = 0
+    stage(As, g.A, g.lda, g.transA, m0, g.M, k0, ke, tid);
+    stage(Bt, g.B, g.ldb, !g.transB, n0, g.N, k0, ke, tid);
     __syncthreads();
     const int fr = lane & 15, fk = 8*(lane >> 4);
     bf16x8 a[2], b[2];
-    for (int i = 0; i < 2; i ++)
-      for (int t = 0; t < 8; t ++) a[i][t] = As[wm + 16*i + fr][fk + t];
-    for (int j = 0; j < 2; j ++)
-      for (int t = 0; t < 8; t ++) b[j][t] = Bt[wn + 16*j + fr][fk + t];
+    for (int i = 0; i < 2; i ++) a[i] = *reinterpret_cast<const bf16x8*>(&As[wm + 16*i + fr][fk]);
+    for (int j = 0; j < 2; j ++) b[j] = *reinterpret_cast<const bf16x8*>(&Bt[wn + 16*j + fr][fk]);
     for (int i = 0; i < 2; i ++)
       for (int j = 0; j < 2; j ++)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
@@ -107,7 +139,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
         float v = acc[i][j][q];
         switch (g.epi)
         {
-        case EPI_F32: g.C[(size_t)m*g.ldc + n] = v; break;
+        case EPI_F32: g.C[((size_t)blockIdx.z*g.M + m)*g.ldc + n] = v; break;
         case EPI_BIAS_F32: g.C[(size_t)m*g.ldc + n] = v + g.bias[n]; break;
         case EPI_BIAS_GELU:
         {
@@ -142,6 +174,7 @@ struct Model {
   int cap = 0;
   bf16 *X = nullptr, *H1 = nullptr, *H2 = nullptr, *H3 = nullptr, *dO = nullptr, *dZ = nullptr, *dZ2 = nullptr;
   float *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr, *O = nullptr, *Y = nullptr, *loss = nullptr;
+  float *part = nullptr;   // split-K partials of the weight gradients: ceil(cap / KSLICE) x 128 x 128
   // device-resident training set
   float *tX = nullptr, *tY = nullptr;
   long long tn = 0, tcap = 0;
@@ -156,6 +189,16 @@ static void layout(Model &m)
   m.off[4] = o; o += (size_t)NH*NH;  m.off[5] = o; o += NH;
   m.off[6] = o; o += (size_t)NH*NOP; m.off[7] = o; o += NOP;
   m.total = o;
+}
+
+// out[i] = sum over the S split-K partials part[z n + i], in z order (deterministic)
+__global__ void k_sum_parts(int S, int n, const float *part, float *out)
+{
+  int i = blockIdx.x*blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float v = 0.0f;
+  for (int z = 0; z < S; z ++) v += part[(size_t)z*n + i];
+  out[i] = v;
 }
 
 __global__ void k_to_bf16(size_t n, const float *src, bf16 *dst)
@@ -269,17 +312,34 @@ void sfgs_mlp_release(sfgs_ctx *ctx)
 {
   Model *m = (Model*)ctx->mlp;
   if (!m) return;
-  void *ptrs[] = {m->param, m->grad, m->m1, m->m2, m->wbf, m->dnorm, m->X, m->H1, m->H2, m->H3, m->dO, m->dZ, m->dZ2,
+  void *ptrs[] = {m->param, m->grad, m->m1, m->m2, m->wbf, m->dnorm, m->part, m->X, m->H1, m->H2, m->H3, m->dO, m->dZ, m->dZ2,
                   m->Z1, m->Z2, m->Z3, m->O, m->Y, m->loss, m->tX, m->tY, m->idx};
   for (void *p : ptrs) if (p) (void)hipFree(p);
   delete m;
   ctx->mlp = nullptr;
 }
 
+static const int KSLICE = 512;   // K per workgroup of a split-K weight-gradient product
+
 static int gemm(sfgs_ctx *ctx, const GemmArgs &g)
 {
-  dim3 grid((g.N + BN - 1)/BN, (g.M + BM - 1)/BM);
+  dim3 grid((g.N + BN - 1)/BN, (g.M + BM - 1)/BM, g.kslice ? (g.K + g.kslice - 1)/g.kslice : 1);
   hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, ctx->stream, g);
+  return 0;
+}
+
+// a weight gradient C = A' B (small M x N, K = the batch): K split over
+// workgroups in slices of KSLICE, the partials summed in slice order into out
+static int gemm_wgrad(sfgs_ctx *ctx, Model *m, GemmArgs g, float *out)
+{
+  const int S = (g.K + KSLICE - 1)/KSLICE;
+  g.ldc = g.N;
+  if (S <= 1) { g.C = out; return gemm(ctx, g); }
+  g.kslice = KSLICE;
+  g.C = m->part;
+  gemm(ctx, g);
+  const int n = g.M*g.N;
+  hipLaunchKernelGGL(k_sum_parts, dim3((n + 255)/256), dim3(256), 0, ctx->stream, S, n, m->part, out);
   return 0;
 }
 
@@ -308,6 +368,9 @@ static int ensure_batch(sfgs_ctx *ctx, Model *m, int B)
     *bufs[k] = nullptr;
     HIPCHK(hipMalloc(bufs[k], sz[k]*(size_t)cap));
   }
+  if (m->part) (void)hipFree(m->part);
+  m->part = nullptr;
+  HIPCHK(hipMalloc(&m->part, sizeof(float)*(size_t)NH*NH*(size_t)((cap + KSLICE - 1)/KSLICE)));
   m->cap = cap;
   return 0;
 }
@@ -339,32 +402,28 @@ static int backward(sfgs_ctx *ctx, Model *m, int B)
   float *G = m->grad;
   // layer 4
   GemmArgs g = gargs(NH, NOP, B, m->H3, NH, 1, m->dO, NOP, 0, EPI_F32);     // dW4 = H3' dO
-  g.C = G + m->off[6]; g.ldc = NOP;
-  gemm(ctx, g);
+  gemm_wgrad(ctx, m, g, G + m->off[6]);
   hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, ctx->stream, B, NOP, m->dO, G + m->off[7]);
   g = gargs(B, NH, NOP, m->dO, NOP, 0, W4, NOP, 1, EPI_GELU_BWD);          // dZ3 = (dO W4') * gelu'(Z3)
   g.H = m->dZ; g.ldh = NH; g.Z = m->Z3; g.ldz = NH;
   gemm(ctx, g);
   // layer 3
   g = gargs(NH, NH, B, m->H2, NH, 1, m->dZ, NH, 0, EPI_F32);               // dW3 = H2' dZ3
-  g.C = G + m->off[4]; g.ldc = NH;
-  gemm(ctx, g);
+  gemm_wgrad(ctx, m, g, G + m->off[4]);
   hipLaunchKernelGGL(k_colsum, dim3(2), dim3(64), 0, ctx->stream, B, NH, m->dZ, G + m->off[5]);
   g = gargs(B, NH, NH, m->dZ, NH, 0, W3, NH, 1, EPI_GELU_BWD);             // dZ2 = (dZ3 W3') * gelu'(Z2)
   g.H = m->dZ2; g.ldh = NH; g.Z = m->Z2; g.ldz = NH;
   gemm(ctx, g);
   // layer 2
   g = gargs(NH, NH, B, m->H1, NH, 1, m->dZ2, NH, 0, EPI_F32);              // dW2 = H1' dZ2
-  g.C = G + m->off[2]; g.ldc = NH;
-  gemm(ctx, g);
+  gemm_wgrad(ctx, m, g, G + m->off[2]);
   hipLaunchKernelGGL(k_colsum, dim3(2), dim3(64), 0, ctx->stream, B, NH, m->dZ2, G + m->off[3]);
   g = gargs(B, NH, NH, m->dZ2, NH, 0, W2, NH, 1, EPI_GELU_BWD);            // dZ1 = (dZ2 W2') * gelu'(Z1)
   g.H = m->dZ; g.ldh = NH; g.Z = m->Z1; g.ldz = NH;
   gemm(ctx, g);
   // layer 1
   g = gargs(NI, NH, B, m->X, NI, 1, m->dZ, NH, 0, EPI_F32);                // dW1 = X' dZ1
-  g.C = G + m->off[0]; g.ldc = NH;
-  gemm(ctx, g);
+  gemm_wgrad(ctx, m, g, G + m->off[0]);
   hipLaunchKernelGGL(k_colsum, dim3(2), dim3(64), 0, ctx->stream, B, NH, m->dZ, G + m->off[1]);
   return 0;
 }

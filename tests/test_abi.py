@@ -13,6 +13,7 @@ LIB = os.path.join(ROOT, "solfec-1.0_amd", "lib")
 def declared(header):
     txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    txt = re.sub(r"^\s*#.*$", "", txt, flags=re.M)          # macros are not functions
     return sorted(set(re.findall(r"\b([A-Z][A-Za-z0-9]*_[A-Za-z0-9_]+)\s*\(", txt)) - {"SFGS_H", "SOLFEC_H"})
 
 

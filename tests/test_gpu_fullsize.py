@@ -1,0 +1,124 @@
+"""GPU parity at the full size of the packing workloads: 200,000 rigid spheres
+(radii U[0.045, 0.055] m, density 2000, jittered lattice in a 6 m box, mu 0.5)
+and the 200,000 rigid-ellipsoid variant (semi-axes U[0.03, 0.06] m, random
+orientations), each settled for 500 steps through the device loop with
+GAUSS_SEIDEL_SOLVER(1e-4, 1000).  On the next step's system the GPU and the
+oracle (bgs.c:913-1069 restated in C) start from identical W, B, R0 and U0:
+
+* the LOCDYN assembly agrees (adjacency and colouring bit for bit, W, W_ii, A,
+  B, V, rho to 1e-12 relative);
+* 25 FP64 Gauss-Seidel iterations with epsilon = meritval = 1e-6 in the GPU's
+  colour-major order agree per contact within 1e-6 of max|R|, and so do U and
+  the relative-error and merit histories;
+* 25 mixed-precision iterations (23 FP32, 2 FP64) follow the FP64 merit history
+  within 1e-3 relative, and the merit they report is the oracle's FP64 merit of
+  the state they return.
+
+Each case settles its packing on the GPU (~40-80 s) and runs the oracle's
+assembly and iterations on one host core (~10-20 s)."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+import solfec
+
+pytestmark = pytest.mark.gpu
+
+H = 1e-3
+ITERS = 25
+
+
+def settle(shape, n=200000, L=6.0, steps=500, seed=2, friction=0.5):
+    s = solfec.SOLFEC("DYNAMIC", H, "/tmp/solfec_test")
+    solfec.SURFACE_MATERIAL(s, friction=friction, restitution=0.0)
+    solfec.GRAVITY(s, (0.0, 0.0, -9.81))
+    if shape == "ellipsoid":
+        solfec.ELLIP_LATTICE(s, n, L, 0.03, 0.06, 0.005, 2000.0, seed)
+    else:
+        solfec.LATTICE(s, n, L, 0.045, 0.055, 0.005, 2000.0, seed)
+    s.device_loop(True)
+    gs = solfec.GAUSS_SEIDEL_SOLVER(1e-4, 1000)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        rc = s.step_begin() or s.step_solve(gs) or s.step_end()
+        assert rc == 0, s.lib.SOLFEC_Last_Error(s.sol).decode()
+        if (k + 1) % 100 == 0:
+            print(f"[settle {shape}] step {k + 1}: {s.ncon} contacts, {time.perf_counter() - t0:.1f} s", flush=True)
+    s.step_begin()            # the next step's contacts, R and U carried from the last solve
+    s.device_loop(False)      # bodies and contacts back into the host arrays
+    return s
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("shape", ["sphere", "ellipsoid"])
+def test_fullsize_assembly_and_gs_iterations_match_oracle(shape):
+    print(f"[fullsize] {shape}: settling 200000 bodies for 500 steps", flush=True)
+    s = settle(shape)
+    b, c = s.bodies(), s.contacts()
+    n = len(c["master"])
+    assert n > 250000, n
+    dev = solfec.Device(0)
+    try:
+        dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
+        dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
+                     c["restitution"], c["cohesion"], c["R"], c["U"])
+        dev.assemble(H)
+        G = dev.locdyn()
+        t0 = time.perf_counter()
+        O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), H)
+        print(f"[fullsize] {shape}: {n} contacts, {O['nnz']} blocks, oracle assembly "
+              f"{time.perf_counter() - t0:.1f} s", flush=True)
+        assert np.array_equal(G["rowptr"], O["rowptr"])
+        assert np.array_equal(G["col"], O["col"])
+        assert np.array_equal(G["via"], O["via"])
+        ncol, col = oracle.colour(O["rowptr"], O["col"])
+        assert np.array_equal(G["colour"], col)
+        for k in ("Wd", "A", "B", "V", "rho", "W"):
+            err = np.abs(G[k] - O[k]).max() / max(np.abs(O[k]).max(), 1e-300)
+            assert err < 1e-12, (k, err)
+        assert abs(G["free_energy"] - O["free_energy"]) <= 1e-12 * abs(O["free_energy"])
+        del G
+
+        kw = dict(epsilon=1e-6, maxiter=ITERS, meritval=1e-6)
+        rep, rer, mer = dev.solve(solfec.make_param(**kw))
+        Rg, Ug = dev.get_RU()
+        order, _ = oracle.colour_order(col, ncol)
+        t0 = time.perf_counter()
+        rc, orep, Ro, Uo, orer, omer = oracle.gs_solve(oracle.System(O), oracle.gs_param(**kw), order,
+                                                       c["R"], c["U"])
+        print(f"[fullsize] {shape}: {ITERS} oracle iterations {time.perf_counter() - t0:.1f} s, "
+              f"GPU {rep.solve_ms:.1f} ms", flush=True)
+        assert rep.iters == orep.iters == ITERS, (rep.iters, orep.iters)
+        errR = np.abs(Rg - Ro).max() / np.abs(Ro).max()
+        errU = np.abs(Ug - Uo).max() / np.abs(Uo).max()
+        print(f"[fullsize] {shape}: max|dR|/max|R| {errR:.3e}, max|dU|/max|U| {errU:.3e}, merit GPU "
+              f"{rep.merit:.6e} oracle {orep.merit:.6e}", flush=True)
+        assert errR < 1e-6, errR
+        assert errU < 1e-6, errU
+        assert np.allclose(rer, orer, rtol=1e-6, atol=0.0)
+        assert np.allclose(mer, omer, rtol=1e-6, atol=0.0)
+        # local-solver failures (rows whose SSN solve hits diagmaxiter and goes down the fallback
+        # chain, bgs.c:981-1002): the GPU counts those of the merit's closing sweep too
+        print(f"[fullsize] {shape}: local-solver failures GPU {rep.fail_rows} oracle {orep.diag_fail}", flush=True)
+        assert orep.diag_fail <= rep.fail_rows <= orep.diag_fail * (ITERS + 1) / ITERS + 1
+        assert orep.error == rep.error
+
+        # mixed precision on the same system: 23 FP32 iterations and 2 FP64 ones.  Its
+        # merit history (FP64 merits of the FP32 iterates) follows the FP64 one, and the
+        # reported merit is the oracle's FP64 merit of the returned state
+        dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
+                     c["restitution"], c["cohesion"], c["R"], c["U"])
+        dev.assemble(H)
+        mrep, mrer, mmer = dev.solve(solfec.make_param(**kw, precision=solfec.device.MIXED, refine=2))
+        Rm, Um = dev.get_RU()
+        m64 = oracle.merit(oracle.System(O), Rm.ravel(), Um.ravel(), 1)[0]
+        dm = np.abs(mmer - omer).max() / omer.max()
+        print(f"[fullsize] {shape}: mixed merit history vs FP64 {dm:.2e}, merit {mrep.merit:.6e} "
+              f"(oracle of its state {m64:.6e}), {mrep.fp64_sweeps} FP64 sweeps", flush=True)
+        assert mrep.iters == ITERS and len(mmer) == ITERS
+        assert dm < 1e-3
+        assert abs(m64 - mrep.merit) <= 1e-9 * m64
+    finally:
+        dev.close()

@@ -130,13 +130,18 @@ def test_gs_warm_start_and_U_state(dev):
 
 @pytest.mark.parametrize("solver", [0, 1])
 def test_other_local_solvers(dev, solver):
+    """PROJECTED_GRADIENT and DE_SAXCE_FENG as the primary local solver with epsilon
+    1e-8 (PROJECTED_GRADIENT, an inexact local solver, does not reach it within the 4000
+    iterations on either side): the reactions agree within 1e-6 of max|R|."""
     s = lattice(125, seed=11)
     b, c = s.bodies(), s.contacts()
     O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), 1e-3)
-    kw = dict(epsilon=1e-5, maxiter=2000, diagsolver=solver)
+    kw = dict(epsilon=1e-8, maxiter=4000, diagsolver=solver)
     rep, rer, mer, Rg, Ug, orep, orer, omer, Ro, Uo = _solve_both(dev, O, kw)
+    err = np.abs(Rg - Ro).max() / np.abs(Ro).max()
+    print(f"[local solver {solver}] iterations gpu {rep.iters} oracle {orep.iters}, |dR|/max|R| {err:.2e}", flush=True)
     assert np.all(np.isfinite(Rg)) and np.all(np.isfinite(Ro))
-    assert np.abs(Rg - Ro).max() <= 1e-5 * np.abs(Ro).max()
+    assert err <= RTOL_R
     assert abs(rep.iters - orep.iters) <= 2
 
 

@@ -187,6 +187,21 @@ int SFGS_Set_Row_Order(sfgs_ctx *ctx, int order);
  * colouring kernel; returns the number of colours or a negative code. */
 int SFGS_Colour(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, int *colour);
 
+/* SFGS_Colour without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  rowptr (n + 1), col
+ * (nnz) and colour (n) are DEVICE pointers the caller owns; colour receives the
+ * same colours as SFGS_Colour (bit for bit).  The rounds' scratch is a
+ * file-scope device array of this library, one per GPU, held until the process
+ * ends: n is limited to SFGS_STATIC_COLOUR_MAX (larger n returns SFGS_E_ARG),
+ * and a graph that is not valid CSR of nnz entries (rowptr[0] = 0,
+ * non-decreasing, rowptr[n] = nnz, columns in [0, n)) is refused with
+ * SFGS_E_ARG before any colouring round runs.  Calls that overlap on one GPU
+ * share that array and race; each call clears what it needs.  Returns the
+ * number of colours or a negative code. */
+#define SFGS_STATIC_COLOUR_MAX (1 << 20)
+int SFGS_Colour_Static(sfgs_ctx *ctx, int n, long long nnz, const int *rowptr, const int *col, int *colour);
+
 /* Body velocity update of BODY_Dynamic_Step_End (bod.c:1446-1447 with
  * rig_constraints_force, bod.c:418-435): velo += h iM r, r = sum(+-H' R) over
  * the contacts of every rigid body, using the current R; velo_out receives 6

@@ -48,6 +48,7 @@ struct sfgs_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  int *static_scratch = nullptr;   // this device's copy of the file-scope scratch array (SFGS_Colour_Static)
 
   // ---- bodies, natural order
   int nb = 0;

@@ -20,6 +20,7 @@
 #include <cfloat>
 #include <climits>
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include "sfgs_ctx.hpp"
 #include "sfgs_math.hpp"
@@ -1232,6 +1233,48 @@ const char* SFGS_Error_String(int code)
   return "unknown error";
 }
 
+} // extern "C"
+
+// ---- colouring without allocation (SFGS_Colour_Static) ----
+// The scratch of the colouring rounds lives in one file-scope device array:
+// the next round's colours (SFGS_STATIC_COLOUR_MAX ints) and, at a 256-byte
+// aligned offset after them, 4 counters.  Every device that runs the library
+// holds its own copy until the process ends (4 MiB + 256 B).  In host code the
+// array's name is not a device address: kernels get the address that
+// hipGetSymbolAddress returns for the current device, kept per device.
+static __device__ __align__(256) int scratch[SFGS_STATIC_COLOUR_MAX + 64];
+
+static int *scratch_ptr[64];
+static std::mutex scratch_mu;
+
+static int* static_scratch_address(int device)
+{
+  if (device < 0 || device >= 64) return nullptr;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[device])
+  {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) { (void)hipGetLastError(); return nullptr; }
+    scratch_ptr[device] = static_cast<int*>(p);
+  }
+  return scratch_ptr[device];
+}
+
+// CSR check on the device: rowptr[0] = 0, non-decreasing, rowptr[n] = nnz
+// (the length of col), columns in [0, n); col is read only inside [0, nnz).
+// bad[0] counts the rows that fail (the colouring rounds read cin[col[p]])
+__global__ void k_check_csr(int n, long long nnz, const int *rowptr, const int *col, int *bad)
+{
+  int v = blockIdx.x*blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  long long a = rowptr[v], b = rowptr[v + 1];
+  if (a < 0 || b < a || b > nnz || (v == 0 && a != 0) || (v == n - 1 && b != nnz)) { atomicAdd(bad, 1); return; }
+  for (long long p = a; p < b; p ++)
+    if (col[p] < 0 || col[p] >= n) { atomicAdd(bad, 1); return; }
+}
+
+extern "C" {
+
 sfgs_ctx* SFGS_Create(int device)
 {
   int count = 0;
@@ -1239,6 +1282,9 @@ sfgs_ctx* SFGS_Create(int device)
   if (hipSetDevice(device) != hipSuccess) return nullptr;
   sfgs_ctx *ctx = new sfgs_ctx();
   ctx->device = device;
+  // the scratch array of SFGS_Colour_Static on this device (a failure is
+  // reported by that entry point, which then refuses to run)
+  ctx->static_scratch = static_scratch_address(device);
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return nullptr; }
   (void)hipEventCreate(&ctx->ev0);
   (void)hipEventCreate(&ctx->ev1);
@@ -2106,6 +2152,45 @@ int SFGS_Colour(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, int *co
   if (n > 0) HIPCHK(hipMemcpyAsync(colour, ctx->colour.p, sizeof(int)*(size_t)n, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return ctx->ncolours;
+}
+
+int SFGS_Colour_Static(sfgs_ctx *ctx, int n, long long nnz, const int *rowptr, const int *col, int *colour)
+{
+  if (!ctx || n < 0 || nnz < 0 || nnz > INT_MAX || (n > 0 && (!rowptr || !colour)) || (nnz > 0 && !col))
+    return SFGS_E_ARG;
+  if (n > SFGS_STATIC_COLOUR_MAX)
+  {
+    ctx->err = "SFGS_Colour_Static colours at most " + std::to_string(SFGS_STATIC_COLOUR_MAX) + " vertices";
+    return SFGS_E_ARG;
+  }
+  if (!ctx->static_scratch) { ctx->err = "no address for the static scratch array on this device"; return SFGS_E_HIP; }
+  if (n == 0) return 0;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  int *next = ctx->static_scratch, *counter = ctx->static_scratch + SFGS_STATIC_COLOUR_MAX;
+  int h[2] = {0, 0};
+  // counters cleared by a kernel (no memset node if a caller captures the stream)
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, 4, 0, counter);
+  hipLaunchKernelGGL(k_check_csr, dim3(nblk(n)), dim3(TPB), 0, st, n, nnz, rowptr, col, counter + 2);
+  HIPCHK(hipMemcpyAsync(h, counter + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  if (h[0]) { ctx->err = std::to_string(h[0]) + " rows of the graph are not valid CSR"; return SFGS_E_ARG; }
+  hipLaunchKernelGGL(k_fill_int, dim3(nblk(n)), dim3(TPB), 0, st, n, -1, colour);
+  hipLaunchKernelGGL(k_fill_int, dim3(nblk(n)), dim3(TPB), 0, st, n, -1, next);
+  for (int round = 0; ; round ++)
+  {
+    if (round > 100000) { ctx->err = "colouring did not finish"; return SFGS_E_STATE; }
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, 1, 0, counter);
+    hipLaunchKernelGGL(k_jp_round, dim3(nblk(n)), dim3(TPB), 0, st, n, rowptr, col, colour, next, counter, counter + 1);
+    hipLaunchKernelGGL(k_copy_int, dim3(nblk(n)), dim3(TPB), 0, st, n, next, colour);
+    HIPCHK(hipMemcpyAsync(h, counter, 2*sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipGetLastError());
+    if (h[0] == 0) break;
+  }
+  if (h[1] >= SFGS_MAX_COLOURS) { ctx->err = "graph needs more than 256 colours"; return SFGS_E_COLOURS; }
+  return h[1] + 1;
 }
 
 int SFGS_Body_Update(sfgs_ctx *ctx, double step, double *velo_out, double *force_out)

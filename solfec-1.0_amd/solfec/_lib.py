@@ -69,6 +69,7 @@ _SFGS_PROTOS = {
     "SFGS_Get_Locdyn": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "SFGS_Get_Sizes": (_i, [_p, _p, _p, _p]),
     "SFGS_Colour": (_i, [_p, _i, _p, _p, _p]),
+    "SFGS_Colour_Static": (_i, [_p, _i, C.c_longlong, _p, _p, _p]),
     "SFGS_Body_Update": (_i, [_p, _d, _p, _p]),
     "SFGS_Penalty_Solve": (_i, [_p, _d, _i, _p, _p, _p]),
     "SFGS_Newton_Solve": (_i, [_p, C.POINTER(SFGS_NEWTON_PARAM), _p, _p, C.POINTER(SFGS_NEWTON_REPORT)]),

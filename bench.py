@@ -501,9 +501,18 @@ def main():
     ap.add_argument("--host-loop", action="store_true", help="settle with the host loop instead of the device loop")
     ap.add_argument("--nt-maxiter", type=int, default=500, help="Newton steps bound of --workload consumers (DIAG: 4x)")
     ap.add_argument("--scenes", type=int, default=256)
-    ap.add_argument("--ws-new-only", type=int, default=1, help="learned warm start only on new contacts")
-    ap.add_argument("--ws-scale", type=float, default=0.5, help="trust factor of the learned warm start")
+    ap.add_argument("--ws-new-only", type=int, default=None,
+                    help="learned warm start only on new contacts (default: the shipped model's variant)")
+    ap.add_argument("--ws-scale", type=float, default=None,
+                    help="trust factor of the learned warm start (default: the shipped model's variant)")
     args = ap.parse_args()
+    if args.ws_new_only is None or args.ws_scale is None:
+        from solfec import learned
+        meta = learned.load_meta() if os.path.exists(learned.DEFAULT_MODEL) else {}
+        if args.ws_new_only is None:
+            args.ws_new_only = int(meta.get("new_only", 1))
+        if args.ws_scale is None:
+            args.ws_scale = float(meta.get("trust", 0.5))
     if args.workload in ("batch", "run", "consumers"):
         if args.gpus != 1:
             print("bench.py measures one GPU", file=sys.stderr)
@@ -666,7 +675,8 @@ def main():
             "epsilon": 1e-6, "meritval": 1e-6, "maxiter": args.maxiter, "friction": args.friction,
             "iters_per_step": iters, "merit": modes[head]["merit"], "rel_error": modes[head]["rel_error"],
             "precision": "mixed" if is32 else "fp64",
-            "warm_start": "learned (new contacts, trust %.2g)" % args.ws_scale if head == "learned" else "previous-step R",
+            "warm_start": ("learned (%s, trust %.2g)" % ("new contacts" if args.ws_new_only else "all contacts", args.ws_scale)
+                           if head == "learned" else "previous-step R"),
             "lanes_per_row": args.lanes,
             "waves_per_workgroup": args.wpb,
             "schedule": ["per-colour launches, XCD-aware", "dataflow", "per-colour launches",

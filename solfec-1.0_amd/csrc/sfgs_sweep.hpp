@@ -104,28 +104,6 @@ __device__ __forceinline__ void r_st(T *p, T v)
   else *p = v;
 }
 
-// The three reaction components of one row (stored 4 per row, 16-byte
-// aligned in FP32 and 32-byte aligned in FP64): one 16-byte load in FP32, two
-// loads in FP64; the coherent variant loads them one by one.
-template <bool COH, typename T>
-__device__ __forceinline__ void r_ld3(const T *p, T &x, T &y, T &z)
-{
-  if constexpr (COH)
-  {
-    x = r_ld<COH>(p); y = r_ld<COH>(p + 1); z = r_ld<COH>(p + 2);
-  }
-  else if constexpr (sizeof(T) == 4)
-  {
-    const float4 v = *reinterpret_cast<const float4*>(p);
-    x = v.x; y = v.y; z = v.z;
-  }
-  else
-  {
-    const double2 v = *reinterpret_cast<const double2*>(p);
-    x = v.x; y = v.y; z = p[2];
-  }
-}
-
 // One slice (64/G rows of one colour, one wavefront) of a sweep.  eu, el and
 // up receive this lane's contributions to the error sums and to the merit
 // numerator.  A failed local solve is recorded in failkey / failcount, keyed by
@@ -155,7 +133,8 @@ __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool v
     Bi[0] = a.B[p]; Bi[1] = a.B[(size_t)n + p]; Bi[2] = a.B[2*(size_t)n + p];
 #pragma unroll
     for (int e = 0; e < 9; e ++) Wi[e] = a.Wd[(size_t)e*n + p];
-    r_ld3<COH>(a.P + 4*(size_t)p, R[0], R[1], R[2]);
+#pragma unroll
+    for (int k = 0; k < 3; k ++) R[k] = r_ld<COH>(a.P + 4*(size_t)p + k);
     if (MERIT)
     {
 #pragma unroll
@@ -204,16 +183,14 @@ __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool v
       {
         const int j = jj[q];
         const bool upd = j >= upd_lo && j < upd_hi;
+        const T *xj = (upd ? a.Q : a.P) + 4*(size_t)j;
+        rv[q][0] = r_ld<COH>(xj); rv[q][1] = r_ld<COH>(xj + 1); rv[q][2] = r_ld<COH>(xj + 2);
         if (MERIT)
         {
-          // the previous iterate of every neighbour, and the current one where it
-          // was updated in this sweep (the same values elsewhere)
-          r_ld3<COH>(a.P + 4*(size_t)j, pv[q][0], pv[q][1], pv[q][2]);
-          if (upd) r_ld3<COH>(a.Q + 4*(size_t)j, rv[q][0], rv[q][1], rv[q][2]);
-          else { rv[q][0] = pv[q][0]; rv[q][1] = pv[q][1]; rv[q][2] = pv[q][2]; }
+          // the previous iterate of every neighbour (equal to rv where not yet updated)
+          const T *pj = a.P + 4*(size_t)j;
+          pv[q][0] = r_ld<COH>(pj); pv[q][1] = r_ld<COH>(pj + 1); pv[q][2] = r_ld<COH>(pj + 2);
         }
-        else
-          r_ld3<COH>((upd ? a.Q : a.P) + 4*(size_t)j, rv[q][0], rv[q][1], rv[q][2]);
       }
 #pragma unroll
       for (int q = 0; q < CH; q ++)

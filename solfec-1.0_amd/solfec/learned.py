@@ -115,6 +115,14 @@ def load(path=DEFAULT_MODEL):
     return z["params"], z["norm"]
 
 
+def load_meta(path=DEFAULT_MODEL):
+    """The training record saved with a model (a dict literal; read without
+    evaluating code); the variant to apply is (meta["new_only"], meta["trust"])."""
+    import ast
+    z = np.load(path, allow_pickle=False)
+    return ast.literal_eval(str(z["meta"])) if "meta" in z.files else {}
+
+
 def install(dev_or_solfec, path=DEFAULT_MODEL, params=None, norm=None):
     """Load a model into a Device or into the device context of a SOLFEC domain."""
     if params is None:
@@ -146,35 +154,87 @@ def compare_iterations(dev, b, c, step=1e-3, epsilon=1e-6, maxiter=5000, meritva
     return base, learned
 
 
-def train_default(dev, seeds=(1, 2, 3, 4, 5, 6), steps=100, new_only=True, epochs=60, batch=1024, lr=2e-3, seed=1,
-                  log=print):
+def held_out_iterations(dev, params, norm, seed, mu, variants, steps=48, every=4, epsilon=1e-6, maxiter=3000):
+    """GS iterations to epsilon on the systems of one config-1 trajectory (every
+    `every`-th step of the first `steps`), from the previous-step R (the
+    reference) and from the network's R0 for each (new_only, trust) variant.
+    Returns {"previous": total, variant: total, ...}."""
+    s = solfec.SOLFEC("DYNAMIC", 1e-3, "/tmp/solfec_eval")
+    solfec.SURFACE_MATERIAL(s, friction=mu, restitution=0.0)
+    solfec.GRAVITY(s, (0.0, 0.0, -9.81))
+    solfec.LATTICE(s, 1000, 1.0, 0.045, 0.055, 0.005, 2000.0, seed)
+    gs = solfec.GAUSS_SEIDEL_SOLVER(epsilon, maxiter)
+    par = solfec.make_param(epsilon, maxiter, meritval=1.0)
+    out = {"previous": 0}
+    out.update({v: 0 for v in variants})
+    for k in range(steps):
+        s.step_begin()
+        if k % every == every - 1:
+            b, c = s.bodies(), s.contacts()
+            for v in ("previous",) + tuple(variants):
+                dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
+                dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
+                             c["restitution"], c["cohesion"], c["R"], c["U"])
+                dev.assemble(1e-3)
+                if v != "previous":
+                    dev.mlp_set(params, norm)
+                    dev.warm_start_mode(int(v[0]), float(v[1]))
+                    dev.warm_start()
+                rep, _, _ = dev.solve(par, history=False)
+                out[v] += rep.iters
+        s.step_solve(gs)
+        s.step_end()
+    return out
+
+
+VARIANTS = ((0, 1.0), (0, 0.5), (1, 1.0), (1, 0.5))   # (new contacts only, trust factor)
+
+
+def train_default(dev, seeds=(1, 2, 3, 4, 5, 6, 7), steps=200, samples=2_000_000, epochs=60, batch=4096, lr=2e-3,
+                  seed=1, validation=(8, 0.45), log=print):
     """The shipped model: snapshots of config-1 trajectories (seeded lattices of
-    1000 spheres, friction U[0.1, 0.8] drawn from each seed, GS to 1e-6), the
-    samples of new contacts only, Adam on the GPU."""
-    Xs, Ys = [], []
+    1000 spheres, friction U[0.1, 0.8] drawn for each trajectory, GS to 1e-6), the
+    first `samples` of them, Adam on the GPU.  Two networks are trained, on all
+    samples and on the new-contact samples only; the (network, variant) pair with
+    the fewest iterations on the validation trajectory is returned."""
+    rng = np.random.default_rng(seed)
+    Xs, Ys, total = [], [], 0
     for sd in seeds:
-        mu = float(np.random.default_rng(sd).uniform(0.1, 0.8))
+        mu = float(rng.uniform(0.1, 0.8))
         x, y = snapshots(int(sd), mu, steps=steps, epsilon=1e-6, maxiter=3000)
         Xs.append(x)
         Ys.append(y)
+        total += len(x)
         log(f"[learned] seed {sd} mu {mu:.3f}: {len(x)} samples")
-    X, Y = np.concatenate(Xs), np.concatenate(Ys)
-    if new_only:
-        keep = np.all(X[:, 13:16] == 0, axis=1)
-        X, Y = X[keep], Y[keep]
-    params, norm, hist = train(dev, X, Y, epochs=epochs, batch=batch, lr=lr, seed=seed)
-    log(f"[learned] {len(X)} samples, loss {hist[0]:.4f} -> {hist[-1]:.4f}")
-    return params, norm, dict(seeds=list(seeds), steps=steps, new_only=new_only, epochs=epochs, batch=batch, lr=lr,
-                              samples=len(X), loss_first=float(hist[0]), loss_last=float(hist[-1]))
+        if total >= samples:
+            break
+    X, Y = np.concatenate(Xs)[:samples], np.concatenate(Ys)[:samples]
+    new = np.all(X[:, 13:16] == 0, axis=1)
+    best = None
+    for name, keep in (("all", np.ones(len(X), bool)), ("new", new)):
+        params, norm, hist = train(dev, X[keep], Y[keep], epochs=epochs, batch=batch, lr=lr, seed=seed)
+        log(f"[learned] trained on {name} samples ({keep.sum()}): loss {hist[0]:.4f} -> {hist[-1]:.4f}")
+        res = held_out_iterations(dev, params, norm, validation[0], validation[1], VARIANTS)
+        log(f"[learned] validation seed {validation[0]}: {res}")
+        for v in VARIANTS:
+            if best is None or res[v] < best[0]:
+                best = (res[v], name, v, params, norm, dict(samples_trained=int(keep.sum()), loss_first=float(hist[0]),
+                                                             loss_last=float(hist[-1]), validation=res))
+    _, name, v, params, norm, info = best
+    meta = dict(seeds=list(seeds[:len(Xs)]), steps=steps, samples=int(len(X)), trained_on=name, new_only=v[0],
+                trust=v[1], epochs=epochs, batch=batch, lr=lr, **info)
+    log(f"[learned] chosen: network trained on {name} samples, variant new_only={v[0]} trust={v[1]}")
+    return params, norm, meta
 
 
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser(description="train the warm-start network on the GPU and save it")
     ap.add_argument("--out", default=DEFAULT_MODEL)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--samples", type=int, default=2_000_000)
     args = ap.parse_args()
     d = solfec.Device(0)
-    p, nrm, meta = train_default(d, steps=args.steps)
+    p, nrm, meta = train_default(d, steps=args.steps, samples=args.samples)
     save(args.out, p, nrm, meta)
-    print(f"saved {args.out}")
+    print(f"saved {args.out}: {meta}")

@@ -34,34 +34,22 @@ def test_snapshots_and_training(dev):
 
 
 def test_shipped_model_against_previous_reaction(dev):
+    """Held-out trajectories (seeds outside the training and validation sets, mu 0.3
+    and 0.6): GS iterations to epsilon 1e-6 on every fourth system of the first 48
+    steps, from the previous-step R (the reference) and from the shipped network's
+    R0 in the variant chosen on the validation trajectory.  The gain is printed; the
+    network may not cost more than 5 % (the plain GS runs after it, so the answer
+    does not depend on it)."""
     params, norm = learned.load()
-    s = solfec.SOLFEC("DYNAMIC", 1e-3, "/tmp/solfec_test")
-    solfec.SURFACE_MATERIAL(s, friction=0.3, restitution=0.0)
-    solfec.GRAVITY(s, (0.0, 0.0, -9.81))
-    solfec.LATTICE(s, 1000, 1.0, 0.045, 0.055, 0.005, 2000.0, 9)   # a seed outside the training set
-    gs = solfec.GAUSS_SEIDEL_SOLVER(1e-6, 3000)
-    par = solfec.make_param(1e-6, 3000, meritval=1.0)
+    meta = learned.load_meta()
+    v = (int(meta.get("new_only", 1)), float(meta.get("trust", 0.5)))
     base = lrn = 0
-    for k in range(48):
-        s.step_begin()
-        if k % 4 == 3:
-            b, c = s.bodies(), s.contacts()
-            its = []
-            for use in (False, True):
-                dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
-                dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
-                             c["restitution"], c["cohesion"], c["R"], c["U"])
-                dev.assemble(1e-3)
-                if use:
-                    dev.mlp_set(params, norm)
-                    dev.warm_start_mode(1, 0.5)
-                    dev.warm_start()
-                rep, _, _ = dev.solve(par, history=False)
-                its.append(rep.iters)
-            base += its[0]
-            lrn += its[1]
-        s.step_solve(gs)
-        s.step_end()
-    gain = base / max(lrn, 1)
-    print(f"[learned] held-out: previous-step R {base} iterations, learned {lrn}; gain x{gain:.3f}")
-    assert lrn <= 1.10 * base
+    for seed, mu in ((9, 0.3), (10, 0.6)):
+        res = learned.held_out_iterations(dev, params, norm, seed, mu, (v,))
+        print(f"[learned] held-out seed {seed} mu {mu}: previous-step R {res['previous']} iterations, "
+              f"network {res[v]}", flush=True)
+        base += res["previous"]
+        lrn += res[v]
+    print(f"[learned] held-out total: previous-step R {base}, network {lrn}, gain x{base / max(lrn, 1):.3f} "
+          f"(variant new_only={v[0]} trust={v[1]}, trained on {meta.get('samples', '?')} samples)", flush=True)
+    assert lrn <= 1.05 * base

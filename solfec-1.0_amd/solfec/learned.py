@@ -188,15 +188,19 @@ def held_out_iterations(dev, params, norm, seed, mu, variants, steps=48, every=4
 
 
 VARIANTS = ((0, 1.0), (0, 0.5), (1, 1.0), (1, 0.5))   # (new contacts only, trust factor)
+SHIPPED_VARIANT = (1, 0.5)
 
 
 def train_default(dev, seeds=(1, 2, 3, 4, 5, 6, 7), steps=200, samples=2_000_000, epochs=60, batch=4096, lr=2e-3,
-                  seed=1, validation=(8, 0.45), log=print):
+                  seed=1, validation=((8, 0.45), (11, 0.55)), log=print):
     """The shipped model: snapshots of config-1 trajectories (seeded lattices of
     1000 spheres, friction U[0.1, 0.8] drawn for each trajectory, GS to 1e-6), the
     first `samples` of them, Adam on the GPU.  Two networks are trained, on all
-    samples and on the new-contact samples only; the (network, variant) pair with
-    the fewest iterations on the validation trajectory is returned."""
+    samples and on the new-contact samples only, and every variant of each is
+    measured on the validation trajectories.  The variant is fixed to the smaller
+    perturbation, new contacts only with trust 0.5 (the variants' validation totals
+    differ by less than the spread between trajectories); the network with the
+    fewer validation iterations in that variant is returned."""
     rng = np.random.default_rng(seed)
     Xs, Ys, total = [], [], 0
     for sd in seeds:
@@ -214,16 +218,21 @@ def train_default(dev, seeds=(1, 2, 3, 4, 5, 6, 7), steps=200, samples=2_000_000
     for name, keep in (("all", np.ones(len(X), bool)), ("new", new)):
         params, norm, hist = train(dev, X[keep], Y[keep], epochs=epochs, batch=batch, lr=lr, seed=seed)
         log(f"[learned] trained on {name} samples ({keep.sum()}): loss {hist[0]:.4f} -> {hist[-1]:.4f}")
-        res = held_out_iterations(dev, params, norm, validation[0], validation[1], VARIANTS)
-        log(f"[learned] validation seed {validation[0]}: {res}")
-        for v in VARIANTS:
-            if best is None or res[v] < best[0]:
-                best = (res[v], name, v, params, norm, dict(samples_trained=int(keep.sum()), loss_first=float(hist[0]),
-                                                             loss_last=float(hist[-1]), validation=res))
-    _, name, v, params, norm, info = best
-    meta = dict(seeds=list(seeds[:len(Xs)]), steps=steps, samples=int(len(X)), trained_on=name, new_only=v[0],
-                trust=v[1], epochs=epochs, batch=batch, lr=lr, **info)
-    log(f"[learned] chosen: network trained on {name} samples, variant new_only={v[0]} trust={v[1]}")
+        val = {}
+        for vs, vmu in validation:
+            res = held_out_iterations(dev, params, norm, vs, vmu, VARIANTS)
+            log(f"[learned] validation seed {vs}: {res}")
+            for k, v in res.items():
+                val[k] = val.get(k, 0) + v
+        if best is None or val[SHIPPED_VARIANT] < best[0]:
+            best = (val[SHIPPED_VARIANT], name, params, norm,
+                    dict(samples_trained=int(keep.sum()), loss_first=float(hist[0]), loss_last=float(hist[-1]),
+                         validation=val))
+    _, name, params, norm, info = best
+    meta = dict(seeds=list(seeds[:len(Xs)]), steps=steps, samples=int(len(X)), trained_on=name,
+                new_only=SHIPPED_VARIANT[0], trust=SHIPPED_VARIANT[1], epochs=epochs, batch=batch, lr=lr, **info)
+    log(f"[learned] chosen: network trained on {name} samples, variant new_only={SHIPPED_VARIANT[0]} "
+        f"trust={SHIPPED_VARIANT[1]}")
     return params, norm, meta
 
 

@@ -147,6 +147,12 @@ int SFGS_Update_Cohesion(sfgs_ctx *ctx);
  * SURFACE_MATERIAL_Cohesion_Get (sps.h:127, sps.c:356) returns per CON. */
 int SFGS_Get_Cohesion(sfgs_ctx *ctx, double *cohesion);
 
+/* Upload the contacts' cohesion states (natural order, 1 per contact), as
+ * SURFACE_MATERIAL_Cohesion_Set (sps.h:130, sps.c:363) sets them per CON; the
+ * host layer restores with it the states that update_cohesion leaves alone
+ * (SPRING_DASHPOT contacts, ldy.c:78) after a NEWTON_SOLVER step. */
+int SFGS_Set_Cohesion(sfgs_ctx *ctx, const double *cohesion);
+
 /* Download the assembled LOCDYN in natural order for inspection: rowptr
  * (n+1), col/via (nnz), W (9 nnz), Wd, A (9 n), B, V (3 n), rho (n), the free
  * energy and the colour of every contact; NULL skips an array. */
@@ -390,6 +396,14 @@ typedef struct {
  * maxiter entries (may be NULL).  R and U follow from SFGS_Get_RU. */
 int SFGS_Newton_Solve(sfgs_ctx *ctx, const SFGS_NEWTON_PARAM *par, double *merhist, int *mvhist,
                       SFGS_NEWTON_REPORT *rep);
+
+/* z = W x formed through the bodies as NEWTON_SOLVER's locdyn 'OFF' mode forms
+ * it (nts.c:396-431): v_b = h M_b^-1 sum of +-H' x over the contacts of body b,
+ * then z_i = H_master v_master - H_slave v_slave per contact.  x and z are in
+ * natural contact order, 3 per contact; needs a LOCDYN assembly from bodies
+ * (SFGS_Locdyn_Update_Begin).  No counterpart of its own in the reference (the
+ * operator of that mode, exposed so that it can be checked against W). */
+int SFGS_Bodyspace_Product(sfgs_ctx *ctx, const double *x, double *z);
 
 /* PENALTY_Solve (pes.c:247-281) for the uploaded bodies and contacts (or the
  * device loop's): per contact W_ii, B and V as SFGS_Locdyn_Update_Begin forms

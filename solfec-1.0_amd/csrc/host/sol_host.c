@@ -1011,6 +1011,50 @@ static int step_assemble(SOLFEC *sol, sfgs_ctx *g, GAUSS_SEIDEL *gs)
   return 0;
 }
 
+/* update_cohesion after a NEWTON step in a domain with SPRING_DASHPOT
+ * materials (their contacts are solved as SIGNORINI_COULOMB ones, nts.c:1072-1081):
+ * the states of the SPRING_DASHPOT contacts stay as they were (ldy.c:78).  The
+ * device updates every contact; the kept states are written back.  Returns an
+ * SFGS error code, or SOL_ERR_MEM */
+static int cohesion_keep_spring_dashpot(SOLFEC *sol, sfgs_ctx *g)
+{
+  const size_t n = (size_t)sol->nc;
+  if (n == 0) return SFGS_Update_Cohesion(g);
+  int *ms = malloc(2*n*sizeof(int));
+  double *before = malloc(n*sizeof(double)), *after = malloc(n*sizeof(double));
+  int rc = 0;
+  if (!ms || !before || !after)
+  {
+    snprintf(sol->err, sizeof(sol->err), "out of host memory for %zu contacts", n);
+    rc = SOL_ERR_MEM;
+  }
+  else
+  {
+    /* the contact list is the device's in the device loop, the host's otherwise */
+    if (sol->devloop) rc = SFGS_Dom_Get_Contacts(g, ms, ms + n, NULL, NULL, NULL, NULL, NULL, NULL, NULL, before,
+                                                 NULL, NULL);
+    else
+    {
+      memcpy(ms, sol->c_master, n*sizeof(int));
+      memcpy(ms + n, sol->c_slave, n*sizeof(int));
+      memcpy(before, sol->c_coh, n*sizeof(double));
+    }
+    if (!rc) rc = SFGS_Update_Cohesion(g);
+    if (!rc) rc = SFGS_Get_Cohesion(g, after);
+    if (!rc)
+    {
+      for (size_t i = 0; i < n; i ++)
+      {
+        const MATPAIR *m = find_material(sol, sol->surf[ms[i]], sol->surf[ms[n + i]]);
+        if (m->model == SOLFEC_SPRING_DASHPOT) after[i] = before[i];
+      }
+      rc = SFGS_Set_Cohesion(g, after);
+    }
+  }
+  free(ms); free(before); free(after);
+  return rc;
+}
+
 /* after the solve, in the host loop: R, U into the contact list and the
  * velocity update (the device loop does both in SFGS_Dom_Update_End) */
 static int step_finish(SOLFEC *sol, sfgs_ctx *g)
@@ -1019,8 +1063,10 @@ static int step_finish(SOLFEC *sol, sfgs_ctx *g)
   if (has_cohesion(sol))
   {
     /* LOCDYN_Update_End (ldy.c:879-889): the cohesion states after the solve */
-    rc = SFGS_Update_Cohesion(g);
+    if (has_spring_dashpot(sol)) rc = cohesion_keep_spring_dashpot(sol, g);
+    else rc = SFGS_Update_Cohesion(g);
     if (!rc && !sol->devloop) rc = SFGS_Get_Cohesion(g, sol->c_coh);
+    if (rc == SOL_ERR_MEM) return rc;
     if (rc)
     {
       snprintf(sol->err, sizeof(sol->err), "GPU cohesion update failed: %s (%s)", SFGS_Error_String(rc), SFGS_Last_Error(g));

@@ -68,7 +68,7 @@ static SweepArgs<T> sweep_args(sfgs_ctx *ctx, const SFGS_PARAM *par, const T *W,
   return a;
 }
 
-static MeritArgs merit_args(sfgs_ctx *ctx, int update_U, double *conmerit)
+[[maybe_unused]] static MeritArgs merit_args(sfgs_ctx *ctx, int update_U, double *conmerit)
 {
   MeritArgs m;
   m.row0 = ctx->slice_row0.p; m.nrows = ctx->slice_nrows.p; m.rowlen = ctx->rowlen.p; m.off = ctx->slice_off.p;

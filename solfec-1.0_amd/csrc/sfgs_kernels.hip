@@ -811,6 +811,12 @@ __global__ __launch_bounds__(TPB) void k_bs_rows(BodyView bv, ConView cv, int n,
     for (int k = 0; k < 3; k ++) U[(size_t)k*n + p] = B[(size_t)k*n + p] + z[k];
     return;
   }
+  if (mode == 2)
+  {
+#pragma unroll
+    for (int k = 0; k < 3; k ++) y[3*(size_t)i + k] = z[k];   // natural order
+    return;
+  }
   const double xp[3] = {x[3*(size_t)p], x[3*(size_t)p + 1], x[3*(size_t)p + 2]};
   double X[9], Y[9];
 #pragma unroll
@@ -824,6 +830,16 @@ __global__ __launch_bounds__(TPB) void k_bs_rows(BodyView bv, ConView cv, int n,
     q = q + alpha*u;
     y[3*(size_t)p + k] = q + ad*xp[k];
   }
+}
+
+// 3 values per row from natural to permuted order
+__global__ void k_perm3(int n, const int *perm, const double *xn, double *xp)
+{
+  const int p = blockIdx.x*blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i = perm[p];
+#pragma unroll
+  for (int k = 0; k < 3; k ++) xp[3*(size_t)p + k] = xn[3*(size_t)i + k];
 }
 
 int sfgs::bodyspace_product(sfgs_ctx *ctx, const double *x, int stride, int mode, const double *X, const double *Y,
@@ -2043,6 +2059,48 @@ int SFGS_Get_Cohesion(sfgs_ctx *ctx, double *cohesion)
   if (n == 0) return 0;
   if (ctx->coh.cap < n) { ctx->err = "no contact records"; return SFGS_E_STATE; }
   HIPCHK(hipMemcpyAsync(cohesion, ctx->coh.p, sizeof(double)*n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int SFGS_Bodyspace_Product(sfgs_ctx *ctx, const double *x, double *z)
+{
+  if (!ctx || (ctx->nc > 0 && (!x || !z))) return SFGS_E_ARG;
+  if (!ctx->assembled || !ctx->from_bodies || ctx->diag_only)
+  {
+    ctx->err = "body-space products need a LOCDYN assembly from bodies";
+    return SFGS_E_STATE;
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  const int n = ctx->nc;
+  if (n == 0) return 0;
+  DBuf<double> xn, xp, zn;
+  int rc = upload(ctx, xn, x, 3*(size_t)n);
+  if (!rc) rc = ensure(ctx, xp, 3*(size_t)n);
+  if (!rc) rc = ensure(ctx, zn, 3*(size_t)n);
+  if (!rc)
+  {
+    hipLaunchKernelGGL(k_perm3, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->perm.p, xn.p, xp.p);
+    rc = bodyspace_product(ctx, xp.p, 3, 2, nullptr, nullptr, 0.0, 0.0, 0.0, zn.p);
+  }
+  if (!rc)
+  {
+    hipError_t e = hipMemcpyAsync(z, zn.p, sizeof(double)*3*(size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) { ctx->err = hipGetErrorString(e); rc = SFGS_E_HIP; }
+  }
+  release(xn); release(xp); release(zn);
+  return rc;
+}
+
+int SFGS_Set_Cohesion(sfgs_ctx *ctx, const double *cohesion)
+{
+  if (!ctx || (ctx->nc > 0 && !cohesion)) return SFGS_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  const size_t n = (size_t)ctx->nc;
+  if (n == 0) return 0;
+  if (ctx->coh.cap < n) { ctx->err = "no contact records"; return SFGS_E_STATE; }
+  HIPCHK(hipMemcpyAsync(ctx->coh.p, cohesion, sizeof(double)*n, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return 0;
 }

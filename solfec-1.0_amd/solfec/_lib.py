@@ -66,6 +66,8 @@ _SFGS_PROTOS = {
     "SFGS_Get_RU": (_i, [_p, _p, _p]),
     "SFGS_Update_Cohesion": (_i, [_p]),
     "SFGS_Get_Cohesion": (_i, [_p, _p]),
+    "SFGS_Set_Cohesion": (_i, [_p, _p]),
+    "SFGS_Bodyspace_Product": (_i, [_p, _p, _p]),
     "SFGS_Get_Locdyn": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "SFGS_Get_Sizes": (_i, [_p, _p, _p, _p]),
     "SFGS_Colour": (_i, [_p, _i, _p, _p, _p]),

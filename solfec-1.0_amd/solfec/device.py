@@ -150,6 +150,15 @@ class Device:
             rer, mer = rer[:rep.iters].copy(), mer[:rep.iters].copy()
         return rep, rer, mer
 
+    def bodyspace_product(self, x):
+        """SFGS_Bodyspace_Product: z = W x through the bodies (locdyn 'OFF', nts.c:396-431),
+        x and z natural, 3 per contact."""
+        n = self.sizes()[0]
+        x = np.ascontiguousarray(x, np.float64).reshape(3 * n)
+        z = np.zeros(3 * n)
+        self._check(self.lib.SFGS_Bodyspace_Product(self.ctx, ptr(x), ptr(z)), "SFGS_Bodyspace_Product")
+        return z.reshape(n, 3)
+
     def merit(self, update_U=1, conmerit=False):
         m = C.c_double(0.0)
         cm = np.zeros(self.sizes()[0]) if conmerit else None

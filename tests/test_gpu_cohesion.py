@@ -199,3 +199,46 @@ def test_cohesive_device_loop_equals_host_loop(solver):
         for k in ("master", "slave", "gap", "R", "U", "cohesion", "mpnt", "spnt"):
             assert np.array_equal(ch[k], cd[k]), k
     assert cohesive[0] > 0
+
+
+@pytest.mark.parametrize("loop", ["host", "device"])
+def test_newton_keeps_spring_dashpot_cohesion(loop):
+    """update_cohesion skips SPRING_DASHPOT contacts (ldy.c:78), which a
+    NEWTON_SOLVER step solves as SIGNORINI_COULOMB ones (nts.c:1072-1081).
+    Spheres sliding on a floor (1 m/s, friction 0.3), glued at time 0 with a
+    cohesion of 0.5 N: surface 1 meets the floor through a SIGNORINI_COULOMB
+    material, surface 2 through a SPRING_DASHPOT one.  After one step the
+    SIGNORINI_COULOMB contacts hold the states that update_cohesion gives for
+    their reactions (the oracle's, bit for bit; the sliding ones break in mode
+    II), and the SPRING_DASHPOT contacts keep 0.5 N although the same test
+    would break some of them."""
+    print(f"[cohesion] NEWTON, SPRING_DASHPOT contacts keep their cohesion, {loop} loop", flush=True)
+    s = solfec.SOLFEC("DYNAMIC", H, "/tmp/solfec_test")
+    solfec.SURFACE_MATERIAL(s, 0, 1, friction=0.3, cohesion=0.5)
+    solfec.SURFACE_MATERIAL(s, 0, 2, model="SPRING_DASHPOT", friction=0.3, cohesion=0.5, spring=1e6, dashpot=-1.0)
+    solfec.GRAVITY(s, (0.0, 0.0, -9.81))
+    mat = solfec.BULK_MATERIAL(s, density=2000.0)
+    solfec.BODY(s, "OBSTACLE", solfec.PLANE((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 0), mat)
+    r = 0.05
+    surf = []
+    for i in range(10):
+        for j in range(10):
+            sid = 1 + (i + j) % 2
+            body = solfec.BODY(s, "RIGID", solfec.SPHERE((0.12 * i, 0.12 * j, r - 1e-4), r, 1, sid), mat)
+            solfec.INITIAL_VELOCITY(body, (1.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+            surf.append(sid)
+    s.device_loop(loop == "device")
+    solfec.RUN(s, solfec.NEWTON_SOLVER(meritval=1e-12, maxiter=500), H)
+    c = s.contacts()
+    assert len(c["master"]) == 100
+    sphere = np.where(c["master"] > 0, c["master"], c["slave"]) - 1
+    sd = np.asarray(surf)[sphere] == 2
+    rule = oracle.update_cohesion(c["R"], c["friction"], np.full(100, 0.5))
+    print(f"[cohesion] SIGNORINI_COULOMB: {(c['cohesion'][~sd] == 0).sum()} of {(~sd).sum()} broken; "
+          f"SPRING_DASHPOT: {(rule[sd] == 0).sum()} of {sd.sum()} would break, "
+          f"{(c['cohesion'][sd] == 0.5).sum()} kept", flush=True)
+    assert sd.sum() == 50
+    np.testing.assert_array_equal(c["cohesion"][~sd], rule[~sd])
+    assert (c["cohesion"][~sd] == 0).sum() > 0
+    assert np.all(c["cohesion"][sd] == 0.5)
+    assert (rule[sd] == 0).sum() > 0

@@ -217,3 +217,52 @@ def test_newton_body_space_run_device_loop_equals_host_loop():
     print(f"[newton] body space against assembled after 10 steps: max |d velo| {d:.2e}"
           f" of {np.abs(ba['velo']).max():.2e}", flush=True)
     assert d <= 1e-3 * np.abs(ba["velo"]).max()
+
+
+def _oracle_W_times(O, x):
+    """W x from the oracle's assembled blocks: W_ii and the off-diagonal blocks
+    (column-major 3x3, natural CSR)."""
+    n = O["n"]
+    Wd = O["Wd"].reshape(n, 3, 3).transpose(0, 2, 1)          # [i, row, col]
+    z = np.einsum("irc,ic->ir", Wd, x)
+    rows = np.repeat(np.arange(n), np.diff(O["rowptr"]))
+    Wb = O["W"].reshape(-1, 3, 3).transpose(0, 2, 1)
+    np.add.at(z, rows, np.einsum("krc,kc->kr", Wb, x[O["col"]]))
+    return z
+
+
+def _ellipsoid_scene(n, seed):
+    s = solfec.SOLFEC("DYNAMIC", H, "/tmp/solfec_test")
+    solfec.SURFACE_MATERIAL(s, friction=0.3)
+    solfec.GRAVITY(s, (0.0, 0.0, -9.81))
+    solfec.ELLIP_LATTICE(s, n, 0.6, 0.03, 0.06, 0.005, 2000.0, seed)
+    s.step_begin()
+    return s
+
+
+@pytest.mark.parametrize("scene", ["spheres", "cohesive spheres", "ellipsoids"])
+def test_bodyspace_product_matches_oracle_W(dev, scene):
+    """The operator of locdyn 'OFF' (nts.c:396-431), h H M^-1 H' x formed through
+    the bodies (k_bs_bodies, k_bs_rows), against the oracle's assembled W
+    (ldy.c:654-844, pinned by impulse responses and closed forms) times the same
+    x: within 1e-12 of max|z|, obstacles included."""
+    print(f"[newton] body-space product against the oracle's W, {scene}", flush=True)
+    s = _ellipsoid_scene(343, 4) if scene == "ellipsoids" else lattice(343, seed=5)
+    b, c = s.bodies(), s.contacts()
+    if scene == "cohesive spheres":
+        c = dict(c)
+        c["cohesion"] = np.where(np.arange(len(c["master"])) % 2 == 0, 0.05, 0.0)
+    n = len(c["master"])
+    dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
+    dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
+                 c["restitution"], c["cohesion"], c["R"], c["U"])
+    dev.assemble(H)
+    O = oracle.assemble(oracle.bodies_from(b), oracle.contacts_from(c), H)
+    rng = np.random.default_rng(11)
+    for k in range(3):
+        x = rng.normal(size=(n, 3)) * (10.0 ** rng.uniform(-2, 2, size=(n, 1)))
+        z = dev.bodyspace_product(x)
+        zo = _oracle_W_times(O, x)
+        err = np.abs(z - zo).max() / np.abs(zo).max()
+        print(f"[newton] {n} contacts, {O['nnz']} blocks: max |z - W x| / max |W x| {err:.2e}", flush=True)
+        assert err < 1e-12

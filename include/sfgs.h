@@ -339,6 +339,12 @@ int SFGS_Dom_Update_Begin(sfgs_ctx *ctx, double step, const double *gravity, int
  * its spatial point following the master body and its previous frame and gap
  * (update_contact, dom.c:452-458). */
 int SFGS_Dom_Set_Time(sfgs_ctx *ctx, double time);
+/* Analysis of the device loop (dom->dynamic): 1 (default) integrates the
+ * rigid bodies with BODY_Dynamic_Step_Begin/End (bod.c:1313-1373, 1425-1492),
+ * 0 with BODY_Static_Step_Begin/End (QUASI_STATIC, bod.c:1588-1645: u(t+h) =
+ * h M^-1 f(t+h) from no carried velocity, x += h v, R = R exp[h W]).  The
+ * assembly takes the same flag from SFGS_Locdyn_Update_Begin. */
+int SFGS_Dom_Set_Analysis(sfgs_ctx *ctx, int dynamic);
 /* DOM_Update_End (dom.c:3818) after SFGS_Solve: R and U back into the
  * contact list, the velocity update by the reactions (as SFGS_Body_Update)
  * and the rigid end of step (BODY_Dynamic_Step_End, bod.c:1425-1492) */

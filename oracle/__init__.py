@@ -108,6 +108,8 @@ def lib():
             "OR_FlexGMRES_Dense": (_i, [_i, _p, _p, _p, _p, _i, _i, _i, _d, _p, _p]),
             "OR_Rig_Step_Begin": (None, [_i, _d, _p, _d, _d, _p, _p, _p, _p, _p, _p, _p]),
             "OR_Rig_Step_End": (None, [_i, _d, _p, _p, _p, _p, _p, _p]),
+            "OR_Rig_Static_Begin": (None, [_i, _d, _p, _d, _d, _p, _p, _p]),
+            "OR_Rig_Static_End": (None, [_i, _d, _p, _p, _p]),
         }
         for fn, (res, args) in protos.items():
             f = getattr(L, fn)
@@ -323,6 +325,20 @@ def rig_step_end(kind, h, I, force, velo, rot, x, aux):
     velo, rot, x, aux = _f(velo).copy(), _f(rot).copy(), _f(x).copy(), _f(aux).copy()
     lib().OR_Rig_Step_End(int(kind), h, _ptr(_f(I)), _ptr(_f(force)), _ptr(velo), _ptr(rot), _ptr(x), _ptr(aux))
     return velo, rot, x, aux
+
+
+def rig_static_begin(kind, h, gravity, mass, minv, J, I, velo):
+    """BODY_Static_Step_Begin of one rigid body: returns the velocity h M^-1 f(t+h)."""
+    velo = _f(velo).copy()
+    lib().OR_Rig_Static_Begin(int(kind), h, _ptr(_f(gravity)), mass, minv, _ptr(_f(J)), _ptr(_f(I)), _ptr(velo))
+    return velo
+
+
+def rig_static_end(kind, h, velo, rot, x):
+    """BODY_Static_Step_End of one rigid body (velocity already updated): rot, x."""
+    rot, x = _f(rot).copy(), _f(x).copy()
+    lib().OR_Rig_Static_End(int(kind), h, _ptr(_f(velo)), _ptr(rot), _ptr(x))
+    return rot, x
 
 
 def penalty_contact(implicit, step, gap, spring, dashpot, hpow, friction, W, B, V, R0=(0.0, 0.0, 0.0),

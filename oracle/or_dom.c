@@ -28,6 +28,17 @@
  * (1 - cos t)/t^2 through t^12 below (10 degrees)^2, libm sin and cos above.
  * Obstacles stay fixed (their velocity is zero).
  *
+ * OR_Rig_Static_Begin / OR_Rig_Static_End follow BODY_Static_Step_Begin
+ * (bod.c:1588-1614, with rig_static_force, bod.c:357-386) and
+ * BODY_Static_Step_End (bod.c:1616-1645) for a rigid body under gravity only:
+ * u(t+h) = h M^-1 f(t+h), the previous velocity not carried, the angular part
+ * of f the gyroscopic term -W1 x J W1 with W1 = J^-1 exp[-hW] J W; at the end
+ * x += h v and R = R exp[h W].  The previous velocity (velo0) is left as it
+ * is.  Pinned in tests/test_oracle_dom.py by closed forms (free fall from any
+ * velocity, spin about a principal axis stopped in one step, Euler's equations
+ * of torque-free motion as h goes to 0, the end step against a Rodrigues
+ * rotation).
+ *
  * Parity: pinned in tests/test_oracle_dom.py by closed forms (two spheres on a
  * line, a sphere on a plane, the orientation rule, free fall, torque-free spin
  * of an isotropic body about a fixed axis, EXPMAP against a Rodrigues matrix
@@ -270,4 +281,35 @@ void OR_Rig_Step_End(int kind, double h, const double *I, const double *force, d
   for (int k = 0; k < 3; k ++) A[k] += h*force[k];
   or_mtv(DR, A, O);
   or_mv(I, O, W);                                      /* W(t+h) = I exp[-(h/2)W](...) */
+}
+
+void OR_Rig_Static_Begin(int kind, double h, const double *gravity, double mass, double minv, const double *J,
+                         const double *I, double *velo)
+{
+  if (kind == OR_OBS) return;
+  double *W = velo, *v = velo + 3, O[3], DR[9], f[6], JW[3], DJW[3], W1[3], JW1[3], c[3], dW[3];
+  for (int k = 0; k < 3; k ++) O[k] = h*W[k];
+  OR_Expmap(O, DR);                                    /* exp[h W(t)] */
+  f[0] = f[1] = f[2] = 0.0;                            /* body-frame torque: none under gravity */
+  for (int k = 0; k < 3; k ++) f[3 + k] = mass*gravity[k];
+  or_mv(J, W, JW);
+  or_mtv(DR, JW, DJW);                                 /* exp[-hW] J W */
+  for (int k = 0; k < 3; k ++) O[k] = h*f[k] + DJW[k];
+  or_mv(I, O, W1);                                     /* W1 = I [exp[-hW] J W + h T] */
+  or_mv(J, W1, JW1);
+  or_cross(W1, JW1, c);
+  for (int k = 0; k < 3; k ++) f[k] -= c[k];           /* T - W1 x J W1 */
+  or_mv(I, f, dW);
+  for (int k = 0; k < 3; k ++) { W[k] = h*dW[k]; v[k] = h*(minv*f[3 + k]); }   /* u = h iM f */
+}
+
+void OR_Rig_Static_End(int kind, double h, double *velo, double *R, double *x)
+{
+  if (kind == OR_OBS) return;
+  double *W = velo, *v = velo + 3, O[3], DR[9], T[9];
+  for (int k = 0; k < 3; k ++) x[k] += h*v[k];         /* x(t+h) = x(t) + h v(t+h) */
+  for (int k = 0; k < 3; k ++) O[k] = h*W[k];
+  OR_Expmap(O, DR);
+  or_mm(R, DR, T);
+  memcpy(R, T, sizeof(T));                             /* R(t+h) = R(t) exp[h W(t+h)] */
 }

@@ -158,6 +158,9 @@ void OR_Rig_Step_Begin(int kind, double h, const double *gravity, double mass, d
                        const double *I, double *velo, double *velo0, double *R, double *x, double *A);
 void OR_Rig_Step_End(int kind, double h, const double *I, const double *force, double *velo, double *R, double *x,
                      double *A);
+void OR_Rig_Static_Begin(int kind, double h, const double *gravity, double mass, double minv, const double *J,
+                         const double *I, double *velo);
+void OR_Rig_Static_End(int kind, double h, double *velo, double *R, double *x);
 
 /* spring-dashpot contact law (or_pes.c); W, B, V of the contact; R in: the
  * carried reaction (its tangential part enters B_N), out: the reaction; coh

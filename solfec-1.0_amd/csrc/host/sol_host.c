@@ -365,22 +365,35 @@ int SOLFEC_Ellipsoid_Lattice_Scene(SOLFEC *sol, int n, double L, double amin, do
 
 /* ---------------- rigid body integration (scheme RIG_NEG) ---------------- */
 
-/* bod.c:1313-1373 for RIG, no point forces: gravity only */
+/* bod.c:1313-1373 for RIG, no point forces: gravity only; QUASI_STATIC
+ * analysis: bod.c:1588-1614 */
 static void body_step_begin(SOLFEC *sol, int b)
 {
   if (sol->kind[b] == SOL_OBS)   /* fixed obstacles */
   {
-    memcpy(sol->velo0 + 6*b, sol->velo + 6*b, 6*sizeof(double));
+    if (sol->dynamic) memcpy(sol->velo0 + 6*b, sol->velo + 6*b, 6*sizeof(double));
+    return;
+  }
+  if (!sol->dynamic)
+  {
+    geom_rig_static_begin(sol->step, sol->gravity, sol->mass[b], sol->minv[b], sol->J + 9*b, sol->jinv + 9*b,
+                          sol->velo + 6*b);
     return;
   }
   geom_rig_step_begin(sol->step, sol->gravity, sol->mass[b], sol->minv[b], sol->J + 9*b, sol->jinv + 9*b,
                       sol->velo + 6*b, sol->velo0 + 6*b, sol->rot + 9*b, sol->x + 3*b, sol->aux + 3*b);
 }
 
-/* bod.c:1425-1492 for RIG: velocity already updated by the reactions (GPU) */
+/* bod.c:1425-1492 for RIG (QUASI_STATIC: bod.c:1616-1645): velocity already
+ * updated by the reactions (GPU) */
 static void body_step_end(SOLFEC *sol, int b)
 {
   if (sol->kind[b] == SOL_OBS) return;
+  if (!sol->dynamic)
+  {
+    geom_rig_static_end(sol->step, sol->velo + 6*b, sol->rot + 9*b, sol->x + 3*b);
+    return;
+  }
   geom_rig_step_end(sol->step, sol->jinv + 9*b, sol->force + 6*b, sol->velo + 6*b, sol->rot + 9*b, sol->x + 3*b,
                     sol->aux + 3*b);
 }
@@ -761,6 +774,7 @@ static int dev_push(SOLFEC *sol, sfgs_ctx *g)
     def.friction = sol->defmat.friction; def.restitution = sol->defmat.restitution; def.cohesion = sol->defmat.cohesion;
     def.spring = sol->defmat.spring; def.dashpot = sol->defmat.dashpot; def.hpow = sol->defmat.hpow;
     rc = SFGS_Dom_Upload(g, &sh, sol->nmat, mat, &def, sol->c_point);
+    if (!rc) rc = SFGS_Dom_Set_Analysis(g, sol->dynamic);
     free(mat);
   }
   if (rc)

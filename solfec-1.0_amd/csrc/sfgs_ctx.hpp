@@ -142,6 +142,7 @@ struct sfgs_ctx {
   sfgs::DBuf<double> mat_v;               // friction, restitution, cohesion, spring, dashpot, hpow, 6 per material
   double defmat[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 1.0};
   double dom_time = 0.0;                  // domain time: contacts created at 0 take their material's cohesion
+  int dom_dynamic = 1;                    // rigid scheme of the device loop: 1 dynamic, 0 quasi-static
   sfgs::DBuf<double> geo, geo2, dpart;
   sfgs::DBuf<int> okf, okpos, ok2, okpos2, ok3, okpos3, cand_m, cand_s;
   sfgs::DBuf<int> t_master, t_slave;

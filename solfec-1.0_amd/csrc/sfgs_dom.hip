@@ -151,24 +151,30 @@ __device__ bool pair_known(const unsigned long long *keys, unsigned long long ma
   return false;
 }
 
+// dynamic = 0: the quasi-static step (bod.c:1588-1614), velo0 left alone
 __global__ void k_dom_begin(int nb, const int *kind, double h, double g0, double g1, double g2, const double *mass,
                             const double *minv, const double *J, const double *I, double *velo, double *velo0,
-                            double *rot, double *x, double *aux)
+                            double *rot, double *x, double *aux, int dynamic)
 {
   int b = blockIdx.x*blockDim.x + threadIdx.x;
   if (b >= nb) return;
   if (kind[b] == 1)
   {
-    for (int k = 0; k < 6; k ++) velo0[6*(size_t)b + k] = velo[6*(size_t)b + k];
+    if (dynamic) for (int k = 0; k < 6; k ++) velo0[6*(size_t)b + k] = velo[6*(size_t)b + k];
     return;
   }
   const double grav[3] = {g0, g1, g2};
+  if (!dynamic)
+  {
+    geom_rig_static_begin(h, grav, mass[b], minv[b], J + 9*(size_t)b, I + 9*(size_t)b, velo + 6*(size_t)b);
+    return;
+  }
   geom_rig_step_begin(h, grav, mass[b], minv[b], J + 9*(size_t)b, I + 9*(size_t)b, velo + 6*(size_t)b,
                       velo0 + 6*(size_t)b, rot + 9*(size_t)b, x + 3*(size_t)b, aux + 3*(size_t)b);
 }
 
 __global__ void k_dom_end(int nb, const int *kind, double h, const double *I, const double *bout, double *velo,
-                          double *force, double *rot, double *x, double *aux)
+                          double *force, double *rot, double *x, double *aux, int dynamic)
 {
   int b = blockIdx.x*blockDim.x + threadIdx.x;
   if (b >= nb) return;
@@ -178,6 +184,11 @@ __global__ void k_dom_end(int nb, const int *kind, double h, const double *I, co
     force[6*(size_t)b + k] = bout[6*(size_t)nb + 6*(size_t)b + k];
   }
   if (kind[b] == 1) return;
+  if (!dynamic)
+  {
+    geom_rig_static_end(h, velo + 6*(size_t)b, rot + 9*(size_t)b, x + 3*(size_t)b);
+    return;
+  }
   geom_rig_step_end(h, I + 9*(size_t)b, force + 6*(size_t)b, velo + 6*(size_t)b, rot + 9*(size_t)b,
                     x + 3*(size_t)b, aux + 3*(size_t)b);
 }
@@ -551,7 +562,7 @@ int SFGS_Dom_Update_Begin(sfgs_ctx *ctx, double step, const double *gravity, int
   if (nb > 0)
     hipLaunchKernelGGL(k_dom_begin, dim3(nblk(nb)), dim3(TPB), 0, st, nb, ctx->kind.p, step, gravity[0], gravity[1],
                        gravity[2], ctx->mass.p, ctx->minv.p, ctx->J.p, ctx->jinv.p, ctx->velo.p, ctx->velo0.p,
-                       ctx->rot.p, ctx->bx.p, ctx->aux.p);
+                       ctx->rot.p, ctx->bx.p, ctx->aux.p, ctx->dom_dynamic);
 
   // existing contacts: geometry, kept flags and their order-preserving positions
   CHK(ensure(ctx, ctx->geo, GEO*(size_t)nc));
@@ -677,6 +688,13 @@ int SFGS_Dom_Set_Time(sfgs_ctx *ctx, double time)
   return 0;
 }
 
+int SFGS_Dom_Set_Analysis(sfgs_ctx *ctx, int dynamic)
+{
+  if (!ctx || (dynamic != 0 && dynamic != 1)) return SFGS_E_ARG;
+  ctx->dom_dynamic = dynamic;
+  return 0;
+}
+
 int SFGS_Dom_Update_End(sfgs_ctx *ctx, double step)
 {
   if (!ctx || !(step > 0.0)) return SFGS_E_ARG;
@@ -688,7 +706,8 @@ int SFGS_Dom_Update_End(sfgs_ctx *ctx, double step)
   if (nb > 0)
   {
     hipLaunchKernelGGL(k_dom_end, dim3(nblk(nb)), dim3(TPB), 0, ctx->stream, nb, ctx->kind.p, step, ctx->jinv.p,
-                       ctx->body_out.p, ctx->velo.p, ctx->force.p, ctx->rot.p, ctx->bx.p, ctx->aux.p);
+                       ctx->body_out.p, ctx->velo.p, ctx->force.p, ctx->rot.p, ctx->bx.p, ctx->aux.p,
+                       ctx->dom_dynamic);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
   }

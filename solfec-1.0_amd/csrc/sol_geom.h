@@ -339,6 +339,40 @@ SOL_GEOM_FN void geom_rig_step_end(double h, const double *I, const double *r, d
   geom_mv(I, t, W);   /* W(t+h) = I exp[-(h/2)W1][exp[-(h/2)W(t)] J W(t) + h T] */
 }
 
+/* begin of a quasi-static rigid step under gravity only (bod.c:1588-1614 with
+ * rig_static_force, bod.c:357-386): u(t+h) = h M^-1 f(t+h), where the
+ * angular part of f is the gyroscopic term -W1 x J W1, W1 = I exp[-hW] J W
+ * (no torque loads), and the linear part m g.  The previous velocity is not
+ * carried, and velo0 is left alone. */
+SOL_GEOM_FN void geom_rig_static_begin(double h, const double *gravity, double mass, double minv,
+                                       const double *J, const double *I, double *velo)
+{
+  double *W = velo, *v = velo + 3;
+  double O[3] = {h*W[0], h*W[1], h*W[2]}, DR[9], JW[3], DJW[3], W1[3], JW1[3], c[3], T[3], f[3], dW[3];
+  geom_expmap(O, DR);                               /* exp[h W(t)] */
+  for (int k = 0; k < 3; k ++) { T[k] = 0.0; f[k] = mass*gravity[k]; }
+  geom_mv(J, W, JW);
+  geom_mtv(DR, JW, DJW);                            /* exp[-hW] J W */
+  for (int k = 0; k < 3; k ++) O[k] = h*T[k] + DJW[k];
+  geom_mv(I, O, W1);                                /* W1 = I [exp[-hW] J W + h T] */
+  geom_mv(J, W1, JW1);
+  geom_cross(W1, JW1, c);
+  for (int k = 0; k < 3; k ++) T[k] -= c[k];        /* T - W1 x J W1 */
+  geom_mv(I, T, dW);
+  for (int k = 0; k < 3; k ++) { W[k] = h*dW[k]; v[k] = h*(minv*f[k]); }   /* u = h iM f */
+}
+
+/* end of a quasi-static rigid step (bod.c:1616-1645), the velocity already
+ * updated by the reactions: x(t+h) = x(t) + h v(t+h), R(t+h) = R(t) exp[h W(t+h)] */
+SOL_GEOM_FN void geom_rig_static_end(double h, const double *velo, double *R, double *x)
+{
+  const double *W = velo, *v = velo + 3;
+  for (int k = 0; k < 3; k ++) x[k] += h*v[k];
+  double O[3] = {h*W[0], h*W[1], h*W[2]}, DR[9];
+  geom_expmap(O, DR);
+  geom_mm(R, DR, R);
+}
+
 /* grid cell coordinate of a position v (in cells from the grid's corner),
  * clamped to [0, dim - 1]; a non-finite position falls into cell 0, where the
  * distance tests reject it */

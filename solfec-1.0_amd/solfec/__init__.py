@@ -27,17 +27,20 @@ _CALLBACK_T = C.CFUNCTYPE(C.c_int, C.c_void_p)
 
 
 class SOLFEC:
-    """SOLFEC(analysis, step, output) -- only 'DYNAMIC' is supported (lng.c:1719-1767)."""
+    """SOLFEC(analysis, step, output) with analysis 'DYNAMIC' or 'QUASI_STATIC' (lng.c:1719-1767).
+
+    QUASI_STATIC integrates the rigid bodies with the static scheme (bod.c:1588-1645:
+    u(t+h) = h M^-1 f(t+h), no carried velocity) and assembles the local dynamics with
+    the static gap shift max(gap, 0)/h and no restitution."""
 
     def __init__(self, analysis, step, output):
         if analysis not in ("DYNAMIC", "QUASI_STATIC"):
             raise ValueError("Invalid analysis kind")
-        if analysis != "DYNAMIC":
-            raise ValueError("only DYNAMIC analysis is supported by this build")
         if not step > 0:
             raise ValueError("'step' must be positive")
         self.lib = solfec_lib()
-        self.sol = self.lib.SOLFEC_Create(1, float(step), str(output).encode())
+        self.analysis = analysis
+        self.sol = self.lib.SOLFEC_Create(1 if analysis == "DYNAMIC" else 0, float(step), str(output).encode())
         if not self.sol:
             raise RuntimeError("SOLFEC_Create failed")
         self.step = float(step)

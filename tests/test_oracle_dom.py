@@ -145,3 +145,81 @@ def test_host_detection_matches_brute_force():
     kept = (O["old"] >= 0).sum()
     assert 0 < kept < len(c1["master"]) and kept < len(O["master"])
     _compare(s, O)
+
+
+# ---------------- QUASI_STATIC rigid step (bod.c:357-386, 1588-1645) ----------------
+
+def _inertia():
+    J = np.diag([0.01, 0.02, 0.035])
+    return J.ravel(), np.linalg.inv(J).ravel()
+
+
+def test_quasi_static_step_forgets_the_velocity():
+    """u(t+h) = h M^-1 f(t+h): under gravity alone a body starts every step from
+    rest, whatever it carried, and its velocity is h g."""
+    h, g, m = 1e-3, np.array([0.0, 0.0, -9.81]), 2.0
+    J, I = _inertia()
+    velo = oracle.rig_static_begin(0, h, g, m, 1 / m, J, I, np.r_[0.0, 0.0, 0.0, 0.5, -0.2, 1.0])
+    np.testing.assert_array_equal(velo[:3], 0.0)
+    np.testing.assert_allclose(velo[3:], h * g, rtol=1e-15)
+    # spin about a principal axis: exp[-hW] J W = J W, W1 = W, W1 x J W1 = 0: the spin stops
+    velo = oracle.rig_static_begin(0, h, g, m, 1 / m, J, I, np.r_[0.0, 0.0, 25.0, 0.0, 0.0, 0.0])
+    assert np.abs(velo[:3]).max() <= 1e-13 * 25.0
+    # obstacles keep their (zero) velocity
+    np.testing.assert_array_equal(oracle.rig_static_begin(1, h, g, m, 1 / m, J, I, np.zeros(6)), np.zeros(6))
+
+
+def test_quasi_static_step_is_eulers_equation_as_h_goes_to_zero():
+    """Off a principal axis the angular velocity after the step is h times the
+    torque-free Euler acceleration -J^-1 (W x J W), up to O(h^2)."""
+    g, m = np.zeros(3), 2.0
+    J, I = _inertia()
+    Jm = J.reshape(3, 3)
+    W = np.array([3.0, -4.0, 5.0])
+    euler = -np.linalg.solve(Jm, np.cross(W, Jm @ W))
+    errs = []
+    for h in (1e-3, 5e-4, 2.5e-4, 1.25e-4):
+        velo = oracle.rig_static_begin(0, h, g, m, 1 / m, J, I, np.r_[W, 0.0, 0.0, 0.0])
+        errs.append(np.abs(velo[:3] / h - euler).max())
+    print("Euler residuals", errs, "of", np.abs(euler).max())
+    assert errs[0] < 1e-2 * np.abs(euler).max()
+    for a, b in zip(errs, errs[1:]):
+        assert 1.9 < a / b < 2.1          # first order in h
+
+
+def test_quasi_static_end_step():
+    """x(t+h) = x(t) + h v, R(t+h) = R(t) exp[h W] (a Rodrigues rotation)."""
+    h = 1e-3
+    velo = np.r_[3.0, -40.0, 25.0, 0.5, -0.2, 1.0]
+    rot0, x0 = rodrigues(np.array([0.2, 0.1, -0.3])), np.array([1.0, 2.0, 3.0])
+    rot, x = oracle.rig_static_end(0, h, velo, rot0, x0)
+    np.testing.assert_allclose(x, x0 + h * velo[3:], rtol=1e-15)
+    Rexp = (rot0.reshape(3, 3).T @ rodrigues(h * velo[:3]).reshape(3, 3).T).T.ravel()
+    np.testing.assert_allclose(rot, Rexp, atol=1e-15)
+    rot, x = oracle.rig_static_end(1, h, velo, rot0, x0)
+    np.testing.assert_array_equal(x, x0)
+
+
+def test_host_quasi_static_step_matches_oracle():
+    """The host layer's QUASI_STATIC step_begin (shared with the device loop)
+    against the oracle, body by body; the previous velocities stay."""
+    n = 343
+    s = lattice(n, L=0.6, seed=5, steps_begin=0, analysis="QUASI_STATIC")
+    rng = np.random.default_rng(2)
+    for k in range(n):
+        s.set_velocity(k, np.r_[rng.normal(scale=5.0, size=3), rng.normal(scale=0.5, size=3)])
+    b0 = s.bodies()
+    s.step_begin()
+    b = s.bodies()
+    np.testing.assert_array_equal(b["velo0"], b0["velo0"])
+    np.testing.assert_array_equal(b["x"], b0["x"])
+    np.testing.assert_array_equal(b["rot"], b0["rot"])
+    mass = np.where(b0["minv"] > 0, 1.0 / np.where(b0["minv"] > 0, b0["minv"], 1.0), 0.0)
+    for k in range(n + 6):
+        I = b0["jinv"][9 * k:9 * k + 9]
+        J = np.linalg.inv(I.reshape(3, 3).T).T.ravel() if b0["kind"][k] == 0 else np.zeros(9)
+        velo = oracle.rig_static_begin(b0["kind"][k], 1e-3, [0, 0, -9.81], mass[k], b0["minv"][k], J, I,
+                                       b0["velo"][6 * k:6 * k + 6])
+        # J here is inv(jinv), not the host's own: the spheres' gyroscopic terms
+        # (zero in exact arithmetic) differ at the rounding level
+        np.testing.assert_allclose(b["velo"][6 * k:6 * k + 6], velo, rtol=1e-13, atol=1e-16)

@@ -172,6 +172,21 @@ def bench_consumers(args):
                                            merit_every_10=[float(x) for x in mer[9::10][:30]])
         log(f"[consumers] NEWTON {lv}: {1e3 * dt:.1f} ms, {rep.iters} steps, {rep.matvecs} products, "
             f"merit {rep.merit:.3e}")
+    # body space (locdyn OFF), DIAG steps, the assembly included: the full update
+    # against UPMIN (ldy.c:59, 644), which forms no off-diagonal blocks
+    for tag, diag_only in (("full", False), ("upmin", True)):
+        fresh(assemble=False)
+        t0 = time.perf_counter()
+        dev.assemble(h, diagonal_only=diag_only)
+        ta = time.perf_counter()
+        p = solfec.newton_param(meritval=1e-6, maxiter=4 * args.nt_maxiter, linver="DIAG", gsflag="OFF")
+        p.bodyspace = 1
+        rep, _, _ = dev.newton(p)
+        t1 = time.perf_counter()
+        out["newton_diag_bodyspace_" + tag] = dict(assembly_ms=1e3 * (ta - t0), solve_ms=1e3 * (t1 - ta),
+                                                   ms=1e3 * (t1 - t0), steps=rep.iters, merit=rep.merit)
+        log(f"[consumers] NEWTON DIAG body space, {tag} assembly: {1e3 * (ta - t0):.1f} + {1e3 * (t1 - ta):.1f} ms, "
+            f"{rep.iters} steps, merit {rep.merit:.3e}")
     # penalty: a linear spring with critical damping on every contact
     fresh(assemble=False)
     k = np.full(n, 1e6)

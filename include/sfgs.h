@@ -112,6 +112,14 @@ int SFGS_Contacts_Upload(sfgs_ctx *ctx, int nc, const int *master, const int *sl
  * colouring, colour-major block-sparse W, W_ii, A = inv(W_ii), rho, B, V and
  * the free energy, all on the device.  dynamic = 1 for DYNAMIC analysis. */
 int SFGS_Locdyn_Update_Begin(sfgs_ctx *ctx, double step, int dynamic);
+/* LOCDYN_Update_Begin with UPMIN (ldy.c:59, 644), as the reference runs it for
+ * NEWTON_SOLVER with locdyn OFF: W_ii, A, rho, B, V and the free energy per
+ * contact and the body incidence, no adjacency and no off-diagonal blocks
+ * (every row in one colour; SFGS_Get_Locdyn returns nnz = 0).  Only
+ * SFGS_Newton_Solve with bodyspace = 1, the merit without the U update, the
+ * body update and the cohesion update accept such a store; the other solves
+ * return SFGS_E_STATE. */
+int SFGS_Locdyn_Update_Min(sfgs_ctx *ctx, double step, int dynamic);
 
 /* Alternative to the two uploads + assembly: install an already assembled
  * system (natural order, off-diagonal blocks in CSR rowptr/col, 9 per block)

@@ -997,8 +997,9 @@ static int has_cohesion(SOLFEC *sol)
 }
 
 /* the assembly of a step: in the host loop bodies and contacts go to the
- * device first; gs (may be NULL) asks for the learned warm start */
-static int step_assemble(SOLFEC *sol, sfgs_ctx *g, GAUSS_SEIDEL *gs)
+ * device first; gs (may be NULL) asks for the learned warm start; upmin: the
+ * diagonal blocks only (UPMIN, ldy.c:59, 644: NEWTON_SOLVER with locdyn OFF) */
+static int step_assemble(SOLFEC *sol, sfgs_ctx *g, GAUSS_SEIDEL *gs, int upmin)
 {
   double t0 = now_s();
   int rc = 0;
@@ -1008,7 +1009,8 @@ static int step_assemble(SOLFEC *sol, sfgs_ctx *g, GAUSS_SEIDEL *gs)
     if (!rc) rc = SFGS_Contacts_Upload(g, sol->nc, sol->c_master, sol->c_slave, sol->c_mpnt, sol->c_spnt, sol->c_base,
                                        sol->c_gap, sol->c_fri, sol->c_rest, sol->c_coh, sol->c_R, sol->c_U);
   }
-  if (!rc) rc = SFGS_Locdyn_Update_Begin(g, sol->step, sol->dynamic);
+  if (!rc) rc = upmin ? SFGS_Locdyn_Update_Min(g, sol->step, sol->dynamic)
+                     : SFGS_Locdyn_Update_Begin(g, sol->step, sol->dynamic);
   sol->t_locdyn += now_s() - t0;
   sol->locdyn_current = rc == 0;
   if (!rc && sol->capture)
@@ -1110,7 +1112,7 @@ int SOLFEC_Step_Solve(SOLFEC *sol, GAUSS_SEIDEL *gs)
   }
   sfgs_ctx *g = SOLFEC_Gpu(sol);
   if (!g) return SOL_ERR_GPU;
-  int rc = step_assemble(sol, g, gs);
+  int rc = step_assemble(sol, g, gs, 0);
   if (rc) return rc;
   double t1 = now_s();
   int err = GAUSS_SEIDEL_Solve_Device(gs, g, sol->err, sizeof(sol->err));
@@ -1127,7 +1129,7 @@ int SOLFEC_Step_Solve_Newton(SOLFEC *sol, NEWTON *ns)
   if (!sol || !ns) return SOL_ERR_ARG;
   sfgs_ctx *g = SOLFEC_Gpu(sol);
   if (!g) return SOL_ERR_GPU;
-  int rc = step_assemble(sol, g, NULL);
+  int rc = step_assemble(sol, g, NULL, ns->par.bodyspace);
   if (rc) return rc;
   double t1 = now_s();
   rc = NEWTON_Solve_Device(ns, g, sol->err, sizeof(sol->err));

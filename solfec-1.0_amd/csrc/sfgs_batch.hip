@@ -696,6 +696,7 @@ static int batch_solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TU
 {
   if (!ctx->assembled) { ctx->err = "solve before LOCDYN assembly"; return SFGS_E_STATE; }
   if (ctx->diag_only) { ctx->err = "the store holds a penalty solve, not an assembled system"; return SFGS_E_STATE; }
+  if (ctx->upmin) { ctx->err = "the local dynamics hold the diagonal blocks only (SFGS_Locdyn_Update_Min)"; return SFGS_E_STATE; }
   if (tn.lanes_per_row != ctx->G_built)
   {
     ctx->err = "batch shape has " + std::to_string(tn.lanes_per_row) + " lanes per row, the system was assembled with " +

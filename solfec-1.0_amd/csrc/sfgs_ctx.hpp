@@ -100,6 +100,7 @@ struct sfgs_ctx {
   int dynamic = 1;
   int have_bodies = 0, have_system = 0, assembled = 0, from_bodies = 0;
   int diag_only = 0;                     // the store holds a penalty solve (R, U per contact), no system
+  int upmin = 0;                         // diagonal blocks only (SFGS_Locdyn_Update_Min): W products through the bodies
   sfgs::DBuf<double> pen;                // penalty solve: spring, dashpot, hpow per contact, or the default six
   sfgs::DBuf<double> nt_mem;             // Newton solve: linearisation, Krylov vectors, partials
   sfgs::DBuf<unsigned char> bar_mem;     // barrier schedule: colour tables, partials, counter

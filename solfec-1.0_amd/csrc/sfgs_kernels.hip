@@ -1428,7 +1428,11 @@ int SFGS_Contacts_Upload(sfgs_ctx *ctx, int nc, const int *master, const int *sl
   return 0;
 }
 
-int SFGS_Locdyn_Update_Begin(sfgs_ctx *ctx, double step, int dynamic)
+// LOCDYN_Update_Begin from the bodies: full = true is UPALL (diagonal and
+// off-diagonal blocks, adjacency, colouring); full = false is UPMIN (ldy.c:59,
+// 644): the diagonal blocks, B and V only, every row in one colour, no
+// adjacency, for the solvers that form W products through the bodies
+static int locdyn_update(sfgs_ctx *ctx, double step, int dynamic, bool full)
 {
   if (!ctx || !(step > 0.0)) return SFGS_E_ARG;
   if (!ctx->have_bodies) { ctx->err = "no bodies"; return SFGS_E_STATE; }
@@ -1438,12 +1442,13 @@ int SFGS_Locdyn_Update_Begin(sfgs_ctx *ctx, double step, int dynamic)
 
   CHK(build_incidence(ctx));
 
-  // contact adjacency CSR (natural order)
+  // contact adjacency CSR (natural order); empty rows for UPMIN
   CHK(ensure(ctx, ctx->rowlen_n, (size_t)nc + 1));
   CHK(ensure(ctx, ctx->rowptr, (size_t)nc + 1));
   HIPCHK(hipMemsetAsync(ctx->rowlen_n.p, 0, sizeof(int)*((size_t)nc + 1), st));
-  if (nc > 0) hipLaunchKernelGGL(k_rowlen, dim3(nblk(nc)), dim3(TPB), 0, st, nc, ctx->master.p, ctx->slave.p, ctx->kind.p,
-                                 ctx->bptr.p, ctx->rowlen_n.p);
+  if (nc > 0 && full)
+    hipLaunchKernelGGL(k_rowlen, dim3(nblk(nc)), dim3(TPB), 0, st, nc, ctx->master.p, ctx->slave.p, ctx->kind.p,
+                       ctx->bptr.p, ctx->rowlen_n.p);
   CHK(scan_int(ctx, ctx->rowlen_n.p, ctx->rowptr.p, nc + 1));
   int nnz = 0;
   HIPCHK(hipMemcpyAsync(&nnz, ctx->rowptr.p + nc, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1451,8 +1456,9 @@ int SFGS_Locdyn_Update_Begin(sfgs_ctx *ctx, double step, int dynamic)
   ctx->nnz = nnz;
   CHK(ensure(ctx, ctx->col, (size_t)nnz));
   CHK(ensure(ctx, ctx->via, (size_t)nnz));
-  if (nc > 0) hipLaunchKernelGGL(k_adj_fill, dim3(nblk(nc)), dim3(TPB), 0, st, nc, ctx->master.p, ctx->slave.p, ctx->kind.p,
-                                 ctx->bptr.p, ctx->blist.p, ctx->rowptr.p, ctx->col.p, ctx->via.p);
+  if (nc > 0 && nnz > 0)
+    hipLaunchKernelGGL(k_adj_fill, dim3(nblk(nc)), dim3(TPB), 0, st, nc, ctx->master.p, ctx->slave.p, ctx->kind.p,
+                       ctx->bptr.p, ctx->blist.p, ctx->rowptr.p, ctx->col.p, ctx->via.p);
 
   // colouring and colour-major layout
   CHK(colour_graph(ctx, nc));
@@ -1490,9 +1496,20 @@ int SFGS_Locdyn_Update_Begin(sfgs_ctx *ctx, double step, int dynamic)
   ctx->assembled = 1;
   ctx->from_bodies = 1;
   ctx->diag_only = 0;
+  ctx->upmin = full ? 0 : 1;
   ctx->mirror_valid = 0;
   if (singular) { ctx->err = std::to_string(singular) + " singular diagonal blocks"; return SFGS_E_SINGULAR; }
   return 0;
+}
+
+int SFGS_Locdyn_Update_Begin(sfgs_ctx *ctx, double step, int dynamic)
+{
+  return locdyn_update(ctx, step, dynamic, true);
+}
+
+int SFGS_Locdyn_Update_Min(sfgs_ctx *ctx, double step, int dynamic)
+{
+  return locdyn_update(ctx, step, dynamic, false);
 }
 
 int SFGS_System_Upload(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, const double *W,
@@ -1550,6 +1567,7 @@ int SFGS_System_Upload(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, 
   ctx->assembled = 1;
   ctx->from_bodies = 0;
   ctx->diag_only = 0;
+  ctx->upmin = 0;
   ctx->mirror_valid = 0;
   return 0;
 }
@@ -1705,6 +1723,7 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   if (par->precision == SFGS_PREC_MIXED && par->refine < 0) return SFGS_E_ARG;
   if (!ctx->assembled) { ctx->err = "solve before LOCDYN assembly"; return SFGS_E_STATE; }
   if (ctx->diag_only) { ctx->err = "the store holds a penalty solve, not an assembled system"; return SFGS_E_STATE; }
+  if (ctx->upmin) { ctx->err = "the local dynamics hold the diagonal blocks only (SFGS_Locdyn_Update_Min)"; return SFGS_E_STATE; }
   if (ctx->nscenes > 1) { ctx->err = "the system holds independent scenes: use SFGS_Batch_Solve"; return SFGS_E_STATE; }
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
@@ -1925,6 +1944,11 @@ int SFGS_Merit(sfgs_ctx *ctx, int update_U, double *merit, double *conmerit)
   if (!ctx) return SFGS_E_ARG;
   if (!ctx->assembled) { ctx->err = "merit before LOCDYN assembly"; return SFGS_E_STATE; }
   if (ctx->diag_only) { ctx->err = "the store holds a penalty solve, not an assembled system"; return SFGS_E_STATE; }
+  if (ctx->upmin && update_U)
+  {
+    ctx->err = "the local dynamics hold the diagonal blocks only (SFGS_Locdyn_Update_Min)";
+    return SFGS_E_STATE;
+  }
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const int n = ctx->nc, mgrid = (ctx->nslices + TPB/SLICE - 1) / (TPB/SLICE);
@@ -1969,7 +1993,11 @@ static int compute_features(sfgs_ctx *ctx, int natural)
 int SFGS_Features(sfgs_ctx *ctx, float *features)
 {
   if (!ctx || !features) return SFGS_E_ARG;
-  if (!ctx->assembled || !ctx->from_bodies || ctx->diag_only) { ctx->err = "features need a system assembled from bodies"; return SFGS_E_STATE; }
+  if (!ctx->assembled || !ctx->from_bodies || ctx->diag_only || ctx->upmin)
+  {
+    ctx->err = "features need a system assembled from bodies (all blocks)";
+    return SFGS_E_STATE;
+  }
   HIPCHK(hipSetDevice(ctx->device));
   CHK(compute_features(ctx, 1));
   if (ctx->nc > 0)
@@ -1981,7 +2009,11 @@ int SFGS_Features(sfgs_ctx *ctx, float *features)
 int SFGS_Warm_Start(sfgs_ctx *ctx)
 {
   if (!ctx) return SFGS_E_ARG;
-  if (!ctx->assembled || !ctx->from_bodies || ctx->diag_only) { ctx->err = "warm start needs a system assembled from bodies"; return SFGS_E_STATE; }
+  if (!ctx->assembled || !ctx->from_bodies || ctx->diag_only || ctx->upmin)
+  {
+    ctx->err = "warm start needs a system assembled from bodies (all blocks)";
+    return SFGS_E_STATE;
+  }
   HIPCHK(hipSetDevice(ctx->device));
   CHK(compute_features(ctx, 0));
   return sfgs_mlp_apply(ctx, ctx->feat.p, ctx->nc, ctx->R.p);
@@ -2319,6 +2351,7 @@ int SFGS_Penalty_Solve(sfgs_ctx *ctx, double step, int implicit, const double *s
   ctx->assembled = 1;
   ctx->from_bodies = 1;
   ctx->diag_only = 1;
+  ctx->upmin = 0;
   ctx->nnz = 0;
   ctx->ncolours = 0;
   ctx->mirror_valid = 0;

@@ -529,6 +529,11 @@ int SFGS_Newton_Solve(sfgs_ctx *ctx, const SFGS_NEWTON_PARAM *par, double *merhi
     ctx->err = "locdyn OFF needs a system assembled from bodies";
     return SFGS_E_STATE;
   }
+  if (!par->bodyspace && ctx->upmin)
+  {
+    ctx->err = "locdyn ON needs the off-diagonal blocks (SFGS_Locdyn_Update_Begin)";
+    return SFGS_E_STATE;
+  }
   // U = W R + B (U_WR_B, nts.c:255-431), then the merit (nts.c:929, 1199); in
   // body space W R is formed through the bodies
   auto merit_of = [&](double &m) -> int {

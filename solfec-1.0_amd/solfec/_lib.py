@@ -60,6 +60,7 @@ _SFGS_PROTOS = {
     "SFGS_Bodies_Upload": (_i, [_p, _i, _p, _p, _p, _p, _p, _p, _p]),
     "SFGS_Contacts_Upload": (_i, [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "SFGS_Locdyn_Update_Begin": (_i, [_p, _d, _i]),
+    "SFGS_Locdyn_Update_Min": (_i, [_p, _d, _i]),
     "SFGS_System_Upload": (_i, [_p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _d, _d, _i, _p, _p]),
     "SFGS_Solve": (_i, [_p, C.POINTER(SFGS_PARAM), _p, _p, C.POINTER(SFGS_REPORT)]),
     "SFGS_Merit": (_i, [_p, _i, _p, _p]),

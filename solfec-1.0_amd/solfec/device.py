@@ -106,8 +106,12 @@ class Device:
              f64(friction, n), f64(restitution, n), f64(cohesion, n), f64(R, 3 * n), f64(U, 3 * n)]
         self._check(self.lib.SFGS_Contacts_Upload(self.ctx, n, *[ptr(x) for x in a]), "SFGS_Contacts_Upload")
 
-    def assemble(self, step, dynamic=1):
-        self._check(self.lib.SFGS_Locdyn_Update_Begin(self.ctx, step, dynamic), "SFGS_Locdyn_Update_Begin")
+    def assemble(self, step, dynamic=1, diagonal_only=False):
+        """SFGS_Locdyn_Update_Begin, or SFGS_Locdyn_Update_Min (UPMIN: diagonal blocks only)."""
+        if diagonal_only:
+            self._check(self.lib.SFGS_Locdyn_Update_Min(self.ctx, step, dynamic), "SFGS_Locdyn_Update_Min")
+        else:
+            self._check(self.lib.SFGS_Locdyn_Update_Begin(self.ctx, step, dynamic), "SFGS_Locdyn_Update_Begin")
 
     def newton(self, param):
         """SFGS_Newton_Solve: returns the report, merhist and mvhist."""

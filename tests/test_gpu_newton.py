@@ -266,3 +266,50 @@ def test_bodyspace_product_matches_oracle_W(dev, scene):
         err = np.abs(z - zo).max() / np.abs(zo).max()
         print(f"[newton] {n} contacts, {O['nnz']} blocks: max |z - W x| / max |W x| {err:.2e}", flush=True)
         assert err < 1e-12
+
+
+@pytest.mark.parametrize("linver", ["GMRES", "DIAG"])
+def test_upmin_assembly_and_body_space_newton(dev, linver):
+    """SFGS_Locdyn_Update_Min (UPMIN, ldy.c:59, 644), the update the reference
+    runs for locdyn OFF: the diagonal blocks, A, rho, B, V and the free energy
+    equal the full assembly's bit for bit (the same per-row kernel), no
+    off-diagonal blocks or adjacency are formed, the solves that need them
+    refuse, and three body-space Newton steps on it give what they give on the
+    full assembly: the same reactions bit for bit with DIAG steps (per-row
+    updates), to rounding with GMRES (the dot products run in another row
+    order)."""
+    print(f"[newton] UPMIN assembly, body-space {linver}", flush=True)
+    s = lattice(343, seed=5)
+    b, c = s.bodies(), s.contacts()
+
+    def run(diagonal_only):
+        dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
+        dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
+                     c["restitution"], c["cohesion"], c["R"], c["U"])
+        dev.assemble(H, diagonal_only=diagonal_only)
+        L = dev.locdyn()
+        sizes = dev.sizes()
+        if diagonal_only:
+            with pytest.raises(solfec.device.SfgsError):
+                dev.solve(solfec.make_param(1e-6, 10))
+            with pytest.raises(solfec.device.SfgsError):
+                dev.newton(solfec.newton_param(meritval=1e-14, maxiter=3, linver=linver, gsflag="OFF"))
+        p = solfec.newton_param(meritval=1e-14, maxiter=3, linver=linver, gsflag="OFF")
+        p.bodyspace = 1
+        rep, mer, _ = dev.newton(p)
+        R, U = dev.get_RU()
+        return L, sizes, rep, np.asarray(mer), R
+    Lf, sf, rf, mf, Rf = run(False)
+    Lm, sm, rm, mm, Rm = run(True)
+    print(f"[newton] full: {sf[1]} blocks, {sf[2]} colours; UPMIN: {sm[1]} blocks, {sm[2]} colours; merits "
+          f"{mf[-1]:.6e} / {mm[-1]:.6e}, max |dR| {np.abs(Rf - Rm).max():.2e}", flush=True)
+    assert sm[1] == 0 and sm[2] == 1 and sf[1] > 0
+    for k in ("Wd", "A", "B", "V", "rho"):
+        assert np.array_equal(Lf[k], Lm[k]), k
+    assert Lf["free_energy"] == Lm["free_energy"]
+    assert rf.iters == rm.iters == 3
+    if linver == "DIAG":
+        assert np.array_equal(Rf, Rm)
+    else:
+        assert np.abs(Rf - Rm).max() <= 1e-10 * np.abs(Rf).max()
+        np.testing.assert_allclose(mm, mf, rtol=1e-8, atol=0.0)

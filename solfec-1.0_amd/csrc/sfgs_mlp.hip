@@ -33,10 +33,33 @@ static const int BM = 64, BN = 64, BK = 32, LPAD = 8;
 
 enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS_F32 = 2, EPI_GELU_BWD = 3, EPI_BF16 = 4 };
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f*x*(1.0f + erff(x*0.70710678118654752f)); }
+// GELU with erf from Abramowitz & Stegun, Handbook of Mathematical Functions,
+// eq. 7.1.26: erf(u) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-u^2),
+// t = 1 / (1 + p u), u >= 0, |error| <= 1.5e-7 (odd extension for u < 0).  The
+// activation's absolute error, 0.75e-7 |x| at most, is far below the bf16
+// resolution of the activations, and every kernel here uses this one form, so
+// the fused inference and the layer-by-layer forward pass agree bit for bit.
+// The exponential also gives the derivative's Gaussian: exp(-u^2) = exp(-x^2/2).
+__device__ __forceinline__ void erf_as(float x, float &erf_u, float &gauss)
+{
+  const float u = fabsf(x)*0.70710678118654752f;
+  const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f*u);   // hardware reciprocal (1 ulp)
+  const float poly = t*(0.254829592f + t*(-0.284496736f + t*(1.421413741f + t*(-1.453152027f + t*1.061405429f))));
+  gauss = __expf(-u*u);
+  const float e = 1.0f - poly*gauss;
+  erf_u = x < 0.0f ? -e : e;
+}
+__device__ __forceinline__ float gelu(float x)
+{
+  float e, g;
+  erf_as(x, e, g);
+  return 0.5f*x*(1.0f + e);
+}
 __device__ __forceinline__ float dgelu(float x)
 {
-  return 0.5f*(1.0f + erff(x*0.70710678118654752f)) + x*0.39894228040143268f*expf(-0.5f*x*x);
+  float e, g;
+  erf_as(x, e, g);
+  return 0.5f*(1.0f + e) + x*0.39894228040143268f*g;
 }
 
 struct GemmArgs {
@@ -273,21 +296,153 @@ __global__ void k_gather(int B, const int *idx, const float *tX, const float *tY
   for (int c = 0; c < NO; c ++) Y[(size_t)i*NO + c] = tY[(size_t)s*NO + c] / norm[2*NF + c];
 }
 
-// R0 = R_prev + scale * f(features), rows in colour-major order (R has 4
-// doubles per row); new_only leaves rows that carry a reaction unchanged
-__global__ void k_apply(int n, const float *O, const float *norm, double *R, int new_only, float scale)
+// ---- fused inference of the warm start ----
+//
+// The whole network for tiles of 64 rows in one persistent launch: normalised
+// features into LDS as bf16, three hidden layers through LDS (ping-pong
+// buffers of 64 x 128 bf16), the output layer and the reaction shift, without
+// any activation in HBM.  Wave w owns columns 32w .. 32w + 31 of every hidden
+// layer and keeps its B fragments of W1, W2, W3 (and the output layer's first
+// 16 columns) in registers for the whole launch, loaded once per workgroup.
+// Every output element is formed as k_gemm forms it (the same 16x16x32 bf16
+// products in the same k order, the same lane placement, bias then GELU), so
+// the result is that of the layer-by-layer forward pass.
+struct InferArgs {
+  int n;
+  const float *feat;       // n x 18, rows in the order of R
+  const float *norm;       // feature mean (18), std (18), output scale (3)
+  const bf16 *W1, *W2, *W3, *W4;   // [in][out]: 32 x 128, 128 x 128, 128 x 128, 128 x 32
+  const float *b1, *b2, *b3, *b4;
+  double *R;               // 4 per row
+  int new_only;
+  float scale;
+};
+
+static const int IT_M = 64;   // rows per tile
+
+// B fragment of a [K][ldw] weight matrix: column n, k = k0 .. k0 + 7
+__device__ __forceinline__ bf16x8 wfrag(const bf16 *W, int ldw, int n, int k0)
 {
-  int p = blockIdx.x*blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  double *r = R + 4*(size_t)p;
-  if (new_only && (r[0] != 0.0 || r[1] != 0.0 || r[2] != 0.0)) return;
-  for (int c = 0; c < NO; c ++) r[c] += (double)(scale * O[(size_t)p*NOP + c] * norm[2*NF + c]);
+  bf16x8 f;
+#pragma unroll
+  for (int t = 0; t < 8; t ++) f[t] = W[(size_t)(k0 + t)*ldw + n];
+  return f;
 }
 
-__global__ void k_iota(int n, int *idx)
+// one hidden layer of the tile: Out = bf16(GELU(In Wt + b)) for this wave's
+// 32 columns, K = 128 in four chunks of 32 (the k order of k_gemm)
+__device__ __forceinline__ void infer_hidden(const bf16 (*In)[NH + LPAD], bf16 (*Out)[NH + LPAD], const bf16x8 (&w)[2][4],
+                                             const float (&b)[2], int wave, int lane)
 {
-  int i = blockIdx.x*blockDim.x + threadIdx.x;
-  if (i < n) idx[i] = i;
+  const int fr = lane & 15, fk = 8*(lane >> 4);
+  floatx4 acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; i ++)
+#pragma unroll
+    for (int j = 0; j < 2; j ++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int c = 0; c < 4; c ++)
+#pragma unroll
+    for (int i = 0; i < 4; i ++)
+    {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&In[16*i + fr][32*c + fk]);
+#pragma unroll
+      for (int j = 0; j < 2; j ++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w[j][c], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+  for (int i = 0; i < 4; i ++)
+#pragma unroll
+    for (int j = 0; j < 2; j ++)
+#pragma unroll
+      for (int q = 0; q < 4; q ++)
+        Out[16*i + 4*(lane >> 4) + q][32*wave + 16*j + fr] = (bf16)gelu(acc[i][j][q] + b[j]);
+}
+
+__global__ __launch_bounds__(256) void k_mlp_infer(InferArgs a)
+{
+  __shared__ __align__(16) bf16 Xs[IT_M][NI + LPAD];
+  __shared__ __align__(16) bf16 Ha[IT_M][NH + LPAD];
+  __shared__ __align__(16) bf16 Hb[IT_M][NH + LPAD];
+  __shared__ float Os[IT_M][NO];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, fr = lane & 15, fk = 8*(lane >> 4);
+  bf16x8 w1[2], w2[2][4], w3[2][4], w4[4];
+  float b1[2], b2[2], b3[2];
+#pragma unroll
+  for (int j = 0; j < 2; j ++)
+  {
+    const int n = 32*wave + 16*j + fr;
+    w1[j] = wfrag(a.W1, NH, n, fk);
+#pragma unroll
+    for (int c = 0; c < 4; c ++) { w2[j][c] = wfrag(a.W2, NH, n, 32*c + fk); w3[j][c] = wfrag(a.W3, NH, n, 32*c + fk); }
+    b1[j] = a.b1[n]; b2[j] = a.b2[n]; b3[j] = a.b3[n];
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c ++) w4[c] = wfrag(a.W4, NOP, fr, 32*c + fk);
+  const float b4 = a.b4[fr];
+
+  for (int row0 = blockIdx.x*IT_M; row0 < a.n; row0 += gridDim.x*IT_M)
+  {
+    // normalised features (k_gather): thread t takes row t/4, features 8(t%4) .. +7
+    {
+      const int r = tid >> 2, f0 = 8*(tid & 3);
+#pragma unroll
+      for (int t = 0; t < 8; t ++)
+      {
+        const int f = f0 + t;
+        float v = 0.f;
+        if (row0 + r < a.n && f < NF) v = (a.feat[(size_t)(row0 + r)*NF + f] - a.norm[f]) / a.norm[NF + f];
+        Xs[r][f] = (bf16)v;
+      }
+    }
+    __syncthreads();
+    // layer 1 (K = 32: one chunk)
+    {
+      floatx4 acc[4][2];
+#pragma unroll
+      for (int i = 0; i < 4; i ++)
+      {
+        const bf16x8 x = *reinterpret_cast<const bf16x8*>(&Xs[16*i + fr][fk]);
+#pragma unroll
+        for (int j = 0; j < 2; j ++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, w1[j], floatx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i ++)
+#pragma unroll
+        for (int j = 0; j < 2; j ++)
+#pragma unroll
+          for (int q = 0; q < 4; q ++)
+            Ha[16*i + 4*(lane >> 4) + q][32*wave + 16*j + fr] = (bf16)gelu(acc[i][j][q] + b1[j]);
+    }
+    __syncthreads();
+    infer_hidden(Ha, Hb, w2, b2, wave, lane);
+    __syncthreads();
+    infer_hidden(Hb, Ha, w3, b3, wave, lane);
+    __syncthreads();
+    // output layer: wave w takes rows 16w .. 16w + 15, the first 16 output columns
+    {
+      floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < 4; c ++)
+      {
+        const bf16x8 h = *reinterpret_cast<const bf16x8*>(&Ha[16*wave + fr][32*c + fk]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h, w4[c], acc, 0, 0, 0);
+      }
+      if (fr < NO)
+#pragma unroll
+        for (int q = 0; q < 4; q ++) Os[16*wave + 4*(lane >> 4) + q][fr] = acc[q] + b4;
+    }
+    __syncthreads();
+    // the shift of R: R0 = R_prev + scale * out * output scale; with new_only,
+    // rows that carry a reaction keep it
+    if (tid < IT_M && row0 + tid < a.n)
+    {
+      double *r = a.R + 4*(size_t)(row0 + tid);
+      if (!(a.new_only && (r[0] != 0.0 || r[1] != 0.0 || r[2] != 0.0)))
+        for (int c = 0; c < NO; c ++) r[c] += (double)(a.scale * Os[tid][c] * a.norm[2*NF + c]);
+    }
+    __syncthreads();   // Xs, Ha and Os are rewritten by the next tile
+  }
 }
 
 } // namespace sfgs_mlp
@@ -435,30 +590,20 @@ static int refresh_bf16(sfgs_ctx *ctx, Model *m)
   return 0;
 }
 
-// apply the model to device features (n x 18, row order of R) and shift R by the prediction
+// apply the model to device features (n x 18, row order of R) and shift R by
+// the prediction: one fused launch (k_mlp_infer), no workspace
 int sfgs_mlp_apply(sfgs_ctx *ctx, const float *feat, int n, double *R)
 {
   Model *m = model_of(ctx);
   if (!m->have) { ctx->err = "no MLP parameters"; return SFGS_E_STATE; }
   if (n == 0) return 0;
-  if (ensure_batch(ctx, m, n)) return SFGS_E_HIP;
-  if (n > m->tcap)
-  {
-    if (m->idx) (void)hipFree(m->idx);
-    m->idx = nullptr;
-    if (m->tX) (void)hipFree(m->tX);
-    if (m->tY) (void)hipFree(m->tY);
-    m->tX = m->tY = nullptr;
-    HIPCHK(hipMalloc(&m->tX, sizeof(float)*NF*(size_t)n));
-    HIPCHK(hipMalloc(&m->tY, sizeof(float)*NO*(size_t)n));
-    HIPCHK(hipMalloc(&m->idx, sizeof(int)*(size_t)n));
-    m->tcap = n;
-  }
-  const float *dn = m->dnorm;
-  hipLaunchKernelGGL(k_iota, dim3((n + 255)/256), dim3(256), 0, ctx->stream, n, m->idx);
-  hipLaunchKernelGGL(k_gather, dim3((n + 255)/256), dim3(256), 0, ctx->stream, n, m->idx, feat, feat, dn, m->X, m->Y);
-  forward(ctx, m, n);
-  hipLaunchKernelGGL(k_apply, dim3((n + 255)/256), dim3(256), 0, ctx->stream, n, m->O, dn, R, m->new_only, m->scale);
+  InferArgs a;
+  a.n = n; a.feat = feat; a.norm = m->dnorm;
+  a.W1 = m->wbf + m->off[0]; a.W2 = m->wbf + m->off[2]; a.W3 = m->wbf + m->off[4]; a.W4 = m->wbf + m->off[6];
+  a.b1 = m->param + m->off[1]; a.b2 = m->param + m->off[3]; a.b3 = m->param + m->off[5]; a.b4 = m->param + m->off[7];
+  a.R = R; a.new_only = m->new_only; a.scale = m->scale;
+  const int tiles = (n + IT_M - 1)/IT_M;
+  hipLaunchKernelGGL(k_mlp_infer, dim3(std::min(tiles, 512)), dim3(256), 0, ctx->stream, a);   // two per CU resident
   HIPCHK(hipGetLastError());
   return 0;
 }

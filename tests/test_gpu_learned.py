@@ -53,3 +53,44 @@ def test_shipped_model_against_previous_reaction(dev):
     print(f"[learned] held-out total: previous-step R {base}, network {lrn}, gain x{base / max(lrn, 1):.3f} "
           f"(variant new_only={v[0]} trust={v[1]}, trained on {meta.get('samples', '?')} samples)", flush=True)
     assert lrn <= 1.05 * base
+
+
+@pytest.mark.parametrize("new_only,scale", [(0, 1.0), (1, 0.5)])
+def test_fused_warm_start_equals_layer_by_layer_forward(dev, new_only, scale):
+    """The warm start runs the whole network in one fused launch (k_mlp_infer);
+    SFGS_Mlp_Forward runs it layer by layer through k_gemm.  Both form every
+    product with the same bf16 MFMA operations in the same order, so the shift
+    of R is the forward pass's output times the trust factor, up to the rounding
+    of the last float product (the forward pass returns out times the output
+    scale, the warm start multiplies scale, out and the output scale in that
+    order).  With new_only, rows that carry a reaction keep it exactly."""
+    from scenes import lattice
+    params, norm = learned.load()
+    learned.install(dev, params=params, norm=norm)
+    s = lattice(1000, seed=2)
+    b, c = s.bodies(), s.contacts()
+    n = len(c["master"])
+    R = np.random.default_rng(4).normal(scale=0.1, size=(n, 3))
+    R[::3] = 0.0                             # a third of the rows start without a reaction
+    dev.bodies(b["kind"], b["rot"], b["x0"], b["jinv"], b["minv"], b["velo"], b["velo0"])
+    dev.contacts(c["master"], c["slave"], c["mpnt"], c["spnt"], c["base"], c["gap"], c["friction"],
+                 c["restitution"], c["cohesion"], R.ravel(), c["U"])
+    dev.assemble(1e-3)
+    F = dev.features()
+    out = dev.mlp_forward(F).reshape(n, 3).astype(np.float64)
+    R0, _ = dev.get_RU()
+    dev.warm_start_mode(new_only, scale)
+    dev.warm_start()
+    R1, _ = dev.get_RU()
+    shift = R1 - R0
+    want = scale * out
+    carried = np.any(R0 != 0.0, axis=1)
+    if new_only:
+        want[carried] = 0.0
+        assert np.array_equal(R1[carried], R0[carried])
+    err = np.abs(shift - want).max() / np.abs(want).max()
+    print(f"[learned] fused warm start, new_only={new_only} scale={scale}: {n} rows, "
+          f"{int((~carried).sum())} without a reaction, max |shift - scale out| / max {err:.2e}", flush=True)
+    assert np.abs(want).max() > 0
+    assert err < 1e-6
+    dev.warm_start_mode(1, 0.5)

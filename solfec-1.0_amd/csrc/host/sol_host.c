@@ -1047,9 +1047,16 @@ static int cohesion_keep_spring_dashpot(SOLFEC *sol, sfgs_ctx *g)
   else
   {
     /* the contact list is the device's in the device loop, the host's otherwise */
-    if (sol->devloop) rc = SFGS_Dom_Get_Contacts(g, ms, ms + n, NULL, NULL, NULL, NULL, NULL, NULL, NULL, before,
-                                                 NULL, NULL);
-    else
+    int ndev = -1;
+    rc = SFGS_Get_Sizes(g, &ndev, NULL, NULL);
+    if (!rc && ndev != (int)n)
+    {
+      snprintf(sol->err, sizeof(sol->err), "device holds %d contacts, the domain %zu", ndev, n);
+      rc = SFGS_E_STATE;
+    }
+    if (!rc && sol->devloop) rc = SFGS_Dom_Get_Contacts(g, ms, ms + n, NULL, NULL, NULL, NULL, NULL, NULL, NULL,
+                                                        before, NULL, NULL);
+    else if (!rc)
     {
       memcpy(ms, sol->c_master, n*sizeof(int));
       memcpy(ms + n, sol->c_slave, n*sizeof(int));

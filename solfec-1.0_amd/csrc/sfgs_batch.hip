@@ -13,18 +13,19 @@
 // Every workgroup of a scene forms the same sums in the same order, so all of
 // them reach the same termination decision without a broadcast.
 //
-// Per scene the kernel runs what SFGS_Solve runs for one system: the free
-// energy, Gauss-Seidel sweeps with the fused merit of the previous iterate and
-// the termination test of bgs.c:1016-1036 (as k_finish), then for MIXED the
-// conversion to FP64, the refinement sweeps and the final FP64 merit with the
-// update of U (mrf.c:61-76).  The sweep itself is sweep_slice, shared with the
-// per-colour kernels.
+// Per scene the kernel runs what SFGS_Solve runs for one system in FP64: the
+// free energy, Gauss-Seidel sweeps with the fused merit of the previous iterate
+// and the termination test of bgs.c:1016-1036 (as k_finish).  MIXED solves run
+// the per-colour schedule below, which has the two phases of the single-system
+// MIXED solve.  The sweep itself is sweep_slice, shared with the per-colour
+// kernels.
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cfloat>
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <type_traits>
 #include "sfgs_ctx.hpp"
 #include "sfgs_math.hpp"
 #include "sfgs_sweep.hpp"
@@ -42,7 +43,7 @@ struct BatchScene {
 struct BatchOut {
   int iters, nref, sweeps, done, aborted, failcount, timeout;
   int final_buf, final_U;   // per-colour schedule: buffer of the final R (0, 1), final U (0 U, 1 Um)
-  int pad;
+  int iters0;               // per-colour MIXED: iterations of the FP32 phase
   unsigned long long failkey;
   double error, merit, energy, prev_error;
 };
@@ -60,7 +61,6 @@ template <typename T>
 struct BatchArgs {
   SweepArgs<T> a;          // sweep data of the first phase (P, Q set per sweep)
   SweepArgs<double> d;     // FP64 sweep data (energy, refinement)
-  MeritArgs m;             // FP64 merit (R set at the end)
   const BatchScene *scenes;
   const int *wg_scene;     // scene of every workgroup
   BatchGeom geo;
@@ -71,7 +71,7 @@ struct BatchArgs {
   BatchOut *out;
   double *wpart;           // 2 parities x 4 doubles per workgroup
   unsigned *bar;           // barrier counter per scene
-  int maxiter, nomerit, reverse, refine, mixed;
+  int maxiter, nomerit, reverse;
   double eps, meritval;
 };
 
@@ -261,88 +261,28 @@ __global__ __launch_bounds__(NW*64) void k_batch(BatchArgs<T> b)
   }
   if (!done) { final_buf = maxsweeps & 1; final_U = 0; }
 
-  int nref = 0;
   if (alive)
   {
+    // the final state into the canonical buffers (own rows only)
     const T *Rf = final_buf ? b.X1 : b.X0;
     const T *Uf = final_U ? b.Umt : b.Ut;
     const int n = b.a.n;
-    if (b.mixed)
+    for (int s = sl_lo + w*NW + wave; s < sl_hi; s += nw*NW)
     {
-      // first-phase state into the FP64 store
-      for (int s = sl_lo + w*NW + wave; s < sl_hi; s += nw*NW)
-      {
-        const int p = slice_row<G>(b.d.row0, b.d.nrows, s);
-        if (p < 0) continue;
+      const int p = slice_row<G>(b.d.row0, b.d.nrows, s);
+      if (p < 0) continue;
+      if (final_buf)
 #pragma unroll
-        for (int k = 0; k < 3; k ++)
-        {
-          b.R64[4*(size_t)p + k] = (double)Rf[4*(size_t)p + k];
-          b.U64[(size_t)k*n + p] = (double)Uf[(size_t)k*n + p];
-        }
-        b.R64[4*(size_t)p + 3] = 0.0;
-      }
-      alive = scene_barrier(bar, nw, b.geo.abort_all, b.geo.spin_limit, &flag);
-      // FP64 refinement sweeps (no merit, no early stop)
-      for (int k = 0; alive && !aborted && k < b.refine; k ++)
-      {
-        SweepArgs<double> a = b.d;
-        a.P = (k & 1) ? b.R64b : b.R64;
-        a.Q = (k & 1) ? b.R64 : b.R64b;
-        if (threadIdx.x == 0) sweepno = iters + k;
-        __syncthreads();
-        double eu = 0.0, el = 0.0, up = 0.0;
-        alive = batch_sweep<double, G, NW, false>(b.geo, a, sc, g0, w, nw, 0, &sweepno, o, bar, &flag, eu, el, up);
-        if (!alive) break;
-        nsw ++;
-        alive = scene_sum(eu, el, up);
-        if (!alive) break;
-        error = sqrt(eu) / sqrt(el == 0.0 ? 1.0 : el);
-        nref = k + 1;
-      }
-      const double *Rfin = (nref & 1) ? b.R64b : b.R64;
-      if (alive && !b.nomerit)
-      {
-        // final FP64 merit with the update of U (mrf.c:61-76)
-        MeritArgs m = b.m;
-        m.R = Rfin;
-        double up = 0.0, z1 = 0.0, z2 = 0.0;
-        for (int s = sl_lo + w*NW + wave; s < sl_hi; s += nw*NW) merit_slice<G>(m, s, true, up);
-        alive = scene_sum(up, z1, z2);
-        merit = 0.5*up / lo_energy;
-      }
-      if (alive && (nref & 1))
-      {
-        // R64b -> R64 for own rows (the merit pass reads R64b only)
-        for (int s = sl_lo + w*NW + wave; alive && s < sl_hi; s += nw*NW)
-        {
-          const int p = slice_row<G>(b.d.row0, b.d.nrows, s);
-          if (p < 0) continue;
+        for (int k = 0; k < 4; k ++) b.R64[4*(size_t)p + k] = (double)Rf[4*(size_t)p + k];
+      if (final_U)
 #pragma unroll
-          for (int k = 0; k < 4; k ++) b.R64[4*(size_t)p + k] = b.R64b[4*(size_t)p + k];
-        }
-      }
-    }
-    else
-    {
-      // FP64 sweeps: the final state into the canonical buffers (own rows only)
-      for (int s = sl_lo + w*NW + wave; s < sl_hi; s += nw*NW)
-      {
-        const int p = slice_row<G>(b.d.row0, b.d.nrows, s);
-        if (p < 0) continue;
-        if (final_buf)
-#pragma unroll
-          for (int k = 0; k < 4; k ++) b.R64[4*(size_t)p + k] = (double)Rf[4*(size_t)p + k];
-        if (final_U)
-#pragma unroll
-          for (int k = 0; k < 3; k ++) b.U64[(size_t)k*n + p] = (double)Uf[(size_t)k*n + p];
-      }
+        for (int k = 0; k < 3; k ++) b.U64[(size_t)k*n + p] = (double)Uf[(size_t)k*n + p];
     }
   }
   if (w == 0 && threadIdx.x == 0)
   {
-    o->iters = iters + nref;
-    o->nref = nref;
+    o->iters = iters;
+    o->nref = 0;
     o->sweeps = nsw;
     o->done = done;
     o->error = error;
@@ -399,9 +339,11 @@ struct UScene {
   int slice0, slice1;     // the scene's slices (all colours)
 };
 
+// sbound: per slice, the rows already updated in this sweep (forward: the
+// scene's earlier colours; backward: its later ones)
 template <typename T, int G, bool MERIT>
 __global__ __launch_bounds__(TPB) void k_usweep(SweepArgs<T> a, const int *clist, int nlist, const int2 *sbound,
-                                                const int *sscene, BatchOut *out, double *spart, int sweepno_base)
+                                                const int *sscene, BatchOut *out, double *spart, int backward)
 {
   const int idx = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6);
   if (idx >= nlist) return;                       // whole wavefront
@@ -410,8 +352,9 @@ __global__ __launch_bounds__(TPB) void k_usweep(SweepArgs<T> a, const int *clist
   if (o->done) return;                            // a finished scene: whole wavefront
   const int2 bd = sbound[s];
   double eu = 0.0, el = 0.0, up = 0.0;
-  int sweepno = sweepno_base + o->sweeps;
-  sweep_slice<T, G, MERIT>(a, s, true, bd.x, bd.y, 0, &sweepno, &o->failkey, &o->failcount, &o->aborted, eu, el, up);
+  int sweepno = o->iters0 + o->sweeps;
+  sweep_slice<T, G, MERIT>(a, s, true, bd.x, bd.y, backward, &sweepno, &o->failkey, &o->failcount, &o->aborted,
+                           eu, el, up);
   eu = wave_sum(eu); el = wave_sum(el); up = wave_sum(up);
   if ((threadIdx.x & 63) == 0)
   {
@@ -430,19 +373,6 @@ __global__ __launch_bounds__(TPB) void k_uenergy(SweepArgs<double> d, int nslice
   if ((threadIdx.x & 63) == 0) spart[3*(size_t)s] = e;
 }
 
-// per-slice merit terms of the final FP64 state (with the update of U)
-template <int G>
-__global__ __launch_bounds__(TPB) void k_umerit(MeritArgs m, int nslices, const int *sscene, const BatchOut *out,
-                                                double *spart)
-{
-  const int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6);
-  if (s >= nslices) return;
-  double up = 0.0;
-  merit_slice<G>(m, s, true, up);
-  up = wave_sum(up);
-  if ((threadIdx.x & 63) == 0) spart[3*(size_t)s + 2] = up;
-}
-
 // deterministic sum of three partials over a scene's slices (one block per scene)
 __device__ __forceinline__ void scene_partials(const double *spart, int s0, int s1, double &a, double &b, double &c)
 {
@@ -456,45 +386,77 @@ __device__ __forceinline__ void scene_partials(const double *spart, int s0, int 
 
 // mode 0: free energy of each scene (before the sweeps)
 // mode 1: termination test after a sweep (bgs.c:1016-1036, as k_finish) per running scene;
-//         running scenes are counted into active[sweep]
-// mode 2: error of a refinement sweep; mode 3: the final merit
+//         running scenes are counted into active[sweep].  The phase's iteration bound is
+//         maxiter - iters0 (iters0: iterations of an earlier phase, reported on top), and
+//         no stop on convergence before miniter iterations of the phase.
 __global__ __launch_bounds__(TPB) void k_uscene(int mode, const UScene *sc, const double *spart, BatchOut *out,
-                                                int maxiter, int nomerit, double eps, double meritval, int sweep,
-                                                int *active)
+                                                int maxiter, int miniter, int nomerit, double eps, double meritval,
+                                                int sweep, int *active)
 {
   BatchOut *o = out + blockIdx.x;
   const UScene u = sc[blockIdx.x];
   if (u.slice1 == u.slice0) { if (threadIdx.x == 0 && mode == 0) o->done = 1; return; }
   if (mode == 1 && o->done) return;
-  if (mode == 2 && o->aborted) return;
   double eu, el, up;
   scene_partials(spart, u.slice0, u.slice1, eu, el, up);
   if (threadIdx.x != 0) return;
   if (mode == 0) { o->energy = 0.5*eu; return; }
-  if (mode == 3) { const double lo = o->energy < DBL_EPSILON ? 1.0 : o->energy; o->merit = 0.5*up/lo; return; }
   const double error_k = sqrt(eu) / sqrt(el == 0.0 ? 1.0 : el);
-  if (mode == 2) { o->error = error_k; o->nref ++; o->sweeps ++; o->iters ++; return; }
-  const int k = o->sweeps;
+  const int k = o->sweeps, base = o->iters0, bound = maxiter - base;
   o->sweeps = k + 1;
   const int aborted = o->aborted != 0;
   int done = 0;
   if (nomerit)
   {
-    o->iters = k + 1; o->error = error_k; o->merit = 0.0; o->final_buf = (k + 1) & 1; o->final_U = 0;
-    done = aborted || !(k + 1 < maxiter && error_k > eps);
+    o->iters = base + k + 1; o->error = error_k; o->merit = 0.0; o->final_buf = (k + 1) & 1; o->final_U = 0;
+    done = aborted || !(k + 1 < bound && (error_k > eps || k + 1 < miniter));
   }
-  else if (aborted) { o->iters = k + 1; o->final_buf = (k + 1) & 1; o->final_U = 0; done = 1; }
+  else if (aborted) { o->iters = base + k + 1; o->final_buf = (k + 1) & 1; o->final_U = 0; done = 1; }
   else if (k == 0) o->prev_error = error_k;
   else
   {
     const double lo = o->energy < DBL_EPSILON ? 1.0 : o->energy;
     const double merit = 0.5*up/lo, error = o->prev_error;
-    o->iters = k; o->error = error; o->merit = merit;
-    if (!(k < maxiter && (error > eps || merit > meritval))) { o->final_buf = k & 1; o->final_U = 1; done = 1; }
+    o->iters = base + k; o->error = error; o->merit = merit;
+    if (!(k < bound && (error > eps || merit > meritval || k < miniter))) { o->final_buf = k & 1; o->final_U = 1; done = 1; }
     else o->prev_error = error_k;
   }
   if (done) o->done = 1;
   else atomicAdd(active + sweep, 1);
+}
+
+// start of the FP64 phase of MIXED: a scene that did not abort and has
+// iterations left runs again from the state k_ufinal wrote into R64 / U64
+__global__ void k_uphase(int nscenes, const UScene *sc, int maxiter, BatchOut *out)
+{
+  const int q = blockIdx.x*blockDim.x + threadIdx.x;
+  if (q >= nscenes) return;
+  BatchOut *o = out + q;
+  o->iters0 = o->iters;
+  o->sweeps = 0;
+  o->prev_error = 0.0;
+  o->final_buf = 0;
+  o->final_U = 0;
+  o->done = (sc[q].slice1 == sc[q].slice0 || o->aborted || o->iters >= maxiter) ? 1 : 0;
+}
+
+// the FP64 phase's final state into the canonical buffers, per scene's final buffers
+template <int G>
+__global__ __launch_bounds__(TPB) void k_ufinal64(int nslices, const int *row0, const int *nrows, const int *sscene,
+                                                  const BatchOut *out, const double *Rb, const double *Um, double *R,
+                                                  double *U, int n)
+{
+  const int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6);
+  if (s >= nslices) return;
+  const int p = slice_row<G>(row0, nrows, s);
+  if (p < 0) return;
+  const BatchOut &o = out[sscene[s]];
+  if (o.final_buf)
+#pragma unroll
+    for (int k = 0; k < 4; k ++) R[4*(size_t)p + k] = Rb[4*(size_t)p + k];
+  if (o.final_U)
+#pragma unroll
+    for (int k = 0; k < 3; k ++) U[(size_t)k*n + p] = Um[(size_t)k*n + p];
 }
 
 // final state of the first phase into the FP64 store, per scene's final buffers
@@ -523,26 +485,6 @@ __global__ __launch_bounds__(TPB) void k_ufinal(int nslices, const int *row0, co
   }
 }
 
-// before the refinement sweeps: every scene that did not abort runs again
-__global__ void k_ureset(int nscenes, BatchOut *out)
-{
-  const int q = blockIdx.x*blockDim.x + threadIdx.x;
-  if (q < nscenes) out[q].done = out[q].aborted ? 1 : 0;
-}
-
-// after an odd number of refinement sweeps the scene's R is in the second buffer
-template <int G>
-__global__ __launch_bounds__(TPB) void k_ucopy(int nslices, const int *row0, const int *nrows, const int *sscene,
-                                               const BatchOut *out, const double *R64b, double *R64)
-{
-  const int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6);
-  if (s >= nslices) return;
-  const int p = slice_row<G>(row0, nrows, s);
-  if (p < 0 || !(out[sscene[s]].nref & 1)) return;
-#pragma unroll
-  for (int k = 0; k < 4; k ++) R64[4*(size_t)p + k] = R64b[4*(size_t)p + k];
-}
-
 struct Tuned { SFGS_BATCH_TUNE t; int valid; };
 static Tuned g_default = {{2, 4, 0, 1}, 0};
 
@@ -556,7 +498,13 @@ static bool tune_ok(const SFGS_BATCH_TUNE &t)
 } // namespace
 
 // the per-colour schedule of the batch (k_usweep, k_uscene); b holds the
-// sweep data and buffers as for the persistent kernel
+// sweep data and buffers as for the persistent kernel.  FP64: Gauss-Seidel
+// sweeps until every scene has met the test of bgs.c:1036 or its iteration
+// bound.  MIXED (T = float), as SFGS_Solve for one system: FP32 sweeps under
+// the test with epsilon and meritval held above FP32 resolution, for at most
+// maxiter - refine iterations; then, per scene, FP64 sweeps from that state
+// under the reference's test, at least refine of them, within the scene's
+// remaining iterations.
 template <typename T, int G>
 static int run_union(sfgs_ctx *ctx, const SFGS_PARAM *par, const BatchArgs<T> &b)
 {
@@ -564,10 +512,11 @@ static int run_union(sfgs_ctx *ctx, const SFGS_PARAM *par, const BatchArgs<T> &b
   const int nsc = ctx->nscenes, ncol = ctx->ncolours, ns = ctx->nslices, n = ctx->nc;
   const bool mixed = par->precision == SFGS_PREC_MIXED;
   const int nomerit = par->meritval >= 1.0 ? 1 : 0, maxiter = par->maxiter;
-  const int refine = mixed ? par->refine : 0;
-  // slices listed colour by colour (all scenes), their scene and update bounds
+  const int refine = mixed ? std::max(par->refine, 0) : 0;
+  // slices listed colour by colour (all scenes), their scene and the rows
+  // already updated when they run in a forward or a backward sweep
   std::vector<int> clist, cstart((size_t)ncol + 1, 0), sscene((size_t)std::max(ns, 1), 0),
-                   sbound(2*(size_t)std::max(ns, 1), 0), uscene(2*(size_t)std::max(nsc, 1), 0);
+                   sbound(4*(size_t)std::max(ns, 1), 0), uscene(2*(size_t)std::max(nsc, 1), 0);
   for (int c = 0; c < ncol; c ++)
   {
     cstart[c] = (int)clist.size();
@@ -578,8 +527,11 @@ static int run_union(sfgs_ctx *ctx, const SFGS_PARAM *par, const BatchArgs<T> &b
       {
         clist.push_back(sl);
         sscene[sl] = q;
-        sbound[2*(size_t)sl] = ctx->h_colptr[(size_t)q*ncol];   // rows of the scene's earlier colours
+        sbound[2*(size_t)sl] = ctx->h_colptr[(size_t)q*ncol];            // forward: the scene's earlier colours
         sbound[2*(size_t)sl + 1] = ctx->h_colptr[g];
+        const size_t bb = 2*(size_t)std::max(ns, 1) + 2*(size_t)sl;      // backward: its later colours
+        sbound[bb] = ctx->h_colptr[g + 1];
+        sbound[bb + 1] = ctx->h_colptr[(size_t)(q + 1)*ncol];
       }
     }
   }
@@ -596,9 +548,8 @@ static int run_union(sfgs_ctx *ctx, const SFGS_PARAM *par, const BatchArgs<T> &b
   CHK(upload(ctx, ctx->u_scene, uscene.data(), uscene.size()));
   CHK(ensure(ctx, ctx->u_spart, 3*(size_t)std::max(ns, 1)));
   CHK(ensure(ctx, ctx->u_active, (size_t)maxsweeps + 1));
-  HIPCHK(hipMemsetAsync(ctx->u_active.p, 0, sizeof(int)*((size_t)maxsweeps + 1), st));
   const UScene *usc = (const UScene*)ctx->u_scene.p;
-  const int2 *sbd = (const int2*)ctx->u_sbound.p;
+  const int2 *sbf = (const int2*)ctx->u_sbound.p, *sbb = sbf + std::max(ns, 1);
   BatchOut *out = (BatchOut*)ctx->bout.p;
   double *spart = ctx->u_spart.p;
   const int sblk = (ns + TPB/SLICE - 1) / (TPB/SLICE);
@@ -606,85 +557,74 @@ static int run_union(sfgs_ctx *ctx, const SFGS_PARAM *par, const BatchArgs<T> &b
 
   // free energy of every scene
   hipLaunchKernelGGL(k_uenergy<G>, dim3(sblk), dim3(TPB), 0, st, b.d, ns, spart);
-  hipLaunchKernelGGL(k_uscene, dim3(nsc), dim3(TPB), 0, st, 0, usc, (const double*)spart, out, maxiter, nomerit,
+  hipLaunchKernelGGL(k_uscene, dim3(nsc), dim3(TPB), 0, st, 0, usc, (const double*)spart, out, maxiter, 0, nomerit,
                      par->epsilon, par->meritval, 0, ctx->u_active.p);
-  // Gauss-Seidel sweeps until every scene has stopped
-  auto sweep = [&](auto tag, SweepArgs<T> a, int kk, bool merit) {
-    (void)tag;
-    for (int c = 0; c < ncol; c ++)
+
+  // sweeps of one phase until every scene has stopped; sweep kk reads X[kk & 1], writes the other
+  auto phase = [&](auto base, auto *X0, auto *X1, int bound, int miniter, double eps, double meritval) -> int {
+    using TT = typename std::remove_pointer<decltype(X0)>::type;
+    HIPCHK(hipMemsetAsync(ctx->u_active.p, 0, sizeof(int)*((size_t)maxsweeps + 1), st));
+    int k = 0, batch = 2;
+    while (k < maxsweeps)
     {
-      const int nl = cstart[c + 1] - cstart[c];
-      if (nl == 0) continue;
-      const dim3 grid((nl + TPB/SLICE - 1) / (TPB/SLICE));
-      if (merit)
-        hipLaunchKernelGGL((k_usweep<T, G, true>), grid, dim3(TPB), 0, st, a, ctx->u_clist.p + cstart[c], nl, sbd,
-                           (const int*)ctx->u_sscene.p, out, spart, 0);
-      else
-        hipLaunchKernelGGL((k_usweep<T, G, false>), grid, dim3(TPB), 0, st, a, ctx->u_clist.p + cstart[c], nl, sbd,
-                           (const int*)ctx->u_sscene.p, out, spart, 0);
-    }
-    (void)kk;
-  };
-  int k = 0, batch = 2;
-  while (k < maxsweeps)
-  {
-    const int nbt = std::min(batch, maxsweeps - k);
-    for (int j = 0; j < nbt; j ++)
-    {
-      const int kk = k + j;
-      SweepArgs<T> a = b.a;
-      a.P = (kk & 1) ? b.X1 : b.X0;
-      a.Q = (kk & 1) ? b.X0 : b.X1;
-      sweep(0, a, kk, !nomerit && kk > 0);
-      hipLaunchKernelGGL(k_uscene, dim3(nsc), dim3(TPB), 0, st, 1, usc, (const double*)spart, out, maxiter, nomerit,
-                         par->epsilon, par->meritval, kk, ctx->u_active.p);
-    }
-    k += nbt;
-    int act = 0;
-    HIPCHK(hipMemcpyAsync(&act, ctx->u_active.p + (k - 1), sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    HIPCHK(hipGetLastError());
-    if (act == 0) break;
-    batch = std::min(batch*2, 32);
-  }
-  hipLaunchKernelGGL((k_ufinal<T, G>), dim3(sblk), dim3(TPB), 0, st, ns, (const int*)ctx->slice_row0.p,
-                     (const int*)ctx->slice_nrows.p, (const int*)ctx->u_sscene.p, (const BatchOut*)out,
-                     (const T*)b.X0, (const T*)b.X1, (const T*)b.Ut, (const T*)b.Umt, b.R64, b.U64, n, mixed ? 1 : 0);
-  if (mixed)
-  {
-    // FP64 refinement sweeps of every scene that did not abort, then the final FP64 merit
-    if (refine > 0)
-    {
-      hipLaunchKernelGGL(k_ureset, dim3((nsc + TPB - 1)/TPB), dim3(TPB), 0, st, nsc, out);
-      for (int kk = 0; kk < refine; kk ++)
+      const int nbt = std::min(batch, maxsweeps - k);
+      for (int j = 0; j < nbt; j ++)
       {
-        SweepArgs<double> a = b.d;
-        a.P = (kk & 1) ? b.R64b : b.R64;
-        a.Q = (kk & 1) ? b.R64 : b.R64b;
-        for (int c = 0; c < ncol; c ++)
+        const int kk = k + j, back = par->reverse && (kk & 1);
+        SweepArgs<TT> a = base;
+        a.P = (kk & 1) ? X1 : X0;
+        a.Q = (kk & 1) ? X0 : X1;
+        const bool merit = !nomerit && kk > 0;
+        for (int ci = 0; ci < ncol; ci ++)
         {
-          const int nl = cstart[c + 1] - cstart[c];
+          const int c = back ? ncol - 1 - ci : ci, nl = cstart[c + 1] - cstart[c];
           if (nl == 0) continue;
-          hipLaunchKernelGGL((k_usweep<double, G, false>), dim3((nl + TPB/SLICE - 1) / (TPB/SLICE)), dim3(TPB), 0, st,
-                             a, ctx->u_clist.p + cstart[c], nl, sbd, (const int*)ctx->u_sscene.p, out, spart, 0);
+          const dim3 grid((nl + TPB/SLICE - 1) / (TPB/SLICE));
+          if (merit)
+            hipLaunchKernelGGL((k_usweep<TT, G, true>), grid, dim3(TPB), 0, st, a, ctx->u_clist.p + cstart[c], nl,
+                               back ? sbb : sbf, (const int*)ctx->u_sscene.p, out, spart, back);
+          else
+            hipLaunchKernelGGL((k_usweep<TT, G, false>), grid, dim3(TPB), 0, st, a, ctx->u_clist.p + cstart[c], nl,
+                               back ? sbb : sbf, (const int*)ctx->u_sscene.p, out, spart, back);
         }
-        hipLaunchKernelGGL(k_uscene, dim3(nsc), dim3(TPB), 0, st, 2, usc, (const double*)spart, out, maxiter, nomerit,
-                           par->epsilon, par->meritval, 0, ctx->u_active.p);
+        hipLaunchKernelGGL(k_uscene, dim3(nsc), dim3(TPB), 0, st, 1, usc, (const double*)spart, out, bound, miniter,
+                           nomerit, eps, meritval, kk, ctx->u_active.p);
       }
-      hipLaunchKernelGGL(k_ucopy<G>, dim3(sblk), dim3(TPB), 0, st, ns, (const int*)ctx->slice_row0.p,
-                         (const int*)ctx->slice_nrows.p, (const int*)ctx->u_sscene.p, (const BatchOut*)out,
-                         (const double*)b.R64b, b.R64);
+      k += nbt;
+      int act = 0;
+      HIPCHK(hipMemcpyAsync(&act, ctx->u_active.p + (k - 1), sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      HIPCHK(hipGetLastError());
+      if (act == 0) break;
+      batch = std::min(batch*2, 32);
     }
-    if (!nomerit)
-    {
-      MeritArgs m = b.m;
-      m.R = b.R64;
-      hipLaunchKernelGGL(k_umerit<G>, dim3(sblk), dim3(TPB), 0, st, m, ns, (const int*)ctx->u_sscene.p,
-                         (const BatchOut*)out, spart);
-      hipLaunchKernelGGL(k_uscene, dim3(nsc), dim3(TPB), 0, st, 3, usc, (const double*)spart, out, maxiter, nomerit,
-                         par->epsilon, par->meritval, 0, ctx->u_active.p);
-    }
+    return 0;
+  };
+
+  if (!mixed)
+  {
+    CHK(phase(b.a, b.X0, b.X1, maxiter, 0, par->epsilon, par->meritval));
+    // the final state into the canonical buffers
+    hipLaunchKernelGGL((k_ufinal<T, G>), dim3(sblk), dim3(TPB), 0, st, ns, (const int*)ctx->slice_row0.p,
+                       (const int*)ctx->slice_nrows.p, (const int*)ctx->u_sscene.p, (const BatchOut*)out,
+                       (const T*)b.X0, (const T*)b.X1, (const T*)b.Ut, (const T*)b.Umt, b.R64, b.U64, n, 0);
+    HIPCHK(hipGetLastError());
+    return 0;
   }
+  // FP32 phase (none when maxiter <= refine: the whole solve is FP64, from the FP64 state)
+  if (maxiter > refine)
+  {
+    CHK(phase(b.a, b.X0, b.X1, maxiter - refine, 0, std::max(par->epsilon, 1e-6), std::max(par->meritval, 1e-10)));
+    hipLaunchKernelGGL((k_ufinal<T, G>), dim3(sblk), dim3(TPB), 0, st, ns, (const int*)ctx->slice_row0.p,
+                       (const int*)ctx->slice_nrows.p, (const int*)ctx->u_sscene.p, (const BatchOut*)out,
+                       (const T*)b.X0, (const T*)b.X1, (const T*)b.Ut, (const T*)b.Umt, b.R64, b.U64, n, 1);
+  }
+  // FP64 phase per scene under the reference's test
+  hipLaunchKernelGGL(k_uphase, dim3((nsc + TPB - 1)/TPB), dim3(TPB), 0, st, nsc, usc, maxiter, out);
+  CHK(phase(b.d, b.R64, b.R64b, maxiter, refine, par->epsilon, par->meritval));
+  hipLaunchKernelGGL(k_ufinal64<G>, dim3(sblk), dim3(TPB), 0, st, ns, (const int*)ctx->slice_row0.p,
+                     (const int*)ctx->slice_nrows.p, (const int*)ctx->u_sscene.p, (const BatchOut*)out,
+                     (const double*)b.R64b, (const double*)b.Um64, b.R64, b.U64, n);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -715,10 +655,9 @@ static int batch_solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TU
     return SFGS_E_STATE;
   }
 
-  // workgroups per scene
+  // workgroups per scene (the persistent launch runs FP64 solves only)
   int cap = 0;
-  if (mixed) CHK((dispatch_batch<float>(ctx, G, NW, BatchArgs<float>(), 0, &cap)));
-  else CHK((dispatch_batch<double>(ctx, G, NW, BatchArgs<double>(), 0, &cap)));
+  CHK((dispatch_batch<double>(ctx, G, NW, BatchArgs<double>(), 0, &cap)));
   if (cap < 1) { ctx->err = "batched kernel cannot be resident"; return SFGS_E_HIP; }
   std::vector<int> rows((size_t)nsc), want((size_t)nsc), slices((size_t)nsc);
   long long total_rows = 0;
@@ -769,13 +708,11 @@ static int batch_solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TU
   auto fill = [&](auto &b) {
     b.d = sweep_args<double>(ctx, &pd, ctx->W.p, ctx->Wd.p, ctx->B.p, ctx->rho.p, ctx->pfri.p, ctx->pcoh.p, ctx->shift.p,
                              ctx->U.p, ctx->Um.p, par->diagepsilon);
-    b.m = merit_args(ctx, 1, nullptr);
     b.geo.gslice = ctx->bgslice.p; b.geo.gptr = ctx->bgptr.p; b.geo.ncol = ncol;
     b.geo.abort_all = ctx->babort.p; b.geo.spin_limit = 1u << 22;
     b.R64 = ctx->R.p; b.R64b = ctx->Rb.p; b.U64 = ctx->U.p; b.Um64 = ctx->Um.p;
     b.out = (BatchOut*)ctx->bout.p; b.bar = ctx->bbar.p;
     b.maxiter = par->maxiter; b.nomerit = par->meritval >= 1.0 ? 1 : 0; b.reverse = par->reverse;
-    b.refine = mixed ? par->refine : 0; b.mixed = mixed ? 1 : 0;
     b.eps = par->epsilon; b.meritval = par->meritval;
   };
   fill(bd);
@@ -790,9 +727,10 @@ static int batch_solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TU
     bf.X0 = ctx->R32.p; bf.X1 = ctx->R32b.p; bf.Ut = ctx->U32.p; bf.Umt = ctx->Um32.p;
   }
 
-  // launches over chunks of scenes that fit the resident capacity, or the per-colour schedule
+  // launches over chunks of scenes that fit the resident capacity, or the per-colour
+  // schedule, which every MIXED solve takes (its FP64 phase runs per scene)
   std::vector<int> wgs((size_t)nsc, 0);
-  const bool per_colour = tn.schedule == 1 && !par->reverse;
+  const bool per_colour = tn.schedule == 1 || mixed;
   HIPCHK(hipEventRecord(ctx->ev0, st));
   if (per_colour)
   {
@@ -822,11 +760,10 @@ static int batch_solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TU
     CHK(upload(ctx, ctx->bscene_tab, (const int*)tab.data(), tab.size()*sizeof(BatchScene)/sizeof(int)));
     CHK(upload(ctx, ctx->bwg_scene, wg_scene.data(), wg_scene.size()));
     CHK(ensure(ctx, ctx->bpart, 8*(size_t)used));
-    bd.scenes = bf.scenes = (const BatchScene*)ctx->bscene_tab.p;
-    bd.wg_scene = bf.wg_scene = ctx->bwg_scene.p;
-    bd.wpart = bf.wpart = ctx->bpart.p;
-    if (mixed) CHK(dispatch_batch<float>(ctx, G, NW, bf, used, nullptr));
-    else CHK(dispatch_batch<double>(ctx, G, NW, bd, used, nullptr));
+    bd.scenes = (const BatchScene*)ctx->bscene_tab.p;
+    bd.wg_scene = ctx->bwg_scene.p;
+    bd.wpart = ctx->bpart.p;
+    CHK(dispatch_batch<double>(ctx, G, NW, bd, used, nullptr));
     // the tables are re-uploaded for the next chunk only after this launch has run
     HIPCHK(hipStreamSynchronize(st));
     HIPCHK(hipGetLastError());
@@ -928,8 +865,8 @@ int SFGS_Batch_Autotune(sfgs_ctx *ctx, const SFGS_PARAM *par, int trial_iters, S
       ctx->G = lanes[li];
       if ((rc = reassemble(ctx))) break;
     }
-    // shape index 16 is the per-colour schedule (no workgroup shape)
-    for (int wr = 0; wr <= 16 && !rc; wr ++)
+    // shape index 16 is the per-colour schedule (no workgroup shape), the only one of MIXED
+    for (int wr = par->precision == SFGS_PREC_MIXED ? 16 : 0; wr <= 16 && !rc; wr ++)
       {
         const int wi = wr / 4, ri = wr % 4;
         SFGS_BATCH_TUNE t = wr == 16 ? SFGS_BATCH_TUNE{lanes[li], 4, 0, 1}

@@ -254,28 +254,32 @@ int SFGS_Warm_Start_Mode(sfgs_ctx *ctx, int new_only, float scale);
  * which solves one domain per GAUSS_SEIDEL_Solve call, bgs.c:913-1069) ----
  * The uploaded bodies are split into scenes by a scene id per body; contacts
  * belong to their master body's scene and may not couple rigid bodies of two
- * scenes.  All scenes are solved by one persistent kernel launch: each scene
- * owns a set of workgroups that sweep its colours and synchronise among
- * themselves only, and each scene runs its own termination test (bgs.c:1036),
- * so a scene that converges stops while the others go on.  MIXED precision
- * runs the FP32 sweeps, the refinement sweeps and the final FP64 merit inside
- * the same launch.  Call SFGS_Batch_Scenes after SFGS_Bodies_Upload and before
- * SFGS_Locdyn_Update_Begin (a body upload resets the split); SFGS_Solve
- * refuses a split system. */
+ * scenes.  Each scene runs its own termination test (bgs.c:1036), so a scene
+ * that converges stops while the others go on.  FP64 solves run either as one
+ * persistent kernel launch, each scene owning a set of workgroups that sweep
+ * its colours and synchronise among themselves only, or with one launch per
+ * colour over every running scene (the tune's schedule).  MIXED solves take
+ * the per-colour schedule and, per scene, the two phases of SFGS_Solve's
+ * MIXED: FP32 sweeps under the test with epsilon and meritval held at or
+ * above 1e-6 and 1e-10, for at most maxiter - refine iterations, then FP64
+ * sweeps under the reference's test, at least refine of them, within the
+ * scene's remaining iterations.  Call SFGS_Batch_Scenes after
+ * SFGS_Bodies_Upload and before SFGS_Locdyn_Update_Begin (a body upload
+ * resets the split); SFGS_Solve refuses a split system. */
 typedef struct {
   int lanes_per_row;    /* layout the system was assembled with (SFGS_Set_Layout): 1 or 2 */
   int waves_per_group;  /* wavefronts per workgroup: 1, 2, 4 or 8 */
   int rows_per_group;   /* target rows of a scene per workgroup; 0 = share the resident workgroups
                            among the scenes in proportion to their rows (one launch) */
-  int schedule;         /* 0 = one persistent launch (the fields above); 1 = one launch per colour
-                           over the colour's slices of every running scene, a scene-wise
-                           termination test after each sweep (forward sweeps; reverse uses 0) */
+  int schedule;         /* 0 = one persistent launch (the fields above; FP64 only, MIXED uses 1);
+                           1 = one launch per colour over the colour's slices of every running
+                           scene, a scene-wise termination test after each sweep */
 } SFGS_BATCH_TUNE;
 
 typedef struct {
   int rows;             /* contacts of the scene */
   int groups;           /* workgroups that solved it */
-  int iters;            /* Gauss-Seidel iterations (MIXED: FP32 iterations + refinement sweeps) */
+  int iters;            /* Gauss-Seidel iterations (MIXED: FP32 + FP64 iterations) */
   int error_code;       /* SFGS_GS_* */
   int fail_rows;        /* local solver failures */
   int aborted;          /* FAILURE_EXIT raised */
@@ -291,9 +295,10 @@ int SFGS_Batch_Scenes(sfgs_ctx *ctx, int nscenes, const int *body_scene);
 int SFGS_Batch_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TUNE *tune, SFGS_SCENE_REPORT *reports,
                      SFGS_REPORT *total);
 /* time trial_iters iterations of every candidate shape (lanes 1 and 2, which
- * re-assembles the system; the persistent launch with waves 1..8 and rows per
- * group proportional, 1024, 4096 or all; the per-colour schedule) and keep the
- * fastest; R and U are restored after each trial */
+ * re-assembles the system; for FP64 the persistent launch with waves 1..8 and
+ * rows per group proportional, 1024, 4096 or all, and the per-colour schedule;
+ * for MIXED the per-colour schedule) and keep the fastest; R and U are
+ * restored after each trial */
 int SFGS_Batch_Autotune(sfgs_ctx *ctx, const SFGS_PARAM *par, int trial_iters, SFGS_BATCH_TUNE *best, double *best_ms);
 
 /* ---- device-resident time stepping: DOM_Update_Begin / DOM_Update_End on the

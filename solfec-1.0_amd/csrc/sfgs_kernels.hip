@@ -1326,7 +1326,7 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   release(ctx->rowlen); release(ctx->W); release(ctx->Wd); release(ctx->A); release(ctx->B); release(ctx->V);
   release(ctx->rho); release(ctx->pfri); release(ctx->pcoh); release(ctx->shift); release(ctx->R); release(ctx->U); release(ctx->pen); sfgs_newton_release(ctx); sfgs_bar_release(ctx);
   release(ctx->Rb); release(ctx->Um); release(ctx->sep); release(ctx->wcol);
-  release(ctx->W32); release(ctx->Wd32); release(ctx->B32); release(ctx->R32); release(ctx->R32b); release(ctx->U32); release(ctx->Um32);
+  release(ctx->W32); release(ctx->Wd32); release(ctx->A32); release(ctx->B32); release(ctx->R32); release(ctx->R32b); release(ctx->U32); release(ctx->Um32);
   release(ctx->rho32); release(ctx->pfri32); release(ctx->pcoh32); release(ctx->shift32);
   release(ctx->partials); release(ctx->mpartials); release(ctx->hist); release(ctx->failkey);
   release(ctx->failcount); release(ctx->ctrl); release(ctx->fin);
@@ -1690,6 +1690,7 @@ int sfgs::build_f32_mirror(sfgs_ctx *ctx)
   hipStream_t st = ctx->stream;
   CHK(ensure(ctx, ctx->W32, (size_t)std::max(nb, 1LL)));
   CHK(ensure(ctx, ctx->Wd32, 9*(size_t)n));
+  CHK(ensure(ctx, ctx->A32, 9*(size_t)n));
   CHK(ensure(ctx, ctx->B32, 3*(size_t)n));
   CHK(ensure(ctx, ctx->rho32, (size_t)n));
   CHK(ensure(ctx, ctx->pfri32, (size_t)n));
@@ -1703,6 +1704,7 @@ int sfgs::build_f32_mirror(sfgs_ctx *ctx)
   if (n > 0)
   {
     hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(9*n)), dim3(TPB), 0, st, 9*n, ctx->Wd.p, ctx->Wd32.p);
+    hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(9*n)), dim3(TPB), 0, st, 9*n, ctx->A.p, ctx->A32.p);
     hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(3*n)), dim3(TPB), 0, st, 3*n, ctx->B.p, ctx->B32.p);
     hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(n)), dim3(TPB), 0, st, n, ctx->rho.p, ctx->rho32.p);
     hipLaunchKernelGGL(k_to_f32, dim3(nblk_ll(n)), dim3(TPB), 0, st, n, ctx->pfri.p, ctx->pfri32.p);

@@ -45,7 +45,8 @@ struct SweepArgs {
   const T *W;
   const int *wcol;
   const T *Wd, *B, *rho, *fri, *coh, *shift;
-  const double *A;      // merit weights stay FP64
+  const double *A;      // merit weights (FP64 sweeps)
+  const float *A32;     // the same rounded to FP32 (FP32 sweeps); the merit arithmetic stays FP64
   const unsigned char *sep;
   const T *P;
   T *Q, *U, *Um;
@@ -138,7 +139,7 @@ __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool v
     if (MERIT)
     {
 #pragma unroll
-      for (int e = 0; e < 9; e ++) Ai[e] = a.A[(size_t)e*n + p];
+      for (int e = 0; e < 9; e ++) Ai[e] = sizeof(T) == 4 ? (double)a.A32[(size_t)e*n + p] : a.A[(size_t)e*n + p];
     }
     else
     {

@@ -94,7 +94,8 @@ struct sfgs_ctx {
   int G_built = 1;                       // layout of the current store
   sfgs::DBuf<unsigned char> sep;
   sfgs::DBuf<int> wcol;
-  sfgs::DBuf<float> W32, Wd32, A32, B32, R32, R32b, U32, Um32, rho32, pfri32, pcoh32, shift32;
+  sfgs::DBuf<float> W32, Wd32, B32, R32, R32b, U32, Um32, rho32, pfri32, pcoh32, shift32;
+  sfgs::DBuf<float> rec32;               // FP32 row records of the FP32 sweeps (five float4 per row)
   int mirror_valid = 0;                  // FP32 copies match the FP64 store
   double energy = 0.0, step = 0.0;
   int dynamic = 1;

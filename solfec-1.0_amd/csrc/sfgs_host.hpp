@@ -60,7 +60,7 @@ static SweepArgs<T> sweep_args(sfgs_ctx *ctx, const SFGS_PARAM *par, const T *W,
   a.row0 = ctx->slice_row0.p; a.nrows = ctx->slice_nrows.p; a.rowlen = ctx->rowlen.p; a.off = ctx->slice_off.p;
   a.slen = ctx->slice_len.p;
   a.W = W; a.wcol = ctx->wcol.p; a.Wd = Wd; a.A = ctx->A.p; a.B = B; a.rho = rho;
-  a.A32 = sizeof(T) == 4 ? ctx->A32.p : nullptr;
+  a.rec = sizeof(T) == 4 ? reinterpret_cast<const float4 *>(ctx->rec32.p) : nullptr;
   a.fri = fri; a.coh = coh; a.shift = shift; a.sep = ctx->sep.p;
   a.P = nullptr; a.Q = nullptr; a.U = U; a.Um = Um; a.Uin = U; a.n = ctx->nc;
   a.diageps = (T)diageps; a.diagmaxiter = par->diagmaxiter; a.diagsolver = par->diagsolver;

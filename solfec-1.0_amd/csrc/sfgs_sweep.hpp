@@ -46,7 +46,7 @@ struct SweepArgs {
   const int *wcol;
   const T *Wd, *B, *rho, *fri, *coh, *shift;
   const double *A;      // merit weights (FP64 sweeps)
-  const float *A32;     // the same rounded to FP32 (FP32 sweeps); the merit arithmetic stays FP64
+  const float4 *rec;    // FP32 sweeps: the row records (B, law, W_ii, A; k_rec32); the merit arithmetic stays FP64
   const unsigned char *sep;
   const T *P;
   T *Q, *U, *Um;
@@ -124,29 +124,58 @@ __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool v
   long long off = 0;
   if (valid) { nr = a.nrows[s]; r0 = a.row0[s]; off = a.off[s]; }   // one round of loads
   const bool act = valid && run && r < nr;
-  if (act) { p = r0 + r; len = a.rowlen[p]; }
   // the row's own data first, so that its loads overlap the gather below
   T Bi[3] = {T(0), T(0), T(0)}, Wi[9], R[3] = {T(0), T(0), T(0)}, U[3] = {T(0), T(0), T(0)};
   double Ai[9];
   RowLaw<T> L{T(0), T(0), T(0), T(0), false};
   if (act)
   {
-    Bi[0] = a.B[p]; Bi[1] = a.B[(size_t)n + p]; Bi[2] = a.B[2*(size_t)n + p];
-#pragma unroll
-    for (int e = 0; e < 9; e ++) Wi[e] = a.Wd[(size_t)e*n + p];
-#pragma unroll
-    for (int k = 0; k < 3; k ++) R[k] = r_ld<COH>(a.P + 4*(size_t)p + k);
-    if (MERIT)
+    p = r0 + r;
+    if constexpr (sizeof(T) == 4)
     {
+      // the row record: five 16-byte loads (k_rec32), and R_i as one when not shared
+      const float4 *rc = a.rec + 5*(size_t)p;
+      const float4 q0 = rc[0], q1 = rc[1], q2 = rc[2], q3 = rc[3], q4 = rc[4];
+      const int ls = __float_as_int(q1.w);
+      len = ls >> 1;
+      Bi[0] = q0.x; Bi[1] = q0.y; Bi[2] = q0.z;
+      Wi[0] = q2.x; Wi[1] = Wi[3] = q2.y; Wi[2] = Wi[6] = q2.z; Wi[4] = q2.w; Wi[5] = Wi[7] = q3.x; Wi[8] = q3.y;
+      if (MERIT)
+      {
+        Ai[0] = q3.z; Ai[1] = Ai[3] = q3.w; Ai[2] = Ai[6] = q4.x; Ai[4] = q4.y; Ai[5] = Ai[7] = q4.z; Ai[8] = q4.w;
+      }
+      L = RowLaw<T>{q1.x, q1.y, q1.z, q0.w, (ls & 1) != 0};
+      if (COH)
+      {
 #pragma unroll
-      for (int e = 0; e < 9; e ++) Ai[e] = sizeof(T) == 4 ? (double)a.A32[(size_t)e*n + p] : a.A[(size_t)e*n + p];
+        for (int k = 0; k < 3; k ++) R[k] = r_ld<COH>(a.P + 4*(size_t)p + k);
+      }
+      else
+      {
+        const float4 rr = *reinterpret_cast<const float4 *>(a.P + 4*(size_t)p);
+        R[0] = rr.x; R[1] = rr.y; R[2] = rr.z;
+      }
     }
     else
+    {
+      len = a.rowlen[p];
+      Bi[0] = a.B[p]; Bi[1] = a.B[(size_t)n + p]; Bi[2] = a.B[2*(size_t)n + p];
+#pragma unroll
+      for (int e = 0; e < 9; e ++) Wi[e] = a.Wd[(size_t)e*n + p];
+#pragma unroll
+      for (int k = 0; k < 3; k ++) R[k] = r_ld<COH>(a.P + 4*(size_t)p + k);
+      if (MERIT)
+      {
+#pragma unroll
+        for (int e = 0; e < 9; e ++) Ai[e] = a.A[(size_t)e*n + p];
+      }
+      L = RowLaw<T>{a.fri[p], a.coh[p], a.shift[p], a.rho[p], a.sep[p] != 0};
+    }
+    if (!MERIT)
     {
 #pragma unroll
       for (int k = 0; k < 3; k ++) U[k] = a.Uin[(size_t)k*n + p];
     }
-    L = RowLaw<T>{a.fri[p], a.coh[p], a.shift[p], a.rho[p], a.sep[p] != 0};
   }
   T b0 = T(0), b1 = T(0), b2 = T(0), u0 = T(0), u1 = T(0), u2 = T(0);
   if (act)

@@ -1671,6 +1671,8 @@ static int run_gs(sfgs_ctx *ctx, SweepArgs<T> sa, T *X0, T *X1, int maxiter, int
   hc.miniter = miniter;
   HIPCHK(hipMemcpyAsync(ctx->ctrl.p, &hc, sizeof(Ctrl), hipMemcpyHostToDevice, st));
   const int maxsweeps = nomerit ? maxiter : maxiter + 1;   // with merit, sweep k closes iteration k-1
+  // with merit the final U is the merit pass's (Um) unless FAILURE_EXIT stops a sweep
+  sa.store_U = sa.failure == SFGS_FAILURE_EXIT;
   int batch = pl.prof ? 1 : 2, k = 0;
   while (k < maxsweeps)
   {

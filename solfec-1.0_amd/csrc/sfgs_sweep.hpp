@@ -54,6 +54,8 @@ struct SweepArgs {
   int n;
   T diageps;
   int diagmaxiter, diagsolver, failure;
+  int store_U;          // write the local solver's U; a merit sweep's U is not the final state unless
+                        // the solve can abort inside it (FAILURE_EXIT), then U is that state
   double *partials;     // 3 per block: errup, errlo, merit numerator
   unsigned long long *failkey;
   int *failcount;
@@ -298,9 +300,19 @@ __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool v
     }
     if (g == 0)
     {
+      if constexpr (sizeof(T) == 4 && !COH)
+        *reinterpret_cast<float4 *>(a.Q + 4*(size_t)p) = make_float4(R[0], R[1], R[2], 0.0f);
+      else
+      {
 #pragma unroll
-      for (int k = 0; k < 3; k ++) { r_st<COH>(a.Q + 4*(size_t)p + k, R[k]); a.U[(size_t)k*n + p] = U[k]; }
-      r_st<COH>(a.Q + 4*(size_t)p + 3, T(0));
+        for (int k = 0; k < 3; k ++) r_st<COH>(a.Q + 4*(size_t)p + k, R[k]);
+        r_st<COH>(a.Q + 4*(size_t)p + 3, T(0));
+      }
+      if (!MERIT || a.store_U)
+      {
+#pragma unroll
+        for (int k = 0; k < 3; k ++) a.U[(size_t)k*n + p] = U[k];
+      }
       const double d0 = (double)R[0] - R0[0], d1 = (double)R[1] - R0[1], d2 = (double)R[2] - R0[2];
       eu += d0*d0 + d1*d1 + d2*d2;
       el += (double)R[0]*R[0] + (double)R[1]*R[1] + (double)R[2]*R[2];

@@ -350,8 +350,10 @@ static int run_flow_t(sfgs_ctx *ctx, const SweepArgs<T> &sa, T *X0, T *X1, int m
   hipLaunchKernelGGL(k_flow_final<T>, dim3(nblk(n)), dim3(TPB), 0, st, n, Rf, Uf, X0, sa.U);
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipGetLastError());
+  // the state is now in X0 and sa.U; final_U = 1 tells the caller that U is a
+  // merit pass's, which it forms again from R as it does for the colour launches
   hc.final_buf = 0;
-  hc.final_U = 0;
+  hc.final_U = hc.final_U >= 2 ? 1 : 0;
   return 0;
 }
 

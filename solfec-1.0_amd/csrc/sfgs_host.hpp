@@ -64,7 +64,7 @@ static SweepArgs<T> sweep_args(sfgs_ctx *ctx, const SFGS_PARAM *par, const T *W,
   a.fri = fri; a.coh = coh; a.shift = shift; a.sep = ctx->sep.p;
   a.P = nullptr; a.Q = nullptr; a.U = U; a.Um = Um; a.Uin = U; a.n = ctx->nc;
   a.diageps = (T)diageps; a.diagmaxiter = par->diagmaxiter; a.diagsolver = par->diagsolver;
-  a.failure = par->failure; a.store_U = 1; a.partials = ctx->partials.p; a.failkey = ctx->failkey.p; a.failcount = ctx->failcount.p;
+  a.failure = par->failure; a.store_U = 1; a.store_Um = 1; a.partials = ctx->partials.p; a.failkey = ctx->failkey.p; a.failcount = ctx->failcount.p;
   a.ctrl = ctx->ctrl.p;
   return a;
 }

@@ -1671,8 +1671,11 @@ static int run_gs(sfgs_ctx *ctx, SweepArgs<T> sa, T *X0, T *X1, int maxiter, int
   hc.miniter = miniter;
   HIPCHK(hipMemcpyAsync(ctx->ctrl.p, &hc, sizeof(Ctrl), hipMemcpyHostToDevice, st));
   const int maxsweeps = nomerit ? maxiter : maxiter + 1;   // with merit, sweep k closes iteration k-1
-  // with merit the final U is the merit pass's (Um) unless FAILURE_EXIT stops a sweep
+  // With merit the final U is the merit pass's unless FAILURE_EXIT stops a
+  // sweep.  The merit pass's U is not stored either: the caller forms it from
+  // the final R (final_U = 1: k_merit with update_U)
   sa.store_U = sa.failure == SFGS_FAILURE_EXIT;
+  sa.store_Um = 0;
   int batch = pl.prof ? 1 : 2, k = 0;
   while (k < maxsweeps)
   {
@@ -1805,6 +1808,14 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
 
   Ctrl hc, hr;
   std::memset(&hr, 0, sizeof(hr));
+  // U = B + W R of the final R in ctx->R (MERIT_Function's update of U,
+  // mrf.c:61-76), when the merit pass's U is the final one (final_U = 1)
+  auto merit_U = [&]() {
+    if (mgrid == 0) return;
+    MeritArgs ma = merit_args(ctx, 1, nullptr);
+    SFGS_DISPATCH_G(ctx->G_built, auto kern = k_merit<GG>; hipLaunchKernelGGL(kern, dim3(mgrid), dim3(TPB), 0, st, ma,
+                    ctx->nslices));
+  };
   HIPCHK(hipEventRecord(ctx->ev0, st));
   if (!mixed)
   {
@@ -1830,8 +1841,7 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
                          ctx->hist.p, pl, hc));
     if (n > 0 && hc.final_buf == 1)
       HIPCHK(hipMemcpyAsync(ctx->R.p, ctx->Rb.p, sizeof(double)*4*(size_t)n, hipMemcpyDeviceToDevice, st));
-    if (n > 0 && hc.final_U == 1)
-      HIPCHK(hipMemcpyAsync(ctx->U.p, ctx->Um.p, sizeof(double)*3*(size_t)n, hipMemcpyDeviceToDevice, st));
+    if (n > 0 && hc.final_U == 1) merit_U();
   }
   else
   {
@@ -1872,9 +1882,10 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
                         par->reverse, ctx->hist.p, pl, hc));
     if (n > 0)
     {
-      float *Rf = hc.final_buf == 1 ? ctx->R32b.p : ctx->R32.p, *Uf = hc.final_U == 1 ? ctx->Um32.p : ctx->U32.p;
+      float *Rf = hc.final_buf == 1 ? ctx->R32b.p : ctx->R32.p;
       hipLaunchKernelGGL(k_to_f64, dim3(nblk_ll(4LL*n)), dim3(TPB), 0, st, 4LL*n, Rf, ctx->R.p);
-      hipLaunchKernelGGL(k_to_f64, dim3(nblk_ll(3LL*n)), dim3(TPB), 0, st, 3LL*n, Uf, ctx->U.p);
+      if (hc.final_U == 1) merit_U();   // U = B + W R in FP64
+      else hipLaunchKernelGGL(k_to_f64, dim3(nblk_ll(3LL*n)), dim3(TPB), 0, st, 3LL*n, ctx->U32.p, ctx->U.p);
     }
     if (refine > 0 && !hc.aborted)
     {
@@ -1886,8 +1897,7 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
                          par->reverse, ctx->hist.p + 2*(size_t)maxiter, pl, hr, refine));
       if (n > 0 && hr.final_buf == 1)
         HIPCHK(hipMemcpyAsync(ctx->R.p, ctx->Rb.p, sizeof(double)*4*(size_t)n, hipMemcpyDeviceToDevice, st));
-      if (n > 0 && hr.final_U == 1)
-        HIPCHK(hipMemcpyAsync(ctx->U.p, ctx->Um.p, sizeof(double)*3*(size_t)n, hipMemcpyDeviceToDevice, st));
+      if (n > 0 && hr.final_U == 1) merit_U();
     }
     else if (!nomerit && mgrid > 0)
     {

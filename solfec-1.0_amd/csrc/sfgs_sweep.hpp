@@ -56,6 +56,8 @@ struct SweepArgs {
   int diagmaxiter, diagsolver, failure;
   int store_U;          // write the local solver's U; a merit sweep's U is not the final state unless
                         // the solve can abort inside it (FAILURE_EXIT), then U is that state
+  int store_Um;         // write the merit pass's U (Um); a caller that forms the final U = B + W R
+                        // afterwards (k_merit with update_U) leaves it off
   double *partials;     // 3 per block: errup, errlo, merit numerator
   unsigned long long *failkey;
   int *failcount;
@@ -268,8 +270,11 @@ __device__ __forceinline__ void sweep_slice(const SweepArgs<T> &a, int s, bool v
       if (g == 0)
       {
         up += q0*C[0] + q1*C[1] + q2*C[2];
+        if (a.store_Um)
+        {
 #pragma unroll
-        for (int k = 0; k < 3; k ++) a.Um[(size_t)k*n + p] = U[k];
+          for (int k = 0; k < 3; k ++) a.Um[(size_t)k*n + p] = U[k];
+        }
       }
     }
 
@@ -360,9 +365,10 @@ __device__ __forceinline__ void merit_slice(const MeritArgs &a, int s, bool vali
       const int j = cb[(size_t)m*SLICE];
       const double *w = Wb + 9*(size_t)m*SLICE;
       const double r0 = a.R[4*(size_t)j], r1 = a.R[4*(size_t)j + 1], r2 = a.R[4*(size_t)j + 2];
-      u0 += w[0]*r0 + w[3*SLICE]*r1 + w[6*SLICE]*r2;
-      u1 += w[SLICE]*r0 + w[4*SLICE]*r1 + w[7*SLICE]*r2;
-      u2 += w[2*SLICE]*r0 + w[5*SLICE]*r1 + w[8*SLICE]*r2;
+      // the sweep's multiply-adds (sweep_slice), so that U equals the fused merit pass's
+      u0 = u0 + fma(w[6*SLICE], r2, fma(w[3*SLICE], r1, w[0]*r0));
+      u1 = u1 + fma(w[7*SLICE], r2, fma(w[4*SLICE], r1, w[SLICE]*r0));
+      u2 = u2 + fma(w[8*SLICE], r2, fma(w[5*SLICE], r1, w[2*SLICE]*r0));
     }
   }
   u0 = gsum<G>(u0); u1 = gsum<G>(u1); u2 = gsum<G>(u2);

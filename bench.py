@@ -497,8 +497,9 @@ def main():
     ap.add_argument("--modes", default="fp64,mixed,learned",
                     help="solve modes to time (comma list of fp64, mixed, learned = learned warm start + mixed)")
     ap.add_argument("--maxiter", type=int, default=5000, help="iteration bound (5000 is the workload's)")
-    ap.add_argument("--row-order", type=int, default=2, choices=[0, 1, 2],
-                    help="rows inside a colour: 0 by decreasing length, 1 by contact index, 2 by body block")
+    ap.add_argument("--row-order", type=int, default=3, choices=[0, 1, 2, 3],
+                    help="rows inside a colour: 0 by decreasing length, 1 by contact index, 2 by body block, "
+                         "3 by the Z-order of the master body's position")
     ap.add_argument("--schedule", type=int, default=0, choices=[0, 1, 2, 3],
                     help="sweep schedule: 0 one launch per colour (XCD-aware), 1 persistent dataflow, "
                          "2 one launch per colour (hardware order), 3 persistent with a barrier per colour")
@@ -696,7 +697,8 @@ def main():
             "waves_per_workgroup": args.wpb,
             "schedule": ["per-colour launches, XCD-aware", "dataflow", "per-colour launches",
                          "persistent, barrier per colour"][args.schedule],
-            "row_order": ["decreasing row length", "contact index", "body block, row length"][args.row_order],
+            "row_order": ["decreasing row length", "contact index", "body block, row length",
+                          "Z-order of the master body's position"][args.row_order],
             "modes": modes,
             "speedup_vs_plain_fp64": modes["fp64"]["ms_per_step"] / ms_per_step if "fp64" in modes else None,
             "merit_1e-6_reached": {k: any(f is not None for f in m["first_iter_merit_le_1e6"]) for k, m in modes.items()},

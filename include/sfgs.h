@@ -189,12 +189,14 @@ int SFGS_Set_Schedule(sfgs_ctx *ctx, int schedule);
 int SFGS_Set_Sweep_Block(sfgs_ctx *ctx, int waves_per_block);
 
 /* Order of the rows inside a colour (no counterpart in the reference): 0 =
- * decreasing row length (least slice padding); 1 = contact index; 2 (default)
- * = master body in blocks of 64 bodies, then decreasing row length, which
- * keeps the rows of nearby bodies together (their neighbours' reactions then
- * share cache lines) while slices stay of even length (needs a system
- * assembled from bodies; otherwise 0).  Takes effect at the next assembly; the
- * iterates do not depend on it (rows of a colour are independent). */
+ * decreasing row length (least slice padding); 1 = contact index; 2 = master
+ * body in blocks of 64 bodies, then decreasing row length; 3 (default) = the
+ * Z-order (Morton code on a 1024^3 grid over the bodies' bounding box) of the
+ * master body's position, which keeps the rows of bodies close in space
+ * together, so that their neighbours' reactions share cache lines.  Orders 2
+ * and 3 need a system assembled from bodies (otherwise 0).  Takes effect at the
+ * next assembly; the iterates do not depend on it (rows of a colour are
+ * independent). */
 int SFGS_Set_Row_Order(sfgs_ctx *ctx, int order);
 
 /* Colour an arbitrary graph given in CSR (no self loops) with the device

@@ -88,8 +88,10 @@ struct sfgs_ctx {
   int wpb = 4;                           // wavefronts per workgroup of the per-colour sweep launches (1, 2, 4)
   int schedule = 0;                      // sweeps: 0 one launch per colour (XCD-aware), 1 persistent dataflow,
                                          // 2 one launch per colour (hardware order), 3 persistent with barriers
-  int row_order = 2;                     // inside a colour: 0 by decreasing row length, 1 by contact index,
-                                         // 2 by master body block then decreasing row length
+  int row_order = 3;                     // inside a colour: 0 by decreasing row length, 1 by contact index,
+                                         // 2 by master body block then decreasing row length, 3 by the
+                                         // Z-order of the master body's position
+  sfgs::DBuf<unsigned long long> bbox;   // row order 3: bounding box of the bodies (order keys)
   sfgs::DBuf<unsigned char> flow_mem;    // dataflow schedule: third R buffer, U pairs, versions, partials
   int G_built = 1;                       // layout of the current store
   sfgs::DBuf<unsigned char> sep;

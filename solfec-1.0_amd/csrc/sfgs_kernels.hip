@@ -239,19 +239,81 @@ __global__ void k_fill_int(int n, int v, int *b)
   if (i < n) b[i] = v;
 }
 
+// monotone map of a double onto unsigned 64-bit integers (a < b <=> key(a) < key(b))
+__device__ __forceinline__ unsigned long long order_key(double d)
+{
+  const unsigned long long b = (unsigned long long)__double_as_longlong(d);
+  return (b >> 63) ? ~b : b | 0x8000000000000000ULL;
+}
+
+__device__ __forceinline__ double order_val(unsigned long long k)
+{
+  return __longlong_as_double((long long)((k >> 63) ? k & 0x7fffffffffffffffULL : ~k));
+}
+
+__global__ void k_bbox_init(unsigned long long *bb)
+{
+  if (threadIdx.x < 6) bb[threadIdx.x] = threadIdx.x < 3 ? ~0ULL : 0ULL;
+}
+
+// bounding box of the positions of the bodies that take part in the dynamics
+// (kind 0), exact and order-independent: bb[0..2] minima, bb[3..5] maxima of
+// order_key (k_bbox_init first)
+__global__ void k_bbox(int nb, const int *kind, const double *x, unsigned long long *bb)
+{
+  int b = blockIdx.x*blockDim.x + threadIdx.x;
+  if (b >= nb || kind[b] != 0) return;
+#pragma unroll
+  for (int k = 0; k < 3; k ++)
+  {
+    const unsigned long long v = order_key(x[3*(size_t)b + k]);
+    atomicMin(bb + k, v);
+    atomicMax(bb + 3 + k, v);
+  }
+}
+
+// the 10 low bits of v spread to every third bit (Morton code of one axis)
+__device__ __forceinline__ unsigned spread3(unsigned v)
+{
+  unsigned x = v & 0x3ffu;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
 // sort key of row i, by order mode: 0 (scene, colour, -row length); 1 (scene,
 // colour), the stable sort keeping contact order inside a colour; 2 (scene,
 // colour, master body / 64, -row length), which keeps rows of nearby bodies
-// together while slices stay of even length.  scene = 0 without scenes
+// together while slices stay of even length; 3 (scene, colour, Z-order of the
+// master body's position on a 1024^3 grid over the bounding box bb of the
+// bodies), which keeps rows of bodies that are close in space together.
+// scene = 0 without scenes
 __global__ void k_sort_keys(int n, const int *colour, const int *rowptr, const int *scene, int mode,
-                            const int *master, unsigned long long *keys, int *vals)
+                            const int *master, const double *x, const unsigned long long *bb,
+                            unsigned long long *keys, int *vals)
 {
   int i = blockIdx.x*blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int len = rowptr[i + 1] - rowptr[i];
   const unsigned long long sc = scene ? (unsigned long long)(unsigned)scene[i] : 0ULL;
   const unsigned long long col = (unsigned long long)(unsigned)colour[i];
-  if (mode == 2)
+  if (mode == 3)
+  {
+    const double *p = x + 3*(size_t)master[i];
+    unsigned q[3];
+#pragma unroll
+    for (int k = 0; k < 3; k ++)
+    {
+      const double lo = order_val(bb[k]), hi = order_val(bb[3 + k]);
+      const double t = hi > lo ? (p[k] - lo) / (hi - lo) * 1024.0 : 0.0;
+      q[k] = (unsigned)fmin(fmax(t, 0.0), 1023.0);
+    }
+    const unsigned long long z = (unsigned long long)(spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2));
+    keys[i] = (sc << 40) | (col << 32) | z;
+  }
+  else if (mode == 2)
   {
     const unsigned long long blk = (unsigned long long)((unsigned)master[i] >> 6) & 0xffffffULL;
     keys[i] = (sc << 40) | (col << 32) | (blk << 8) | (0xffu - (unsigned)min(len, 0xff));
@@ -1102,9 +1164,17 @@ static int build_order(sfgs_ctx *ctx, int n, bool have_master)
   CHK(ensure(ctx, ctx->svals, n));
   CHK(ensure(ctx, ctx->perm, n));
   CHK(ensure(ctx, ctx->iperm, n));
-  // order 2 needs the master bodies of the contacts (a system uploaded as W has none)
-  const int mode = ctx->row_order == 2 && !have_master ? 0 : ctx->row_order;
-  int keybits = mode == 2 ? 40 : 32;
+  // orders 2 and 3 need the master bodies of the contacts (a system uploaded as W has none)
+  const int mode = ctx->row_order >= 2 && !have_master ? 0 : ctx->row_order;
+  int keybits = mode >= 2 ? 40 : 32;
+  if (mode == 3)
+  {
+    CHK(ensure(ctx, ctx->bbox, 6));
+    hipLaunchKernelGGL(k_bbox_init, dim3(1), dim3(64), 0, ctx->stream, ctx->bbox.p);
+    if (ctx->nb > 0)
+      hipLaunchKernelGGL(k_bbox, dim3(nblk(ctx->nb)), dim3(TPB), 0, ctx->stream, ctx->nb, ctx->kind.p, ctx->x0.p,
+                         ctx->bbox.p);
+  }
   if (scenes)
   {
     for (int v = ctx->nscenes - 1; v > 0; v >>= 1) keybits ++;
@@ -1122,8 +1192,8 @@ static int build_order(sfgs_ctx *ctx, int n, bool have_master)
   if (n > 0)
   {
     hipLaunchKernelGGL(k_sort_keys, dim3(nblk(n)), dim3(TPB), 0, ctx->stream, n, ctx->colour.p, ctx->rowptr.p,
-                       scenes ? ctx->row_scene.p : (const int*)nullptr, mode, ctx->master.p, ctx->skeys.p,
-                       ctx->svals.p);
+                       scenes ? ctx->row_scene.p : (const int*)nullptr, mode, ctx->master.p, ctx->x0.p,
+                       mode == 3 ? ctx->bbox.p : (const unsigned long long *)nullptr, ctx->skeys.p, ctx->svals.p);
     size_t bytes = 0;
     HIPCHK(sfgs::sort_pairs_u64(nullptr, bytes, ctx->skeys.p, ctx->skeys2.p, ctx->svals.p, ctx->perm.p, n, keybits, ctx->stream));
     CHK(ensure(ctx, ctx->scratch, bytes));
@@ -1342,7 +1412,7 @@ void SFGS_Destroy(sfgs_ctx *ctx)
   release(ctx->bptr); release(ctx->blist); release(ctx->bkeys); release(ctx->bkeys2); release(ctx->bvals); release(ctx->bvals2);
   release(ctx->rowptr); release(ctx->rowlen_n); release(ctx->col); release(ctx->via);
   release(ctx->colour); release(ctx->colour2); release(ctx->perm); release(ctx->iperm); release(ctx->counter);
-  release(ctx->skeys); release(ctx->skeys2); release(ctx->svals); release(ctx->svals2); release(ctx->scratch);
+  release(ctx->skeys); release(ctx->skeys2); release(ctx->svals); release(ctx->svals2); release(ctx->scratch); release(ctx->bbox);
   release(ctx->slice_row0); release(ctx->slice_nrows); release(ctx->slice_len); release(ctx->slice_off);
   release(ctx->rowlen); release(ctx->W); release(ctx->Wd); release(ctx->A); release(ctx->B); release(ctx->V);
   release(ctx->rho); release(ctx->pfri); release(ctx->pcoh); release(ctx->shift); release(ctx->R); release(ctx->U); release(ctx->pen); sfgs_newton_release(ctx); sfgs_bar_release(ctx);
@@ -2079,7 +2149,7 @@ int SFGS_Set_Layout(sfgs_ctx *ctx, int lanes_per_row)
 
 int SFGS_Set_Row_Order(sfgs_ctx *ctx, int order)
 {
-  if (!ctx || order < 0 || order > 2) return SFGS_E_ARG;
+  if (!ctx || order < 0 || order > 3) return SFGS_E_ARG;
   ctx->row_order = order;
   return 0;
 }

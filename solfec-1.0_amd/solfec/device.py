@@ -82,7 +82,8 @@ class Device:
         self._check(self.lib.SFGS_Set_Layout(self.ctx, int(lanes_per_row)), "SFGS_Set_Layout")
 
     def set_row_order(self, order):
-        """0: rows of a colour by decreasing length; 1: by contact index (spatial order)."""
+        """SFGS_Set_Row_Order: rows of a colour by 0 decreasing length, 1 contact index, 2 master body
+        block, 3 (default) Z-order of the master body's position."""
         self._check(self.lib.SFGS_Set_Row_Order(self.ctx, int(order)), "SFGS_Set_Row_Order")
 
     def set_sweep_block(self, waves_per_block):

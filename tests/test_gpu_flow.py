@@ -114,8 +114,9 @@ def test_flow_matches_oracle(dev):
 
 @pytest.mark.parametrize("lanes", [1, 2])
 def test_row_order_and_block_mapping_same_iterates(dev, lanes):
-    """Rows of a colour are independent: ordering them by contact index or by body
-    block instead of by length, mapping workgroups to XCDs differently, or
+    """Rows of a colour are independent: ordering them by contact index, by body
+    block or by the Z-order of the master body instead of by length, mapping
+    workgroups to XCDs differently, or
     grouping 1, 2 or 4 slices per workgroup changes only the layout, not the
     reactions."""
     s = lattice(1000, seed=2)
@@ -123,7 +124,7 @@ def test_row_order_and_block_mapping_same_iterates(dev, lanes):
     kw = dict(epsilon=1e-6, maxiter=3000, meritval=1e-6)
     out = []
     dev.set_layout(lanes)
-    for order, sched, wpb in ((0, 0, 4), (1, 0, 4), (2, 0, 4), (2, 2, 4), (2, 0, 1), (2, 0, 2)):
+    for order, sched, wpb in ((0, 0, 4), (1, 0, 4), (2, 0, 4), (3, 0, 4), (3, 2, 4), (3, 0, 1), (3, 0, 2)):
         dev.set_row_order(order)
         dev.set_schedule(sched)
         dev.set_sweep_block(wpb)

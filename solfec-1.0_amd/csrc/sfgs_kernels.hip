@@ -259,16 +259,41 @@ __global__ void k_bbox_init(unsigned long long *bb)
 // bounding box of the positions of the bodies that take part in the dynamics
 // (kind 0), exact and order-independent: bb[0..2] minima, bb[3..5] maxima of
 // order_key (k_bbox_init first)
-__global__ void k_bbox(int nb, const int *kind, const double *x, unsigned long long *bb)
+__global__ __launch_bounds__(TPB) void k_bbox(int nb, const int *kind, const double *x, unsigned long long *bb)
 {
   int b = blockIdx.x*blockDim.x + threadIdx.x;
-  if (b >= nb || kind[b] != 0) return;
-#pragma unroll
-  for (int k = 0; k < 3; k ++)
+  unsigned long long lo[3] = {~0ULL, ~0ULL, ~0ULL}, hi[3] = {0ULL, 0ULL, 0ULL};
+  if (b < nb && kind[b] == 0)
   {
-    const unsigned long long v = order_key(x[3*(size_t)b + k]);
-    atomicMin(bb + k, v);
-    atomicMax(bb + 3 + k, v);
+#pragma unroll
+    for (int k = 0; k < 3; k ++) lo[k] = hi[k] = order_key(x[3*(size_t)b + k]);
+  }
+  // wavefront, then workgroup minima and maxima; one atomic per value and workgroup
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+  {
+#pragma unroll
+    for (int k = 0; k < 3; k ++)
+    {
+      lo[k] = min(lo[k], (unsigned long long)__shfl_xor(lo[k], o, 64));
+      hi[k] = max(hi[k], (unsigned long long)__shfl_xor(hi[k], o, 64));
+    }
+  }
+  __shared__ unsigned long long sh[TPB/64][6];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+  {
+#pragma unroll
+    for (int k = 0; k < 3; k ++) { sh[w][k] = lo[k]; sh[w][3 + k] = hi[k]; }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6)
+  {
+    const int k = threadIdx.x;
+    unsigned long long v = sh[0][k];
+    for (int q = 1; q < TPB/64; q ++) v = k < 3 ? min(v, sh[q][k]) : max(v, sh[q][k]);
+    if (k < 3) { if (v != ~0ULL) atomicMin(bb + k, v); }
+    else if (v != 0ULL) atomicMax(bb + k, v);
   }
 }
 

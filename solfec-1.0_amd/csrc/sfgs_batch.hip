@@ -44,6 +44,7 @@ struct BatchOut {
   int iters, nref, sweeps, done, aborted, failcount, timeout;
   int final_buf, final_U;   // per-colour schedule: buffer of the final R (0, 1), final U (0 U, 1 Um)
   int iters0;               // per-colour MIXED: iterations of the FP32 phase
+  int sweeps0;              // per-colour MIXED: sweeps of the FP32 phase
   unsigned long long failkey;
   double error, merit, energy, prev_error;
 };
@@ -433,6 +434,7 @@ __global__ void k_uphase(int nscenes, const UScene *sc, int maxiter, BatchOut *o
   if (q >= nscenes) return;
   BatchOut *o = out + q;
   o->iters0 = o->iters;
+  o->sweeps0 = o->sweeps;
   o->sweeps = 0;
   o->prev_error = 0.0;
   o->final_buf = 0;
@@ -803,7 +805,7 @@ static int batch_solve(sfgs_ctx *ctx, const SFGS_PARAM *par, const SFGS_BATCH_TU
       if (rows[q] > 0 && !b.aborted && b.iters - b.nref >= par->maxiter) r.error_code = SFGS_GS_DIVERGED;
     }
     maxit = std::max(maxit, b.iters);
-    updates += (long long)b.sweeps * rows[q];
+    updates += (long long)(b.sweeps0 + b.sweeps) * rows[q];   // both phases of MIXED
   }
   if (total)
   {

@@ -98,6 +98,12 @@ def test_batch_matches_individual_oracle(dev, precision):
     if precision == 1:
         assert checked >= 4
     assert tot.iters == reps["iters"].max()
+    # row updates count every sweep of both phases: an iteration is one sweep, and each
+    # phase adds the sweep that closes its last iteration (two phases in MIXED)
+    lo = int(np.sum(reps["iters"].astype(np.int64) * reps["rows"]))
+    hi = int(np.sum((reps["iters"].astype(np.int64) + 1 + precision) * reps["rows"]))
+    print(f"[batch] row updates {tot.row_updates}, iterations x rows {lo}", flush=True)
+    assert lo <= tot.row_updates <= hi
 
 
 def test_batch_workgroups_do_not_change_result(dev):

@@ -503,8 +503,8 @@ def main():
     ap.add_argument("--schedule", type=int, default=0, choices=[0, 1, 2, 3],
                     help="sweep schedule: 0 one launch per colour (XCD-aware), 1 persistent dataflow, "
                          "2 one launch per colour (hardware order), 3 persistent with a barrier per colour")
-    ap.add_argument("--wpb", type=int, default=4, choices=[1, 2, 4],
-                    help="wavefronts per workgroup of the per-colour sweep launches")
+    ap.add_argument("--wpb", type=int, default=0, choices=[0, 1, 2, 4],
+                    help="wavefronts per workgroup of the per-colour sweep launches (0: 2 for FP64, 4 for mixed)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (scale of the traffic counters) and exit")
     ap.add_argument("--shape", default="sphere", choices=["sphere", "ellipsoid"],
@@ -694,7 +694,7 @@ def main():
             "warm_start": ("learned (%s, trust %.2g)" % ("new contacts" if args.ws_new_only else "all contacts", args.ws_scale)
                            if head == "learned" else "previous-step R"),
             "lanes_per_row": args.lanes,
-            "waves_per_workgroup": args.wpb,
+            "waves_per_workgroup": args.wpb if args.wpb else "2 for FP64 solves, 4 for mixed",
             "schedule": ["per-colour launches, XCD-aware", "dataflow", "per-colour launches",
                          "persistent, barrier per colour"][args.schedule],
             "row_order": ["decreasing row length", "contact index", "body block, row length",

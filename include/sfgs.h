@@ -183,9 +183,10 @@ int SFGS_Set_Layout(sfgs_ctx *ctx, int lanes_per_row);
  * meet at a device-wide barrier after every colour, with the same iterates
  * (1 or 2 lanes per row; otherwise schedule 0). */
 int SFGS_Set_Schedule(sfgs_ctx *ctx, int schedule);
-/* Wavefronts per workgroup of the per-colour sweep launches (1, 2 or 4,
- * default 4; one slice of 64 lanes per wavefront); no counterpart in the
- * reference.  Same iterates for every value. */
+/* Wavefronts per workgroup of the per-colour sweep launches (1, 2 or 4; one
+ * slice of 64 lanes per wavefront), or 0 (default): 2 for FP64 solves and 4
+ * for MIXED, the faster of each at the 200k-sphere packing.  No counterpart
+ * in the reference.  Same iterates for every value. */
 int SFGS_Set_Sweep_Block(sfgs_ctx *ctx, int waves_per_block);
 
 /* Order of the rows inside a colour (no counterpart in the reference): 0 =

@@ -85,7 +85,9 @@ struct sfgs_ctx {
   sfgs::DBuf<int> rowlen;                // per permuted row
   sfgs::DBuf<double> W, Wd, A, B, V, rho, pfri, pcoh, shift, R, U, Rb, Um;
   int G = 2;                             // lanes per row of the sweep layout (1, 2, 4, 8)
-  int wpb = 4;                           // wavefronts per workgroup of the per-colour sweep launches (1, 2, 4)
+  int wpb = 0;                           // wavefronts per workgroup of the per-colour sweep launches (1, 2, 4;
+                                         // 0: by precision, 2 for FP64 solves and 4 for MIXED)
+  int wpb_run = 4;                       // the value of the current solve
   int schedule = 0;                      // sweeps: 0 one launch per colour (XCD-aware), 1 persistent dataflow,
                                          // 2 one launch per colour (hardware order), 3 persistent with barriers
   int row_order = 3;                     // inside a colour: 0 by decreasing row length, 1 by contact index,

@@ -1696,7 +1696,7 @@ int SFGS_System_Upload(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, 
 // (then k_finish takes it)
 // one colour launch of k_sweep with the context's lanes per row and wavefronts per workgroup
 #define SFGS_SWEEP_WPB(MER, GG) \
-  switch (ctx->wpb) { \
+  switch (ctx->wpb_run) { \
   case 1: hipLaunchKernelGGL((k_sweep<T, GG, MER, 1>), dim3(grid[c]), dim3(64), 0, ctx->stream, a, ctx->h_cslice[c], \
                              ctx->h_cslice[c + 1], pbase[c], lo, hi, backward, xcd, f, np, hist); break; \
   case 2: hipLaunchKernelGGL((k_sweep<T, GG, MER, 2>), dim3(grid[c]), dim3(128), 0, ctx->stream, a, ctx->h_cslice[c], \
@@ -1866,13 +1866,16 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
   // the barrier schedule: one persistent launch, a device-wide barrier per colour
   const bool bar = ctx->schedule == 3 && n > 0 && ctx->G_built <= 2;
 
+  // wavefronts per workgroup: the FP64 sweeps (239 VGPRs, two wavefronts per
+  // SIMD) run faster in workgroups of 2, the FP32 ones in workgroups of 4
+  ctx->wpb_run = ctx->wpb ? ctx->wpb : (mixed ? 4 : 2);
   GsPlan pl;
   pl.grid.assign((size_t)ncol, 0);
   pl.pbase.assign((size_t)ncol + 1, 0);
   for (int c = 0; c < ncol; c ++)
   {
     int ns = ctx->h_cslice[c + 1] - ctx->h_cslice[c];
-    pl.grid[c] = (ns + ctx->wpb - 1) / ctx->wpb;
+    pl.grid[c] = (ns + ctx->wpb_run - 1) / ctx->wpb_run;
     pl.pbase[c + 1] = pl.pbase[c] + pl.grid[c];
   }
   pl.np = pl.pbase[ncol];
@@ -2160,7 +2163,8 @@ int SFGS_Set_Schedule(sfgs_ctx *ctx, int schedule)
 
 int SFGS_Set_Sweep_Block(sfgs_ctx *ctx, int waves_per_block)
 {
-  if (!ctx || (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)) return SFGS_E_ARG;
+  if (!ctx || (waves_per_block != 0 && waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4))
+    return SFGS_E_ARG;
   ctx->wpb = waves_per_block;
   return 0;
 }

@@ -87,7 +87,8 @@ class Device:
         self._check(self.lib.SFGS_Set_Row_Order(self.ctx, int(order)), "SFGS_Set_Row_Order")
 
     def set_sweep_block(self, waves_per_block):
-        """SFGS_Set_Sweep_Block: wavefronts per workgroup of the per-colour sweep launches (1, 2, 4)."""
+        """SFGS_Set_Sweep_Block: wavefronts per workgroup of the per-colour sweep launches (1, 2, 4;
+        0, the default: 2 for FP64 solves, 4 for MIXED)."""
         self._check(self.lib.SFGS_Set_Sweep_Block(self.ctx, int(waves_per_block)), "SFGS_Set_Sweep_Block")
 
     def set_schedule(self, schedule):

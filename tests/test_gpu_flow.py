@@ -124,7 +124,8 @@ def test_row_order_and_block_mapping_same_iterates(dev, lanes):
     kw = dict(epsilon=1e-6, maxiter=3000, meritval=1e-6)
     out = []
     dev.set_layout(lanes)
-    for order, sched, wpb in ((0, 0, 4), (1, 0, 4), (2, 0, 4), (3, 0, 4), (3, 2, 4), (3, 0, 1), (3, 0, 2)):
+    for order, sched, wpb in ((0, 0, 4), (1, 0, 4), (2, 0, 4), (3, 0, 4), (3, 2, 4), (3, 0, 1), (3, 0, 2),
+                             (3, 0, 0)):
         dev.set_row_order(order)
         dev.set_schedule(sched)
         dev.set_sweep_block(wpb)
@@ -137,7 +138,7 @@ def test_row_order_and_block_mapping_same_iterates(dev, lanes):
         out.append((rep, R, U, dev.locdyn()["colour"]))
     dev.set_row_order(0)
     dev.set_schedule(0)
-    dev.set_sweep_block(4)
+    dev.set_sweep_block(0)
     dev.set_layout(1)
     r0, R0, U0, c0 = out[0]
     for r1, R1, U1, c1 in out[1:]:

@@ -594,7 +594,8 @@ def main():
                    converged=[bool(r.iters < args.maxiter) for r in reps],
                    merit_at=dict((str(k), float(mers[-1][k - 1])) for k in (100, 1000, 2000, 5000)
                                  if k <= len(mers[-1])),
-                   device_ms=[float(r.solve_ms) for r in reps], sweeps=sweeps, fp64_sweeps=fp64)
+                   device_ms=[float(r.solve_ms) for r in reps], sweeps=sweeps, fp64_sweeps=fp64,
+                   launches_per_sweep=int(reps[-1].launches_per_sweep))
         res["merhist_last"] = mers[-1]
         if ref_merit is not None:
             iso_merit(res, ref_merit)
@@ -640,19 +641,24 @@ def main():
     # bytes of every sweep of every timed solve (FP32 sweeps at s = 4, FP64 at
     # s = 8) over the solves' device time (HIP events on the library's stream
     # around each solve, so the termination kernels and launch gaps count
-    # against the sweeps); per launch, the same device time over the launches
+    # against the sweeps); per colour and per launch, the same device time over
+    # the colours and over the launches (the trailing colours of a few slices
+    # each share one launch)
     is32 = head in ("mixed", "learned")
     hm = modes[head]
     b32 = 4 * (9 * nnz + 26 * n) + 4 * nnz + 4 * n
     b64 = 8 * (9 * nnz + 26 * n) + 4 * nnz + 4 * n
     nbytes = sum((sw - f) * (b32 if is32 else b64) + f * b64 for sw, f in zip(hm["sweeps"], hm["fp64_sweeps"]))
     dev_s = 1e-3 * sum(hm["device_ms"])
-    launches = sum(hm["sweeps"]) * ncol
+    colour_steps = sum(hm["sweeps"]) * ncol
+    launches = sum(hm["sweeps"]) * (hm["launches_per_sweep"] or ncol)
     achieved = nbytes / dev_s / 1e9
     kname = ({1: "k_flow<%s>", 3: "k_bar<%s>"}.get(args.schedule, "k_sweep<%s>")) % ("float" if is32 else "double")
     roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, args),
-                kernel=kname, avg_launch_us=round(1e6 * dev_s / max(launches, 1), 3), launches=int(launches),
+                kernel=kname, avg_colour_us=round(1e6 * dev_s / max(colour_steps, 1), 3),
+                avg_launch_us=round(1e6 * dev_s / max(launches, 1), 3), launches=int(launches),
+                launches_per_sweep=int(hm["launches_per_sweep"] or ncol),
                 bytes_per_sweep=int(b32 if is32 else b64),
                 how="timed steps: algorithmic bytes of all sweeps / device time of the solves (HIP events)")
     log(f"[bench] sweeps of the timed {head} steps: {launches} launches, {1e3 * dev_s:.2f} ms device, "

@@ -83,6 +83,8 @@ typedef struct {
   long long row_updates; /* rows visited = sweeps x rows */
   long long nnz;       /* off-diagonal blocks */
   int fp64_sweeps;     /* MIXED: sweeps of the FP64 phase (the rest of row_updates / n are FP32) */
+  int launches_per_sweep; /* sweep-kernel launches per sweep of the colour-launch schedules (the
+                             trailing colours of a few slices each share one launch) */
 } SFGS_REPORT;
 
 /* context: LOCDYN_Create, ldy.h:93 (one device-resident W/B/R store) */

@@ -660,6 +660,50 @@ __global__ __launch_bounds__(WPB*64) void k_sweep(SweepArgs<T> a, int s0, int s1
   }
 }
 
+// Several consecutive colours of a sweep, each of at most a few slices, in one
+// launch of one workgroup: colour after colour in sweep order, wavefront w
+// taking slices w, w + NW, ... of each, with a workgroup barrier between
+// colours.  The rows of a colour then read the reactions the colour before
+// wrote through the workgroup's own L1, which the barrier's workgroup-scope
+// fence makes visible: the same iterates as one launch per colour, one launch
+// fewer per colour merged.  One partial (3 sums) for the group; fin as in
+// k_sweep (the group's block is the launch's last).
+struct ColourGroup {
+  static const int MAX = 8;
+  int n;
+  int s0[MAX], s1[MAX], lo[MAX], hi[MAX];
+};
+
+template <typename T, int G, bool MERIT, int NW>
+__global__ __launch_bounds__(NW*64) void k_sweep_group(SweepArgs<T> a, ColourGroup cg, int pbase, int backward,
+                                                        int *fin, int np, double *hist)
+{
+  const int done = a.ctrl->done;
+  const int w = threadIdx.x >> 6;
+  double eu = 0.0, el = 0.0, up = 0.0;
+  for (int q = 0; q < cg.n; q ++)
+  {
+    for (int s = cg.s0[q] + w; s < cg.s1[q]; s += NW)
+      sweep_slice<T, G, MERIT>(a, s, true, cg.lo[q], cg.hi[q], backward, &a.ctrl->sweeps, a.failkey, a.failcount,
+                               &a.ctrl->aborted, eu, el, up, done == 0);
+    __syncthreads();   // this colour's reactions before the next colour's gathers
+  }
+  if (done) return;
+  block_sum3<NW*64>(eu, el, up);
+  if (threadIdx.x == 0)
+  {
+    double *pt = a.partials + 3*(size_t)pbase;
+    pt[0] = eu; pt[1] = el; pt[2] = up;
+  }
+  if (fin)
+  {
+    __syncthreads();
+    if (threadIdx.x == 0) __threadfence();   // the partial before the sums
+    __syncthreads();
+    finish_sweep<NW*64>(a.ctrl, a.partials, np, hist);
+  }
+}
+
 template <int G>
 __global__ __launch_bounds__(TPB) void k_merit(MeritArgs a, int nslices)
 {
@@ -1710,29 +1754,74 @@ int SFGS_System_Upload(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, 
   case 8: SFGS_SWEEP_WPB(MER, 8) break; \
   default: SFGS_SWEEP_WPB(MER, 4) break; }
 
+// the trailing colours merged into one k_sweep_group launch: wavefronts of its
+// workgroup, and the most slices a colour may have to join it (one slice per
+// wavefront)
+static const int GROUP_NW = 8;
+
+#define SFGS_GROUP_LAUNCH(MER, GG) \
+  hipLaunchKernelGGL((k_sweep_group<T, GG, MER, GROUP_NW>), dim3(1), dim3(GROUP_NW*64), 0, ctx->stream, a, cg, \
+                     gslot, backward, f, np, hist);
+#define SFGS_DISPATCH_GROUP(MER) \
+  switch (ctx->G_built) { \
+  case 1: SFGS_GROUP_LAUNCH(MER, 1) break; \
+  case 2: SFGS_GROUP_LAUNCH(MER, 2) break; \
+  case 8: SFGS_GROUP_LAUNCH(MER, 8) break; \
+  default: SFGS_GROUP_LAUNCH(MER, 4) break; }
+
+// one sweep: a launch per colour c < gc in sweep order, and the colours
+// gc .. ncol-1 (at most GROUP_NW slices each) in one k_sweep_group launch,
+// last in a forward sweep and first in a backward one; gslot is the group's
+// partial.  The last launch takes the termination test (fin)
 template <typename T>
 static bool launch_sweep(sfgs_ctx *ctx, const SweepArgs<T> &base, T *X0, T *X1, int k, int backward, bool merit,
-                         const std::vector<int> &grid, const std::vector<int> &pbase, bool prof, int *fin, int np,
-                         double *hist)
+                         const std::vector<int> &grid, const std::vector<int> &pbase, int gc, int gslot, bool prof,
+                         int *fin, int np, double *hist)
 {
   const int ncol = ctx->ncolours, n = ctx->nc;
   const int xcd = ctx->schedule == 2 ? 0 : 1;
   SweepArgs<T> a = base;
   if (k & 1) { a.P = X1; a.Q = X0; }
   else { a.P = X0; a.Q = X1; }
-  int lastc = -1;
+  // launch units in sweep order: a colour, or -1 for the group
+  int units[SFGS_MAX_COLOURS + 1], nu = 0;
+  bool grouped = false;
   for (int ci = 0; ci < ncol; ci ++)
   {
     int c = backward ? ncol - 1 - ci : ci;
-    if (grid[c] > 0) lastc = c;
+    if (c >= gc) { if (!grouped) { units[nu ++] = -1; grouped = true; } continue; }
+    if (grid[c] > 0) units[nu ++] = c;
   }
-  for (int ci = 0; ci < ncol; ci ++)
+  for (int u = 0; u < nu; u ++)
   {
-    int c = backward ? ncol - 1 - ci : ci;
-    if (grid[c] == 0) continue;
+    int *f = u == nu - 1 ? fin : nullptr;
+    const int c = units[u];
+    if (c < 0)
+    {
+      ColourGroup cg;
+      cg.n = 0;
+      for (int ci = 0; ci < ncol; ci ++)
+      {
+        int cc = backward ? ncol - 1 - ci : ci;
+        if (cc < gc || ctx->h_cslice[cc + 1] == ctx->h_cslice[cc]) continue;
+        cg.s0[cg.n] = ctx->h_cslice[cc];
+        cg.s1[cg.n] = ctx->h_cslice[cc + 1];
+        cg.lo[cg.n] = backward ? ctx->h_colptr[cc + 1] : 0;
+        cg.hi[cg.n] = backward ? n : ctx->h_colptr[cc];
+        cg.n ++;
+      }
+      if (merit)
+      {
+        SFGS_DISPATCH_GROUP(true);
+      }
+      else
+      {
+        SFGS_DISPATCH_GROUP(false);
+      }
+      continue;
+    }
     int lo = backward ? ctx->h_colptr[c + 1] : 0, hi = backward ? n : ctx->h_colptr[c];
-    int *f = c == lastc ? fin : nullptr;
-    if (prof) (void)hipEventRecord(ctx->prof_ev[2*ci], ctx->stream);
+    if (prof) (void)hipEventRecord(ctx->prof_ev[2*c], ctx->stream);
     if (merit)
     {
       SFGS_DISPATCH_SWEEP(true);
@@ -1741,14 +1830,15 @@ static bool launch_sweep(sfgs_ctx *ctx, const SweepArgs<T> &base, T *X0, T *X1, 
     {
       SFGS_DISPATCH_SWEEP(false);
     }
-    if (prof) (void)hipEventRecord(ctx->prof_ev[2*ci + 1], ctx->stream);
+    if (prof) (void)hipEventRecord(ctx->prof_ev[2*c + 1], ctx->stream);
   }
-  return lastc >= 0;
+  return nu > 0;
 }
 
 struct GsPlan {
   std::vector<int> grid, pbase;
   int np = 0;
+  int gc = 0, gslot = 0;   // first colour of the merged trailing group (ncolours: none), its partial
   bool prof = false;
   double sweep_ms = 0.0;
   long long launches = 0;
@@ -1778,8 +1868,8 @@ static int run_gs(sfgs_ctx *ctx, SweepArgs<T> sa, T *X0, T *X1, int maxiter, int
     for (int j = 0; j < nbt; j ++)
     {
       int kk = k + j;
-      if (!launch_sweep<T>(ctx, sa, X0, X1, kk, reverse && (kk % 2), !nomerit && kk > 0, pl.grid, pl.pbase, pl.prof,
-                           ctx->fin.p, pl.np, hist))
+      if (!launch_sweep<T>(ctx, sa, X0, X1, kk, reverse && (kk % 2), !nomerit && kk > 0, pl.grid, pl.pbase, pl.gc,
+                           pl.gslot, pl.prof, ctx->fin.p, pl.np, hist))
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(TPB), 0, st, ctx->ctrl.p, ctx->partials.p, pl.np, hist);
     }
     k += nbt;
@@ -1878,8 +1968,23 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
     pl.grid[c] = (ns + ctx->wpb_run - 1) / ctx->wpb_run;
     pl.pbase[c + 1] = pl.pbase[c] + pl.grid[c];
   }
-  pl.np = pl.pbase[ncol];
   pl.prof = par->profile != 0;
+  // the trailing colours of at most GROUP_NW slices each (two or more of them,
+  // at most ColourGroup::MAX) run as one launch; not in profile mode, which
+  // times every colour
+  pl.gc = ncol;
+  if (!pl.prof)
+  {
+    int c = ncol, nonempty = 0;
+    while (c > 0 && ncol - (c - 1) <= ColourGroup::MAX && ctx->h_cslice[c] - ctx->h_cslice[c - 1] <= GROUP_NW)
+    {
+      c --;
+      if (ctx->h_cslice[c + 1] > ctx->h_cslice[c]) nonempty ++;
+    }
+    if (nonempty >= 2) pl.gc = c;
+  }
+  pl.gslot = pl.pbase[pl.gc];
+  pl.np = pl.gc < ncol ? pl.gslot + 1 : pl.pbase[ncol];
   const int mgrid = (ctx->nslices + TPB/SLICE - 1) / (TPB/SLICE);
   CHK(ensure(ctx, ctx->partials, 3*(size_t)std::max(pl.np, 1)));
   CHK(ensure(ctx, ctx->mpartials, (size_t)std::max(mgrid, 1)));
@@ -2068,6 +2173,9 @@ int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *me
     rep->row_updates = (long long)(hc.sweeps + hr.sweeps) * n;   // every sweep run, the merit's extra one included
     rep->nnz = ctx->nnz;
     rep->fp64_sweeps = mixed ? hr.sweeps : hc.sweeps;
+    int lps = pl.gc < ncol ? 1 : 0;
+    for (int c = 0; c < std::min(pl.gc, ncol); c ++) lps += pl.grid[c] > 0 ? 1 : 0;
+    rep->launches_per_sweep = flow || bar ? 0 : lps;
     rep->error = fk ? (int)(fk & 3ull) : SFGS_GS_OK;
     if (!rep->aborted && iters >= maxiter) rep->error = SFGS_GS_DIVERGED;
   }

@@ -29,7 +29,8 @@ class SFGS_PARAM(C.Structure):
 class SFGS_REPORT(C.Structure):
     _fields_ = [("iters", _i), ("error", _i), ("rel_error", _d), ("merit", _d), ("ncolours", _i),
                 ("fail_rows", _i), ("aborted", _i), ("solve_ms", _d), ("sweep_ms", _d),
-                ("sweep_launches", _ll), ("row_updates", _ll), ("nnz", _ll), ("fp64_sweeps", _i)]
+                ("sweep_launches", _ll), ("row_updates", _ll), ("nnz", _ll), ("fp64_sweeps", _i),
+                ("launches_per_sweep", _i)]
 
 
 class SFGS_NEWTON_PARAM(C.Structure):

@@ -487,6 +487,17 @@ __global__ __launch_bounds__(TPB) void k_ufinal(int nslices, const int *row0, co
   }
 }
 
+// U = B + W R (the merit's update of U, mrf.c:61-76) of the rows of every
+// scene whose final U is a merit pass's (final_U = 1), from the final R in the
+// canonical buffer: the per-colour sweeps do not store that U every sweep
+template <int G>
+__global__ __launch_bounds__(TPB) void k_umerit(MeritArgs a, int nslices, const int *sscene, const BatchOut *out)
+{
+  const int s = blockIdx.x*(TPB/SLICE) + (threadIdx.x >> 6);
+  double up = 0.0;
+  merit_slice<G>(a, s, s < nslices && out[sscene[s < nslices ? s : 0]].final_U == 1, up);
+}
+
 struct Tuned { SFGS_BATCH_TUNE t; int valid; };
 static Tuned g_default = {{2, 4, 0, 1}, 0};
 
@@ -574,6 +585,10 @@ static int run_union(sfgs_ctx *ctx, const SFGS_PARAM *par, const BatchArgs<T> &b
       {
         const int kk = k + j, back = par->reverse && (kk & 1);
         SweepArgs<TT> a = base;
+        // a merit sweep's U is final only if FAILURE_EXIT stops it; the merit
+        // pass's U is formed again from the final R (k_umerit)
+        a.store_U = a.failure == SFGS_FAILURE_EXIT;
+        a.store_Um = 0;
         a.P = (kk & 1) ? X1 : X0;
         a.Q = (kk & 1) ? X0 : X1;
         const bool merit = !nomerit && kk > 0;
@@ -603,6 +618,12 @@ static int run_union(sfgs_ctx *ctx, const SFGS_PARAM *par, const BatchArgs<T> &b
     return 0;
   };
 
+  // the final U of the scenes that stopped on the test, from the canonical R
+  const MeritArgs ma = merit_args(ctx, 1, nullptr);
+  auto merit_U = [&]() {
+    hipLaunchKernelGGL(k_umerit<G>, dim3(sblk), dim3(TPB), 0, st, ma, ns, (const int*)ctx->u_sscene.p,
+                       (const BatchOut*)out);
+  };
   if (!mixed)
   {
     CHK(phase(b.a, b.X0, b.X1, maxiter, 0, par->epsilon, par->meritval));
@@ -610,6 +631,7 @@ static int run_union(sfgs_ctx *ctx, const SFGS_PARAM *par, const BatchArgs<T> &b
     hipLaunchKernelGGL((k_ufinal<T, G>), dim3(sblk), dim3(TPB), 0, st, ns, (const int*)ctx->slice_row0.p,
                        (const int*)ctx->slice_nrows.p, (const int*)ctx->u_sscene.p, (const BatchOut*)out,
                        (const T*)b.X0, (const T*)b.X1, (const T*)b.Ut, (const T*)b.Umt, b.R64, b.U64, n, 0);
+    merit_U();
     HIPCHK(hipGetLastError());
     return 0;
   }
@@ -620,6 +642,7 @@ static int run_union(sfgs_ctx *ctx, const SFGS_PARAM *par, const BatchArgs<T> &b
     hipLaunchKernelGGL((k_ufinal<T, G>), dim3(sblk), dim3(TPB), 0, st, ns, (const int*)ctx->slice_row0.p,
                        (const int*)ctx->slice_nrows.p, (const int*)ctx->u_sscene.p, (const BatchOut*)out,
                        (const T*)b.X0, (const T*)b.X1, (const T*)b.Ut, (const T*)b.Umt, b.R64, b.U64, n, 1);
+    merit_U();   // in FP64, the FP64 phase's start
   }
   // FP64 phase per scene under the reference's test
   hipLaunchKernelGGL(k_uphase, dim3((nsc + TPB - 1)/TPB), dim3(TPB), 0, st, nsc, usc, maxiter, out);
@@ -627,6 +650,7 @@ static int run_union(sfgs_ctx *ctx, const SFGS_PARAM *par, const BatchArgs<T> &b
   hipLaunchKernelGGL(k_ufinal64<G>, dim3(sblk), dim3(TPB), 0, st, ns, (const int*)ctx->slice_row0.p,
                      (const int*)ctx->slice_nrows.p, (const int*)ctx->u_sscene.p, (const BatchOut*)out,
                      (const double*)b.R64b, (const double*)b.Um64, b.R64, b.U64, n);
+  merit_U();
   HIPCHK(hipGetLastError());
   return 0;
 }

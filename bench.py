@@ -308,9 +308,10 @@ def stream_copy_rate(mib=1024, reps=10):
 
 
 def measured_traffic(kernel, args):
-    """HBM bytes per launch of the headline kernel from the committed rocprofv3 --pmc
-    summary of this workload and shape (FETCH_SIZE and WRITE_SIZE scaled by a
-    device copy of known size), or None when none matches."""
+    """HBM bytes per colour of the headline sweep (a sweep's bytes over its
+    colours) from the committed rocprofv3 --pmc summary of this workload and shape
+    (FETCH_SIZE and WRITE_SIZE of every launch, scaled by a device copy of known
+    size), or None when none matches."""
     import re
     path = os.path.join(ROOT, "profiles", "k_sweep_config2_traffic.md")
     if args.schedule == 1 or args.shape != "sphere" or not os.path.exists(path):

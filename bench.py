@@ -478,7 +478,7 @@ def iso_merit(res, ref_merit):
     res["ms_to_merit_ref"] = float(np.mean(res["device_ms"])) * k / res["iters"][-1] if k else None
 
 
-def main():
+def make_parser():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -522,7 +522,11 @@ def main():
                     help="learned warm start only on new contacts (default: the shipped model's variant)")
     ap.add_argument("--ws-scale", type=float, default=None,
                     help="trust factor of the learned warm start (default: the shipped model's variant)")
-    args = ap.parse_args()
+    return ap
+
+
+def main():
+    args = make_parser().parse_args()
     if args.ws_new_only is None or args.ws_scale is None:
         from solfec import learned
         meta = learned.load_meta() if os.path.exists(learned.DEFAULT_MODEL) else {}

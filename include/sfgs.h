@@ -133,7 +133,11 @@ int SFGS_System_Upload(sfgs_ctx *ctx, int n, const int *rowptr, const int *col, 
                        double step, int dynamic, const double *R, const double *U);
 
 /* GAUSS_SEIDEL_Solve (bgs.h:119, serial bgs.c:913-1069) on the current
- * system.  rerhist / merhist (may be NULL) receive par->maxiter doubles. */
+ * system.  rerhist / merhist (may be NULL) receive par->maxiter doubles.  On
+ * return R and U (SFGS_Get_RU) are the reference's final state: when the solve
+ * stops on its test or at maxiter with the merit on, U is the merit's update
+ * U = B + W R of the final R (mrf.c:61-76); otherwise (no merit, or
+ * FAILURE_EXIT) the local solver's U of the last sweep. */
 int SFGS_Solve(sfgs_ctx *ctx, const SFGS_PARAM *par, double *rerhist, double *merhist, SFGS_REPORT *rep);
 
 /* MERIT_Function (mrf.h:31, mrf.c:35-184) for the current R; update_U as in
